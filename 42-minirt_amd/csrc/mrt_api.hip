@@ -1,0 +1,793 @@
+// C ABI of libmrt_hip.so: context, scene upload, the wavefront render loop
+// and read-back.  Everything here is host code around the kernels of
+// mrt_kernels.hip; see include/mrt.h for the contract of each entry point.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mrt_kernels.h"
+
+using namespace mrt;
+
+struct mrt_ctx {
+    int          device = 0;
+    hipStream_t  stream = nullptr;
+    uint32_t     W = 0, H = 0;
+    char         err[512] = {0};
+    int          num_cu = 0;
+    uint32_t     trace_blocks = 0, shadow_blocks = 0, grid_blocks = 0;
+
+    // scene
+    float4      *d_nodes = nullptr, *d_prims = nullptr, *d_planes = nullptr;
+    float4      *d_mats = nullptr, *d_lights = nullptr;
+    float       *d_nrm = nullptr;
+    uint32_t     n_nodes = 0, n_prims = 0, n_planes = 0, n_mats = 0;
+    uint32_t     bvh_depth = 0;
+    std::vector<mrt_light> lights;
+    bool         has_scene = false;
+    mrt_camera   cam{};
+    float        ambient_ratio = 0.0f, ambient_color[3] = {0, 0, 0};
+    float        ior_all = 1.0f;
+    uint32_t     flags = 0;
+
+    // accumulation
+    float       *d_accum = nullptr;
+    uint32_t    *d_ctr = nullptr;
+    unsigned long long *d_stats = nullptr;
+
+    // wavefront buffers
+    uint32_t     cap_paths = 0, cap_shadow = 0;
+    float4      *q_o[2] = {nullptr, nullptr}, *q_d[2] = {nullptr, nullptr};
+    float4      *q_t[2] = {nullptr, nullptr};
+    float4      *hit = nullptr, *acc = nullptr;
+    float4      *s_o = nullptr, *s_d = nullptr;
+    float2      *s_c = nullptr;
+    uint8_t     *vis = nullptr;
+    float4      *p_a = nullptr, *p_c = nullptr;
+    uint4       *p_b = nullptr;
+
+    // options
+    uint64_t     seed = 0x6D696E6952542121ull;
+    int          estimator = MRT_EST_REFERENCE;
+    int          stats_on = 0;
+    uint32_t     path_budget = 1u << 22;
+
+    // statistics
+    uint64_t     paths = 0;
+    double       ms[6] = {0, 0, 0, 0, 0, 0};
+};
+
+static int fail(mrt_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof(c->err), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                    \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            return fail((ctx), MRT_E_HIP, "%s failed: %s", #call,            \
+                        hipGetErrorString(e_));                              \
+    } while (0)
+
+static inline float bits_f(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, sizeof(f));
+    return f;
+}
+
+template <typename T>
+static void dfree(T *&p)
+{
+    if (p)
+        (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T>
+static hipError_t dalloc(T *&p, size_t count)
+{
+    dfree(p);
+    if (count == 0)
+        count = 1;
+    return hipMalloc((void **)&p, count * sizeof(T));
+}
+
+extern "C" const char *mrt_version(void) { return "mrt_hip 0.1 gfx950"; }
+
+extern "C" const char *mrt_last_error(const mrt_ctx *ctx)
+{
+    return ctx ? ctx->err : "null context";
+}
+
+extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
+                        mrt_ctx **out)
+{
+    if (!out || width == 0 || height == 0
+        || (uint64_t)width * height > (1ull << 28))
+        return MRT_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return MRT_E_NODEVICE;
+    if (device < 0 || device >= ndev)
+        return MRT_E_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return MRT_E_NODEVICE;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return MRT_E_NODEVICE;
+    mrt_ctx *c = new (std::nothrow) mrt_ctx();
+    if (!c)
+        return MRT_E_NOMEM;
+    c->device = device;
+    c->W = width;
+    c->H = height;
+    c->num_cu = prop.multiProcessorCount;
+    int rc = MRT_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess
+            || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)
+                   != hipSuccess) {
+            rc = MRT_E_HIP;
+            break;
+        }
+        size_t npx = (size_t)width * height;
+        if (dalloc(c->d_accum, npx * 3) != hipSuccess
+            || dalloc(c->d_ctr, CTR_COUNT) != hipSuccess
+            || dalloc(c->d_stats, STAT_COUNT) != hipSuccess) {
+            rc = MRT_E_NOMEM;
+            break;
+        }
+        if (hipMemsetAsync(c->d_accum, 0, npx * 3 * sizeof(float), c->stream)
+                != hipSuccess
+            || hipMemsetAsync(c->d_ctr, 0, CTR_COUNT * sizeof(uint32_t),
+                              c->stream) != hipSuccess
+            || hipMemsetAsync(c->d_stats, 0, STAT_COUNT * 8, c->stream)
+                   != hipSuccess
+            || hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = MRT_E_HIP;
+            break;
+        }
+        c->trace_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu(false));
+        c->shadow_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu(true));
+        c->grid_blocks = (uint32_t)c->num_cu * 8u;
+    } while (0);
+    if (rc != MRT_OK) {
+        mrt_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return MRT_OK;
+}
+
+static void free_wavefront(mrt_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        dfree(c->q_o[k]);
+        dfree(c->q_d[k]);
+        dfree(c->q_t[k]);
+    }
+    dfree(c->hit);
+    dfree(c->acc);
+    dfree(c->s_o);
+    dfree(c->s_d);
+    dfree(c->s_c);
+    dfree(c->vis);
+    dfree(c->p_a);
+    dfree(c->p_b);
+    dfree(c->p_c);
+    c->cap_paths = c->cap_shadow = 0;
+}
+
+static void free_scene(mrt_ctx *c)
+{
+    dfree(c->d_nodes);
+    dfree(c->d_prims);
+    dfree(c->d_nrm);
+    dfree(c->d_planes);
+    dfree(c->d_mats);
+    dfree(c->d_lights);
+    c->has_scene = false;
+}
+
+extern "C" void mrt_destroy(mrt_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    free_wavefront(c);
+    free_scene(c);
+    dfree(c->d_accum);
+    dfree(c->d_ctr);
+    dfree(c->d_stats);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int upload(mrt_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes)
+        return MRT_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
+{
+    if (!c || !s)
+        return MRT_E_INVALID;
+    if (s->n_materials == 0 || !s->materials)
+        return fail(c, MRT_E_INVALID, "scene has no materials");
+    if (s->n_lights > 64 || (s->n_lights && !s->lights))
+        return fail(c, MRT_E_LIMIT, "at most 64 lights");
+    for (uint32_t i = 0; i < s->n_triangles; i++)
+        if (s->tri_mat[i] >= s->n_materials)
+            return fail(c, MRT_E_INVALID, "triangle %u: material out of range", i);
+    for (uint32_t i = 0; i < s->n_spheres; i++)
+        if (s->sph_mat[i] >= s->n_materials || !(s->sph[4 * i + 3] > 0.0f))
+            return fail(c, MRT_E_INVALID, "sphere %u: bad material or radius", i);
+    if (s->n_planes && (!s->pln || !s->pln_mat))
+        return fail(c, MRT_E_INVALID, "planes without arrays");
+    for (uint32_t i = 0; i < s->n_planes; i++)
+        if (s->pln_mat[i] >= s->n_materials)
+            return fail(c, MRT_E_INVALID, "plane %u: material out of range", i);
+    for (uint32_t k = 0; k < s->n_lights; k++)
+        if (s->lights[k].shadow_samples > 4096)
+            return fail(c, MRT_E_LIMIT, "light %u: more than 4096 shadow samples", k);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_scene(c);
+
+    mrt_bvh bvh;
+    int rc = mrt_bvh_build(s, MRT_BVH_MAX_LEAF, &bvh);
+    if (rc != MRT_OK)
+        return fail(c, rc, "BVH build failed (%d)", rc);
+    if (bvh.depth > MRT_BVH_MAX_DEPTH) {
+        mrt_bvh_free(&bvh);
+        return fail(c, MRT_E_LIMIT, "BVH depth %u exceeds %u", bvh.depth,
+                    MRT_BVH_MAX_DEPTH);
+    }
+    std::vector<float4> planes(2 * (size_t)s->n_planes);
+    for (uint32_t k = 0; k < s->n_planes; k++) {
+        const float *q = s->pln + 6 * (size_t)k;
+        planes[2 * k] = make_float4(q[0], q[1], q[2], bits_f(s->pln_mat[k]));
+        planes[2 * k + 1] = make_float4(q[3], q[4], q[5], 0.0f);
+    }
+    std::vector<float4> mats(3 * (size_t)s->n_materials);
+    for (uint32_t k = 0; k < s->n_materials; k++) {
+        const mrt_material &m = s->materials[k];
+        mats[3 * k] = make_float4(m.color[0], m.color[1], m.color[2], m.metallic);
+        mats[3 * k + 1] = make_float4(m.roughness, m.ior, m.transmission, m.ao);
+        mats[3 * k + 2] = make_float4(m.emission_strength, m.emission_color[0],
+                                      m.emission_color[1], m.emission_color[2]);
+    }
+    std::vector<float4> lights(3 * (size_t)(s->n_lights ? s->n_lights : 1));
+    for (uint32_t k = 0; k < s->n_lights; k++) {
+        const mrt_light &l = s->lights[k];
+        lights[3 * k] = make_float4(l.vec[0], l.vec[1], l.vec[2], l.brightness);
+        lights[3 * k + 1] = make_float4(l.color[0], l.color[1], l.color[2], l.radius);
+        lights[3 * k + 2] = make_float4(bits_f((uint32_t)l.type),
+                                        bits_f((uint32_t)l.shadow_samples), 0.0f, 0.0f);
+    }
+    rc = MRT_OK;
+    if (dalloc(c->d_nodes, (size_t)bvh.n_nodes * 4) != hipSuccess
+        || dalloc(c->d_prims, (size_t)bvh.n_prims * 3) != hipSuccess
+        || dalloc(c->d_nrm, (size_t)bvh.n_prims * 9) != hipSuccess
+        || dalloc(c->d_planes, planes.size()) != hipSuccess
+        || dalloc(c->d_mats, mats.size()) != hipSuccess
+        || dalloc(c->d_lights, lights.size()) != hipSuccess)
+        rc = fail(c, MRT_E_NOMEM, "device allocation for the scene failed");
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_nodes, bvh.nodes, (size_t)bvh.n_nodes * 64);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_prims, bvh.prims, (size_t)bvh.n_prims * 48);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_nrm, bvh.shade_nrm, (size_t)bvh.n_prims * 36);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_planes, planes.data(), planes.size() * 16);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_mats, mats.data(), mats.size() * 16);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_lights, lights.data(), lights.size() * 16);
+    if (rc == MRT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(c, MRT_E_HIP, "scene upload failed");
+    c->n_nodes = bvh.n_nodes;
+    c->n_prims = bvh.n_prims;
+    c->bvh_depth = bvh.depth;
+    mrt_bvh_free(&bvh);
+    if (rc != MRT_OK) {
+        free_scene(c);
+        return rc;
+    }
+    c->n_planes = s->n_planes;
+    c->n_mats = s->n_materials;
+    c->lights.assign(s->lights, s->lights + s->n_lights);
+    c->cam = s->camera;
+    c->ambient_ratio = s->ambient_ratio;
+    memcpy(c->ambient_color, s->ambient_color, sizeof(c->ambient_color));
+    c->ior_all = s->ior_all > 0.0f ? s->ior_all : 1.0f;
+    c->flags = s->flags;
+    c->has_scene = true;
+    return MRT_OK;
+}
+
+extern "C" int mrt_set_camera(mrt_ctx *c, const mrt_camera *cam)
+{
+    if (!c || !cam)
+        return MRT_E_INVALID;
+    c->cam = *cam;
+    return MRT_OK;
+}
+
+extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *, uint32_t, uint32_t,
+                               float)
+{
+    return fail(c, MRT_E_UNSUPPORTED, "HDRI environment not built yet");
+}
+
+extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t, float)
+{
+    return fail(c, MRT_E_UNSUPPORTED, "photon map not built yet");
+}
+
+// init_viewport_values / init_viewport_vectors (src/ray_tracing/viewport.c:
+// 38-88) in double, rounded to float for the kernels.
+static FrameParams frame_params(const mrt_ctx *c)
+{
+    FrameParams fp;
+    memset(&fp, 0, sizeof(fp));
+    const mrt_camera &cam = c->cam;
+    double dir[3] = {cam.dir[0], cam.dir[1], cam.dir[2]};
+    double l = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (int k = 0; k < 3; k++)
+        dir[k] = l > 0.0 ? dir[k] / l : 0.0;
+    double focus = cam.focus_dist > 0.0f ? cam.focus_dist : 1.0;
+    double width = 2.0 * tan((double)cam.fov_deg * (M_PI / 180.0) / 2.0) * focus;
+    double height = width / ((double)c->W / (double)c->H);
+    double up[3] = {0, 1, 0};
+    if (fabs(dir[1]) > 0.999) {
+        up[0] = 1;
+        up[1] = 0;
+    }
+    double nd[3] = {-dir[0], -dir[1], -dir[2]};
+    double u[3] = {up[1] * nd[2] - up[2] * nd[1], up[2] * nd[0] - up[0] * nd[2],
+                   up[0] * nd[1] - up[1] * nd[0]};
+    l = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int k = 0; k < 3; k++)
+        u[k] = l > 0.0 ? u[k] / l : 0.0;
+    double cv[3] = {dir[1] * u[2] - dir[2] * u[1], dir[2] * u[0] - dir[0] * u[2],
+                    dir[0] * u[1] - dir[1] * u[0]};
+    double rad = focus * tan((double)cam.defocus_angle_deg * M_PI / 360.0);
+    for (int k = 0; k < 3; k++) {
+        double vu = u[k] * width, vv = cv[k] * height;
+        double du = vu / (double)c->W, dv = vv / (double)c->H;
+        double ul = ((double)cam.pos[k] + dir[k] * focus) - vu / 2 - vv / 2;
+        fp.cam_o[k] = cam.pos[k];
+        fp.p00[k] = (float)(ul + (du + dv) * 0.5);
+        fp.du[k] = (float)du;
+        fp.dv[k] = (float)dv;
+        fp.disk_u[k] = (float)(u[k] * rad);
+        fp.disk_v[k] = (float)(cv[k] * rad);
+        fp.cam_pos[k] = cam.pos[k];
+        double bg = c->ambient_color[k] / 255.0 * (double)c->ambient_ratio;
+        float bgf = (float)bg;
+        fp.bg_clamped[k] = bgf < 0.0f ? 0.0f : (bgf > 16.0f ? 16.0f : bgf);
+    }
+    fp.lens = rad > 0.0;
+    fp.ior_all = c->ior_all;
+    fp.W = c->W;
+    fp.H = c->H;
+    fp.WH = c->W * c->H;
+    fp.n_lights = (uint32_t)c->lights.size();
+    fp.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
+    fp.flags = c->flags;
+    fp.seed = c->seed;
+    return fp;
+}
+
+static SceneDev scene_dev(const mrt_ctx *c)
+{
+    SceneDev S;
+    S.nodes = c->d_nodes;
+    S.prims = c->d_prims;
+    S.planes = c->d_planes;
+    S.n_nodes = c->n_nodes;
+    S.n_planes = c->n_planes;
+    S.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
+    return S;
+}
+
+static uint32_t shadow_rays_per_path(const mrt_ctx *c)
+{
+    uint32_t n = 0;
+    for (const mrt_light &l : c->lights) {
+        bool soft = l.type == MRT_LIGHT_POINT && l.radius != 0.0f
+                    && l.shadow_samples > 1;
+        n += soft ? (uint32_t)l.shadow_samples : 1u;
+    }
+    return n ? n : 1u;
+}
+
+static int ensure_wavefront(mrt_ctx *c, uint32_t paths, uint32_t shadow)
+{
+    if (paths <= c->cap_paths && shadow <= c->cap_shadow)
+        return MRT_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_wavefront(c);
+    bool ok = true;
+    for (int k = 0; k < 2; k++)
+        ok = ok && dalloc(c->q_o[k], paths) == hipSuccess
+             && dalloc(c->q_d[k], paths) == hipSuccess
+             && dalloc(c->q_t[k], paths) == hipSuccess;
+    ok = ok && dalloc(c->hit, paths) == hipSuccess
+         && dalloc(c->acc, paths) == hipSuccess
+         && dalloc(c->p_a, paths) == hipSuccess
+         && dalloc(c->p_b, paths) == hipSuccess
+         && dalloc(c->p_c, paths) == hipSuccess
+         && dalloc(c->s_o, shadow) == hipSuccess
+         && dalloc(c->s_d, shadow) == hipSuccess
+         && dalloc(c->s_c, shadow) == hipSuccess
+         && dalloc(c->vis, shadow) == hipSuccess;
+    if (!ok) {
+        free_wavefront(c);
+        return fail(c, MRT_E_NOMEM, "wavefront buffers (%u paths) do not fit",
+                    paths);
+    }
+    c->cap_paths = paths;
+    c->cap_shadow = shadow;
+    return MRT_OK;
+}
+
+namespace {
+enum { K_RAYGEN = 0, K_TRACE, K_SHADE, K_SHADOW, K_ACCUM, K_RESOLVE };
+struct EvPair {
+    int         cls;
+    hipEvent_t  a, b;
+};
+struct Timer {
+    mrt_ctx              *c;
+    bool                  on;
+    std::vector<EvPair>   ev;
+    void begin(int cls)
+    {
+        if (!on)
+            return;
+        EvPair p{cls, nullptr, nullptr};
+        (void)hipEventCreate(&p.a);
+        (void)hipEventCreate(&p.b);
+        (void)hipEventRecord(p.a, c->stream);
+        ev.push_back(p);
+    }
+    void end()
+    {
+        if (on && !ev.empty())
+            (void)hipEventRecord(ev.back().b, c->stream);
+    }
+    void collect()
+    {
+        if (!on)
+            return;
+        (void)hipStreamSynchronize(c->stream);
+        for (EvPair &p : ev) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                int slot = p.cls == K_RESOLVE ? K_SHADE : p.cls;
+                c->ms[slot] += ms;
+            }
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        ev.clear();
+    }
+};
+}  // namespace
+
+#define LAUNCH(ctx, tm, cls, expr)                                           \
+    do {                                                                     \
+        (tm).begin(cls);                                                     \
+        hipError_t e_ = (expr);                                              \
+        (tm).end();                                                          \
+        if (e_ != hipSuccess) {                                              \
+            (tm).collect();                                                  \
+            return fail((ctx), MRT_E_HIP, "kernel launch failed: %s",        \
+                        hipGetErrorString(e_));                              \
+        }                                                                    \
+    } while (0)
+
+extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
+                          uint32_t max_depth)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "mrt_render before mrt_upload_scene");
+    if (max_depth > 4095)
+        return fail(c, MRT_E_LIMIT, "max_depth above 4095");
+    if (n_samples == 0 || max_depth == 0)
+        return MRT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t WH = c->W * c->H;
+    uint32_t nb = c->path_budget / WH;
+    if (nb < 1)
+        nb = 1;
+    if (nb > n_samples)
+        nb = n_samples;
+    uint64_t P64 = (uint64_t)nb * WH;
+    uint64_t S64 = P64 * shadow_rays_per_path(c);
+    if (S64 > 0xFFFFFFF0ull)
+        return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
+    int rc = ensure_wavefront(c, (uint32_t)P64, (uint32_t)S64);
+    if (rc != MRT_OK)
+        return rc;
+    FrameParams fp = frame_params(c);
+    SceneDev S = scene_dev(c);
+    const bool count = (c->stats_on & MRT_STATS_COUNT) != 0;
+    const bool timing = (c->stats_on & MRT_STATS_TIME) != 0;
+    Timer tm{c, timing, {}};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing) {
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        (void)hipEventRecord(t0, c->stream);
+    }
+    for (uint32_t s = 0; s < n_samples; s += nb) {
+        uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
+        uint32_t P = n * WH;
+        fp.sample0 = first_sample + s;
+        LAUNCH(c, tm, K_RAYGEN,
+               launch_raygen(c->stream, fp, c->q_o[0], c->q_d[0], c->q_t[0],
+                             c->acc, P, c->d_ctr));
+        for (uint32_t b = 0; b < max_depth; b++) {
+            int cur = b & 1, nxt = cur ^ 1;
+            fp.bounce = b;
+            fp.last_bounce = (b + 1 == max_depth) ? 1u : 0u;
+            ShadeBufs B;
+            B.q_o = c->q_o[cur];
+            B.q_d = c->q_d[cur];
+            B.q_t = c->q_t[cur];
+            B.hit = c->hit;
+            B.qn_o = c->q_o[nxt];
+            B.qn_d = c->q_d[nxt];
+            B.qn_t = c->q_t[nxt];
+            B.acc = c->acc;
+            B.planes = c->d_planes;
+            B.prims = c->d_prims;
+            B.mats = c->d_mats;
+            B.lights = c->d_lights;
+            B.nrm = c->d_nrm;
+            B.s_o = c->s_o;
+            B.s_d = c->s_d;
+            B.s_c = c->s_c;
+            B.vis = c->vis;
+            B.p_a = c->p_a;
+            B.p_b = c->p_b;
+            B.p_c = c->p_c;
+            uint32_t gb = (P + 255) / 256;
+            if (gb > c->grid_blocks)
+                gb = c->grid_blocks;
+            LAUNCH(c, tm, K_TRACE,
+                   launch_trace(c->stream, c->trace_blocks, count, S,
+                                c->q_o[cur], c->q_d[cur], c->hit, c->d_ctr,
+                                c->d_stats));
+            LAUNCH(c, tm, K_SHADE, launch_shade(c->stream, gb, fp, B, c->d_ctr));
+            if (!c->lights.empty()) {
+                LAUNCH(c, tm, K_SHADOW,
+                       launch_shadow(c->stream, c->shadow_blocks, count, S,
+                                     c->s_o, c->s_d, c->vis, c->d_ctr,
+                                     c->d_stats));
+                LAUNCH(c, tm, K_RESOLVE,
+                       launch_resolve(c->stream, gb, fp, B, c->d_ctr));
+            }
+            LAUNCH(c, tm, K_SHADE, launch_advance(c->stream, c->d_ctr, c->d_stats));
+        }
+        LAUNCH(c, tm, K_ACCUM,
+               launch_accumulate(c->stream, c->acc, c->d_accum, WH, n,
+                                 c->d_stats));
+    }
+    c->paths += (uint64_t)n_samples * WH;
+    if (timing) {
+        (void)hipEventRecord(t1, c->stream);
+        tm.collect();
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess)
+            c->ms[5] += ms;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+    }
+    return MRT_OK;
+}
+
+extern "C" int mrt_synchronize(mrt_ctx *c)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_readback_f32(mrt_ctx *c, float *out)
+{
+    if (!c || !out)
+        return MRT_E_INVALID;
+    size_t n = (size_t)c->W * c->H * 3;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_accum, n * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_readback_add(mrt_ctx *c, double *float_render)
+{
+    if (!c || !float_render)
+        return MRT_E_INVALID;
+    size_t n = (size_t)c->W * c->H * 3;
+    std::vector<float> tmp(n);
+    int rc = mrt_readback_f32(c, tmp.data());
+    if (rc != MRT_OK)
+        return rc;
+    for (size_t i = 0; i < n; i++)
+        float_render[i] += (double)tmp[i];
+    return MRT_OK;
+}
+
+extern "C" int mrt_clear(mrt_ctx *c)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    size_t n = (size_t)c->W * c->H * 3;
+    HIPCHK(c, hipMemsetAsync(c->d_accum, 0, n * sizeof(float), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_render_aov(mrt_ctx *c, int aov, uint32_t sample, void *out)
+{
+    if (!c || !out || aov < MRT_AOV_PRIM_ID || aov > MRT_AOV_PRIMS)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "mrt_render_aov before mrt_upload_scene");
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t WH = (size_t)c->W * c->H;
+    size_t bytes = WH * (aov == MRT_AOV_NORMAL ? 12 : 4);
+    void *d = nullptr;
+    HIPCHK(c, hipMalloc(&d, bytes));
+    FrameParams fp = frame_params(c);
+    SceneDev S = scene_dev(c);
+    hipError_t e = launch_aov(c->stream, fp, S, c->d_prims, c->d_nrm, sample,
+                              aov, d);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return fail(c, MRT_E_HIP, "AOV render failed: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+extern "C" int mrt_trace_rays(mrt_ctx *c, uint32_t n, const float *orig,
+                              const float *dir, uint32_t *prim_id, float *t)
+{
+    if (!c || (n && (!orig || !dir || !prim_id || !t)))
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "mrt_trace_rays before mrt_upload_scene");
+    if (n == 0)
+        return MRT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    float *d_o = nullptr, *d_d = nullptr, *d_t = nullptr;
+    uint32_t *d_p = nullptr;
+    size_t v = (size_t)n * 3 * sizeof(float);
+    hipError_t e = hipMalloc((void **)&d_o, v);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&d_d, v);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&d_t, (size_t)n * 4);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&d_p, (size_t)n * 4);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_o, orig, v, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_d, dir, v, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = launch_trace_rays(c->stream, scene_dev(c), c->d_prims, n, d_o, d_d,
+                              d_p, d_t);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(prim_id, d_p, (size_t)n * 4, hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_o);
+    (void)hipFree(d_d);
+    (void)hipFree(d_t);
+    (void)hipFree(d_p);
+    if (e != hipSuccess)
+        return fail(c, MRT_E_HIP, "trace_rays failed: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    switch (option) {
+    case MRT_OPT_SEED:
+        c->seed = (uint64_t)value;
+        return MRT_OK;
+    case MRT_OPT_ESTIMATOR:
+        if (value != MRT_EST_REFERENCE && value != MRT_EST_MIS)
+            return fail(c, MRT_E_INVALID, "unknown estimator");
+        c->estimator = (int)value;
+        return MRT_OK;
+    case MRT_OPT_STATS:
+        if (!(value >= 0.0) || value > 3.0)
+            return fail(c, MRT_E_INVALID, "stats flags out of range");
+        c->stats_on = (int)value;
+        return MRT_OK;
+    case MRT_OPT_PATH_BUDGET:
+        if (!(value >= 1.0) || value > 1.0e9)
+            return fail(c, MRT_E_INVALID, "path budget out of range");
+        c->path_budget = (uint32_t)value;
+        return MRT_OK;
+    case MRT_OPT_BLOCK:
+        return fail(c, MRT_E_UNSUPPORTED, "block size is fixed in this build");
+    default:
+        return fail(c, MRT_E_INVALID, "unknown option %d", option);
+    }
+}
+
+extern "C" int mrt_get_stats(mrt_ctx *c, mrt_stats *out)
+{
+    if (!c || !out)
+        return MRT_E_INVALID;
+    unsigned long long st[STAT_COUNT];
+    HIPCHK(c, hipMemcpyAsync(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    out->paths = c->paths;
+    out->rays_closest = st[STAT_RAYS_CLOSEST];
+    out->rays_shadow = st[STAT_RAYS_SHADOW];
+    out->nodes_closest = st[STAT_NODES_CLOSEST];
+    out->prims_closest = st[STAT_PRIMS_CLOSEST];
+    out->nodes_shadow = st[STAT_NODES_SHADOW];
+    out->prims_shadow = st[STAT_PRIMS_SHADOW];
+    out->nonfinite = st[STAT_NONFINITE];
+    out->ms_raygen = c->ms[K_RAYGEN];
+    out->ms_trace = c->ms[K_TRACE];
+    out->ms_shade = c->ms[K_SHADE];
+    out->ms_shadow = c->ms[K_SHADOW];
+    out->ms_accum = c->ms[K_ACCUM];
+    out->ms_total = c->ms[5];
+    return MRT_OK;
+}
+
+extern "C" int mrt_reset_stats(mrt_ctx *c)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    HIPCHK(c, hipMemsetAsync(c->d_stats, 0, STAT_COUNT * 8, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->paths = 0;
+    for (double &m : c->ms)
+        m = 0.0;
+    return MRT_OK;
+}

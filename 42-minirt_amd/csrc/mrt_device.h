@@ -1,0 +1,342 @@
+// Device-side building blocks of the wavefront path tracer (gfx950):
+// counter RNG, primitive tests and the ordered BVH traversal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mrt.h"
+#include "mrt_host.h"
+
+namespace mrt {
+
+constexpr float IT_MIN_F = 0.001f;    // IT_MIN, includes/minirt.h:44
+constexpr float IT_MAX_F = 1000.0f;   // IT_MAX, includes/minirt.h:43
+constexpr float BOX_WIDEN = 1.0000004f;  // 1 + 2 gamma_3 (conservative slab)
+constexpr int STACK_MAX = 64;         // >= MRT_BVH_MAX_DEPTH + 1
+constexpr uint32_t HIT_NONE = 0xFFFFFFFFu;
+constexpr uint32_t HIT_PLANE = 0x80000000u;   // hit code of plane k: flag | k
+
+// ---- counter-based RNG -----------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter = {pixel, sample,
+// bounce << 20 | block, stream}, key = the 64-bit scene seed.
+struct U4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox_u4(uint32_t c0, uint32_t c1, uint32_t c2,
+                                        uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    const float s = 5.9604644775390625e-08f;   // 2^-24
+    return U4{(float)(c0 >> 8) * s, (float)(c1 >> 8) * s,
+              (float)(c2 >> 8) * s, (float)(c3 >> 8) * s};
+}
+
+__device__ __forceinline__ U4 rng4(uint64_t seed, uint32_t pixel,
+                                   uint32_t sample, uint32_t bounce,
+                                   uint32_t block)
+{
+    return philox_u4(pixel, sample, (bounce << 20) | block, 0u,
+                     (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// ---- small vector helpers (explicit evaluation order) ---------------------
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 had(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b)
+{
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+// vec3_unit: v / |v| per component, zero stays zero
+__device__ __forceinline__ V3 unit(V3 a)
+{
+    float l = sqrtf(dot(a, a));
+    if (l > 0.0f)
+        return v3(a.x / l, a.y / l, a.z / l);
+    return v3(0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ V3 xyz(float4 f) { return v3(f.x, f.y, f.z); }
+
+// ---- primitive tests ------------------------------------------------------
+// Moller-Trumbore as hit_triangle (src/math/triangle/hit_triangle.c:68-96);
+// B = v1 - v0, C = v2 - v0 stored in the record.
+__device__ __forceinline__ bool isect_tri(V3 o, V3 d, float4 A, float4 B,
+                                          float4 C, bool one_sided, float &t,
+                                          float &bu, float &bv)
+{
+    V3 e1 = xyz(B), e2 = xyz(C);
+    V3 h = cross(d, e2);
+    float a = dot(e1, h);
+    if (one_sided ? (a < 1e-6f) : (fabsf(a) < 1e-6f))
+        return false;
+    float f = 1.0f / a;
+    V3 s = sub(o, xyz(A));
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f)
+        return false;
+    V3 q = cross(s, e1);
+    float v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f)
+        return false;
+    float tt = f * dot(e2, q);
+    if (tt < IT_MIN_F || tt > IT_MAX_F)
+        return false;
+    t = tt;
+    bu = u;
+    bv = v;
+    return true;
+}
+
+// hit_sphere's roots (src/math/sphere/hit_sphere.c:31-47, valid_t.c:16-26)
+// in the cancellation-free form (Haines et al., Ray Tracing Gems ch. 7).
+__device__ __forceinline__ bool isect_sph(V3 o, V3 d, float4 A, float4 B,
+                                          float &t)
+{
+    V3 oc = sub(o, xyz(A));
+    float r2 = B.x * B.x;
+    float bq = -dot(oc, d);
+    V3 l = add(oc, mul(d, bq));
+    float disc = r2 - dot(l, l);
+    if (disc < 0.0f)
+        return false;
+    float sq = sqrtf(disc);
+    float qv = bq + copysignf(sq, bq);
+    float cc = dot(oc, oc) - r2;
+    float ta = qv != 0.0f ? cc / qv : 0.0f;
+    float tb = qv;
+    float t0 = fminf(ta, tb), t1 = fmaxf(ta, tb);
+    float tt = t0;
+    if (tt < IT_MIN_F || tt > IT_MAX_F) {
+        tt = t1;
+        if (tt < IT_MIN_F || tt > IT_MAX_F)
+            return false;
+    }
+    t = tt;
+    return true;
+}
+
+// hit_plane (src/math/plane/hit_plane.c:43-67); P = point | mat, N = normal.
+__device__ __forceinline__ bool isect_pln(V3 o, V3 d, float4 P, float4 N, float &t)
+{
+    float denom = N.x * d.x + N.y * d.y + N.z * d.z;
+    float tt = ((P.x - o.x) * N.x + (P.y - o.y) * N.y + (P.z - o.z) * N.z) / denom;
+    if (fabsf(denom) < 1e-8f)
+        return false;
+    if (tt < IT_MIN_F || tt > IT_MAX_F)
+        return false;
+    t = tt;
+    return true;
+}
+
+__device__ __forceinline__ float inv_dir(float d)
+{
+    return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
+}
+
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx,
+                                     float hy, float hz, V3 o, V3 inv,
+                                     float tclose, float &tnear)
+{
+    float tx1 = (lx - o.x) * inv.x, tx2 = (hx - o.x) * inv.x;
+    float ty1 = (ly - o.y) * inv.y, ty2 = (hy - o.y) * inv.y;
+    float tz1 = (lz - o.z) * inv.z, tz2 = (hz - o.z) * inv.z;
+    float tmin = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)),
+                       fmaxf(fminf(tz1, tz2), 0.0f));
+    float tmax = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)),
+                       fminf(fmaxf(tz1, tz2), tclose));
+    tnear = tmin;
+    return tmin <= tmax * BOX_WIDEN;
+}
+
+struct SceneDev {
+    const float4 *nodes;      // 4 per node
+    const float4 *prims;      // 3 per primitive (leaf order)
+    const float4 *planes;     // 2 per plane: point | mat, normal | 0
+    uint32_t      n_nodes;
+    uint32_t      n_planes;
+    uint32_t      one_sided;
+};
+
+// Ordered traversal: near child first, far child on the stack with its
+// entry distance, dropped at pop when beyond the closest hit.  The first
+// SS stack entries live in LDS (lds_code / lds_t, stride `stride`), deeper
+// ones in a private array.  ANY: stop at the first hit below tlimit.
+template <bool ANY, bool COUNT, int SS>
+__device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
+                                         float tlimit, uint32_t *lds_code,
+                                         float *lds_t, int stride,
+                                         float &t_out, uint32_t &hit_slot,
+                                         float &bu_out, float &bv_out,
+                                         uint32_t &nodes, uint32_t &prims)
+{
+    if (S.n_nodes == 0)
+        return false;
+    V3 inv = v3(inv_dir(d.x), inv_dir(d.y), inv_dir(d.z));
+    float tclose = tlimit;
+    bool hit = false;
+    uint32_t ovf_code[STACK_MAX - SS];
+    float ovf_t[STACK_MAX - SS];
+    int sp = 0;
+    int32_t code = 0;
+    for (;;) {
+        if (code >= 0) {
+            const float4 *n = S.nodes + (size_t)code * 4;
+            float4 a = n[0], b = n[1], c = n[2], e = n[3];
+            if (COUNT)
+                nodes++;
+            float t0, t1;
+            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tclose, t0);
+            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tclose, t1);
+            int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+            if (h0 && h1) {
+                bool near1 = t1 < t0;
+                int32_t farc = near1 ? c0 : c1;
+                float fart = near1 ? t0 : t1;
+                if (sp < SS) {
+                    lds_code[sp * stride] = (uint32_t)farc;
+                    lds_t[sp * stride] = fart;
+                } else if (sp < STACK_MAX) {
+                    ovf_code[sp - SS] = (uint32_t)farc;
+                    ovf_t[sp - SS] = fart;
+                }
+                sp = sp < STACK_MAX ? sp + 1 : sp;
+                code = near1 ? c1 : c0;
+                continue;
+            }
+            if (h0) {
+                code = c0;
+                continue;
+            }
+            if (h1) {
+                code = c1;
+                continue;
+            }
+        } else {
+            uint32_t v = ~(uint32_t)code, first = v >> 3, cnt = (v & 7u) + 1u;
+            for (uint32_t i = 0; i < cnt; i++) {
+                const float4 *p = S.prims + (size_t)(first + i) * 3;
+                float4 A = p[0], B = p[1];
+                if (COUNT)
+                    prims++;
+                float tt, uu = 0.0f, vv = 0.0f;
+                bool h;
+                if (__float_as_uint(B.w) == MRT_PRIMKIND_SPHERE) {
+                    h = isect_sph(o, d, A, B, tt);
+                } else {
+                    float4 C = p[2];
+                    h = isect_tri(o, d, A, B, C, S.one_sided != 0, tt, uu, vv);
+                }
+                if (h && tt < tclose) {
+                    tclose = tt;
+                    hit = true;
+                    hit_slot = first + i;
+                    bu_out = uu;
+                    bv_out = vv;
+                    if (ANY) {
+                        t_out = tt;
+                        return true;
+                    }
+                }
+            }
+        }
+        // pop, skipping entries beyond the closest hit
+        for (;;) {
+            if (sp == 0) {
+                t_out = tclose;
+                return hit;
+            }
+            sp--;
+            float et;
+            uint32_t ec;
+            if (sp < SS) {
+                ec = lds_code[sp * stride];
+                et = lds_t[sp * stride];
+            } else {
+                ec = ovf_code[sp - SS];
+                et = ovf_t[sp - SS];
+            }
+            if (et <= tclose * BOX_WIDEN) {
+                code = (int32_t)ec;
+                break;
+            }
+        }
+    }
+}
+
+// Closest hit over BVH + planes (hit_register_all, hit_register.c:109-123).
+// code: leaf-order slot, HIT_PLANE | k, or HIT_NONE.
+template <bool COUNT, int SS>
+__device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
+                                            uint32_t *lds_code, float *lds_t,
+                                            int stride, float &t, uint32_t &code,
+                                            float &bu, float &bv,
+                                            uint32_t &nodes, uint32_t &prims)
+{
+    float tb = IT_MAX_F;
+    uint32_t slot = HIT_NONE;
+    float u = 0.0f, v = 0.0f;
+    bool hit = traverse<false, COUNT, SS>(S, o, d, IT_MAX_F, lds_code, lds_t,
+                                          stride, tb, slot, u, v, nodes, prims);
+    for (uint32_t k = 0; k < S.n_planes; k++) {
+        float tp;
+        if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
+            && (!hit || tp < tb)) {
+            hit = true;
+            tb = tp;
+            slot = HIT_PLANE | k;
+        }
+    }
+    t = tb;
+    code = hit ? slot : HIT_NONE;
+    bu = u;
+    bv = v;
+    return hit;
+}
+
+// Occlusion below tlimit: BVH primitives below min(tlimit, 1000), planes
+// below tlimit (check_plight_hit, src/ray_tracing/light/p_light.c:48-64).
+template <bool COUNT, int SS>
+__device__ __forceinline__ bool occluded(const SceneDev &S, V3 o, V3 d,
+                                         float tlimit, uint32_t *lds_code,
+                                         float *lds_t, int stride,
+                                         uint32_t &nodes, uint32_t &prims)
+{
+    float tl = fminf(tlimit, IT_MAX_F);
+    float t;
+    uint32_t slot;
+    float u, v;
+    if (traverse<true, COUNT, SS>(S, o, d, tl, lds_code, lds_t, stride, t, slot,
+                                  u, v, nodes, prims))
+        return true;
+    for (uint32_t k = 0; k < S.n_planes; k++) {
+        float tp;
+        if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
+            && tp < tlimit)
+            return true;
+    }
+    return false;
+}
+
+}  // namespace mrt

@@ -1,0 +1,95 @@
+// Kernel-side structures shared by mrt_kernels.hip and mrt_api.hip.
+#pragma once
+
+#include "mrt_device.h"
+#include "mrt_host.h"
+
+namespace mrt {
+
+constexpr int TRACE_BLOCK = 128;   // threads per block of the trace kernels
+constexpr int TRACE_SS = 8;        // traversal stack entries kept in LDS
+
+// device counters (uint32)
+enum : int {
+    CTR_CUR = 0,          // paths in the current queue
+    CTR_NEXT = 1,         // paths appended to the next queue
+    CTR_SHADOW = 2,       // shadow rays appended this bounce
+    CTR_PEND = 3,         // pending light sums this bounce
+    CTR_WORK_TRACE = 4,   // work-stealing cursor of k_trace
+    CTR_WORK_SHADOW = 5,  // work-stealing cursor of k_shadow
+    CTR_COUNT = 16
+};
+
+// device statistics (uint64)
+enum : int {
+    STAT_RAYS_CLOSEST = 0,
+    STAT_RAYS_SHADOW = 1,
+    STAT_NODES_CLOSEST = 2,
+    STAT_PRIMS_CLOSEST = 3,
+    STAT_NODES_SHADOW = 4,
+    STAT_PRIMS_SHADOW = 5,
+    STAT_NONFINITE = 6,
+    STAT_COUNT = 16
+};
+
+struct FrameParams {
+    float    cam_o[3], p00[3], du[3], dv[3], disk_u[3], disk_v[3];
+    float    cam_pos[3];
+    float    bg_clamped[3];   // clamp(ambient colour / 255 * ratio, 0, 16)
+    float    ior_all;
+    uint32_t lens;
+    uint32_t W, H, WH;
+    uint32_t sample0;         // sample index of batch slot 0
+    uint32_t bounce;
+    uint32_t last_bounce;
+    uint32_t n_lights;
+    uint32_t one_sided;
+    uint32_t flags;           // MRT_SCENE_*
+    uint64_t seed;
+};
+
+struct ShadeBufs {
+    const float4 *q_o, *q_d, *q_t, *hit;
+    float4       *qn_o, *qn_d, *qn_t;
+    float4       *acc;
+    const float4 *planes, *prims, *mats, *lights;
+    const float  *nrm;
+    float4       *s_o, *s_d;
+    float2       *s_c;
+    uint8_t      *vis;
+    float4       *p_a;
+    uint4        *p_b;
+    float4       *p_c;
+};
+
+hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
+                         float4 *q_d, float4 *q_t, float4 *acc, uint32_t P,
+                         uint32_t *ctr);
+hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
+                        const SceneDev &S, const float4 *q_o,
+                        const float4 *q_d, float4 *hit, uint32_t *ctr,
+                        unsigned long long *stats);
+hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
+                         const SceneDev &S, const float4 *s_o,
+                         const float4 *s_d, uint8_t *vis, uint32_t *ctr,
+                         unsigned long long *stats);
+hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
+                        const ShadeBufs &B, uint32_t *ctr);
+hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
+                          const FrameParams &fp, const ShadeBufs &B,
+                          uint32_t *ctr);
+hipError_t launch_advance(hipStream_t st, uint32_t *ctr,
+                          unsigned long long *stats);
+hipError_t launch_accumulate(hipStream_t st, const float4 *acc, float *accum,
+                             uint32_t WH, uint32_t nb,
+                             unsigned long long *stats);
+hipError_t launch_aov(hipStream_t st, const FrameParams &fp, const SceneDev &S,
+                      const float4 *prims, const float *nrm, uint32_t sample,
+                      int aov, void *out);
+hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
+                             const float4 *prims, uint32_t n,
+                             const float *orig, const float *dir,
+                             uint32_t *prim_id, float *t);
+int trace_blocks_per_cu(bool shadow);
+
+}  // namespace mrt

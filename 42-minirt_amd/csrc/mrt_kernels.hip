@@ -1,0 +1,804 @@
+// Wavefront path-tracing kernels for gfx950 (MI355X).
+//
+// One pass of a batch of paths (nb samples x W*H pixels) runs as
+//   raygen -> { trace -> shade -> shadow -> resolve -> advance } x depth
+//   -> accumulate
+// with every queue in HBM as structure-of-arrays of 16-byte words:
+//   path queue   q_o = (origin, ao of the last material)
+//                q_d = (direction, path slot)
+//                q_t = (throughput, unused)
+//   hit record   hit = (t, hit code, barycentric u, v)
+//   shadow queue s_o = (origin, t limit), s_d = (direction, light tag),
+//                s_c = (n.l, (r.v)^64), vis (1 = unoccluded)
+//   pending      p_a = (albedo * throughput, slot), p_b = (next-queue index,
+//                first shadow ray, shadow-ray count, flags),
+//                p_c = (HDRI radiance term, 1 - cos theta)
+//   per-slot sums acc (float4), summed per pixel in sample order at the end.
+// A path owns its slot, so every sum is race-free and deterministic; queue
+// positions come from wave-aggregated atomics.
+#include "mrt_kernels.h"
+
+namespace mrt {
+
+// ---- wave-aggregated allocation (wave64) ----------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Every lane of the wave calls this (n = 0 for lanes that want nothing);
+// returns the first of the n entries reserved for this lane.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, uint32_t n)
+{
+    uint32_t lane = lane_id();
+    uint32_t incl = n;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off)
+            incl += o;
+    }
+    uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total)
+        base = atomicAdd(counter, total);
+    base = __shfl(base, 63, 64);
+    return base + incl - n;
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v)
+{
+    uint64_t s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor((unsigned long long)s, off, 64);
+    if (lane_id() == 0 && s)
+        atomicAdd(dst, (unsigned long long)s);
+}
+
+// ---- camera rays (calc_one_sample_task, src/ray_tracing/calc_sample.c:44-58)
+__device__ __forceinline__ void primary_ray(const FrameParams &fp,
+                                            uint32_t pixel, uint32_t sample,
+                                            V3 &o, V3 &d)
+{
+    U4 r = rng4(fp.seed, pixel, sample, 0u, 0u);
+    V3 op = v3(fp.cam_o[0], fp.cam_o[1], fp.cam_o[2]);
+    if (fp.lens) {
+        // concentric map of block 0 dims 2,3 (random_in_unit_disk's law)
+        float sx = 2.0f * r.z - 1.0f, sy = 2.0f * r.w - 1.0f;
+        float lx = 0.0f, ly = 0.0f;
+        if (sx != 0.0f || sy != 0.0f) {
+            float rr, phi;
+            if (fabsf(sx) > fabsf(sy)) {
+                rr = sx;
+                phi = 0.78539816339744831f * (sy / sx);
+            } else {
+                rr = sy;
+                phi = 1.57079632679489662f - 0.78539816339744831f * (sx / sy);
+            }
+            lx = rr * cosf(phi);
+            ly = rr * sinf(phi);
+        }
+        op = v3(op.x + (fp.disk_u[0] * lx + fp.disk_v[0] * ly),
+                op.y + (fp.disk_u[1] * lx + fp.disk_v[1] * ly),
+                op.z + (fp.disk_u[2] * lx + fp.disk_v[2] * ly));
+    }
+    float fx = (float)(pixel % fp.W) + r.x;
+    float fy = (float)(pixel / fp.W) + r.y;
+    V3 dd = v3(((fp.p00[0] + fp.du[0] * fx) + fp.dv[0] * fy) - op.x,
+               ((fp.p00[1] + fp.du[1] * fx) + fp.dv[1] * fy) - op.y,
+               ((fp.p00[2] + fp.du[2] * fx) + fp.dv[2] * fy) - op.z);
+    float len = sqrtf((dd.x * dd.x + dd.y * dd.y) + dd.z * dd.z);
+    o = op;
+    d = len > 0.0f ? v3(dd.x / len, dd.y / len, dd.z / len) : v3(0.f, 0.f, 0.f);
+}
+
+__global__ void __launch_bounds__(256)
+k_raygen(FrameParams fp, float4 *__restrict__ q_o, float4 *__restrict__ q_d,
+         float4 *__restrict__ q_t, float4 *__restrict__ acc, uint32_t P,
+         uint32_t *__restrict__ ctr)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctr[CTR_CUR] = P;
+        ctr[CTR_NEXT] = 0;
+        ctr[CTR_SHADOW] = 0;
+        ctr[CTR_PEND] = 0;
+        ctr[CTR_WORK_TRACE] = 0;
+        ctr[CTR_WORK_SHADOW] = 0;
+    }
+    if (i >= P)
+        return;
+    uint32_t pixel = i % fp.WH, sample = fp.sample0 + i / fp.WH;
+    V3 o, d;
+    primary_ray(fp, pixel, sample, o, d);
+    q_o[i] = make_float4(o.x, o.y, o.z, 1.0f);
+    q_d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
+    q_t[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// ---- closest-hit trace (persistent, wave-granular work stealing) ----------
+template <bool COUNT>
+__global__ void __launch_bounds__(TRACE_BLOCK)
+k_trace(SceneDev S, const float4 *__restrict__ q_o,
+        const float4 *__restrict__ q_d, float4 *__restrict__ hit,
+        uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats)
+{
+    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    const uint32_t n = ctr[CTR_CUR];
+    uint32_t *lc = s_code + threadIdx.x;
+    float *lt = s_t + threadIdx.x;
+    uint32_t nodes = 0, prims = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0)
+            base = atomicAdd(&ctr[CTR_WORK_TRACE], 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n)
+            break;
+        uint32_t i = base + lane_id();
+        if (i < n) {
+            float4 o4 = q_o[i], d4 = q_d[i];
+            float t, bu, bv;
+            uint32_t code;
+            closest_hit<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), lc, lt,
+                                         TRACE_BLOCK, t, code, bu, bv, nodes,
+                                         prims);
+            hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
+        }
+    }
+    if (COUNT) {
+        wave_add_u64(&stats[STAT_NODES_CLOSEST], nodes);
+        wave_add_u64(&stats[STAT_PRIMS_CLOSEST], prims);
+    }
+}
+
+// ---- shadow / visibility trace --------------------------------------------
+template <bool COUNT>
+__global__ void __launch_bounds__(TRACE_BLOCK)
+k_shadow(SceneDev S, const float4 *__restrict__ s_o,
+         const float4 *__restrict__ s_d, uint8_t *__restrict__ vis,
+         uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats)
+{
+    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    const uint32_t n = ctr[CTR_SHADOW];
+    uint32_t *lc = s_code + threadIdx.x;
+    float *lt = s_t + threadIdx.x;
+    uint32_t nodes = 0, prims = 0;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0)
+            base = atomicAdd(&ctr[CTR_WORK_SHADOW], 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n)
+            break;
+        uint32_t i = base + lane_id();
+        if (i < n) {
+            float4 o4 = s_o[i], d4 = s_d[i];
+            bool occ = occluded<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), o4.w, lc,
+                                                 lt, TRACE_BLOCK, nodes, prims);
+            vis[i] = occ ? 0 : 1;
+        }
+    }
+    if (COUNT) {
+        wave_add_u64(&stats[STAT_NODES_SHADOW], nodes);
+        wave_add_u64(&stats[STAT_PRIMS_SHADOW], prims);
+    }
+}
+
+// ---- shading -----------------------------------------------------------------
+struct Mat {
+    float4 a, b, c;   // (color, metallic) (rough, ior, trans, ao) (emis, ecolor)
+};
+
+__device__ __forceinline__ float pow64(float x)
+{
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+        x = x * x;
+    return x;
+}
+
+// powf(1 - cos, 5) of fresnel_schlick_* (src/math/fresnel_schlick.c:19-38)
+__device__ __forceinline__ float schlick_pow5(float c) { return powf(1.0f - c, 5.0f); }
+
+// ggx_sample_hemisphere (src/material/utils.c:39-63)
+__device__ __forceinline__ V3 ggx_sample(V3 n, float rough, float u2, float uphi)
+{
+    float phi = 6.28318530717958648f * uphi;
+    float r2 = rough * rough;
+    float cth = sqrtf((1.0f - u2) / (1.0f + (r2 * r2 - 1.0f) * u2));
+    float sth = sqrtf(1.0f - cth * cth);
+    V3 tg = unit(cross(n, v3(1.0f, 0.0f, 0.0f)));
+    if (dot(tg, tg) < 0.0001f)
+        tg = unit(cross(n, v3(0.0f, 1.0f, 0.0f)));
+    V3 bt = unit(cross(n, tg));
+    return add(mul(n, cth), add(mul(tg, cosf(phi) * sth), mul(bt, sinf(phi) * sth)));
+}
+
+// cos_weighted_sample_hemishphere (src/material/material_default.c:20-39)
+__device__ __forceinline__ V3 cos_hemi(V3 n, float r1, float r2)
+{
+    float phi = 6.28318530717958648f * r1;
+    V3 l = v3(cosf(phi) * sqrtf(r2), sqrtf(1.0f - r2), sinf(phi) * sqrtf(r2));
+    n = unit(n);
+    V3 u = fabsf(n.y) < 0.999f ? unit(cross(v3(0.f, 1.f, 0.f), n))
+                               : unit(cross(v3(1.f, 0.f, 0.f), n));
+    return unit(add(add(mul(u, l.x), mul(n, l.y)), mul(cross(n, u), l.z)));
+}
+
+// random_in_unit_sphere (src/math/random.c:42-50) on four counter draws
+__device__ __forceinline__ V3 unit_sphere_pt(U4 u)
+{
+    V3 p = v3(u.x * 2.0f - 1.0f, u.y * 2.0f - 1.0f, u.z * 2.0f - 1.0f);
+    if (dot(p, p) >= 1.0f)
+        p = mul(unit(p), cbrtf(u.w));
+    return p;
+}
+
+// metallic_color (src/material/metallic.c:25-48)
+__device__ __forceinline__ void metal(const FrameParams &fp, uint32_t pixel,
+                                      uint32_t sample, V3 n, const Mat &m,
+                                      V3 &d, V3 &thr)
+{
+    float cth = -dot(unit(d), n);
+    V3 mn = n;
+    if (m.b.x > 0.0f) {
+        U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+        mn = ggx_sample(n, m.b.x, g.x, g.y);
+    }
+    d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
+    float p5 = schlick_pow5(cth);
+    thr = had(thr, v3(m.a.x + (1.0f - m.a.x) * p5, m.a.y + (1.0f - m.a.y) * p5,
+                      m.a.z + (1.0f - m.a.z) * p5));
+}
+
+// refracted_ray / refracted_value / refracted_v (src/material/refraction.c:16-76)
+__device__ __forceinline__ void refract(const FrameParams &fp, uint32_t pixel,
+                                        uint32_t sample, V3 n, bool front,
+                                        const Mat &m, V3 &d, V3 &thr)
+{
+    float ior = m.b.y;
+    float eta = front ? fp.ior_all / ior : ior / fp.ior_all;
+    d = unit(d);
+    float cth = -dot(d, n);
+    if (cth > 1.0f)
+        cth = 1.0f;
+    float sth = sqrtf(1.0f - cth * cth);
+    if (eta * sth > 1.0f) {
+        metal(fp, pixel, sample, n, m, d, thr);
+        return;
+    }
+    float c2 = dot(neg(d), n);
+    if (c2 > 1.0f)
+        c2 = 1.0f;
+    V3 perp = mul(add(d, mul(n, c2)), eta);
+    V3 par = mul(n, -sqrtf(fabsf(1.0f - dot(perp, perp))));
+    V3 out = unit(add(par, perp));
+    if (m.b.x > 0.0f) {
+        U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+        out = ggx_sample(out, m.b.x, g.x, g.y);
+    }
+    d = out;
+    thr = had(thr, v3(m.a.x, m.a.y, m.a.z));
+}
+
+enum : uint32_t { SC_NONE = 0, SC_DIFFUSE = 1 };
+
+__global__ void __launch_bounds__(256)
+k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
+{
+    const uint32_t n = ctr[CTR_CUR];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        uint32_t want_next = 0, n_rays = 0, want_pend = 0;
+        uint32_t kind = SC_NONE, slot = 0, pixel = 0, sample = 0;
+        V3 p = v3(0.f, 0.f, 0.f), nrm = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 0.f);
+        V3 thr = v3(0.f, 0.f, 0.f), cp = v3(0.f, 0.f, 0.f);
+        float ao = 1.0f;
+        if (active) {
+            float4 o4 = B.q_o[i], d4 = B.q_d[i], t4 = B.q_t[i], h4 = B.hit[i];
+            slot = __float_as_uint(d4.w);
+            pixel = slot % fp.WH;
+            sample = fp.sample0 + slot / fp.WH;
+            d = xyz(d4);
+            thr = xyz(t4);
+            uint32_t code = __float_as_uint(h4.y);
+            if (code == HIT_NONE) {
+                // add_skybox (src/ray_tracing/path_trace.c:37-48)
+                float4 a = B.acc[slot];
+                float ao_last = o4.w;
+                V3 bg = v3(fp.bg_clamped[0] * ao_last, fp.bg_clamped[1] * ao_last,
+                           fp.bg_clamped[2] * ao_last);
+                a.x = a.x + bg.x * thr.x;
+                a.y = a.y + bg.y * thr.y;
+                a.z = a.z + bg.z * thr.z;
+                B.acc[slot] = a;
+            } else {
+                float t = h4.x;
+                V3 o = xyz(o4);
+                p = add(o, mul(d, t));
+                uint32_t mat;
+                bool front;
+                if (code & HIT_PLANE) {
+                    uint32_t k = code & ~HIT_PLANE;
+                    float4 P = B.planes[2 * k], N = B.planes[2 * k + 1];
+                    mat = __float_as_uint(P.w);
+                    front = (d.x * N.x + d.y * N.y + d.z * N.z) < 0.0f;
+                    nrm = front ? xyz(N) : neg(xyz(N));
+                } else {
+                    const float4 *rec = B.prims + (size_t)code * 3;
+                    float4 A = rec[0], Bv = rec[1], C = rec[2];
+                    mat = __float_as_uint(C.w);
+                    if (__float_as_uint(Bv.w) == MRT_PRIMKIND_SPHERE) {
+                        float rad = Bv.x;
+                        V3 ow = sub(p, xyz(A));
+                        ow = v3(ow.x / rad, ow.y / rad, ow.z / rad);
+                        front = dot(d, ow) < 0.0f;
+                        nrm = front ? ow : neg(ow);
+                    } else {
+                        const float *N = B.nrm + (size_t)code * 9;
+                        float bu = h4.z, bv = h4.w, w = 1.0f - bu - bv;
+                        V3 nn = v3(w * N[0] + bu * N[3] + bv * N[6],
+                                   w * N[1] + bu * N[4] + bv * N[7],
+                                   w * N[2] + bu * N[5] + bv * N[8]);
+                        nn = unit(nn);
+                        if (!fp.one_sided && dot(d, nn) > 0.0f) {
+                            front = false;
+                            nrm = neg(nn);
+                        } else {
+                            front = true;
+                            nrm = nn;
+                        }
+                    }
+                }
+                Mat m;
+                m.a = B.mats[3 * mat];
+                m.b = B.mats[3 * mat + 1];
+                m.c = B.mats[3 * mat + 2];
+                ao = m.b.w;
+                // material_manager_v3 (src/ray_tracing/path_trace.c:15-35)
+                if (m.c.x > 0.0f) {
+                    float4 a = B.acc[slot];
+                    a.x = a.x + (m.c.y * m.c.x) * thr.x;
+                    a.y = a.y + (m.c.z * m.c.x) * thr.y;
+                    a.z = a.z + (m.c.w * m.c.x) * thr.z;
+                    B.acc[slot] = a;
+                }
+                bool is_metal;
+                if (m.a.w == 1.0f) {
+                    is_metal = true;
+                } else if (m.a.w == 0.0f) {
+                    is_metal = false;
+                } else {
+                    U4 u = rng4(fp.seed, pixel, sample, fp.bounce, 1u);
+                    is_metal = m.a.w > u.x;
+                }
+                bool diffuse = false;
+                if (is_metal) {
+                    metal(fp, pixel, sample, nrm, m, d, thr);
+                } else if (!(m.b.y > 0.0f)) {
+                    diffuse = true;
+                } else {
+                    // dielectric_mat (src/material/dielectric.c:40-66)
+                    U4 u = rng4(fp.seed, pixel, sample, fp.bounce, 1u);
+                    float cth = -dot(unit(d), nrm);
+                    float q = (m.b.y - 1.0f) / (m.b.y + 1.0f);
+                    float f0 = powf(q, 2.0f);
+                    float refl = f0 + (1.0f - f0) * schlick_pow5(cth);
+                    if (refl >= u.y && m.b.y != 1.0f) {
+                        V3 mn = nrm;
+                        if (m.b.x > 0.0f) {
+                            U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+                            mn = ggx_sample(nrm, m.b.x, g.x, g.y);
+                        }
+                        d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
+                    } else if (m.b.z == 1.0f) {
+                        refract(fp, pixel, sample, nrm, front, m, d, thr);
+                    } else if (m.b.z == 0.0f) {
+                        diffuse = true;
+                    } else if (m.b.z < u.z) {
+                        diffuse = true;
+                    } else {
+                        refract(fp, pixel, sample, nrm, front, m, d, thr);
+                    }
+                }
+                if (diffuse) {
+                    // default_mat (src/material/material_default.c:71-92)
+                    kind = SC_DIFFUSE;
+                    V3 col = v3(m.a.x, m.a.y, m.a.z);
+                    cp = had(col, thr);
+                    thr = had(thr, col);
+                    for (uint32_t k = 0; k < fp.n_lights; k++) {
+                        float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1],
+                               l2 = B.lights[3 * k + 2];
+                        uint32_t ltype = __float_as_uint(l2.x);
+                        int32_t lss = __float_as_int(l2.y);
+                        V3 ld = ltype == MRT_LIGHT_POINT ? unit(sub(xyz(l0), p))
+                                                         : neg(xyz(l0));
+                        if (dot(nrm, ld) <= 0.0f)
+                            continue;
+                        bool soft = ltype == MRT_LIGHT_POINT && l1.w != 0.0f && lss > 1;
+                        n_rays += soft ? (uint32_t)lss : 1u;
+                    }
+                    U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+                    d = cos_hemi(nrm, g.z, g.w);
+                    want_pend = n_rays ? 1u : 0u;
+                }
+                want_next = fp.last_bounce ? 0u : 1u;
+            }
+        }
+        // allocations: every lane of the wave takes part
+        uint32_t qn = wave_alloc(&ctr[CTR_NEXT], want_next);
+        uint32_t j0 = wave_alloc(&ctr[CTR_SHADOW], n_rays);
+        uint32_t pr = wave_alloc(&ctr[CTR_PEND], want_pend);
+        if (!active)
+            continue;
+        if (want_next) {
+            B.qn_o[qn] = make_float4(p.x, p.y, p.z, ao);
+            B.qn_d[qn] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
+            B.qn_t[qn] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+        }
+        if (kind == SC_DIFFUSE && n_rays) {
+            V3 view = unit(sub(v3(fp.cam_pos[0], fp.cam_pos[1], fp.cam_pos[2]), p));
+            uint32_t j = j0;
+            for (uint32_t k = 0; k < fp.n_lights; k++) {
+                float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1],
+                       l2 = B.lights[3 * k + 2];
+                uint32_t ltype = __float_as_uint(l2.x);
+                int32_t lss = __float_as_int(l2.y);
+                bool point = ltype == MRT_LIGHT_POINT;
+                V3 lpos = xyz(l0);
+                V3 ld = point ? unit(sub(lpos, p)) : neg(lpos);
+                float ndl = dot(nrm, ld);
+                if (ndl <= 0.0f)
+                    continue;
+                V3 nl = neg(ld);
+                V3 refl = sub(nl, mul(nrm, 2.0f * dot(nl, nrm)));
+                float rdv = dot(refl, view);
+                if (rdv < 0.0f)
+                    rdv = 0.0f;
+                float sp64 = pow64(rdv);
+                bool soft = point && l1.w != 0.0f && lss > 1;
+                uint32_t cnt = soft ? (uint32_t)lss : 1u;
+                for (uint32_t s = 0; s < cnt; s++, j++) {
+                    V3 sd;
+                    float tmax;
+                    if (!point) {
+                        sd = unit(ld);
+                        tmax = 2.0f * IT_MAX_F;
+                    } else {
+                        V3 tgt = lpos;
+                        if (soft) {
+                            U4 u = rng4(fp.seed, pixel, sample, fp.bounce,
+                                        16u + k * 4096u + s);
+                            tgt = add(lpos, mul(unit_sphere_pt(u), l1.w));
+                        }
+                        V3 dl = sub(tgt, p);
+                        sd = unit(dl);
+                        tmax = sqrtf(dot(dl, dl));
+                    }
+                    B.s_o[j] = make_float4(p.x, p.y, p.z, tmax);
+                    B.s_d[j] = make_float4(sd.x, sd.y, sd.z,
+                                           __uint_as_float((k << 12) | s));
+                    B.s_c[j] = make_float2(ndl, sp64);
+                }
+            }
+        }
+        if (want_pend) {
+            B.p_a[pr] = make_float4(cp.x, cp.y, cp.z, __uint_as_float(slot));
+            B.p_b[pr] = make_uint4(want_next ? qn : HIT_NONE, j0, n_rays, 0u);
+        }
+    }
+}
+
+// ---- direct-light resolve (light_manager.c:20-70, p_light.c:67-91,
+//      d_light.c:35-57): the quantised sums of the visible lights ----------
+__device__ __forceinline__ float lq(bool exact, float acc, float add)
+{
+    float s = acc + add;
+    return exact ? s : floorf(s);
+}
+
+__global__ void __launch_bounds__(256)
+k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
+{
+    const uint32_t n = ctr[CTR_PEND];
+    const bool exact = (fp.flags & MRT_SCENE_LIGHT_EXACT) != 0;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += gridDim.x * blockDim.x) {
+        float4 a = B.p_a[r];
+        uint4 b = B.p_b[r];
+        uint32_t slot = __float_as_uint(a.w);
+        uint32_t j = b.y, end = b.y + b.z;
+        float lr = 0.0f, lg = 0.0f, lb = 0.0f;
+        while (j < end) {
+            uint32_t tag = __float_as_uint(B.s_d[j].w);
+            uint32_t k = tag >> 12;
+            float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1],
+                   l2 = B.lights[3 * k + 2];
+            uint32_t ltype = __float_as_uint(l2.x);
+            int32_t lss = __float_as_int(l2.y);
+            bool point = ltype == MRT_LIGHT_POINT;
+            bool soft = point && l1.w != 0.0f && lss > 1;
+            uint32_t cnt = soft ? (uint32_t)lss : 1u;
+            uint32_t nvis = 0;
+            for (uint32_t s = 0; s < cnt; s++)
+                nvis += B.vis[j + s];
+            float2 c = B.s_c[j];
+            j += cnt;
+            float br = l0.w;
+            if (point) {
+                float factor = soft ? (float)nvis / (float)cnt : 1.0f;
+                if (!(nvis > 0))
+                    continue;
+                float diff = br * c.x * factor;
+                lr = lq(exact, lr, l1.x * diff);
+                lg = lq(exact, lg, l1.y * diff);
+                lb = lq(exact, lb, l1.z * diff);
+                float spec = c.y * 1.0f * factor;
+                lr = lq(exact, lr, l1.x * br * spec);
+                lg = lq(exact, lg, l1.y * br * spec);
+                lb = lq(exact, lb, l1.z * br * spec);
+            } else {
+                if (!nvis)
+                    continue;
+                lr = lq(exact, lr, l1.x * br * c.x);
+                lg = lq(exact, lg, l1.y * br * c.x);
+                lb = lq(exact, lb, l1.z * br * c.x);
+                lr = lq(exact, lr, l1.x * br * 1.0f * c.y);
+                lg = lq(exact, lg, l1.y * br * 1.0f * c.y);
+                lb = lq(exact, lb, l1.z * br * 1.0f * c.y);
+            }
+        }
+        float4 s = B.acc[slot];
+        s.x = s.x + a.x * (lr / 255.0f);
+        s.y = s.y + a.y * (lg / 255.0f);
+        s.z = s.z + a.z * (lb / 255.0f);
+        B.acc[slot] = s;
+    }
+}
+
+__global__ void k_advance(uint32_t *__restrict__ ctr,
+                          unsigned long long *__restrict__ stats)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        stats[STAT_RAYS_CLOSEST] += ctr[CTR_CUR];
+        stats[STAT_RAYS_SHADOW] += ctr[CTR_SHADOW];
+        ctr[CTR_CUR] = ctr[CTR_NEXT];
+        ctr[CTR_NEXT] = 0;
+        ctr[CTR_SHADOW] = 0;
+        ctr[CTR_PEND] = 0;
+        ctr[CTR_WORK_TRACE] = 0;
+        ctr[CTR_WORK_SHADOW] = 0;
+    }
+}
+
+// Per pixel, in sample order: accum += sum of the batch's finite samples.
+__global__ void __launch_bounds__(256)
+k_accumulate(const float4 *__restrict__ acc, float *__restrict__ accum,
+             uint32_t WH, uint32_t nb, unsigned long long *__restrict__ stats)
+{
+    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bad = 0;
+    if (px < WH) {
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        for (uint32_t k = 0; k < nb; k++) {
+            float4 v = acc[(size_t)k * WH + px];
+            if (isfinite(v.x) && isfinite(v.y) && isfinite(v.z)) {
+                r += v.x;
+                g += v.y;
+                b += v.z;
+            } else {
+                bad++;
+            }
+        }
+        accum[(size_t)px * 3] += r;
+        accum[(size_t)px * 3 + 1] += g;
+        accum[(size_t)px * 3 + 2] += b;
+    }
+    wave_add_u64(&stats[STAT_NONFINITE], bad);
+}
+
+// ---- primary-ray AOVs and explicit rays ------------------------------------
+__global__ void __launch_bounds__(TRACE_BLOCK)
+k_aov(FrameParams fp, SceneDev S, const float4 *__restrict__ prims,
+      const float *__restrict__ nrm, uint32_t sample, int aov,
+      void *__restrict__ out)
+{
+    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= fp.WH)
+        return;
+    V3 o, d;
+    primary_ray(fp, px, sample, o, d);
+    float t, bu, bv;
+    uint32_t code, nodes = 0, np = 0;
+    bool hit = closest_hit<true, TRACE_SS>(S, o, d, s_code + threadIdx.x,
+                                           s_t + threadIdx.x, TRACE_BLOCK, t,
+                                           code, bu, bv, nodes, np);
+    switch (aov) {
+    case MRT_AOV_PRIM_ID: {
+        uint32_t id = MRT_PRIM_NONE;
+        if (hit)
+            id = (code & HIT_PLANE) ? (MRT_PRIM_PLANE | (code & ~HIT_PLANE))
+                                    : __float_as_uint(prims[(size_t)code * 3].w);
+        ((uint32_t *)out)[px] = id;
+        break;
+    }
+    case MRT_AOV_T:
+        ((float *)out)[px] = hit ? t : -1.0f;
+        break;
+    case MRT_AOV_NODES:
+        ((uint32_t *)out)[px] = nodes;
+        break;
+    case MRT_AOV_PRIMS:
+        ((uint32_t *)out)[px] = np;
+        break;
+    case MRT_AOV_NORMAL: {
+        V3 n = v3(0.f, 0.f, 0.f);
+        if (hit) {
+            V3 p = add(o, mul(d, t));
+            if (code & HIT_PLANE) {
+                float4 N = S.planes[2 * (code & ~HIT_PLANE) + 1];
+                n = (d.x * N.x + d.y * N.y + d.z * N.z) < 0.0f ? xyz(N) : neg(xyz(N));
+            } else {
+                float4 A = prims[(size_t)code * 3], Bv = prims[(size_t)code * 3 + 1];
+                if (__float_as_uint(Bv.w) == MRT_PRIMKIND_SPHERE) {
+                    V3 ow = sub(p, xyz(A));
+                    ow = v3(ow.x / Bv.x, ow.y / Bv.x, ow.z / Bv.x);
+                    n = dot(d, ow) < 0.0f ? ow : neg(ow);
+                } else {
+                    const float *N = nrm + (size_t)code * 9;
+                    float w = 1.0f - bu - bv;
+                    V3 nn = unit(v3(w * N[0] + bu * N[3] + bv * N[6],
+                                    w * N[1] + bu * N[4] + bv * N[7],
+                                    w * N[2] + bu * N[5] + bv * N[8]));
+                    n = (!fp.one_sided && dot(d, nn) > 0.0f) ? neg(nn) : nn;
+                }
+            }
+        }
+        float *o3 = (float *)out + (size_t)px * 3;
+        o3[0] = n.x;
+        o3[1] = n.y;
+        o3[2] = n.z;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+__global__ void __launch_bounds__(TRACE_BLOCK)
+k_trace_rays(SceneDev S, const float4 *__restrict__ prims, uint32_t n,
+             const float *__restrict__ orig, const float *__restrict__ dir,
+             uint32_t *__restrict__ prim_id, float *__restrict__ tout)
+{
+    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    V3 o = v3(orig[3 * (size_t)i], orig[3 * (size_t)i + 1], orig[3 * (size_t)i + 2]);
+    V3 d = v3(dir[3 * (size_t)i], dir[3 * (size_t)i + 1], dir[3 * (size_t)i + 2]);
+    float t, bu, bv;
+    uint32_t code, nodes = 0, np = 0;
+    bool hit = closest_hit<false, TRACE_SS>(S, o, d, s_code + threadIdx.x,
+                                            s_t + threadIdx.x, TRACE_BLOCK, t,
+                                            code, bu, bv, nodes, np);
+    uint32_t id = MRT_PRIM_NONE;
+    if (hit)
+        id = (code & HIT_PLANE) ? (MRT_PRIM_PLANE | (code & ~HIT_PLANE))
+                                : __float_as_uint(prims[(size_t)code * 3].w);
+    prim_id[i] = id;
+    tout[i] = hit ? t : -1.0f;
+}
+
+// ---- launchers -----------------------------------------------------------------
+hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
+                         float4 *q_d, float4 *q_t, float4 *acc, uint32_t P,
+                         uint32_t *ctr)
+{
+    dim3 grid((P + 255) / 256);
+    hipLaunchKernelGGL(k_raygen, grid, dim3(256), 0, st, fp, q_o, q_d, q_t,
+                       acc, P, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
+                        const SceneDev &S, const float4 *q_o,
+                        const float4 *q_d, float4 *hit, uint32_t *ctr,
+                        unsigned long long *stats)
+{
+    if (count)
+        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(TRACE_BLOCK), 0,
+                           st, S, q_o, q_d, hit, ctr, stats);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(TRACE_BLOCK), 0,
+                           st, S, q_o, q_d, hit, ctr, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
+                         const SceneDev &S, const float4 *s_o,
+                         const float4 *s_d, uint8_t *vis, uint32_t *ctr,
+                         unsigned long long *stats)
+{
+    if (count)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(TRACE_BLOCK), 0,
+                           st, S, s_o, s_d, vis, ctr, stats);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(TRACE_BLOCK), 0,
+                           st, S, s_o, s_d, vis, ctr, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
+                        const ShadeBufs &B, uint32_t *ctr)
+{
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, fp, B, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
+                          const FrameParams &fp, const ShadeBufs &B,
+                          uint32_t *ctr)
+{
+    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, st, fp, B, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(hipStream_t st, uint32_t *ctr,
+                          unsigned long long *stats)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctr, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(hipStream_t st, const float4 *acc, float *accum,
+                             uint32_t WH, uint32_t nb,
+                             unsigned long long *stats)
+{
+    hipLaunchKernelGGL(k_accumulate, dim3((WH + 255) / 256), dim3(256), 0, st,
+                       acc, accum, WH, nb, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_aov(hipStream_t st, const FrameParams &fp, const SceneDev &S,
+                      const float4 *prims, const float *nrm, uint32_t sample,
+                      int aov, void *out)
+{
+    hipLaunchKernelGGL(k_aov, dim3((fp.WH + TRACE_BLOCK - 1) / TRACE_BLOCK),
+                       dim3(TRACE_BLOCK), 0, st, fp, S, prims, nrm, sample, aov,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
+                             const float4 *prims, uint32_t n,
+                             const float *orig, const float *dir,
+                             uint32_t *prim_id, float *t)
+{
+    hipLaunchKernelGGL(k_trace_rays, dim3((n + TRACE_BLOCK - 1) / TRACE_BLOCK),
+                       dim3(TRACE_BLOCK), 0, st, S, prims, n, orig, dir,
+                       prim_id, t);
+    return hipGetLastError();
+}
+
+int trace_blocks_per_cu(bool shadow)
+{
+    int nb = 0;
+    hipError_t e = shadow
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shadow<false>,
+                                                       TRACE_BLOCK, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false>,
+                                                       TRACE_BLOCK, 0);
+    if (e != hipSuccess || nb < 1)
+        nb = 4;
+    return nb;
+}
+
+}  // namespace mrt

@@ -1,0 +1,437 @@
+/*
+ * Binned-SAH BVH builder (host, C).
+ *
+ * Replaces the reference build (src/ray_tracing/bvh/bvh_build.c:17-124,
+ * qshort_axis.c:88-111): that one splits at the object median of the longest
+ * axis down to one primitive per leaf and recomputes bounds per node.  Here:
+ * 32-bin SAH on centroids per node, leaves of at most max_leaf primitives,
+ * and an object-median fallback that bounds the depth so the device stack
+ * never overflows.  The output is the node / primitive layout in mrt_host.h.
+ */
+#include "mrt_host_internal.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NBINS 32
+
+typedef struct {
+    float lo[3], hi[3];
+} box3;
+
+typedef struct {
+    const mrt_scene *sc;
+    uint32_t  max_leaf;
+    uint32_t  n;
+    box3     *pbox;      /* per source primitive */
+    float    *cent;      /* 3 per source primitive */
+    uint32_t *perm;      /* source index per leaf-order slot */
+    uint32_t *tmp;
+    mrt_bvh_node *nodes;
+    uint32_t  n_nodes, cap_nodes;
+    uint32_t  max_depth, n_leaves;
+    double    sah;
+    double    root_area;
+    int       err;
+} builder;
+
+static inline void box_empty(box3 *b)
+{
+    for (int k = 0; k < 3; k++) {
+        b->lo[k] = FLT_MAX;
+        b->hi[k] = -FLT_MAX;
+    }
+}
+
+static inline void box_grow(box3 *b, const box3 *o)
+{
+    for (int k = 0; k < 3; k++) {
+        b->lo[k] = fminf(b->lo[k], o->lo[k]);
+        b->hi[k] = fmaxf(b->hi[k], o->hi[k]);
+    }
+}
+
+static inline double box_area(const box3 *b)
+{
+    double d[3];
+    for (int k = 0; k < 3; k++) {
+        d[k] = (double)b->hi[k] - (double)b->lo[k];
+        if (d[k] < 0.0)
+            return 0.0;
+    }
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+
+static void prim_bounds(builder *B)
+{
+    const mrt_scene *s = B->sc;
+    for (uint32_t i = 0; i < s->n_triangles; i++) {
+        const float *v = s->tri_pos + (size_t)i * 9;
+        box3 *b = &B->pbox[i];
+        for (int k = 0; k < 3; k++) {
+            b->lo[k] = fminf(v[k], fminf(v[3 + k], v[6 + k]));
+            b->hi[k] = fmaxf(v[k], fmaxf(v[3 + k], v[6 + k]));
+            B->cent[(size_t)i * 3 + k] = 0.5f * (b->lo[k] + b->hi[k]);
+        }
+    }
+    for (uint32_t j = 0; j < s->n_spheres; j++) {
+        const float *q = s->sph + (size_t)j * 4;
+        uint32_t i = s->n_triangles + j;
+        box3 *b = &B->pbox[i];
+        for (int k = 0; k < 3; k++) {
+            /* one ulp outward so the float box holds the exact sphere */
+            b->lo[k] = nextafterf(q[k] - q[3], -INFINITY);
+            b->hi[k] = nextafterf(q[k] + q[3], INFINITY);
+            B->cent[(size_t)i * 3 + k] = q[k];
+        }
+    }
+}
+
+static int32_t leaf_code(uint32_t first, uint32_t count)
+{
+    return ~(int32_t)((first << 3) | (count - 1));
+}
+
+static uint32_t new_node(builder *B)
+{
+    if (B->n_nodes == B->cap_nodes) {
+        uint32_t nc = B->cap_nodes ? B->cap_nodes * 2 : 1024;
+        mrt_bvh_node *nn = realloc(B->nodes, (size_t)nc * sizeof(*nn));
+        if (!nn) {
+            B->err = MRT_E_NOMEM;
+            return UINT32_MAX;
+        }
+        B->nodes = nn;
+        B->cap_nodes = nc;
+    }
+    memset(&B->nodes[B->n_nodes], 0, sizeof(mrt_bvh_node));
+    return B->n_nodes++;
+}
+
+static void set_child(builder *B, uint32_t node, int side, const box3 *b,
+                      int32_t code)
+{
+    mrt_bvh_node *n = &B->nodes[node];
+    float *lo = side ? n->lo1 : n->lo0;
+    float *hi = side ? n->hi1 : n->hi0;
+    memcpy(lo, b->lo, sizeof(b->lo));
+    memcpy(hi, b->hi, sizeof(b->hi));
+    n->child[side] = code;
+}
+
+static uint32_t ceil_log2(uint32_t x)
+{
+    uint32_t l = 0;
+    while ((1u << l) < x)
+        l++;
+    return l;
+}
+
+static int cmp_axis;
+static const float *cmp_cent;
+static int cmp_prim(const void *a, const void *b)
+{
+    float ca = cmp_cent[(size_t)(*(const uint32_t *)a) * 3 + cmp_axis];
+    float cb = cmp_cent[(size_t)(*(const uint32_t *)b) * 3 + cmp_axis];
+    if (ca < cb)
+        return -1;
+    if (ca > cb)
+        return 1;
+    uint32_t ia = *(const uint32_t *)a, ib = *(const uint32_t *)b;
+    return (ia > ib) - (ia < ib);
+}
+
+/* Chooses the split of perm[begin,end); returns the split point mid with
+ * begin < mid < end after partitioning. */
+static uint32_t split_range(builder *B, uint32_t begin, uint32_t end,
+                            uint32_t depth)
+{
+    uint32_t count = end - begin;
+    box3 cb;
+    box_empty(&cb);
+    for (uint32_t i = begin; i < end; i++) {
+        const float *c = &B->cent[(size_t)B->perm[i] * 3];
+        for (int k = 0; k < 3; k++) {
+            cb.lo[k] = fminf(cb.lo[k], c[k]);
+            cb.hi[k] = fmaxf(cb.hi[k], c[k]);
+        }
+    }
+    int axis_long = 0;
+    float ext_long = cb.hi[0] - cb.lo[0];
+    for (int k = 1; k < 3; k++)
+        if (cb.hi[k] - cb.lo[k] > ext_long) {
+            ext_long = cb.hi[k] - cb.lo[k];
+            axis_long = k;
+        }
+
+    uint32_t leaves_needed = (count + B->max_leaf - 1) / B->max_leaf;
+    int median = (depth + 1 + ceil_log2(leaves_needed) >= MRT_BVH_MAX_DEPTH)
+                 || !(ext_long > 0.0f);
+    if (!median) {
+        double best = INFINITY;
+        int best_axis = -1, best_bin = -1;
+        for (int axis = 0; axis < 3; axis++) {
+            float ext = cb.hi[axis] - cb.lo[axis];
+            if (!(ext > 0.0f))
+                continue;
+            box3 bb[NBINS];
+            uint32_t bc[NBINS];
+            for (int k = 0; k < NBINS; k++) {
+                box_empty(&bb[k]);
+                bc[k] = 0;
+            }
+            float scale = (float)NBINS / ext;
+            for (uint32_t i = begin; i < end; i++) {
+                uint32_t p = B->perm[i];
+                int bin = (int)((B->cent[(size_t)p * 3 + axis] - cb.lo[axis])
+                                * scale);
+                if (bin >= NBINS)
+                    bin = NBINS - 1;
+                if (bin < 0)
+                    bin = 0;
+                bc[bin]++;
+                box_grow(&bb[bin], &B->pbox[p]);
+            }
+            double right_area[NBINS];
+            uint32_t right_cnt[NBINS];
+            box3 acc;
+            box_empty(&acc);
+            uint32_t cnt = 0;
+            for (int k = NBINS - 1; k > 0; k--) {
+                box_grow(&acc, &bb[k]);
+                cnt += bc[k];
+                right_area[k] = box_area(&acc);
+                right_cnt[k] = cnt;
+            }
+            box_empty(&acc);
+            cnt = 0;
+            for (int k = 0; k < NBINS - 1; k++) {
+                box_grow(&acc, &bb[k]);
+                cnt += bc[k];
+                if (cnt == 0 || right_cnt[k + 1] == 0)
+                    continue;
+                double cost = box_area(&acc) * cnt
+                              + right_area[k + 1] * right_cnt[k + 1];
+                if (cost < best) {
+                    best = cost;
+                    best_axis = axis;
+                    best_bin = k;
+                }
+            }
+        }
+        if (best_axis >= 0) {
+            float ext = cb.hi[best_axis] - cb.lo[best_axis];
+            float scale = (float)NBINS / ext;
+            uint32_t i = begin, j = end;
+            while (i < j) {
+                uint32_t p = B->perm[i];
+                int bin = (int)((B->cent[(size_t)p * 3 + best_axis]
+                                 - cb.lo[best_axis]) * scale);
+                if (bin >= NBINS)
+                    bin = NBINS - 1;
+                if (bin < 0)
+                    bin = 0;
+                if (bin <= best_bin) {
+                    i++;
+                } else {
+                    j--;
+                    B->perm[i] = B->perm[j];
+                    B->perm[j] = p;
+                }
+            }
+            if (i > begin && i < end)
+                return i;
+        }
+    }
+    /* object median on the longest centroid axis (ties by index) */
+    cmp_axis = axis_long;
+    cmp_cent = B->cent;
+    qsort(B->perm + begin, count, sizeof(uint32_t), cmp_prim);
+    return begin + count / 2;
+}
+
+static void range_box(builder *B, uint32_t begin, uint32_t end, box3 *out)
+{
+    box_empty(out);
+    for (uint32_t i = begin; i < end; i++)
+        box_grow(out, &B->pbox[B->perm[i]]);
+}
+
+/* Builds the subtree of perm[begin,end) whose box is bx; returns its child
+ * code (leaf or inner node index). depth = inner levels above it. */
+static int32_t build_rec(builder *B, uint32_t begin, uint32_t end,
+                         const box3 *bx, uint32_t depth)
+{
+    uint32_t count = end - begin;
+    if (B->err)
+        return 0;
+    if (count <= B->max_leaf) {
+        B->n_leaves++;
+        B->sah += box_area(bx) / B->root_area * (double)count;
+        return leaf_code(begin, count);
+    }
+    uint32_t node = new_node(B);
+    if (node == UINT32_MAX)
+        return 0;
+    if (depth + 1 > B->max_depth)
+        B->max_depth = depth + 1;
+    B->sah += box_area(bx) / B->root_area;
+    uint32_t mid = split_range(B, begin, end, depth);
+    box3 lb, rb;
+    range_box(B, begin, mid, &lb);
+    range_box(B, mid, end, &rb);
+    int32_t lc = build_rec(B, begin, mid, &lb, depth + 1);
+    set_child(B, node, 0, &lb, lc);
+    int32_t rc = build_rec(B, mid, end, &rb, depth + 1);
+    set_child(B, node, 1, &rb, rc);
+    return (int32_t)node;
+}
+
+static inline float bits_f(uint32_t u)
+{
+    float f;
+    memcpy(&f, &u, sizeof(f));
+    return f;
+}
+
+int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
+{
+    if (!s || !out || max_leaf < 1 || max_leaf > MRT_BVH_MAX_LEAF)
+        return MRT_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    if ((s->n_triangles && (!s->tri_pos || !s->tri_mat))
+        || (s->n_spheres && (!s->sph || !s->sph_mat)))
+        return MRT_E_INVALID;
+    uint64_t n64 = (uint64_t)s->n_triangles + s->n_spheres;
+    if (n64 >= (1u << 28))
+        return MRT_E_LIMIT;
+    uint32_t n = (uint32_t)n64;
+    if (n == 0)
+        return MRT_OK;
+
+    builder B;
+    memset(&B, 0, sizeof(B));
+    B.sc = s;
+    B.max_leaf = max_leaf;
+    B.n = n;
+    B.pbox = malloc((size_t)n * sizeof(box3));
+    B.cent = malloc((size_t)n * 3 * sizeof(float));
+    B.perm = malloc((size_t)n * sizeof(uint32_t));
+    if (!B.pbox || !B.cent || !B.perm) {
+        free(B.pbox);
+        free(B.cent);
+        free(B.perm);
+        return MRT_E_NOMEM;
+    }
+    prim_bounds(&B);
+    for (uint32_t i = 0; i < n; i++)
+        B.perm[i] = i;
+    box3 root;
+    range_box(&B, 0, n, &root);
+    B.root_area = box_area(&root);
+    if (!(B.root_area > 0.0))
+        B.root_area = 1.0;
+
+    int32_t rc;
+    if (n <= max_leaf) {
+        /* a single leaf: one inner node with an absent second child */
+        uint32_t node = new_node(&B);
+        B.max_depth = 1;
+        B.n_leaves = 1;
+        B.sah = 1.0 + (double)n;
+        set_child(&B, node, 0, &root, leaf_code(0, n));
+        box3 far;
+        for (int k = 0; k < 3; k++)
+            far.lo[k] = far.hi[k] = 1e30f;
+        set_child(&B, node, 1, &far, leaf_code(0, 1));
+        rc = 0;
+    } else {
+        rc = build_rec(&B, 0, n, &root, 0);
+    }
+    (void)rc;
+    free(B.pbox);
+    free(B.cent);
+    if (B.err) {
+        free(B.nodes);
+        free(B.perm);
+        return B.err;
+    }
+
+    out->prims = malloc((size_t)n * sizeof(mrt_prim));
+    out->shade_nrm = malloc((size_t)n * 9 * sizeof(float));
+    if (!out->prims || !out->shade_nrm) {
+        free(out->prims);
+        free(out->shade_nrm);
+        free(B.nodes);
+        free(B.perm);
+        memset(out, 0, sizeof(*out));
+        return MRT_E_NOMEM;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t p = B.perm[i];
+        mrt_prim *r = &out->prims[i];
+        float *nrm = out->shade_nrm + (size_t)i * 9;
+        if (p < s->n_triangles) {
+            const float *v = s->tri_pos + (size_t)p * 9;
+            for (int k = 0; k < 3; k++) {
+                r->a[k] = v[k];
+                r->b[k] = v[3 + k] - v[k];
+                r->c[k] = v[6 + k] - v[k];
+            }
+            r->a[3] = bits_f(MRT_PRIM_TRIANGLE | p);
+            r->b[3] = bits_f(MRT_PRIMKIND_TRIANGLE);
+            r->c[3] = bits_f(s->tri_mat[p]);
+            if (s->tri_nrm) {
+                memcpy(nrm, s->tri_nrm + (size_t)p * 9, 9 * sizeof(float));
+            } else {
+                double e1[3], e2[3];
+                for (int k = 0; k < 3; k++) {
+                    e1[k] = r->b[k];
+                    e2[k] = r->c[k];
+                }
+                double c[3] = {e1[1] * e2[2] - e1[2] * e2[1],
+                               e1[2] * e2[0] - e1[0] * e2[2],
+                               e1[0] * e2[1] - e1[1] * e2[0]};
+                double l = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                l = l > 0.0 ? 1.0 / l : 0.0;
+                for (int k = 0; k < 3; k++)
+                    for (int j = 0; j < 3; j++)
+                        nrm[k * 3 + j] = (float)(c[j] * l);
+            }
+        } else {
+            uint32_t j = p - s->n_triangles;
+            const float *q = s->sph + (size_t)j * 4;
+            r->a[0] = q[0];
+            r->a[1] = q[1];
+            r->a[2] = q[2];
+            r->a[3] = bits_f(MRT_PRIM_SPHERE | j);
+            r->b[0] = q[3];
+            r->b[1] = q[3] * q[3];
+            r->b[2] = 0.0f;
+            r->b[3] = bits_f(MRT_PRIMKIND_SPHERE);
+            r->c[0] = r->c[1] = r->c[2] = 0.0f;
+            r->c[3] = bits_f(s->sph_mat[j]);
+            memset(nrm, 0, 9 * sizeof(float));
+        }
+    }
+    free(B.perm);
+    out->nodes = B.nodes;
+    out->n_nodes = B.n_nodes;
+    out->n_prims = n;
+    out->depth = B.max_depth;
+    out->n_leaves = B.n_leaves;
+    out->sah_cost = B.sah;
+    return MRT_OK;
+}
+
+void mrt_bvh_free(mrt_bvh *b)
+{
+    if (!b)
+        return;
+    free(b->nodes);
+    free(b->prims);
+    free(b->shade_nrm);
+    memset(b, 0, sizeof(*b));
+}

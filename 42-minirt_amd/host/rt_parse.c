@@ -1,0 +1,386 @@
+/*
+ * .rt scene reader for the elements the path tracer renders (the grammar of
+ * src/parsing/parse_scene.c:99-125 and parse_elements*.c):
+ *   A ratio r,g,b                       (parse_elements_utils5.c:36-53)
+ *   C pos dir fov [defocus [focus]]     (parse_elements2.c:41-67)
+ *   L|l pos brightness r,g,b [radius [shadow_samples]]   (:69-122)
+ *   dl dir brightness r,g,b             (parse_elements5.c:16-42)
+ *   sp pos diameter r,g,b|mat           (parse_elements2.c:124-149)
+ *   pl pos normal r,g,b|mat             (parse_elements6.c:16-39)
+ *   mat name color metallic roughness ior transmission ao emission_strength
+ *       emission_color [scale]          (parse_elements3.c:59-136)
+ *   W win_w win_h [render_w render_h]   (parse_elements5.c:44-83)
+ * Textures, meshes and quadrics (tex cu cy co hb) are reported as
+ * unsupported.  Numbers go through strtod; the reference's ft_atod is not
+ * available, so the rounding of parsed decimals is not pinned to it.
+ */
+#include "mrt_host_internal.h"
+
+#include <ctype.h>
+#include <errno.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXTOK 16
+
+typedef struct {
+    mrt_host_scene *hs;
+    char           *err;
+    size_t          errlen;
+    int             line;
+    int             n_amb, n_cam, n_L, n_W;
+} parser;
+
+static int perr(parser *P, int code, const char *msg, const char *tok)
+{
+    if (P->err && P->errlen)
+        snprintf(P->err, P->errlen, "line %d: %s%s%s", P->line, msg,
+                 tok ? ": " : "", tok ? tok : "");
+    return code;
+}
+
+static int num(const char *s, double *out)
+{
+    char *end;
+    errno = 0;
+    double v = strtod(s, &end);
+    if (end == s || *end || errno || !isfinite(v))
+        return 0;
+    *out = v;
+    return 1;
+}
+
+static int vec3(const char *s, double v[3])
+{
+    char buf[256];
+    if (strlen(s) >= sizeof(buf))
+        return 0;
+    strcpy(buf, s);
+    char *p = buf;
+    for (int k = 0; k < 3; k++) {
+        char *c = strchr(p, ',');
+        if (k < 2) {
+            if (!c)
+                return 0;
+            *c = 0;
+        } else if (c) {
+            return 0;
+        }
+        if (!num(p, &v[k]))
+            return 0;
+        p = c ? c + 1 : p;
+    }
+    return 1;
+}
+
+static int color(const char *s, float c[3])
+{
+    double v[3];
+    if (!vec3(s, v))
+        return 0;
+    for (int k = 0; k < 3; k++) {
+        if (v[k] < 0 || v[k] > 255 || v[k] != floor(v[k]))
+            return 0;
+        c[k] = (float)v[k];
+    }
+    return 1;
+}
+
+static int unit_vec(const char *s, float out[3])
+{
+    double v[3];
+    if (!vec3(s, v))
+        return 0;
+    for (int k = 0; k < 3; k++)
+        if (v[k] < -1.0 || v[k] > 1.0)
+            return 0;
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(l > 0.0))
+        return 0;
+    for (int k = 0; k < 3; k++)
+        out[k] = (float)(v[k] / l);
+    return 1;
+}
+
+static int pos3(const char *s, float out[3])
+{
+    double v[3];
+    if (!vec3(s, v))
+        return 0;
+    for (int k = 0; k < 3; k++)
+        out[k] = (float)v[k];
+    return 1;
+}
+
+/* A colour creates the reference's "no material" object: diffuse with
+ * albedo colour / 255 (get_hit_register_color, get_hit_color.c:17-36). */
+static int color_or_mat(parser *P, const char *s, uint32_t *mat)
+{
+    if (strchr(s, ',')) {
+        float c[3];
+        if (!color(s, c))
+            return perr(P, MRT_E_INVALID, "bad colour", s);
+        mrt_material m;
+        memset(&m, 0, sizeof(m));
+        for (int k = 0; k < 3; k++)
+            m.color[k] = (float)(c[k] / 255.0);
+        m.ao = 1.0f;
+        int r = mrt_hs_add_material(P->hs, &m, NULL);
+        if (r < 0)
+            return r;
+        *mat = (uint32_t)r;
+        return MRT_OK;
+    }
+    int r = mrt_hs_find_material(P->hs, s);
+    if (r < 0)
+        return perr(P, MRT_E_INVALID, "unknown material", s);
+    *mat = (uint32_t)r;
+    return MRT_OK;
+}
+
+static int p_mat(parser *P, char **t, int n)
+{
+    if (n < 10 || n > 13)
+        return perr(P, MRT_E_INVALID, "mat takes 9 to 12 fields", NULL);
+    if (mrt_hs_find_material(P->hs, t[1]) >= 0)
+        return perr(P, MRT_E_INVALID, "material defined twice", t[1]);
+    mrt_material m;
+    memset(&m, 0, sizeof(m));
+    float c[3], e[3];
+    double v[6];
+    if (!color(t[2], c))
+        return perr(P, MRT_E_UNSUPPORTED, "material colour (textures unsupported)", t[2]);
+    for (int k = 0; k < 6; k++)
+        if (!num(t[3 + k], &v[k]))
+            return perr(P, MRT_E_UNSUPPORTED, "material value (textures unsupported)",
+                        t[3 + k]);
+    if (!color(t[9], e))
+        return perr(P, MRT_E_INVALID, "emission colour", t[9]);
+    if (v[0] < 0 || v[0] > 1 || v[1] < 0 || v[1] > 1 || v[2] < 0 || v[3] < 0
+        || v[3] > 1 || v[4] < 0 || v[4] > 1 || v[5] < 0)
+        return perr(P, MRT_E_INVALID, "material value out of range", t[1]);
+    if (n >= 12)
+        return perr(P, MRT_E_UNSUPPORTED, "normal maps", t[11]);
+    for (int k = 0; k < 3; k++) {
+        m.color[k] = (float)(c[k] / 255.0);
+        m.emission_color[k] = (float)(e[k] / 255.0);
+    }
+    m.metallic = (float)v[0];
+    m.roughness = (float)v[1];
+    m.ior = (float)v[2];
+    m.transmission = (float)v[3];
+    m.ao = (float)v[4];
+    m.emission_strength = (float)v[5];
+    int r = mrt_hs_add_material(P->hs, &m, t[1]);
+    return r < 0 ? r : MRT_OK;
+}
+
+static int p_line(parser *P, char **t, int n, int pass)
+{
+    const char *id = t[0];
+    if (pass == 0)
+        return strcmp(id, "mat") == 0 ? p_mat(P, t, n) : MRT_OK;
+    if (strcmp(id, "mat") == 0)
+        return MRT_OK;
+    if (strcmp(id, "A") == 0) {
+        double r;
+        float c[3];
+        if (n != 3 || !num(t[1], &r) || r < 0 || r > 1 || !color(t[2], c))
+            return perr(P, MRT_E_INVALID, "bad ambient light", NULL);
+        mrt_hs_set_ambient(P->hs, (float)r, c);
+        P->n_amb++;
+        return MRT_OK;
+    }
+    if (strcmp(id, "C") == 0) {
+        mrt_camera cam;
+        memset(&cam, 0, sizeof(cam));
+        double fov, a = 0, f = 0;
+        if (n < 4 || n > 6 || !pos3(t[1], cam.pos) || !unit_vec(t[2], cam.dir)
+            || !num(t[3], &fov) || fov < 0 || fov > 180 || fov != floor(fov)
+            || (n > 4 && (!num(t[4], &a) || a < 0))
+            || (n > 5 && (!num(t[5], &f) || f < 0)))
+            return perr(P, MRT_E_INVALID, "bad camera", NULL);
+        cam.fov_deg = (float)fov;
+        cam.defocus_angle_deg = (float)a;
+        cam.focus_dist = f > 0 ? (float)f : 1.0f;
+        mrt_hs_set_camera(P->hs, &cam);
+        P->n_cam++;
+        return MRT_OK;
+    }
+    if (strcmp(id, "L") == 0 || strcmp(id, "l") == 0) {
+        mrt_light L;
+        memset(&L, 0, sizeof(L));
+        double b, rad = 0, ss = 0;
+        L.type = MRT_LIGHT_POINT;
+        if (n < 4 || n > 6 || !pos3(t[1], L.vec) || !num(t[2], &b) || b < 0
+            || b > 1 || !color(t[3], L.color)
+            || (n > 4 && (!num(t[4], &rad) || rad < 0))
+            || (n > 5 && (!num(t[5], &ss) || ss < 0 || ss != floor(ss))))
+            return perr(P, MRT_E_INVALID, "bad light", NULL);
+        L.brightness = (float)b;
+        L.radius = (float)rad;
+        L.shadow_samples = n > 4 ? (n > 5 ? (int32_t)ss : 5) : 0;
+        if (id[0] == 'L')
+            P->n_L++;
+        int r = mrt_hs_add_light(P->hs, &L);
+        return r < 0 ? perr(P, r, "too many lights", NULL) : MRT_OK;
+    }
+    if (strcmp(id, "dl") == 0) {
+        mrt_light L;
+        memset(&L, 0, sizeof(L));
+        double b;
+        L.type = MRT_LIGHT_DIRECTIONAL;
+        if (n != 4 || !unit_vec(t[1], L.vec) || !num(t[2], &b) || b < 0 || b > 1
+            || !color(t[3], L.color))
+            return perr(P, MRT_E_INVALID, "bad directional light", NULL);
+        L.brightness = (float)b;
+        int r = mrt_hs_add_light(P->hs, &L);
+        return r < 0 ? perr(P, r, "too many lights", NULL) : MRT_OK;
+    }
+    if (strcmp(id, "sp") == 0) {
+        float c[3];
+        double d;
+        uint32_t mat;
+        if (n != 4 || !pos3(t[1], c) || !num(t[2], &d) || !(d > 0))
+            return perr(P, MRT_E_INVALID, "bad sphere", NULL);
+        int r = color_or_mat(P, t[3], &mat);
+        if (r)
+            return r;
+        r = mrt_hs_add_sphere(P->hs, c, (float)(d * 0.5), mat);
+        return r < 0 ? perr(P, r, "sphere rejected", NULL) : MRT_OK;
+    }
+    if (strcmp(id, "pl") == 0) {
+        float p[3], nn[3];
+        uint32_t mat;
+        double v[3];
+        if (n != 4 || !pos3(t[1], p) || !vec3(t[2], v))
+            return perr(P, MRT_E_INVALID, "bad plane", NULL);
+        /* the normal is kept as written (parse_elements6.c:16-39) */
+        for (int k = 0; k < 3; k++) {
+            if (v[k] < -1.0 || v[k] > 1.0)
+                return perr(P, MRT_E_INVALID, "bad plane normal", t[2]);
+            nn[k] = (float)v[k];
+        }
+        int r = color_or_mat(P, t[3], &mat);
+        if (r)
+            return r;
+        r = mrt_hs_add_plane(P->hs, p, nn, mat);
+        return r < 0 ? perr(P, r, "plane rejected", NULL) : MRT_OK;
+    }
+    if (strcmp(id, "W") == 0) {
+        double v[4];
+        if ((n != 3 && n != 5))
+            return perr(P, MRT_E_INVALID, "bad window", NULL);
+        for (int k = 1; k < n; k++)
+            if (!num(t[k], &v[k - 1]) || v[k - 1] < 1 || v[k - 1] > 16384
+                || v[k - 1] != floor(v[k - 1]))
+                return perr(P, MRT_E_INVALID, "bad window size", t[k]);
+        if (n == 5)
+            mrt_hs_set_resolution(P->hs, (uint32_t)v[2], (uint32_t)v[3]);
+        else
+            mrt_hs_set_resolution(P->hs, (uint32_t)v[0], (uint32_t)v[1]);
+        P->n_W++;
+        return MRT_OK;
+    }
+    if (!strcmp(id, "cy") || !strcmp(id, "co") || !strcmp(id, "hb")
+        || !strcmp(id, "cu") || !strcmp(id, "tex"))
+        return perr(P, MRT_E_UNSUPPORTED, "element not supported by this renderer", id);
+    return perr(P, MRT_E_INVALID, "unknown element", id);
+}
+
+int mrt_parse_rt_string(const char *text, mrt_host_scene **out, char *err,
+                        size_t errlen)
+{
+    if (!text || !out)
+        return MRT_E_INVALID;
+    *out = NULL;
+    parser P;
+    memset(&P, 0, sizeof(P));
+    P.err = err;
+    P.errlen = errlen;
+    if (err && errlen)
+        err[0] = 0;
+    P.hs = mrt_hs_new();
+    if (!P.hs)
+        return MRT_E_NOMEM;
+    size_t len = strlen(text);
+    char *buf = malloc(len + 1);
+    if (!buf) {
+        mrt_hs_free(P.hs);
+        return MRT_E_NOMEM;
+    }
+    int rc = MRT_OK;
+    for (int pass = 0; pass < 2 && rc == MRT_OK; pass++) {
+        memcpy(buf, text, len + 1);
+        P.line = 0;
+        char *save = NULL;
+        for (char *ln = buf; ln && rc == MRT_OK; ln = save) {
+            save = strchr(ln, '\n');
+            if (save)
+                *save++ = 0;
+            P.line++;
+            char *hash = strchr(ln, '#');
+            if (hash)
+                *hash = 0;
+            char *t[MAXTOK];
+            int n = 0;
+            for (char *p = strtok(ln, " \t\r"); p; p = strtok(NULL, " \t\r")) {
+                if (n == MAXTOK) {
+                    rc = perr(&P, MRT_E_INVALID, "too many fields", NULL);
+                    break;
+                }
+                t[n++] = p;
+            }
+            if (rc == MRT_OK && n > 0)
+                rc = p_line(&P, t, n, pass);
+        }
+    }
+    free(buf);
+    if (rc == MRT_OK) {
+        if (P.n_amb != 1)
+            rc = perr(&P, MRT_E_INVALID, "exactly one A is required", NULL);
+        else if (P.n_cam != 1)
+            rc = perr(&P, MRT_E_INVALID, "exactly one C is required", NULL);
+        else if (P.n_L > 1 || P.n_W > 1)
+            rc = perr(&P, MRT_E_INVALID, "at most one L and one W", NULL);
+    }
+    if (rc != MRT_OK) {
+        mrt_hs_free(P.hs);
+        return rc;
+    }
+    *out = P.hs;
+    return MRT_OK;
+}
+
+int mrt_parse_rt_file(const char *path, mrt_host_scene **out, char *err,
+                      size_t errlen)
+{
+    if (!path || !out)
+        return MRT_E_INVALID;
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        if (err && errlen)
+            snprintf(err, errlen, "cannot open %s", path);
+        return MRT_E_INVALID;
+    }
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    if (sz < 0 || sz > (64L << 20)) {
+        fclose(f);
+        return MRT_E_LIMIT;
+    }
+    char *text = malloc((size_t)sz + 1);
+    if (!text) {
+        fclose(f);
+        return MRT_E_NOMEM;
+    }
+    size_t rd = fread(text, 1, (size_t)sz, f);
+    fclose(f);
+    text[rd] = 0;
+    int rc = mrt_parse_rt_string(text, out, err, errlen);
+    free(text);
+    return rc;
+}

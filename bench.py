@@ -1,0 +1,188 @@
+"""Throughput of the wavefront path tracer on one MI355X.
+
+Workload: the procedural mesh scene at 1920x1080 -- a displaced icosphere
+subdivided 8 times (1,310,720 triangles, fBm amplitude 0.15) over a 256x256
+heightfield floor (131,072 triangles), per-triangle 60/25/15 lambert / GGX /
+dielectric materials, one point light, depth 8, fp32.  One step renders
+`--spp-per-step` samples of every pixel through the whole wavefront
+(raygen, trace, shade, shadow, resolve, accumulate) with the scene resident
+in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "42-minirt_amd" / "python"))
+
+METRIC = ("Msamples/s @1080p (batched vs plain GPU vs host CPU) at ≤1% "
+          "image error; adjoint/fwd time")
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of the encodings the kernels read and write
+B_NODE = 64          # one inner node: both child boxes + child links
+B_PRIM = 48          # one primitive record
+Q_RAY = 32           # ray read by the trace kernel (origin, direction)
+Q_HIT = 16           # hit record written
+Q_SRAY = 32          # shadow ray read
+Q_VIS = 1            # visibility byte written
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(hs, width, height, max_depth, seed, budget_s):
+    """The oracle (fp64, the reference's recursive traversal) on a band of
+    rows of the same frame, all host cores, then one thread."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import mrt_hip as M
+    import oracle_ffi as O
+    view = hs.view()
+    bvh = O.make_bvh(M.BuiltBvh(view))
+    cores = min(16, os.cpu_count() or 1)
+    res = {}
+    for threads, share in ((cores, 0.75), (1, 0.25)):
+        rows, t = 4, 0.0
+        while True:
+            r0 = height // 2 - rows // 2
+            t0 = time.perf_counter()
+            O.render(view, bvh, width, height, 0, 1, max_depth, seed,
+                     f64=True, traversal=O.TRAV_REFERENCE, threads=threads,
+                     rows=(r0, r0 + rows))
+            t = time.perf_counter() - t0
+            if t >= budget_s * share * 0.4 or rows >= height:
+                break
+            grow = max(2.0, min(8.0, budget_s * share * 0.5 / max(t, 1e-3)))
+            rows = min(height, int(rows * grow))
+        res[threads] = (rows * width / t / 1e6, rows, t)
+    v, rows, t = res[cores]
+    v1, rows1, t1 = res[1]
+    return {"value": round(v, 4), "unit": "Msamples/s", "cores": cores,
+            "kind": "port",
+            "sample": f"rows {rows} of the {width}x{height} frame at 1 spp, "
+                      f"depth {max_depth}, oracle fp64 with the reference's "
+                      f"recursive BVH traversal, {t:.1f} s",
+            "single_thread": {"value": round(v1, 4), "cores": 1,
+                              "sample": f"{rows1} rows, {t1:.1f} s"}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--spp-per-step", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--subdiv", type=int, default=8)
+    ap.add_argument("--floor", type=int, default=256)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(rendering shards by sample index; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mrt_hip as M
+    W, H = args.width, args.height
+    t0 = time.perf_counter()
+    hs = M.HostScene.mesh(args.seed, W, H, subdiv=args.subdiv,
+                          floor_cells=args.floor, amp=0.15)
+    ntri = hs.view().contents.n_triangles
+    log(f"scene: {ntri} triangles in {time.perf_counter() - t0:.1f} s")
+    r = M.Renderer(W, H)
+    t0 = time.perf_counter()
+    r.upload(hs)
+    log(f"upload (BVH build + copy): {time.perf_counter() - t0:.1f} s")
+    r.set_seed(0x5EED)
+    spp = args.spp_per_step
+    r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
+
+    # 1) counting pass: nodes / primitives per ray (same sample indices)
+    r.set_option(M.OPT_STATS, M.STATS_COUNT)
+    r.reset_stats()
+    r.render(0, spp, args.depth)
+    cnt = r.stats()
+    r.set_option(M.OPT_STATS, 0)
+    log(f"counts: {json.dumps({k: cnt[k] for k in ('rays_closest', 'rays_shadow', 'nodes_closest', 'prims_closest', 'nodes_shadow', 'prims_shadow')})}")
+
+    # 2) timed steps
+    for s in range(args.warmup):
+        r.render(spp * s, spp, args.depth)
+    r.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        r.render(spp * (args.warmup + s), spp, args.depth)
+        if s % 8 == 7:
+            log(f"step {s + 1}/{args.steps}")
+    r.synchronize()
+    dt = time.perf_counter() - t0
+    samples = args.steps * spp * W * H
+    value = samples / dt / 1e6
+
+    # 3) per-kernel HIP-event times on the library's stream
+    r.set_option(M.OPT_STATS, M.STATS_TIME)
+    r.reset_stats()
+    prof_steps = 2
+    for s in range(prof_steps):
+        r.render(spp * s, spp, args.depth)
+    tm = r.stats()
+    r.set_option(M.OPT_STATS, 0)
+    launches = prof_steps * args.depth
+    trace_ms = tm["ms_trace"] / launches
+    rays_c = cnt["rays_closest"]
+    bytes_trace = (rays_c * (Q_RAY + Q_HIT) + cnt["nodes_closest"] * B_NODE
+                   + cnt["prims_closest"] * B_PRIM)
+    bytes_shadow = (cnt["rays_shadow"] * (Q_SRAY + Q_VIS)
+                    + cnt["nodes_shadow"] * B_NODE
+                    + cnt["prims_shadow"] * B_PRIM)
+    per_launch = bytes_trace / args.depth
+    achieved = per_launch / (trace_ms * 1e-3) / 1e9
+    rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
+        * value * 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain GPU path, mesh scene {W}x{H}, "
+                               f"{ntri} triangles (icosphere subdiv "
+                               f"{args.subdiv} + {args.floor}^2 floor), depth "
+                               f"{args.depth}, {spp} spp per step",
+                   "resolution": [W, H], "triangles": ntri,
+                   "max_depth": args.depth, "spp_per_step": spp},
+        "rays_per_s": round(rays_per_s, 1),
+        "rays_per_sample": round((cnt["rays_closest"] + cnt["rays_shadow"])
+                                 / (spp * W * H), 4),
+        "bytes_per_sample": round((bytes_trace + bytes_shadow)
+                                  / (spp * W * H), 1),
+        "kernel_ms_per_step": {k: round(tm[f"ms_{k}"] / prof_steps, 3)
+                               for k in ("raygen", "trace", "shade", "shadow",
+                                         "accum", "total")},
+        "roofline": {"bound": "hbm", "kernel": "k_trace (closest hit)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "bytes_per_launch": round(per_launch),
+                     "avg_launch_ms": round(trace_ms, 4)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(hs, W, H, args.depth, 0x5EED,
+                                           args.cpu_budget)
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

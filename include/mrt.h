@@ -1,0 +1,226 @@
+/*
+ * mrt.h - C ABI of libmrt_hip.so, the MI355X (gfx950) wavefront path tracer
+ * that replaces miniRT's render back-ends (src/ray_tracing, src/gpu,
+ * src/caustic and the HDRI sampler) behind the reference's render-frame /
+ * accumulation-buffer contract.
+ *
+ * Plain C types only: pointers, sizes and the POD structs below.  Every entry
+ * point returns an int (MRT_OK = 0, negative MRT_E_* on failure) and never
+ * calls exit(); mrt_last_error() gives the message of the last failure on a
+ * context.  One context per host thread; calls on one context are not
+ * thread-safe (the caller serialises them, as the reference's main thread
+ * does with its GL context).
+ */
+#ifndef MRT_H
+#define MRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MRT_OK              0
+#define MRT_E_INVALID      -1   /* bad argument or scene data               */
+#define MRT_E_NOMEM        -2   /* host or device allocation failed         */
+#define MRT_E_HIP          -3   /* a HIP runtime call failed                */
+#define MRT_E_STATE        -4   /* call out of order (e.g. render w/o scene) */
+#define MRT_E_UNSUPPORTED  -5   /* feature not built in this library        */
+#define MRT_E_LIMIT        -6   /* a size exceeds a documented maximum      */
+#define MRT_E_NODEVICE     -7   /* no usable gfx950 device                  */
+
+/* ---- scene contract (host-owned; the library copies on upload) --------- */
+
+/* One material.  Mirrors t_mat (includes/scene_structs.h:84-106) with the
+ * texture slots dropped: color is color_value / 255 (color_to_fcolor,
+ * src/math/color/conversions.c:15-18), emission_color likewise.  An object
+ * given a bare colour in the .rt file is a material with metallic = 0,
+ * ior = 0, ao = 1 and no emission, which takes exactly the reference's
+ * "no material" branch (default_mat, src/ray_tracing/path_trace.c:18-19). */
+typedef struct mrt_material {
+    float color[3];
+    float metallic;           /* 1 metal, 0 dielectric/diffuse, else random */
+    float roughness;          /* GGX alpha = roughness^2                     */
+    float ior;                /* 0 => diffuse                                */
+    float transmission;
+    float ao;
+    float emission_strength;
+    float emission_color[3];
+} mrt_material;               /* 48 bytes */
+
+#define MRT_LIGHT_POINT        0
+#define MRT_LIGHT_DIRECTIONAL  1
+
+/* A light: t_light / t_dlight (includes/scene_structs.h:137-152). */
+typedef struct mrt_light {
+    int32_t type;             /* MRT_LIGHT_*                                 */
+    float   vec[3];           /* position (point) or orientation (dir.)      */
+    float   brightness;       /* 0..1                                        */
+    float   color[3];         /* 0..255, as t_color                          */
+    float   radius;           /* soft shadows when > 0 and samples > 1       */
+    int32_t shadow_samples;
+} mrt_light;                  /* 40 bytes */
+
+/* Camera: t_camera (includes/scene_structs.h:128-135). */
+typedef struct mrt_camera {
+    float pos[3];
+    float dir[3];             /* normalised by the library                   */
+    float fov_deg;            /* horizontal field of view                    */
+    float defocus_angle_deg;  /* 0 = pinhole                                 */
+    float focus_dist;         /* 0 => 1, as the reference's UI default       */
+} mrt_camera;
+
+#define MRT_SCENE_TRIANGLE_ONE_SIDED  0x1u  /* --triangle-one-sided         */
+#define MRT_SCENE_LIGHT_EXACT         0x2u  /* do not truncate light sums to
+                                               integers (reference t_lcolor
+                                               is uint64_t and truncates)   */
+
+typedef struct mrt_scene {
+    uint32_t            n_triangles;
+    const float        *tri_pos;      /* 9 floats / tri: v0 v1 v2             */
+    const float        *tri_nrm;      /* 9 floats / tri: vertex normals, or
+                                         NULL for the geometric normal        */
+    const uint32_t     *tri_mat;      /* material index / tri                 */
+    uint32_t            n_spheres;
+    const float        *sph;          /* 4 floats / sphere: centre, radius    */
+    const uint32_t     *sph_mat;
+    uint32_t            n_planes;
+    const float        *pln;          /* 6 floats / plane: point, normal      */
+    const uint32_t     *pln_mat;
+    uint32_t            n_materials;
+    const mrt_material *materials;
+    uint32_t            n_lights;
+    const mrt_light    *lights;
+    mrt_camera          camera;
+    float               ambient_ratio;     /* A ratio                          */
+    float               ambient_color[3];  /* A colour, 0..255                 */
+    float               ior_all;           /* scene->ior_all (1.0)             */
+    uint32_t            flags;             /* MRT_SCENE_*                      */
+} mrt_scene;
+
+/* Primitive ids reported by MRT_AOV_PRIM_ID: the index into the array the
+ * primitive came from, tagged with its kind.  Miss = MRT_PRIM_NONE. */
+#define MRT_PRIM_NONE      0xFFFFFFFFu
+#define MRT_PRIM_TRIANGLE  0x00000000u
+#define MRT_PRIM_SPHERE    0x40000000u
+#define MRT_PRIM_PLANE     0x80000000u
+#define MRT_PRIM_KIND_MASK 0xC0000000u
+
+/* Estimator for the HDRI term (see mrt_upload_hdri). */
+#define MRT_EST_REFERENCE  0   /* default_mat's importance_sampling term      */
+#define MRT_EST_MIS        1   /* proper solid-angle pdf + balance heuristic  */
+
+/* Options for mrt_set_option. */
+#define MRT_OPT_SEED        1  /* 64-bit RNG key (value cast to uint64)      */
+#define MRT_OPT_ESTIMATOR   2  /* MRT_EST_*                                  */
+#define MRT_OPT_STATS       3  /* MRT_STATS_* flags                          */
+#define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront batch        */
+#define MRT_OPT_BLOCK       5  /* threads per block for the trace kernels    */
+
+/* Flags of MRT_OPT_STATS. */
+#define MRT_STATS_COUNT     1  /* count BVH nodes / primitives per ray       */
+#define MRT_STATS_TIME      2  /* time every kernel launch with HIP events   */
+
+/* Arbitrary output variables of primary rays (mrt_render_aov). */
+#define MRT_AOV_PRIM_ID     0  /* uint32 per pixel                           */
+#define MRT_AOV_T           1  /* float per pixel, distance along unit dir   */
+#define MRT_AOV_NORMAL      2  /* 3 floats per pixel, facing the ray         */
+#define MRT_AOV_NODES       3  /* uint32 per pixel: BVH nodes visited        */
+#define MRT_AOV_PRIMS       4  /* uint32 per pixel: primitives tested        */
+
+/* Counters and kernel times since the last mrt_reset_stats. */
+typedef struct mrt_stats {
+    uint64_t paths;            /* camera paths started                        */
+    uint64_t rays_closest;     /* closest-hit segments traced                 */
+    uint64_t rays_shadow;      /* shadow / visibility rays traced             */
+    uint64_t nodes_closest;    /* BVH nodes visited by closest-hit rays       */
+    uint64_t prims_closest;    /* primitives tested by closest-hit rays       */
+    uint64_t nodes_shadow;
+    uint64_t prims_shadow;
+    uint64_t launches_trace;   /* closest-hit kernel launches                 */
+    uint64_t launches_shadow;
+    uint64_t launches_shade;
+    uint64_t nonfinite;        /* NaN/Inf samples zeroed                      */
+    double   ms_raygen;        /* summed HIP-event times per kernel class     */
+    double   ms_trace;
+    double   ms_shade;
+    double   ms_shadow;
+    double   ms_accum;
+    double   ms_total;         /* mrt_render wall time on the stream          */
+} mrt_stats;
+
+typedef struct mrt_ctx mrt_ctx;
+
+/* Replaces init_shader (src/gpu/init_shader.c:133-159, includes/gpu.h:32):
+ * picks the device, creates the stream and the W x H accumulation buffer. */
+int  mrt_init(int device, uint32_t width, uint32_t height, mrt_ctx **out);
+
+/* Replaces convert_scene_build (src/gpu/gpu_scene.c:104-124, includes/gpu.h:86)
+ * together with init_bvh / build_bvh (src/ray_tracing/bvh/bvh_manager.c:17-37):
+ * builds the BVH on the host and uploads every array.  The library copies;
+ * the caller keeps ownership of the scene. */
+int  mrt_upload_scene(mrt_ctx *ctx, const mrt_scene *scene);
+
+/* Replaces convert_scene (camera/viewport only, src/gpu/gpu_scene.c:62-75). */
+int  mrt_set_camera(mrt_ctx *ctx, const mrt_camera *camera);
+
+/* Replaces send_importance_sampling (src/gpu/importance_sampling.c:69-96) and
+ * make_importance_sampling_map (src/ray_tracing/importance_sampling/
+ * importance_sampling_manager.c:41-61): rgb is width*height linear texels,
+ * row 0 at the nadir (the reference's hdr.pixels order), exposure as
+ * t_hdr.exposure.  Builds the inverse-CDF tables and uploads them. */
+int  mrt_upload_hdri(mrt_ctx *ctx, const float *rgb, uint32_t width,
+                     uint32_t height, float exposure);
+
+/* Replaces caustic_manager (src/caustic/caustic_manager.c:83-102) and
+ * build_caustic_grid (src/gpu/convert_caustic.c:85-102). */
+int  mrt_build_photons(mrt_ctx *ctx, uint32_t n_photons, float radius);
+
+/* Replaces compute_frame_gpu (src/gpu/use_shader.c:81-100): traces
+ * n_samples paths per pixel, sample indices first_sample .. +n_samples-1,
+ * each of at most max_depth segments, and adds them to the device
+ * accumulation buffer.  Asynchronous on the context's stream. */
+int  mrt_render(mrt_ctx *ctx, uint32_t first_sample, uint32_t n_samples,
+                uint32_t max_depth);
+
+/* Replaces get_result (src/gpu/use_shader.c:53-79): waits for the stream and
+ * adds the device sums into the host-owned W*H*3 double buffer, as
+ * add_fcolor_float3 does into screen.float_render. */
+int  mrt_readback_add(mrt_ctx *ctx, double *float_render);
+
+/* Copies the device sums (W*H*3 floats) to out. */
+int  mrt_readback_f32(mrt_ctx *ctx, float *out);
+
+/* Zeroes the device accumulation buffer. */
+int  mrt_clear(mrt_ctx *ctx);
+
+/* Render modes 1/2 of the reference (src/gpu/use_shader.c:69-71,
+ * src/shaders/shader.comp:52-57) generalised: traces the primary rays of
+ * sample `sample` and writes one MRT_AOV_* value per pixel to host `out`. */
+int  mrt_render_aov(mrt_ctx *ctx, int aov, uint32_t sample, void *out);
+
+/* Traces caller-supplied rays (host arrays of n origins and n unit
+ * directions, 3 floats each) and returns closest-hit primitive ids and t;
+ * used by parity tests and by hosts that pick objects with a ray
+ * (src/edition_mod/select_obj.c:40-66 calls hit_register_all the same way). */
+int  mrt_trace_rays(mrt_ctx *ctx, uint32_t n, const float *orig,
+                    const float *dir, uint32_t *prim_id, float *t);
+
+int  mrt_set_option(mrt_ctx *ctx, int option, double value);
+int  mrt_get_stats(mrt_ctx *ctx, mrt_stats *out);
+int  mrt_reset_stats(mrt_ctx *ctx);
+int  mrt_synchronize(mrt_ctx *ctx);
+
+/* Replaces clean_shaders (src/gpu/clean_shaders.c:69-84). */
+void mrt_destroy(mrt_ctx *ctx);
+
+const char *mrt_last_error(const mrt_ctx *ctx);
+const char *mrt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRT_H */

@@ -1,0 +1,179 @@
+/*
+ * mrt_host.h - host-side C helpers exported by libmrt_hip.so: scene
+ * containers, the SAH BVH builder, procedural scene generators, the .rt
+ * reader, the HDRI inverse-CDF tables, PPM / .rts output and the
+ * render-frame loop that drives the device entry points of mrt.h.
+ *
+ * None of these touch the GPU; they run (and are tested) on any host.
+ */
+#ifndef MRT_HOST_H
+#define MRT_HOST_H
+
+#include "mrt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- BVH (replaces src/ray_tracing/bvh/bvh_build.c:17-124) ------------ */
+
+/* A binary node holding the boxes of BOTH children, so one 64-byte fetch
+ * decides which children a ray enters and in which order.
+ * child[k] >= 0 : index of an inner node.
+ * child[k] <  0 : leaf; v = ~child[k], first primitive = v >> 3,
+ *                 count = (v & 7) + 1 (at most MRT_BVH_MAX_LEAF).
+ * An absent child has a zero-size box at 1e30 that no ray reaches. */
+typedef struct mrt_bvh_node {
+    float   lo0[3], hi0[3];
+    float   lo1[3], hi1[3];
+    int32_t child[2];
+    uint32_t pad[2];
+} mrt_bvh_node;                       /* 64 bytes */
+
+/* One primitive in leaf order (48 bytes, three 16-byte words).
+ * triangle: a = v0.xyz | src id,  b = (v1-v0).xyz | kind 0,
+ *           c = (v2-v0).xyz | material
+ * sphere:   a = centre.xyz | src id,  b = (radius, radius^2, 0) | kind 1,
+ *           c = (0, 0, 0) | material
+ * "src id" is the MRT_PRIM_* tagged index into the scene arrays. */
+typedef struct mrt_prim {
+    float a[4], b[4], c[4];
+} mrt_prim;
+
+#define MRT_PRIMKIND_TRIANGLE 0u
+#define MRT_PRIMKIND_SPHERE   1u
+#define MRT_BVH_MAX_LEAF      4u
+#define MRT_BVH_MAX_DEPTH     60u     /* the device traversal stack holds 64 */
+
+typedef struct mrt_bvh {
+    uint32_t      n_nodes;
+    mrt_bvh_node *nodes;              /* node 0 is the root                  */
+    uint32_t      n_prims;
+    mrt_prim     *prims;              /* leaf order                          */
+    float        *shade_nrm;          /* leaf order, 9 floats: vertex normals */
+    uint32_t      depth;              /* deepest inner-node level (root = 1)  */
+    uint32_t      n_leaves;
+    double        sah_cost;           /* C_trav = 1, C_isect = 1, root area   */
+} mrt_bvh;
+
+/* Binned SAH over the scene's triangles and spheres, leaves of at most
+ * max_leaf primitives (1..MRT_BVH_MAX_LEAF).  Planes stay out of the tree
+ * (the reference tests them linearly, hit_register.c:79-107). */
+int  mrt_bvh_build(const mrt_scene *scene, uint32_t max_leaf, mrt_bvh *out);
+void mrt_bvh_free(mrt_bvh *bvh);
+
+/* ---- scene container ---------------------------------------------------- */
+
+typedef struct mrt_host_scene mrt_host_scene;
+
+mrt_host_scene *mrt_hs_new(void);
+void            mrt_hs_free(mrt_host_scene *hs);
+/* Returns the index of the new element, or a negative MRT_E_*. */
+int  mrt_hs_add_material(mrt_host_scene *hs, const mrt_material *m,
+                         const char *name);
+int  mrt_hs_find_material(const mrt_host_scene *hs, const char *name);
+int  mrt_hs_add_sphere(mrt_host_scene *hs, const float centre[3], float radius,
+                       uint32_t mat);
+int  mrt_hs_add_plane(mrt_host_scene *hs, const float point[3],
+                      const float normal[3], uint32_t mat);
+int  mrt_hs_add_triangle(mrt_host_scene *hs, const float v[9],
+                         const float n[9] /* NULL: flat */, uint32_t mat);
+int  mrt_hs_add_light(mrt_host_scene *hs, const mrt_light *l);
+void mrt_hs_set_camera(mrt_host_scene *hs, const mrt_camera *c);
+void mrt_hs_set_ambient(mrt_host_scene *hs, float ratio, const float rgb255[3]);
+void mrt_hs_set_resolution(mrt_host_scene *hs, uint32_t w, uint32_t h);
+void mrt_hs_get_resolution(const mrt_host_scene *hs, uint32_t *w, uint32_t *h);
+void mrt_hs_set_flags(mrt_host_scene *hs, uint32_t flags);
+/* The HDRI (width*height*3 linear floats, row 0 at the nadir); copied. */
+int  mrt_hs_set_hdri(mrt_host_scene *hs, const float *rgb, uint32_t w,
+                     uint32_t h, float exposure);
+int  mrt_hs_get_hdri(const mrt_host_scene *hs, const float **rgb, uint32_t *w,
+                     uint32_t *h, float *exposure);
+/* A view of the arrays; valid until the next mrt_hs_add_* call. */
+const mrt_scene *mrt_hs_view(mrt_host_scene *hs);
+
+/* ---- procedural scenes --------------------------------------------------- */
+
+/* Ground plane y=0, three radius-1 spheres (lambert 0.8, GGX metal
+ * roughness 0.1, dielectric IOR 1.5) centred at x in {-2,0,2} plus a seeded
+ * jitter in [-0.5,0.5) on x and z, one point light, ambient 0.1, pinhole
+ * camera with a 60 degree field of view; width x height. */
+int  mrt_gen_plain_scene(uint64_t seed, uint32_t width, uint32_t height,
+                         mrt_host_scene **out);
+
+/* Displaced icosphere (subdiv levels, seeded fBm radial displacement of
+ * amplitude amp) over a cells x cells heightfield floor; per-triangle
+ * materials drawn 60% lambert / 25% GGX (roughness U[0.05,0.6]) / 15%
+ * dielectric; lit by the plain scene's point light and ambient. */
+int  mrt_gen_mesh_scene(uint64_t seed, uint32_t width, uint32_t height,
+                        uint32_t subdiv, uint32_t floor_cells, float amp,
+                        mrt_host_scene **out);
+
+/* Adds a displaced icosphere to hs; returns the triangle count or < 0. */
+int  mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
+                       uint32_t subdiv, float amp, uint64_t seed,
+                       int material_mix /* 1: per-triangle mix */,
+                       uint32_t mat /* used when material_mix == 0 */);
+
+/* Equirectangular sky: analytic gradient plus a sun disc of the given
+ * angular radius (degrees) and radiance at a seeded elevation in
+ * [15, 60] degrees.  rgb receives w*h*3 floats, row 0 at the nadir. */
+int  mrt_gen_hdri_sky(uint64_t seed, uint32_t w, uint32_t h,
+                      float sun_radius_deg, float sun_radiance, float *rgb);
+
+/* ---- HDRI tables (src/ray_tracing/importance_sampling/calc_*.c) --------- */
+
+typedef struct mrt_hdri_tables {
+    uint32_t w, h;
+    float   *pdf_joint;          /* w*h                                    */
+    float   *marg_inv;           /* h   quantised inverse marginal CDF     */
+    float   *cond_inv;           /* w*h quantised inverse conditional CDF  */
+    double   column_average;
+} mrt_hdri_tables;
+
+int  mrt_hdri_build_tables(const float *rgb, uint32_t w, uint32_t h,
+                           float exposure, mrt_hdri_tables *out);
+void mrt_hdri_free_tables(mrt_hdri_tables *t);
+
+/* ---- .rt reader (src/parsing/parse_scene.c:99-125) ---------------------- */
+
+/* Reads a .rt scene: A C L l dl sp pl mat W (cu / cy / co / hb / tex are
+ * rejected with MRT_E_UNSUPPORTED).  err receives a message on failure. */
+int  mrt_parse_rt_file(const char *path, mrt_host_scene **out, char *err,
+                       size_t errlen);
+int  mrt_parse_rt_string(const char *text, mrt_host_scene **out, char *err,
+                         size_t errlen);
+
+/* ---- output (src/utils/ppm/export_ppm.c, src/utils/export_render_state.c) */
+
+/* P6 PPM of clamp01(pow(acc / samples, 1 / gamma)) * 255. */
+int  mrt_export_ppm(const char *path, const double *float_render,
+                    uint32_t w, uint32_t h, uint32_t samples, double gamma);
+/* render_state.rts: [i32 w][i32 h][i32 sample][w*h*3 f64]. */
+int  mrt_export_rts(const char *path, const double *float_render,
+                    uint32_t w, uint32_t h, uint32_t samples);
+int  mrt_load_rts(const char *path, double *float_render, uint32_t w,
+                  uint32_t h, uint32_t *samples);
+
+/* ---- render-frame loop (src/rendering/loop.c:36-63) --------------------- */
+
+typedef struct mrt_frame {
+    mrt_ctx  *ctx;
+    uint32_t  w, h;
+    uint32_t  sample;            /* screen.sample                          */
+    uint32_t  spp;               /* screen.spp (--max-samples)             */
+    uint32_t  max_bounces;       /* controls.max_bounces                   */
+    uint32_t  samples_per_tick;  /* samples rendered per render_frame call */
+    double   *float_render;      /* host-owned W*H*3 sums                  */
+} mrt_frame;
+
+/* One tick: on sample 0 clears the sums, then renders samples_per_tick
+ * samples (never past spp) and adds them into float_render.  Returns the
+ * number of samples added, 0 once spp is reached, < 0 on failure. */
+int  mrt_frame_tick(mrt_frame *f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRT_HOST_H */

@@ -1,0 +1,211 @@
+"""ORACLE (test infrastructure only): ctypes access to liboracle.so.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg,
+as the checker or the timed CPU baseline -- never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent / "liboracle.so"
+_lib = None
+
+TRAV_BRUTE = 0
+TRAV_REFERENCE = 1
+TRAV_ORDERED = 2
+
+
+class OrcBvh(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("n_nodes", C.c_uint32),
+                ("prims", C.c_void_p), ("n_prims", C.c_uint32)]
+
+
+class OrcHdri(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgb", C.c_void_p),
+                ("exposure", C.c_float), ("pdf_joint", C.c_void_p),
+                ("marg_inv", C.c_void_p), ("cond_inv", C.c_void_p)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
+                ("first_sample", C.c_uint32), ("n_samples", C.c_uint32),
+                ("max_depth", C.c_uint32), ("seed", C.c_uint64),
+                ("traversal", C.c_int), ("estimator", C.c_int),
+                ("threads", C.c_int), ("row_begin", C.c_uint32),
+                ("row_end", C.c_uint32)]
+
+
+class OrcCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "rays_closest", "rays_shadow", "nodes_closest", "prims_closest",
+        "nodes_shadow", "prims_shadow")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise ImportError(f"{LIB} not built (make -C oracle)")
+        L = C.CDLL(str(LIB))
+        vp, u32, fp = C.c_void_p, C.c_uint32, C.POINTER(C.c_float)
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.orc_uniform4.argtypes = [C.c_uint64, u32, u32, u32, u32, u32, fp]
+        L.orc_viewport.argtypes = [vp, u32, u32, C.POINTER(C.c_double)]
+        L.orc_raygen_f32.argtypes = [vp, u32, u32, u32, C.c_uint64, fp, fp]
+        for name in ("orc_trace_rays_f32", "orc_trace_rays_f64"):
+            getattr(L, name).argtypes = [vp, vp, C.c_int, u32, fp, fp,
+                                         C.POINTER(C.c_uint32), vp,
+                                         C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32), C.c_int]
+        L.orc_occluded_f32.argtypes = [vp, vp, u32, fp, fp, fp,
+                                       C.POINTER(C.c_uint8),
+                                       C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32)]
+        for name in ("orc_render_f32", "orc_render_f64"):
+            getattr(L, name).argtypes = [vp, vp, vp, C.POINTER(OrcParams),
+                                         C.POINTER(C.c_double),
+                                         C.POINTER(OrcCounters)]
+            getattr(L, name).restype = C.c_int
+        L.orc_hdri_tables.argtypes = [fp, u32, u32, C.c_float, fp, fp, fp,
+                                      C.POINTER(C.c_double)]
+        L.orc_hdri_tables.restype = C.c_int
+        dp = C.POINTER(C.c_double)
+        L.orc_sample_ggx_f64.argtypes = [dp, C.c_double, C.c_float,
+                                         C.c_float, dp]
+        L.orc_sample_cos_f64.argtypes = [dp, C.c_float, C.c_float, dp]
+        L.orc_unit_sphere_f64.argtypes = [fp, dp]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def uniform4(seed, pixel, sample, bounce, block, stream=0):
+    o = np.zeros(4, np.float32)
+    lib().orc_uniform4(seed, pixel, sample, bounce, block, stream, _fp(o))
+    return o
+
+
+def viewport(camera_ptr, w, h):
+    o = (C.c_double * 19)()
+    lib().orc_viewport(camera_ptr, w, h, o)
+    return np.array(o[:])
+
+
+def raygen_f32(scene_view, w, h, sample, seed):
+    cam = C.addressof(scene_view.contents.camera)
+    o = np.zeros((w * h, 3), np.float32)
+    d = np.zeros((w * h, 3), np.float32)
+    lib().orc_raygen_f32(cam, w, h, sample, seed, _fp(o), _fp(d))
+    return o, d
+
+
+def make_bvh(built):
+    """OrcBvh over a mrt_hip.BuiltBvh (keeps a reference to its arrays)."""
+    b = OrcBvh(built.nodes.ctypes.data, built.n_nodes,
+               built.prims.ctypes.data, built.n_prims)
+    b._keep = built
+    return b
+
+
+def trace_rays(scene_view, bvh, mode, orig, dirs, f64=False, threads=0):
+    o = np.ascontiguousarray(orig, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    n = o.size // 3
+    prim = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float64 if f64 else np.float32)
+    nodes = np.zeros(n, np.uint32)
+    prims = np.zeros(n, np.uint32)
+    fn = lib().orc_trace_rays_f64 if f64 else lib().orc_trace_rays_f32
+    fn(C.cast(scene_view, C.c_void_p),
+       C.byref(bvh) if bvh is not None else None, mode, n, _fp(o), _fp(d),
+       _up(prim), t.ctypes.data, _up(nodes), _up(prims), threads)
+    return prim, t, nodes, prims
+
+
+def occluded_f32(scene_view, bvh, orig, dirs, tmax):
+    o = np.ascontiguousarray(orig, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    tm = np.ascontiguousarray(tmax, np.float32)
+    n = o.size // 3
+    occ = np.zeros(n, np.uint8)
+    nodes = np.zeros(n, np.uint32)
+    prims = np.zeros(n, np.uint32)
+    lib().orc_occluded_f32(C.cast(scene_view, C.c_void_p), C.byref(bvh), n,
+                           _fp(o), _fp(d), _fp(tm),
+                           occ.ctypes.data_as(C.POINTER(C.c_uint8)),
+                           _up(nodes), _up(prims))
+    return occ, nodes, prims
+
+
+def render(scene_view, bvh, width, height, first_sample, n_samples,
+           max_depth, seed, f64=False, traversal=TRAV_ORDERED,
+           estimator=0, threads=0, rows=None, hdri=None):
+    acc = np.zeros((height, width, 3), np.float64)
+    p = OrcParams(width, height, first_sample, n_samples, max_depth, seed,
+                  traversal, estimator, threads,
+                  rows[0] if rows else 0, rows[1] if rows else 0)
+    cnt = OrcCounters()
+    fn = lib().orc_render_f64 if f64 else lib().orc_render_f32
+    rc = fn(C.cast(scene_view, C.c_void_p),
+            C.byref(bvh) if bvh is not None else None,
+            C.byref(hdri) if hdri is not None else None, C.byref(p),
+            acc.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError("oracle render failed")
+    return acc, cnt.as_dict()
+
+
+def hdri_tables(rgb, exposure=0.0):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    pdf = np.zeros((h, w), np.float32)
+    marg = np.zeros(h, np.float32)
+    cond = np.zeros((h, w), np.float32)
+    col = C.c_double()
+    rc = lib().orc_hdri_tables(_fp(rgb), w, h, exposure, _fp(pdf), _fp(marg),
+                               _fp(cond), C.byref(col))
+    if rc != 0:
+        raise RuntimeError("oracle hdri tables failed")
+    return pdf, marg, cond, col.value
+
+
+def sample_ggx(n, rough, u1, u2):
+    nn = (C.c_double * 3)(*n)
+    o = (C.c_double * 3)()
+    lib().orc_sample_ggx_f64(nn, rough, u1, u2, o)
+    return np.array(o[:])
+
+
+def sample_cos(n, u1, u2):
+    nn = (C.c_double * 3)(*n)
+    o = (C.c_double * 3)()
+    lib().orc_sample_cos_f64(nn, u1, u2, o)
+    return np.array(o[:])
+
+
+def unit_sphere(u4):
+    u = np.ascontiguousarray(u4, np.float32)
+    o = (C.c_double * 3)()
+    lib().orc_unit_sphere_f64(_fp(u), o)
+    return np.array(o[:])

@@ -1,0 +1,80 @@
+"""SAH BVH builder invariants: every primitive in exactly one leaf, child
+boxes contain their primitives, leaves of at most 4, bounded depth."""
+import numpy as np
+import pytest
+
+import mrt_hip as M
+
+LEAF_BITS = 3
+
+
+def collect(b):
+    nf = b.node_floats()
+    ch = b.node_children()
+    pw = b.prim_words()
+    pf = b.prims.view(np.float32).reshape(-1, 12)
+    seen = np.zeros(b.n_prims, np.int32)
+    stack = [(0, 1)]
+    max_depth = 0
+    while stack:
+        node, depth = stack.pop()
+        max_depth = max(max_depth, depth)
+        for side in range(2):
+            lo = nf[node, 6 * side:6 * side + 3]
+            hi = nf[node, 6 * side + 3:6 * side + 6]
+            c = int(ch[node, side])
+            if c >= 0:
+                # a child inner node's boxes lie inside the parent's box
+                clo = np.minimum(nf[c, 0:3], nf[c, 6:9])
+                chi = np.maximum(nf[c, 3:6], nf[c, 9:12])
+                assert np.all(clo >= lo) and np.all(chi <= hi)
+                stack.append((c, depth + 1))
+            else:
+                v = ~c & 0xFFFFFFFF
+                first, cnt = v >> LEAF_BITS, (v & 7) + 1
+                if lo[0] >= 1e29:          # absent child of a one-leaf tree
+                    continue
+                assert cnt <= 4
+                for i in range(first, first + cnt):
+                    seen[i] += 1
+                    kind = pw[i, 7]
+                    if kind == 0:
+                        v0 = pf[i, 0:3]
+                        verts = np.stack([v0, v0 + pf[i, 4:7], v0 + pf[i, 8:11]])
+                        assert np.all(verts.min(0) >= lo - 1e-6 * (1 + np.abs(lo)))
+                        assert np.all(verts.max(0) <= hi + 1e-6 * (1 + np.abs(hi)))
+                    else:
+                        c0, r = pf[i, 0:3], pf[i, 4]
+                        assert np.all(c0 - r >= lo) and np.all(c0 + r <= hi)
+    return seen, max_depth
+
+
+@pytest.mark.parametrize("subdiv,floor", [(3, 8), (5, 32)])
+def test_mesh_bvh_invariants(subdiv, floor):
+    hs = M.HostScene.mesh(4, 64, 36, subdiv=subdiv, floor_cells=floor)
+    v = hs.view()
+    b = M.BuiltBvh(v)
+    n = v.contents.n_triangles
+    assert b.n_prims == n
+    seen, depth = collect(b)
+    assert np.all(seen == 1)
+    assert depth == b.depth <= 60
+    # every source triangle appears once among the records
+    src = b.prim_words()[:, 3]
+    assert np.array_equal(np.sort(src), np.arange(n, dtype=np.uint32))
+
+
+def test_spheres_and_single_leaf():
+    hs = M.HostScene.plain(3, 32, 18)
+    b = M.BuiltBvh(hs.view())
+    assert b.n_prims == 3 and b.n_nodes == 1
+    seen, _ = collect(b)
+    assert np.all(seen == 1)
+
+
+def test_sah_beats_median_depth_bound():
+    hs = M.HostScene.mesh(5, 64, 36, subdiv=6, floor_cells=0)
+    b = M.BuiltBvh(hs.view())
+    n = b.n_prims
+    assert b.depth <= 2 * int(np.ceil(np.log2(n)))
+    assert b.n_leaves >= n // 4

@@ -1,0 +1,153 @@
+"""HIP path vs the oracle on the same counter-RNG stream (needs a GPU)."""
+import numpy as np
+import pytest
+
+import mrt_hip as M
+import oracle_ffi as O
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x5EED1234
+
+
+def _setup(hs):
+    view = hs.view()
+    built = M.BuiltBvh(view)
+    return view, O.make_bvh(built)
+
+
+@pytest.fixture(scope="module")
+def plain():
+    hs = M.HostScene.plain(7, 160, 90)
+    view, bvh = _setup(hs)
+    r = M.Renderer(160, 90)
+    r.upload(hs)
+    r.set_seed(SEED)
+    yield hs, view, bvh, r
+    r.close()
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    hs = M.HostScene.mesh(3, 192, 108, subdiv=6, floor_cells=48)
+    view, bvh = _setup(hs)
+    r = M.Renderer(192, 108)
+    r.upload(hs)
+    r.set_seed(SEED)
+    yield hs, view, bvh, r
+    r.close()
+
+
+@pytest.mark.parametrize("which", ["plain", "mesh"])
+@pytest.mark.parametrize("sample", [0, 5])
+def test_primary_hits_match_fp32_oracle(request, which, sample):
+    hs, view, bvh, r = request.getfixturevalue(which)
+    w, h = r.width, r.height
+    ids = r.aov(M.AOV_PRIM_ID, sample)
+    t = r.aov(M.AOV_T, sample)
+    nodes = r.aov(M.AOV_NODES, sample)
+    prims = r.aov(M.AOV_PRIMS, sample)
+    o, d = O.raygen_f32(view, w, h, sample, SEED)
+    rid, rt, rn, rp = O.trace_rays(view, bvh, O.TRAV_ORDERED, o, d)
+    print(f"{which} s={sample}: id {np.mean(ids == rid):.6f} "
+          f"nodes {np.mean(nodes == rn):.6f} prims {np.mean(prims == rp):.6f}")
+    assert np.mean(ids == rid) >= 0.9999
+    hit = rid != M.PRIM_NONE
+    rel = np.abs(t[hit] - rt[hit]) / rt[hit]
+    assert np.mean(rel <= 1e-6) >= 0.9999
+    # same node order on the same fp32 rays: identical traversal counts
+    assert np.mean(nodes == rn) >= 0.999
+    assert np.mean(prims == rp) >= 0.999
+
+
+@pytest.mark.parametrize("which", ["plain", "mesh"])
+def test_primary_hits_match_fp64_reference_semantics(request, which):
+    """IDs match the reference-precision oracle (fp64 vectors, fp32 MT
+    scalars, recursive both-children hit_bvh) on >= 99.99% of pixels and t
+    within 1e-5 relative."""
+    hs, view, bvh, r = request.getfixturevalue(which)
+    w, h = r.width, r.height
+    ids = r.aov(M.AOV_PRIM_ID, 0)
+    t = r.aov(M.AOV_T, 0)
+    o, d = O.raygen_f32(view, w, h, 0, SEED)
+    rid, rt, _, _ = O.trace_rays(view, bvh, O.TRAV_REFERENCE, o, d, f64=True)
+    same = ids == rid
+    hit = same & (rid != M.PRIM_NONE)
+    rel = np.abs(t[hit].astype(np.float64) - rt[hit]) / rt[hit]
+    good = same.copy()
+    good[hit] &= rel <= 1e-5
+    print(f"{which}: fp64 id match {same.mean():.6f}, id+t {good.mean():.6f}")
+    assert good.mean() >= 0.9999
+
+
+@pytest.mark.parametrize("which", ["plain", "mesh"])
+def test_image_matches_oracle_same_stream(request, which):
+    hs, view, bvh, r = request.getfixturevalue(which)
+    w, h = r.width, r.height
+    r.clear()
+    r.render(3, 4, 8)
+    img = r.readback().astype(np.float64)
+    ref, _ = O.render(view, bvh, w, h, 3, 4, 8, SEED)
+    diff = np.abs(img - ref)
+    close = diff <= 1e-4 + 1e-4 * np.abs(ref)
+    print(f"{which}: mean abs diff {diff.mean():.3e}, "
+          f"close {close.mean():.6f}")
+    # paths diverge only where an fp32 branch decision flips
+    assert close.mean() >= 0.995
+    assert diff.mean() <= 1e-3 * max(ref.mean(), 1e-6)
+
+
+def test_trace_rays_random(mesh):
+    hs, view, bvh, r = mesh
+    rng = np.random.default_rng(5)
+    n = 20000
+    o = rng.uniform(-3, 3, size=(n, 3)).astype(np.float32)
+    o[:, 1] = rng.uniform(0.2, 3.0, size=n)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    pid, t = r.trace_rays(o, d)
+    rid, rt, _, _ = O.trace_rays(view, bvh, O.TRAV_BRUTE, o, d)
+    assert np.mean(pid == rid) >= 0.9999
+    hit = rid != M.PRIM_NONE
+    assert np.allclose(t[hit], rt[hit], rtol=1e-6)
+
+
+def test_lambert_sphere_closed_form():
+    """Convex lambert sphere (albedo a) under a uniform background B, no
+    lights: every bounce ray escapes, so each sphere pixel is a*B exactly
+    (material_default.c:85-91 with add_skybox, path_trace.c:37-48)."""
+    hs = M.HostScene.empty()
+    hs.set_resolution(64, 64)
+    a = 0.6
+    m = hs.add_material(color=(a, a, a))
+    hs.add_sphere((0.0, 0.0, 0.0), 1.0, m)
+    hs.set_ambient(0.25, (255, 255, 255))
+    hs.set_camera((0.0, 0.0, -4.0), (0.0, 0.0, 1.0), fov=40.0)
+    with M.Renderer(64, 64) as r:
+        r.upload(hs)
+        r.set_seed(9)
+        ids = r.aov(M.AOV_PRIM_ID, 0).reshape(64, 64)
+        r.render(0, 1, 8)
+        img = r.readback()
+    B = np.float32(0.25)
+    on = ids == M.PRIM_SPHERE
+    assert on.mean() > 0.2
+    np.testing.assert_allclose(img[on], np.float32(a) * B, rtol=1e-6)
+    np.testing.assert_allclose(img[~on], B, rtol=1e-6)
+
+
+def test_stats_match_oracle_counts(plain):
+    hs, view, bvh, r = plain
+    w, h = r.width, r.height
+    r.set_option(M.OPT_STATS, M.STATS_COUNT)
+    r.reset_stats()
+    r.render(0, 2, 8)
+    st = r.stats()
+    r.set_option(M.OPT_STATS, 0)
+    _, cnt = O.render(view, bvh, w, h, 0, 2, 8, SEED)
+    for k in ("rays_closest", "rays_shadow", "nodes_closest", "prims_closest",
+              "nodes_shadow", "prims_shadow"):
+        rel = abs(st[k] - cnt[k]) / max(cnt[k], 1)
+        print(k, st[k], cnt[k], rel)
+        assert rel <= 2e-3, k
