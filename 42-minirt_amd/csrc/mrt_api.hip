@@ -34,6 +34,15 @@ struct mrt_ctx {
     float        ior_all = 1.0f;
     uint32_t     flags = 0;
 
+    // environment map
+    float4      *d_env_tex = nullptr;
+    float       *d_env_pdf = nullptr, *d_env_marg_inv = nullptr;
+    float       *d_env_cond_inv = nullptr, *d_mis_marg = nullptr;
+    float       *d_mis_cond = nullptr, *d_mis_pcell = nullptr;
+    uint32_t     env_w = 0, env_h = 0;
+    float        env_exposure = 0.0f;
+    bool         has_env = false;
+
     // accumulation
     float       *d_accum = nullptr;
     uint32_t    *d_ctr = nullptr;
@@ -202,6 +211,8 @@ static void free_scene(mrt_ctx *c)
     c->has_scene = false;
 }
 
+static void free_env(mrt_ctx *c);
+
 extern "C" void mrt_destroy(mrt_ctx *c)
 {
     if (!c)
@@ -211,6 +222,7 @@ extern "C" void mrt_destroy(mrt_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     free_wavefront(c);
     free_scene(c);
+    free_env(c);
     dfree(c->d_accum);
     dfree(c->d_ctr);
     dfree(c->d_stats);
@@ -334,10 +346,81 @@ extern "C" int mrt_set_camera(mrt_ctx *c, const mrt_camera *cam)
     return MRT_OK;
 }
 
-extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *, uint32_t, uint32_t,
-                               float)
+static void free_env(mrt_ctx *c)
 {
-    return fail(c, MRT_E_UNSUPPORTED, "HDRI environment not built yet");
+    dfree(c->d_env_tex);
+    dfree(c->d_env_pdf);
+    dfree(c->d_env_marg_inv);
+    dfree(c->d_env_cond_inv);
+    dfree(c->d_mis_marg);
+    dfree(c->d_mis_cond);
+    dfree(c->d_mis_pcell);
+    c->has_env = false;
+}
+
+extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
+                               uint32_t h, float exposure)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!rgb) {                        /* NULL removes the environment */
+        free_env(c);
+        return MRT_OK;
+    }
+    if (w < 2 || h < 2 || (uint64_t)w * h > (1ull << 26))
+        return fail(c, MRT_E_LIMIT, "HDRI size %ux%u out of range", w, h);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_env(c);
+    mrt_hdri_tables ref;
+    mrt_hdri_mis mis;
+    int rc = mrt_hdri_build_tables(rgb, w, h, exposure, &ref);
+    if (rc != MRT_OK)
+        return fail(c, rc, "HDRI table build failed");
+    rc = mrt_hdri_build_mis(rgb, w, h, exposure, &mis);
+    if (rc != MRT_OK) {
+        mrt_hdri_free_tables(&ref);
+        return fail(c, rc, "HDRI MIS table build failed");
+    }
+    size_t n = (size_t)w * h;
+    std::vector<float4> tex(n);
+    for (size_t i = 0; i < n; i++)
+        tex[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+    if (dalloc(c->d_env_tex, n) != hipSuccess
+        || dalloc(c->d_env_pdf, n) != hipSuccess
+        || dalloc(c->d_env_marg_inv, h) != hipSuccess
+        || dalloc(c->d_env_cond_inv, n) != hipSuccess
+        || dalloc(c->d_mis_marg, h) != hipSuccess
+        || dalloc(c->d_mis_cond, (size_t)(h - 1) * w) != hipSuccess
+        || dalloc(c->d_mis_pcell, (size_t)(h - 1) * (w - 1)) != hipSuccess)
+        rc = fail(c, MRT_E_NOMEM, "device allocation for the HDRI failed");
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_env_tex, tex.data(), n * 16);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_env_pdf, ref.pdf_joint, n * 4);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_env_marg_inv, ref.marg_inv, (size_t)h * 4);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_env_cond_inv, ref.cond_inv, n * 4);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_mis_marg, mis.marg, (size_t)h * 4);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_mis_cond, mis.cond, (size_t)(h - 1) * w * 4);
+    if (rc == MRT_OK)
+        rc = upload(c, c->d_mis_pcell, mis.pcell, (size_t)(h - 1) * (w - 1) * 4);
+    if (rc == MRT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(c, MRT_E_HIP, "HDRI upload failed");
+    mrt_hdri_free_tables(&ref);
+    mrt_hdri_free_mis(&mis);
+    if (rc != MRT_OK) {
+        free_env(c);
+        return rc;
+    }
+    c->env_w = w;
+    c->env_h = h;
+    c->env_exposure = exposure;
+    c->has_env = true;
+    return MRT_OK;
 }
 
 extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t, float)
@@ -397,6 +480,21 @@ static FrameParams frame_params(const mrt_ctx *c)
     fp.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
     fp.flags = c->flags;
     fp.seed = c->seed;
+    fp.estimator = (uint32_t)c->estimator;
+    if (c->has_env) {
+        fp.has_env = 1;
+        fp.env_tex = c->d_env_tex;
+        fp.env_pdf = c->d_env_pdf;
+        fp.env_marg_inv = c->d_env_marg_inv;
+        fp.env_cond_inv = c->d_env_cond_inv;
+        fp.mis_marg = c->d_mis_marg;
+        fp.mis_cond = c->d_mis_cond;
+        fp.mis_pcell = c->d_mis_pcell;
+        fp.env_w = c->env_w;
+        fp.env_h = c->env_h;
+        float ex = c->env_exposure + c->ambient_ratio - 1.0f;
+        fp.env_scale = ex == 0.0f ? 1.0f : powf(2.0f, ex);
+    }
     return fp;
 }
 
@@ -420,6 +518,8 @@ static uint32_t shadow_rays_per_path(const mrt_ctx *c)
                     && l.shadow_samples > 1;
         n += soft ? (uint32_t)l.shadow_samples : 1u;
     }
+    if (c->has_env)
+        n += 1;                        /* the environment ray */
     return n ? n : 1u;
 }
 
@@ -577,7 +677,9 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
             B.p_a = c->p_a;
             B.p_b = c->p_b;
             B.p_c = c->p_c;
-            uint32_t gb = (P + 255) / 256;
+            B.cap_paths = c->cap_paths;
+            B.cap_shadow = c->cap_shadow;
+            uint32_t gb = (P + SHADE_BLOCK - 1) / SHADE_BLOCK;
             if (gb > c->grid_blocks)
                 gb = c->grid_blocks;
             LAUNCH(c, tm, K_TRACE,
@@ -585,15 +687,16 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
                                 c->q_o[cur], c->q_d[cur], c->hit, c->d_ctr,
                                 c->d_stats));
             LAUNCH(c, tm, K_SHADE, launch_shade(c->stream, gb, fp, B, c->d_ctr));
-            if (!c->lights.empty()) {
+            if (!c->lights.empty() || c->has_env) {
                 LAUNCH(c, tm, K_SHADOW,
                        launch_shadow(c->stream, c->shadow_blocks, count, S,
                                      c->s_o, c->s_d, c->vis, c->d_ctr,
-                                     c->d_stats));
+                                     c->d_stats, c->cap_shadow));
                 LAUNCH(c, tm, K_RESOLVE,
                        launch_resolve(c->stream, gb, fp, B, c->d_ctr));
             }
-            LAUNCH(c, tm, K_SHADE, launch_advance(c->stream, c->d_ctr, c->d_stats));
+            LAUNCH(c, tm, K_SHADE, launch_advance(c->stream, c->d_ctr, c->d_stats,
+                                  c->cap_paths));
         }
         LAUNCH(c, tm, K_ACCUM,
                launch_accumulate(c->stream, c->acc, c->d_accum, WH, n,

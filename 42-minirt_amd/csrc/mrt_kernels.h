@@ -6,8 +6,14 @@
 
 namespace mrt {
 
-constexpr int TRACE_BLOCK = 128;   // threads per block of the trace kernels
-constexpr int TRACE_SS = 8;        // traversal stack entries kept in LDS
+#ifndef MRT_TRACE_BLOCK
+#define MRT_TRACE_BLOCK 128
+#endif
+#ifndef MRT_TRACE_SS
+#define MRT_TRACE_SS 8
+#endif
+constexpr int TRACE_BLOCK = MRT_TRACE_BLOCK;  // threads per trace block
+constexpr int TRACE_SS = MRT_TRACE_SS;        // stack entries kept in LDS
 
 // device counters (uint32)
 enum : int {
@@ -15,10 +21,14 @@ enum : int {
     CTR_NEXT = 1,         // paths appended to the next queue
     CTR_SHADOW = 2,       // shadow rays appended this bounce
     CTR_PEND = 3,         // pending light sums this bounce
-    CTR_WORK_TRACE = 4,   // work-stealing cursor of k_trace
-    CTR_WORK_SHADOW = 5,  // work-stealing cursor of k_shadow
-    CTR_COUNT = 16
+    CTR_WORK_TRACE = 16,  // 8 work cursors of k_trace (one per XCD part)
+    CTR_WORK_SHADOW = 24, // 8 work cursors of k_shadow
+    CTR_COUNT = 32
 };
+
+constexpr int NPART = 8;           // work parts: one per XCD
+constexpr uint32_t CHUNK = 256;    // items a wave takes per atomic
+constexpr int SHADE_BLOCK = 512;   // threads per block of k_shade
 
 // device statistics (uint64)
 enum : int {
@@ -46,7 +56,21 @@ struct FrameParams {
     uint32_t one_sided;
     uint32_t flags;           // MRT_SCENE_*
     uint64_t seed;
+    // environment (HDRI), present when has_env
+    const float4 *env_tex;    // env_w * env_h texels, row 0 at the nadir
+    const float  *env_pdf;    // reference tables: pdf_joint, inverse CDFs
+    const float  *env_marg_inv;
+    const float  *env_cond_inv;
+    const float  *mis_marg;   // MIS tables (mrt_hdri_mis)
+    const float  *mis_cond;
+    const float  *mis_pcell;
+    uint32_t env_w, env_h;
+    float    env_scale;       // 2^(exposure + ratio - 1)
+    uint32_t has_env;
+    uint32_t estimator;       // MRT_EST_*
 };
+
+constexpr uint32_t TAG_ENV = 0x40000000u;   // shadow-ray tag of the HDRI ray
 
 struct ShadeBufs {
     const float4 *q_o, *q_d, *q_t, *hit;
@@ -60,6 +84,8 @@ struct ShadeBufs {
     float4       *p_a;
     uint4        *p_b;
     float4       *p_c;
+    uint32_t      cap_paths;   // capacity of the path / pending arrays
+    uint32_t      cap_shadow;  // capacity of the shadow queue
 };
 
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
@@ -72,14 +98,14 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
 hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
                          const SceneDev &S, const float4 *s_o,
                          const float4 *s_d, uint8_t *vis, uint32_t *ctr,
-                         unsigned long long *stats);
+                         unsigned long long *stats, uint32_t cap);
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr);
 hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
                           uint32_t *ctr);
 hipError_t launch_advance(hipStream_t st, uint32_t *ctr,
-                          unsigned long long *stats);
+                          unsigned long long *stats, uint32_t cap);
 hipError_t launch_accumulate(hipStream_t st, const float4 *acc, float *accum,
                              uint32_t WH, uint32_t nb,
                              unsigned long long *stats);

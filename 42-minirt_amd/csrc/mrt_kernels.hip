@@ -43,6 +43,81 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, uint32_t n)
     return base + incl - n;
 }
 
+// Block-wide version for three counters at once: one atomic per counter per
+// block instead of one per wave.  Every thread of the block calls it.
+__device__ __forceinline__ void block_alloc3(uint32_t *ca, uint32_t *cb,
+                                             uint32_t *cc, uint32_t na,
+                                             uint32_t nb, uint32_t nc,
+                                             uint32_t &oa, uint32_t &ob,
+                                             uint32_t &oc)
+{
+    constexpr int NW = SHADE_BLOCK / 64;
+    __shared__ uint32_t s_w[NW][3];
+    __shared__ uint32_t s_base[3];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t ia = na, ib = nb, ic = nc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t xa = __shfl_up(ia, off, 64), xb = __shfl_up(ib, off, 64),
+                 xc = __shfl_up(ic, off, 64);
+        if (lane >= (uint32_t)off) {
+            ia += xa;
+            ib += xb;
+            ic += xc;
+        }
+    }
+    if (lane == 63) {
+        s_w[wave][0] = ia;
+        s_w[wave][1] = ib;
+        s_w[wave][2] = ic;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const uint32_t k = threadIdx.x;
+        uint32_t tot = 0;
+        for (int w = 0; w < NW; w++) {
+            uint32_t v = s_w[w][k];
+            s_w[w][k] = tot;
+            tot += v;
+        }
+        uint32_t *c = k == 0 ? ca : (k == 1 ? cb : cc);
+        s_base[k] = tot ? atomicAdd(c, tot) : 0u;
+    }
+    __syncthreads();
+    oa = s_base[0] + s_w[wave][0] + ia - na;
+    ob = s_base[1] + s_w[wave][1] + ib - nb;
+    oc = s_base[2] + s_w[wave][2] + ic - nc;
+    __syncthreads();
+}
+
+// Work distribution of the persistent trace kernels: the n items form
+// NPART contiguous parts, one per XCD (consecutive blocks go to consecutive
+// XCDs); a wave takes CHUNK items of its own part per atomic and moves on to
+// the next part once its own is used up.  Returns false when all are done.
+__device__ __forceinline__ bool next_chunk(uint32_t *cursors, uint32_t n,
+                                           uint32_t &part, uint32_t &tries,
+                                           uint32_t &begin, uint32_t &end)
+{
+    while (tries < (uint32_t)NPART) {
+        uint32_t p0 = (uint32_t)((uint64_t)n * part / NPART);
+        uint32_t p1 = (uint32_t)((uint64_t)n * (part + 1) / NPART);
+        uint32_t b = 0;
+        if (p0 < p1) {
+            if (lane_id() == 0)
+                b = atomicAdd(&cursors[part], CHUNK);
+            b = __shfl(b, 0, 64);
+            if (b < p1 - p0) {
+                begin = p0 + b;
+                end = min(begin + CHUNK, p1);
+                return true;
+            }
+        }
+        part = part + 1 == (uint32_t)NPART ? 0 : part + 1;
+        tries++;
+    }
+    return false;
+}
+
 __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v)
 {
     uint64_t s = v;
@@ -101,8 +176,10 @@ k_raygen(FrameParams fp, float4 *__restrict__ q_o, float4 *__restrict__ q_d,
         ctr[CTR_NEXT] = 0;
         ctr[CTR_SHADOW] = 0;
         ctr[CTR_PEND] = 0;
-        ctr[CTR_WORK_TRACE] = 0;
-        ctr[CTR_WORK_SHADOW] = 0;
+        for (int k = 0; k < NPART; k++) {
+            ctr[CTR_WORK_TRACE + k] = 0;
+            ctr[CTR_WORK_SHADOW + k] = 0;
+        }
     }
     if (i >= P)
         return;
@@ -128,22 +205,19 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
     uint32_t *lc = s_code + threadIdx.x;
     float *lt = s_t + threadIdx.x;
     uint32_t nodes = 0, prims = 0;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0)
-            base = atomicAdd(&ctr[CTR_WORK_TRACE], 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n)
-            break;
-        uint32_t i = base + lane_id();
-        if (i < n) {
-            float4 o4 = q_o[i], d4 = q_d[i];
-            float t, bu, bv;
-            uint32_t code;
-            closest_hit<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), lc, lt,
-                                         TRACE_BLOCK, t, code, bu, bv, nodes,
-                                         prims);
-            hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
+    uint32_t part = blockIdx.x % NPART, tries = 0, cb, ce;
+    while (next_chunk(&ctr[CTR_WORK_TRACE], n, part, tries, cb, ce)) {
+        for (uint32_t base = cb; base < ce; base += 64) {
+            uint32_t i = base + lane_id();
+            if (i < ce) {
+                float4 o4 = q_o[i], d4 = q_d[i];
+                float t, bu, bv;
+                uint32_t code;
+                closest_hit<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), lc, lt,
+                                             TRACE_BLOCK, t, code, bu, bv,
+                                             nodes, prims);
+                hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
+            }
         }
     }
     if (COUNT) {
@@ -157,27 +231,26 @@ template <bool COUNT>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_shadow(SceneDev S, const float4 *__restrict__ s_o,
          const float4 *__restrict__ s_d, uint8_t *__restrict__ vis,
-         uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats)
+         uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats,
+         uint32_t cap)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
     __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
-    const uint32_t n = ctr[CTR_SHADOW];
+    const uint32_t n = min(ctr[CTR_SHADOW], cap);
     uint32_t *lc = s_code + threadIdx.x;
     float *lt = s_t + threadIdx.x;
     uint32_t nodes = 0, prims = 0;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0)
-            base = atomicAdd(&ctr[CTR_WORK_SHADOW], 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n)
-            break;
-        uint32_t i = base + lane_id();
-        if (i < n) {
-            float4 o4 = s_o[i], d4 = s_d[i];
-            bool occ = occluded<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), o4.w, lc,
-                                                 lt, TRACE_BLOCK, nodes, prims);
-            vis[i] = occ ? 0 : 1;
+    uint32_t part = blockIdx.x % NPART, tries = 0, cb, ce;
+    while (next_chunk(&ctr[CTR_WORK_SHADOW], n, part, tries, cb, ce)) {
+        for (uint32_t base = cb; base < ce; base += 64) {
+            uint32_t i = base + lane_id();
+            if (i < ce) {
+                float4 o4 = s_o[i], d4 = s_d[i];
+                bool occ = occluded<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), o4.w,
+                                                     lc, lt, TRACE_BLOCK, nodes,
+                                                     prims);
+                vis[i] = occ ? 0 : 1;
+            }
         }
     }
     if (COUNT) {
@@ -285,7 +358,55 @@ __device__ __forceinline__ void refract(const FrameParams &fp, uint32_t pixel,
 
 enum : uint32_t { SC_NONE = 0, SC_DIFFUSE = 1 };
 
-__global__ void __launch_bounds__(256)
+// ---- environment -------------------------------------------------------------
+constexpr double PI_D = 3.14159265358979323846;
+
+// Texel of get_background_color_value / get_hdr_color (background.c:15-57):
+// v = 1/2 + asin(d.y)/pi, u = 1/2 - atan2(d.z, d.x)/(2 pi), texel
+// ((int)(u (w-1)), (int)(v (h-1))), scaled by 2^(exposure + ratio - 1)
+// (get_hdr_pixel_skybox, hit_register/get_hdr_value.c:43-70).
+__device__ __forceinline__ V3 env_texel(const FrameParams &fp, V3 dir)
+{
+    V3 d = unit(dir);
+    double v = 0.5 + asin((double)d.y) / PI_D;
+    double u = 0.5 - atan2((double)d.z, (double)d.x) / (2.0 * PI_D);
+    int x = (int)(u * (fp.env_w - 1)), y = (int)(v * (fp.env_h - 1));
+    float4 t = fp.env_tex[(size_t)y * fp.env_w + x];
+    return v3(t.x * fp.env_scale, t.y * fp.env_scale, t.z * fp.env_scale);
+}
+
+// Largest index i in [0, n-1] with cdf[i] <= u (cdf[0] = 0).
+__device__ __forceinline__ uint32_t cdf_search(const float *cdf, uint32_t n, float u)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Solid-angle density of the MIS environment sampler in direction d.
+__device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
+{
+    V3 d = unit(dir);
+    const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
+    double v = 0.5 + asin((double)d.y) / PI_D;
+    double u = 0.5 - atan2((double)d.z, (double)d.x) / (2.0 * PI_D);
+    uint32_t y = (uint32_t)(v * nh), x = (uint32_t)(u * nw);
+    y = y < nh ? y : nh - 1;
+    x = x < nw ? x : nw - 1;
+    float cose = sqrtf(fmaxf(0.0f, 1.0f - d.y * d.y));
+    float pc = fp.mis_pcell[(size_t)y * nw + x];
+    if (!(cose > 0.0f) || !(pc > 0.0f))
+        return 0.0f;
+    return pc * (float)nw * (float)nh / (19.7392088021787172f * cose);
+}
+
+__global__ void __launch_bounds__(SHADE_BLOCK)
 k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
     const uint32_t n = ctr[CTR_CUR];
@@ -297,7 +418,10 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         uint32_t kind = SC_NONE, slot = 0, pixel = 0, sample = 0;
         V3 p = v3(0.f, 0.f, 0.f), nrm = v3(0.f, 0.f, 0.f), d = v3(0.f, 0.f, 0.f);
         V3 thr = v3(0.f, 0.f, 0.f), cp = v3(0.f, 0.f, 0.f);
-        float ao = 1.0f;
+        float ao = 1.0f, pb_next = 0.0f;
+        bool env_ray = false;
+        V3 env_d = v3(0.f, 0.f, 0.f), env_rad = v3(0.f, 0.f, 0.f);
+        float env_omc = 1.0f;
         if (active) {
             float4 o4 = B.q_o[i], d4 = B.q_d[i], t4 = B.q_t[i], h4 = B.hit[i];
             slot = __float_as_uint(d4.w);
@@ -310,11 +434,24 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 // add_skybox (src/ray_tracing/path_trace.c:37-48)
                 float4 a = B.acc[slot];
                 float ao_last = o4.w;
-                V3 bg = v3(fp.bg_clamped[0] * ao_last, fp.bg_clamped[1] * ao_last,
-                           fp.bg_clamped[2] * ao_last);
-                a.x = a.x + bg.x * thr.x;
-                a.y = a.y + bg.y * thr.y;
-                a.z = a.z + bg.z * thr.z;
+                V3 bg = v3(fp.bg_clamped[0], fp.bg_clamped[1], fp.bg_clamped[2]);
+                float wmis = 1.0f;
+                if (fp.has_env) {
+                    bg = env_texel(fp, d);
+                    if (fp.estimator == MRT_EST_REFERENCE) {
+                        bg = v3(fminf(fmaxf(bg.x, 0.0f), 16.0f),
+                                fminf(fmaxf(bg.y, 0.0f), 16.0f),
+                                fminf(fmaxf(bg.z, 0.0f), 16.0f));
+                    } else if (t4.w > 0.0f) {
+                        // escape after a diffuse bounce: balance heuristic
+                        float pl = env_pdf(fp, d);
+                        wmis = t4.w / (t4.w + pl);
+                    }
+                }
+                bg = mul(bg, ao_last);
+                a.x = a.x + bg.x * thr.x * wmis;
+                a.y = a.y + bg.y * thr.y * wmis;
+                a.z = a.z + bg.z * thr.z * wmis;
                 B.acc[slot] = a;
             } else {
                 float t = h4.x;
@@ -423,23 +560,77 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                         bool soft = ltype == MRT_LIGHT_POINT && l1.w != 0.0f && lss > 1;
                         n_rays += soft ? (uint32_t)lss : 1u;
                     }
+                    if (fp.has_env) {
+                        U4 e = rng4(fp.seed, pixel, sample, fp.bounce, 3u);
+                        if (fp.estimator == MRT_EST_REFERENCE) {
+                            // importance_sampling (material_default.c:41-69)
+                            const uint32_t ew = fp.env_w, eh = fp.env_h;
+                            double vy = fp.env_marg_inv[(int)((double)e.x * (eh - 1))];
+                            double vx = fp.env_cond_inv[(int)(vy * (eh - 1) * ew
+                                                              + (double)e.y * (ew - 1))];
+                            float pdf = fp.env_pdf[(int)(vy * (eh - 1) * ew
+                                                         + vx * (ew - 1))];
+                            if (pdf > 0.0f) {
+                                float th = (float)((vy - 0.5) * PI_D);
+                                float ph = (float)((0.5 - vx) * 2.0 * PI_D);
+                                env_d = v3((float)(cos((double)th) * cos((double)ph)),
+                                           (float)sin((double)th),
+                                           (float)(cos((double)th) * sin((double)ph)));
+                                float cth = dot(nrm, env_d);
+                                env_rad = had(mul(env_texel(fp, env_d), ao), thr);
+                                env_rad = mul(env_rad, cth / pdf);
+                                env_omc = 1.0f - cth;
+                                env_ray = true;
+                            }
+                        } else {
+                            // light-sample half of the MIS pair
+                            const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
+                            uint32_t y = cdf_search(fp.mis_marg, nh, e.x);
+                            uint32_t x = cdf_search(fp.mis_cond + (size_t)y * fp.env_w,
+                                                    nw, e.y);
+                            float pc = fp.mis_pcell[(size_t)y * nw + x];
+                            float vv = ((float)y + e.z) / (float)nh;
+                            float uu = ((float)x + e.w) / (float)nw;
+                            float el = (vv - 0.5f) * 3.14159265358979324f;
+                            float ph = (0.5f - uu) * 6.28318530717958648f;
+                            float ce = cosf(el);
+                            env_d = v3(ce * cosf(ph), sinf(el), ce * sinf(ph));
+                            float cn = dot(nrm, env_d);
+                            if (pc > 0.0f && ce > 0.0f && cn > 0.0f) {
+                                float pl = pc * (float)nw * (float)nh
+                                           / (19.7392088021787172f * ce);
+                                float pbs = cn / 3.14159265358979324f;
+                                float wl = pl / (pl + pbs);
+                                env_rad = had(mul(env_texel(fp, env_d), ao), thr);
+                                env_rad = mul(env_rad, (cn / (3.14159265358979324f * pl)) * wl);
+                                env_ray = true;
+                            }
+                        }
+                        n_rays += env_ray ? 1u : 0u;
+                    }
                     U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
                     d = cos_hemi(nrm, g.z, g.w);
+                    pb_next = dot(nrm, d) / 3.14159265358979324f;
                     want_pend = n_rays ? 1u : 0u;
                 }
                 want_next = fp.last_bounce ? 0u : 1u;
             }
         }
         // allocations: every lane of the wave takes part
-        uint32_t qn = wave_alloc(&ctr[CTR_NEXT], want_next);
-        uint32_t j0 = wave_alloc(&ctr[CTR_SHADOW], n_rays);
-        uint32_t pr = wave_alloc(&ctr[CTR_PEND], want_pend);
+        uint32_t qn, j0, pr;
+        block_alloc3(&ctr[CTR_NEXT], &ctr[CTR_SHADOW], &ctr[CTR_PEND],
+                     want_next, n_rays, want_pend, qn, j0, pr);
         if (!active)
             continue;
+        // the host sizes every queue for the worst case; never write past it
+        if (qn >= B.cap_paths)
+            want_next = 0;
+        if (j0 + n_rays > B.cap_shadow || pr >= B.cap_paths)
+            n_rays = want_pend = 0;
         if (want_next) {
             B.qn_o[qn] = make_float4(p.x, p.y, p.z, ao);
             B.qn_d[qn] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
-            B.qn_t[qn] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+            B.qn_t[qn] = make_float4(thr.x, thr.y, thr.z, pb_next);
         }
         if (kind == SC_DIFFUSE && n_rays) {
             V3 view = unit(sub(v3(fp.cam_pos[0], fp.cam_pos[1], fp.cam_pos[2]), p));
@@ -486,10 +677,18 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     B.s_c[j] = make_float2(ndl, sp64);
                 }
             }
+            if (env_ray) {
+                B.s_o[j] = make_float4(p.x, p.y, p.z, 2.0f * IT_MAX_F);
+                B.s_d[j] = make_float4(env_d.x, env_d.y, env_d.z,
+                                       __uint_as_float(TAG_ENV));
+                B.s_c[j] = make_float2(0.0f, 0.0f);
+            }
         }
         if (want_pend) {
             B.p_a[pr] = make_float4(cp.x, cp.y, cp.z, __uint_as_float(slot));
-            B.p_b[pr] = make_uint4(want_next ? qn : HIT_NONE, j0, n_rays, 0u);
+            B.p_b[pr] = make_uint4(want_next ? qn : HIT_NONE, j0, n_rays,
+                                   env_ray ? 1u : 0u);
+            B.p_c[pr] = make_float4(env_rad.x, env_rad.y, env_rad.z, env_omc);
         }
     }
 }
@@ -505,7 +704,7 @@ __device__ __forceinline__ float lq(bool exact, float acc, float add)
 __global__ void __launch_bounds__(256)
 k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
-    const uint32_t n = ctr[CTR_PEND];
+    const uint32_t n = min(ctr[CTR_PEND], B.cap_paths);
     const bool exact = (fp.flags & MRT_SCENE_LIGHT_EXACT) != 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
@@ -514,6 +713,8 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         uint32_t slot = __float_as_uint(a.w);
         uint32_t j = b.y, end = b.y + b.z;
         float lr = 0.0f, lg = 0.0f, lb = 0.0f;
+        if (b.w)
+            end--;                      // the environment ray comes last
         while (j < end) {
             uint32_t tag = __float_as_uint(B.s_d[j].w);
             uint32_t k = tag >> 12;
@@ -557,22 +758,37 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         s.x = s.x + a.x * (lr / 255.0f);
         s.y = s.y + a.y * (lg / 255.0f);
         s.z = s.z + a.z * (lb / 255.0f);
+        if (b.w && B.vis[end]) {
+            float4 c = B.p_c[r];
+            s.x = s.x + c.x;
+            s.y = s.y + c.y;
+            s.z = s.z + c.z;
+            if (fp.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
+                float4 t = B.qn_t[b.x];
+                t.x = t.x * c.w;
+                t.y = t.y * c.w;
+                t.z = t.z * c.w;
+                B.qn_t[b.x] = t;
+            }
+        }
         B.acc[slot] = s;
     }
 }
 
 __global__ void k_advance(uint32_t *__restrict__ ctr,
-                          unsigned long long *__restrict__ stats)
+                          unsigned long long *__restrict__ stats, uint32_t cap)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         stats[STAT_RAYS_CLOSEST] += ctr[CTR_CUR];
         stats[STAT_RAYS_SHADOW] += ctr[CTR_SHADOW];
-        ctr[CTR_CUR] = ctr[CTR_NEXT];
+        ctr[CTR_CUR] = min(ctr[CTR_NEXT], cap);
         ctr[CTR_NEXT] = 0;
         ctr[CTR_SHADOW] = 0;
         ctr[CTR_PEND] = 0;
-        ctr[CTR_WORK_TRACE] = 0;
-        ctr[CTR_WORK_SHADOW] = 0;
+        for (int k = 0; k < NPART; k++) {
+            ctr[CTR_WORK_TRACE + k] = 0;
+            ctr[CTR_WORK_SHADOW + k] = 0;
+        }
     }
 }
 
@@ -725,21 +941,22 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
 hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
                          const SceneDev &S, const float4 *s_o,
                          const float4 *s_d, uint8_t *vis, uint32_t *ctr,
-                         unsigned long long *stats)
+                         unsigned long long *stats, uint32_t cap)
 {
     if (count)
         hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(TRACE_BLOCK), 0,
-                           st, S, s_o, s_d, vis, ctr, stats);
+                           st, S, s_o, s_d, vis, ctr, stats, cap);
     else
         hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(TRACE_BLOCK), 0,
-                           st, S, s_o, s_d, vis, ctr, stats);
+                           st, S, s_o, s_d, vis, ctr, stats, cap);
     return hipGetLastError();
 }
 
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, fp, B, ctr);
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp, B,
+                       ctr);
     return hipGetLastError();
 }
 
@@ -752,9 +969,9 @@ hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
 }
 
 hipError_t launch_advance(hipStream_t st, uint32_t *ctr,
-                          unsigned long long *stats)
+                          unsigned long long *stats, uint32_t cap)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctr, stats);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctr, stats, cap);
     return hipGetLastError();
 }
 

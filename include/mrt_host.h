@@ -135,6 +135,20 @@ int  mrt_hdri_build_tables(const float *rgb, uint32_t w, uint32_t h,
                            float exposure, mrt_hdri_tables *out);
 void mrt_hdri_free_tables(mrt_hdri_tables *t);
 
+/* Tables of the MIS estimator: a piecewise-constant density over the cells
+ * that read each texel, weighted by luminance x solid angle. */
+typedef struct mrt_hdri_mis {
+    uint32_t w, h;
+    float   *marg;               /* h:         row CDF, marg[0] = 0, [h-1] = 1 */
+    float   *cond;               /* (h-1) * w: per-row column CDFs          */
+    float   *pcell;              /* (h-1) * (w-1): cell probabilities       */
+    double   total;              /* sum of the cell weights                 */
+} mrt_hdri_mis;
+
+int  mrt_hdri_build_mis(const float *rgb, uint32_t w, uint32_t h,
+                        float exposure, mrt_hdri_mis *out);
+void mrt_hdri_free_mis(mrt_hdri_mis *m);
+
 /* ---- .rt reader (src/parsing/parse_scene.c:99-125) ---------------------- */
 
 /* Reads a .rt scene: A C L l dl sp pl mat W (cu / cy / co / hb / tex are

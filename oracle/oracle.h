@@ -47,6 +47,9 @@ typedef struct orc_hdri {
     const float *pdf_joint;        /* w*h  (reference tables)                */
     const float *marg_inv;         /* h                                      */
     const float *cond_inv;         /* w*h                                    */
+    const float *mis_marg;         /* h        (orc_hdri_mis_tables)         */
+    const float *mis_cond;         /* (h-1)*w                                */
+    const float *mis_pcell;        /* (h-1)*(w-1)                            */
 } orc_hdri;
 
 typedef struct orc_params {
@@ -118,6 +121,11 @@ int  orc_render_f64(const mrt_scene *s, const orc_bvh *b, const orc_hdri *hd,
 int  orc_hdri_tables(const float *rgb, uint32_t w, uint32_t h, float exposure,
                      float *pdf_joint, float *marg_inv, float *cond_inv,
                      double *column_average);
+
+/* Tables of the MIS environment estimator (not in the reference). */
+int  orc_hdri_mis_tables(const float *rgb, uint32_t w, uint32_t h,
+                         float exposure, float *marg, float *cond,
+                         float *pcell, double *total);
 
 /* The samplers of ggx_sample_hemisphere (src/material/utils.c:39-63),
  * cos_weighted_sample_hemishphere (material_default.c:20-39) and

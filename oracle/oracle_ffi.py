@@ -26,7 +26,9 @@ class OrcBvh(C.Structure):
 class OrcHdri(C.Structure):
     _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgb", C.c_void_p),
                 ("exposure", C.c_float), ("pdf_joint", C.c_void_p),
-                ("marg_inv", C.c_void_p), ("cond_inv", C.c_void_p)]
+                ("marg_inv", C.c_void_p), ("cond_inv", C.c_void_p),
+                ("mis_marg", C.c_void_p), ("mis_cond", C.c_void_p),
+                ("mis_pcell", C.c_void_p)]
 
 
 class OrcParams(C.Structure):
@@ -209,3 +211,33 @@ def unit_sphere(u4):
     o = (C.c_double * 3)()
     lib().orc_unit_sphere_f64(_fp(u), o)
     return np.array(o[:])
+
+
+def hdri_mis_tables(rgb, exposure=0.0):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    marg = np.zeros(h, np.float32)
+    cond = np.zeros((h - 1, w), np.float32)
+    pcell = np.zeros((h - 1, w - 1), np.float32)
+    tot = C.c_double()
+    L = lib()
+    L.orc_hdri_mis_tables.argtypes = [C.POINTER(C.c_float), C.c_uint32,
+                                      C.c_uint32, C.c_float] + \
+        [C.POINTER(C.c_float)] * 3 + [C.POINTER(C.c_double)]
+    if L.orc_hdri_mis_tables(_fp(rgb), w, h, exposure, _fp(marg), _fp(cond),
+                             _fp(pcell), C.byref(tot)) != 0:
+        raise RuntimeError("oracle MIS tables failed")
+    return marg, cond, pcell, tot.value
+
+
+def make_hdri(rgb, exposure=0.0):
+    """OrcHdri with the oracle's own reference and MIS tables."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    pdf, marg, cond, _ = hdri_tables(rgb, exposure)
+    mm, mc, mp, _ = hdri_mis_tables(rgb, exposure)
+    hd = OrcHdri(w, h, rgb.ctypes.data, exposure, pdf.ctypes.data,
+                 marg.ctypes.data, cond.ctypes.data, mm.ctypes.data,
+                 mc.ctypes.data, mp.ctypes.data)
+    hd._keep = (rgb, pdf, marg, cond, mm, mc, mp)
+    return hd

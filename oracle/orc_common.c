@@ -261,3 +261,60 @@ int orc_hdri_tables(const float *rgb, uint32_t w, uint32_t h, float exposure,
     free(cdf);
     return 0;
 }
+
+/* The MIS environment density (a new estimator, not in the reference):
+ * cell (y, x) holds the directions whose texel lookup (background.c:34-57)
+ * reads texel (y, x); its weight is luminance x (sin e_hi - sin e_lo), the
+ * exact integral of cos(elevation) over the cell's elevation band. */
+int orc_hdri_mis_tables(const float *rgb, uint32_t w, uint32_t h,
+                        float exposure, float *marg, float *cond,
+                        float *pcell, double *total_out)
+{
+    const uint32_t nw = w - 1, nh = h - 1;
+    double *wg = malloc((size_t)nw * nh * sizeof(double));
+    double *rs = malloc(nh * sizeof(double));
+    if (!wg || !rs) {
+        free(wg);
+        free(rs);
+        return -1;
+    }
+    double scale = exposure == 0.0f ? 1.0 : (double)powf(2.0f, exposure);
+    double total = 0.0;
+    for (uint32_t y = 0; y < nh; y++) {
+        double band = sin(((double)(y + 1) / nh - 0.5) * PI_D)
+                      - sin(((double)y / nh - 0.5) * PI_D);
+        double s = 0.0;
+        for (uint32_t x = 0; x < nw; x++) {
+            const float *t = rgb + ((size_t)y * w + x) * 3;
+            double lum = 0.2126 * ((double)t[0] * scale)
+                         + 0.7152 * ((double)t[1] * scale)
+                         + 0.0722 * ((double)t[2] * scale);
+            wg[(size_t)y * nw + x] = lum > 0.0 ? lum * band : 0.0;
+            s += wg[(size_t)y * nw + x];
+        }
+        rs[y] = s;
+        total += s;
+    }
+    double a = 0.0;
+    marg[0] = 0.0f;
+    for (uint32_t y = 0; y < nh; y++) {
+        a += rs[y];
+        marg[y + 1] = total > 0.0 ? (float)(a / total) : (float)(y + 1) / nh;
+        double ra = 0.0;
+        cond[(size_t)y * w] = 0.0f;
+        for (uint32_t x = 0; x < nw; x++) {
+            ra += wg[(size_t)y * nw + x];
+            cond[(size_t)y * w + x + 1] =
+                rs[y] > 0.0 ? (float)(ra / rs[y]) : (float)(x + 1) / nw;
+            pcell[(size_t)y * nw + x] =
+                total > 0.0 ? (float)(wg[(size_t)y * nw + x] / total) : 0.0f;
+        }
+        cond[(size_t)y * w + nw] = 1.0f;
+    }
+    marg[nh] = 1.0f;
+    if (total_out)
+        *total_out = total;
+    free(wg);
+    free(rs);
+    return 0;
+}
