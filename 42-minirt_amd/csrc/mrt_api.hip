@@ -9,6 +9,8 @@
 
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "mrt_kernels.h"
 
 using namespace mrt;
@@ -43,6 +45,14 @@ struct mrt_ctx {
     float        env_exposure = 0.0f;
     bool         has_env = false;
 
+    // photon map
+    float4      *d_ph_pos = nullptr, *d_ph_pow = nullptr;
+    uint32_t    *d_cstart = nullptr, *d_cend = nullptr;
+    uint32_t     ph_n = 0, ph_table = 0, ph_k = 15;
+    float        ph_cell = 0.0f, ph_radius = 0.0f, ph_norm = 1.0f;
+    bool         has_photons = false;
+    float        scene_c[3] = {0, 0, 0}, scene_r = 0.0f;
+
     // accumulation
     float       *d_accum = nullptr;
     uint32_t    *d_ctr = nullptr;
@@ -56,7 +66,7 @@ struct mrt_ctx {
     float4      *s_o = nullptr, *s_d = nullptr;
     float2      *s_c = nullptr;
     uint8_t     *vis = nullptr;
-    float4      *p_a = nullptr, *p_c = nullptr;
+    float4      *p_a = nullptr, *p_c = nullptr, *p_x = nullptr, *p_k = nullptr;
     uint4       *p_b = nullptr;
 
     // options
@@ -197,6 +207,8 @@ static void free_wavefront(mrt_ctx *c)
     dfree(c->p_a);
     dfree(c->p_b);
     dfree(c->p_c);
+    dfree(c->p_x);
+    dfree(c->p_k);
     c->cap_paths = c->cap_shadow = 0;
 }
 
@@ -213,6 +225,16 @@ static void free_scene(mrt_ctx *c)
 
 static void free_env(mrt_ctx *c);
 
+static void free_photons(mrt_ctx *c)
+{
+    dfree(c->d_ph_pos);
+    dfree(c->d_ph_pow);
+    dfree(c->d_cstart);
+    dfree(c->d_cend);
+    c->ph_n = 0;
+    c->has_photons = false;
+}
+
 extern "C" void mrt_destroy(mrt_ctx *c)
 {
     if (!c)
@@ -223,6 +245,7 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     free_wavefront(c);
     free_scene(c);
     free_env(c);
+    free_photons(c);
     dfree(c->d_accum);
     dfree(c->d_ctr);
     dfree(c->d_stats);
@@ -335,6 +358,27 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     c->ior_all = s->ior_all > 0.0f ? s->ior_all : 1.0f;
     c->flags = s->flags;
     c->has_scene = true;
+    /* bounding sphere of the BVH primitives (emission disk of environment
+     * photons): box of triangle vertices and sphere extents */
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < s->n_triangles * 3; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], s->tri_pos[3 * (size_t)i + k]);
+            hi[k] = fmaxf(hi[k], s->tri_pos[3 * (size_t)i + k]);
+        }
+    for (uint32_t i = 0; i < s->n_spheres; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], s->sph[4 * i + k] - s->sph[4 * i + 3]);
+            hi[k] = fmaxf(hi[k], s->sph[4 * i + k] + s->sph[4 * i + 3]);
+        }
+    float d2 = 0.0f;
+    for (int k = 0; k < 3; k++) {
+        c->scene_c[k] = 0.5f * (lo[k] + hi[k]);
+        d2 = d2 + (hi[k] - lo[k]) * (hi[k] - lo[k]);
+    }
+    c->scene_r = 0.5f * sqrtf(d2);
+    free_photons(c);
     return MRT_OK;
 }
 
@@ -423,9 +467,218 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     return MRT_OK;
 }
 
-extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t, float)
+static FrameParams frame_params(const mrt_ctx *c);
+static SceneDev scene_dev(const mrt_ctx *c);
+
+static ShadeBufs scene_bufs(const mrt_ctx *c)
 {
-    return fail(c, MRT_E_UNSUPPORTED, "photon map not built yet");
+    ShadeBufs B;
+    memset(&B, 0, sizeof(B));
+    B.planes = c->d_planes;
+    B.prims = c->d_prims;
+    B.mats = c->d_mats;
+    B.lights = c->d_lights;
+    B.nrm = c->d_nrm;
+    return B;
+}
+
+extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t n_per, float radius)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "mrt_build_photons before mrt_upload_scene");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_photons(c);
+    if (n_per == 0)
+        return MRT_OK;
+    if (!(radius > 0.0f) || !(c->ph_norm > 0.0f))
+        return fail(c, MRT_E_INVALID, "caustic radius and norm must be positive");
+    PhotonEmit pe;
+    memset(&pe, 0, sizeof(pe));
+    for (uint32_t k = 0; k < c->lights.size(); k++)
+        if (c->lights[k].type == MRT_LIGHT_POINT) {
+            if (pe.n_point == 64)
+                return fail(c, MRT_E_LIMIT, "caustics support at most 64 point lights");
+            pe.point_idx[pe.n_point++] = k;
+        }
+    pe.has_env = c->has_env ? 1u : 0u;
+    uint64_t total = (uint64_t)(pe.n_point + pe.has_env) * n_per;
+    if (total == 0)
+        return fail(c, MRT_E_INVALID, "caustics need a point light or an HDRI");
+    if (total > (1ull << 31))
+        return fail(c, MRT_E_LIMIT, "more than 2^31 photons");
+    pe.n_per = n_per;
+    pe.n_total = (uint32_t)total;
+    memcpy(pe.sc, c->scene_c, sizeof(pe.sc));
+    pe.sr = c->scene_r;
+    const uint32_t cap = pe.n_total;
+    float4 *raw_pos = nullptr, *raw_pow = nullptr;
+    uint32_t *cnt = nullptr;
+    unsigned long long *k_in = nullptr, *k_out = nullptr;
+    uint32_t *v_in = nullptr, *v_out = nullptr;
+    void *tmp = nullptr;
+    int rc = MRT_OK;
+    uint32_t n = 0;
+    hipError_t e = hipSuccess;
+    do {
+        if (dalloc(raw_pos, cap) != hipSuccess || dalloc(raw_pow, cap) != hipSuccess
+            || dalloc(cnt, 1) != hipSuccess) {
+            rc = fail(c, MRT_E_NOMEM, "photon buffers (%u) do not fit", cap);
+            break;
+        }
+        FrameParams fp = frame_params(c);
+        if ((e = launch_fill_u32(c->stream, cnt, 0u, 1)) != hipSuccess
+            || (e = launch_photon_emit(c->stream, fp, scene_dev(c), pe,
+                                       scene_bufs(c), raw_pos, raw_pow, cnt,
+                                       cap)) != hipSuccess
+            || (e = hipMemcpyAsync(&n, cnt, 4, hipMemcpyDeviceToHost, c->stream))
+                   != hipSuccess
+            || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+            break;
+        if (n > cap)
+            n = cap;
+        if (n == 0)
+            break;
+        const uint32_t table = 2 * n;        /* cells_am = 2 photons */
+        const float cell = c->ph_k ? sqrtf(radius) : radius;
+        if (dalloc(k_in, n) != hipSuccess || dalloc(k_out, n) != hipSuccess
+            || dalloc(v_in, n) != hipSuccess || dalloc(v_out, n) != hipSuccess
+            || dalloc(c->d_ph_pos, n) != hipSuccess
+            || dalloc(c->d_ph_pow, n) != hipSuccess
+            || dalloc(c->d_cstart, table) != hipSuccess
+            || dalloc(c->d_cend, table) != hipSuccess) {
+            rc = fail(c, MRT_E_NOMEM, "photon map (%u) does not fit", n);
+            break;
+        }
+        unsigned end_bit = 32;
+        while (end_bit < 64 && (1ull << (end_bit - 32)) < table)
+            end_bit++;
+        size_t tmp_bytes = 0;
+        if ((e = launch_photon_keys(c->stream, raw_pos, n, cell, table, k_in,
+                                    v_in)) != hipSuccess
+            || (e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, k_in, k_out,
+                                              v_in, v_out, n, 0, end_bit,
+                                              c->stream)) != hipSuccess
+            || (e = hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1)) != hipSuccess
+            || (e = rocprim::radix_sort_pairs(tmp, tmp_bytes, k_in, k_out, v_in,
+                                              v_out, n, 0, end_bit, c->stream))
+                   != hipSuccess
+            || (e = launch_fill_u32(c->stream, c->d_cstart, 0u, table)) != hipSuccess
+            || (e = launch_fill_u32(c->stream, c->d_cend, 0u, table)) != hipSuccess
+            || (e = launch_photon_reorder(c->stream, raw_pos, raw_pow, k_out,
+                                          v_out, n, c->d_ph_pos, c->d_ph_pow,
+                                          c->d_cstart, c->d_cend)) != hipSuccess
+            || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+            break;
+        c->ph_n = n;
+        c->ph_table = table;
+        c->ph_cell = cell;
+        c->ph_radius = radius;
+        c->has_photons = true;
+    } while (0);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(raw_pos);
+    dfree(raw_pow);
+    dfree(cnt);
+    dfree(k_in);
+    dfree(k_out);
+    dfree(v_in);
+    dfree(v_out);
+    if (tmp)
+        (void)hipFree(tmp);
+    if (e != hipSuccess && rc == MRT_OK)
+        rc = fail(c, MRT_E_HIP, "photon map build failed: %s", hipGetErrorString(e));
+    if (rc != MRT_OK || !c->has_photons) {
+        free_photons(c);
+        return rc;
+    }
+    return MRT_OK;
+}
+
+extern "C" int mrt_get_photons(mrt_ctx *c, uint32_t max, float *pos,
+                               float *power, uint32_t *ids, uint32_t *n_out)
+{
+    if (!c || !n_out)
+        return MRT_E_INVALID;
+    *n_out = c->ph_n;
+    uint32_t n = c->ph_n < max ? c->ph_n : max;
+    if (n == 0)
+        return MRT_OK;
+    std::vector<float4> p(n), w(n);
+    HIPCHK(c, hipMemcpyAsync(p.data(), c->d_ph_pos, (size_t)n * 16,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w.data(), c->d_ph_pow, (size_t)n * 16,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n; i++) {
+        if (pos) {
+            pos[3 * i] = p[i].x;
+            pos[3 * i + 1] = p[i].y;
+            pos[3 * i + 2] = p[i].z;
+        }
+        if (power) {
+            power[3 * i] = w[i].x;
+            power[3 * i + 1] = w[i].y;
+            power[3 * i + 2] = w[i].z;
+        }
+        if (ids) {
+            uint32_t g;
+            memcpy(&g, &p[i].w, 4);
+            ids[i] = g;
+        }
+    }
+    return MRT_OK;
+}
+
+static PhotonMapDev photon_map_dev(const mrt_ctx *c)
+{
+    PhotonMapDev pm;
+    memset(&pm, 0, sizeof(pm));
+    pm.pos = c->d_ph_pos;
+    pm.pow = c->d_ph_pow;
+    pm.cstart = c->d_cstart;
+    pm.cend = c->d_cend;
+    pm.n = c->ph_n;
+    pm.table = c->ph_table;
+    pm.cell = c->ph_cell;
+    pm.k = c->ph_k;
+    pm.radius = c->ph_radius;
+    pm.norm = c->ph_norm;
+    return pm;
+}
+
+extern "C" int mrt_caustic_at(mrt_ctx *c, uint32_t n, const float *pts,
+                              float *rgb)
+{
+    if (!c || (n && (!pts || !rgb)))
+        return MRT_E_INVALID;
+    if (n == 0)
+        return MRT_OK;
+    if (!c->has_photons) {
+        memset(rgb, 0, (size_t)n * 12);
+        return MRT_OK;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    float *d_p = nullptr, *d_o = nullptr;
+    hipError_t e = hipMalloc((void **)&d_p, (size_t)n * 12);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&d_o, (size_t)n * 12);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_p, pts, (size_t)n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = launch_caustic_at(c->stream, photon_map_dev(c), n, d_p, d_o);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rgb, d_o, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_p);
+    (void)hipFree(d_o);
+    if (e != hipSuccess)
+        return fail(c, MRT_E_HIP, "caustic query failed: %s", hipGetErrorString(e));
+    return MRT_OK;
 }
 
 // init_viewport_values / init_viewport_vectors (src/ray_tracing/viewport.c:
@@ -495,6 +748,10 @@ static FrameParams frame_params(const mrt_ctx *c)
         float ex = c->env_exposure + c->ambient_ratio - 1.0f;
         fp.env_scale = ex == 0.0f ? 1.0f : powf(2.0f, ex);
     }
+    if (c->has_photons) {
+        fp.has_photons = 1;
+        fp.pm = photon_map_dev(c);
+    }
     return fp;
 }
 
@@ -539,6 +796,8 @@ static int ensure_wavefront(mrt_ctx *c, uint32_t paths, uint32_t shadow)
          && dalloc(c->p_a, paths) == hipSuccess
          && dalloc(c->p_b, paths) == hipSuccess
          && dalloc(c->p_c, paths) == hipSuccess
+         && dalloc(c->p_x, paths) == hipSuccess
+         && dalloc(c->p_k, paths) == hipSuccess
          && dalloc(c->s_o, shadow) == hipSuccess
          && dalloc(c->s_d, shadow) == hipSuccess
          && dalloc(c->s_c, shadow) == hipSuccess
@@ -677,6 +936,8 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
             B.p_a = c->p_a;
             B.p_b = c->p_b;
             B.p_c = c->p_c;
+            B.p_x = c->p_x;
+            B.p_k = c->p_k;
             B.cap_paths = c->cap_paths;
             B.cap_shadow = c->cap_shadow;
             uint32_t gb = (P + SHADE_BLOCK - 1) / SHADE_BLOCK;
@@ -687,7 +948,10 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
                                 c->q_o[cur], c->q_d[cur], c->hit, c->d_ctr,
                                 c->d_stats));
             LAUNCH(c, tm, K_SHADE, launch_shade(c->stream, gb, fp, B, c->d_ctr));
-            if (!c->lights.empty() || c->has_env) {
+            if (c->has_photons)
+                LAUNCH(c, tm, K_SHADE,
+                       launch_caustic(c->stream, c->grid_blocks, fp, B, c->d_ctr));
+            if (!c->lights.empty() || c->has_env || c->has_photons) {
                 LAUNCH(c, tm, K_SHADOW,
                        launch_shadow(c->stream, c->shadow_blocks, count, S,
                                      c->s_o, c->s_d, c->vis, c->d_ctr,
@@ -852,6 +1116,16 @@ extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
         return MRT_OK;
     case MRT_OPT_BLOCK:
         return fail(c, MRT_E_UNSUPPORTED, "block size is fixed in this build");
+    case MRT_OPT_PHOTON_K:
+        if (!(value >= 0.0) || value > 32.0)
+            return fail(c, MRT_E_LIMIT, "photon k must be in 0..32");
+        c->ph_k = (uint32_t)value;
+        return MRT_OK;
+    case MRT_OPT_PHOTON_NORM:
+        if (!(value > 0.0))
+            return fail(c, MRT_E_INVALID, "photon norm must be positive");
+        c->ph_norm = (float)value;
+        return MRT_OK;
     default:
         return fail(c, MRT_E_INVALID, "unknown option %d", option);
     }

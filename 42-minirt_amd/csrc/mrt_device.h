@@ -14,6 +14,12 @@ constexpr float IT_MIN_F = 0.001f;    // IT_MIN, includes/minirt.h:44
 constexpr float IT_MAX_F = 1000.0f;   // IT_MAX, includes/minirt.h:43
 constexpr float BOX_WIDEN = 1.0000004f;  // 1 + 2 gamma_3 (conservative slab)
 constexpr int STACK_MAX = 64;         // >= MRT_BVH_MAX_DEPTH + 1
+#ifndef MRT_POPCULL
+#define MRT_POPCULL 0
+#endif
+// keep each stacked child's entry distance and drop it at pop when it lies
+// beyond the closest hit (8-byte entries) or keep codes only (4 bytes)
+constexpr bool POPCULL = MRT_POPCULL != 0;
 constexpr uint32_t HIT_NONE = 0xFFFFFFFFu;
 constexpr uint32_t HIT_PLANE = 0x80000000u;   // hit code of plane k: flag | k
 
@@ -53,6 +59,17 @@ __device__ __forceinline__ U4 rng4(uint64_t seed, uint32_t pixel,
     return philox_u4(pixel, sample, (bounce << 20) | block, 0u,
                      (uint32_t)seed, (uint32_t)(seed >> 32));
 }
+
+// The counter of one path vertex: stream 0 camera paths, 1 photons.
+struct RngKey {
+    uint64_t seed;
+    uint32_t pixel, sample, bounce, stream;
+    __device__ __forceinline__ U4 operator()(uint32_t block) const
+    {
+        return philox_u4(pixel, sample, (bounce << 20) | block, stream,
+                         (uint32_t)seed, (uint32_t)(seed >> 32));
+    }
+};
 
 // ---- small vector helpers (explicit evaluation order) ---------------------
 struct V3 {
@@ -216,10 +233,12 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                 float fart = near1 ? t0 : t1;
                 if (sp < SS) {
                     lds_code[sp * stride] = (uint32_t)farc;
-                    lds_t[sp * stride] = fart;
+                    if (POPCULL)
+                        lds_t[sp * stride] = fart;
                 } else if (sp < STACK_MAX) {
                     ovf_code[sp - SS] = (uint32_t)farc;
-                    ovf_t[sp - SS] = fart;
+                    if (POPCULL)
+                        ovf_t[sp - SS] = fart;
                 }
                 sp = sp < STACK_MAX ? sp + 1 : sp;
                 code = near1 ? c1 : c0;
@@ -268,16 +287,18 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                 return hit;
             }
             sp--;
-            float et;
+            float et = 0.0f;
             uint32_t ec;
             if (sp < SS) {
                 ec = lds_code[sp * stride];
-                et = lds_t[sp * stride];
+                if (POPCULL)
+                    et = lds_t[sp * stride];
             } else {
                 ec = ovf_code[sp - SS];
-                et = ovf_t[sp - SS];
+                if (POPCULL)
+                    et = ovf_t[sp - SS];
             }
-            if (et <= tclose * BOX_WIDEN) {
+            if (!POPCULL || et <= tclose * BOX_WIDEN) {
                 code = (int32_t)ec;
                 break;
             }

@@ -10,7 +10,7 @@ namespace mrt {
 #define MRT_TRACE_BLOCK 128
 #endif
 #ifndef MRT_TRACE_SS
-#define MRT_TRACE_SS 8
+#define MRT_TRACE_SS 24
 #endif
 constexpr int TRACE_BLOCK = MRT_TRACE_BLOCK;  // threads per trace block
 constexpr int TRACE_SS = MRT_TRACE_SS;        // stack entries kept in LDS
@@ -26,8 +26,14 @@ enum : int {
     CTR_COUNT = 32
 };
 
-constexpr int NPART = 8;           // work parts: one per XCD
-constexpr uint32_t CHUNK = 256;    // items a wave takes per atomic
+#ifndef MRT_NPART
+#define MRT_NPART 1
+#endif
+// Work parts of the persistent trace kernels.  One global cursor keeps the
+// whole GPU on a narrow, coherent window of the queue (neighbouring pixels
+// share BVH nodes in every L2); per-XCD parts measured slower.
+constexpr int NPART = MRT_NPART;
+constexpr uint32_t CHUNK = 64;     // items a wave takes per atomic
 constexpr int SHADE_BLOCK = 512;   // threads per block of k_shade
 
 // device statistics (uint64)
@@ -40,6 +46,27 @@ enum : int {
     STAT_PRIMS_SHADOW = 5,
     STAT_NONFINITE = 6,
     STAT_COUNT = 16
+};
+
+// Photon map in HBM, sorted by hash bucket (hash_cell of
+// src/gpu/convert_caustic_utils.c:15-24 modulo `table`), photon id second.
+struct PhotonMapDev {
+    const float4   *pos;      // (position, photon id)
+    const float4   *pow;      // (power, 0)
+    const uint32_t *cstart;   // first photon of each bucket
+    const uint32_t *cend;     // one past the last
+    uint32_t n, table;
+    float    cell;            // grid cell edge >= the search distance
+    uint32_t k;               // k nearest (0: every photon within radius)
+    float    radius, norm;    // --caustic=N;RADIUS;NORM
+};
+
+// Photon emission: n_per photons per point light, then n_per from the
+// environment through the disk of the scene's bounding sphere (sc, sr).
+struct PhotonEmit {
+    uint32_t n_per, n_point, has_env, n_total;
+    float    sc[3], sr;
+    uint32_t point_idx[64];
 };
 
 struct FrameParams {
@@ -68,6 +95,8 @@ struct FrameParams {
     float    env_scale;       // 2^(exposure + ratio - 1)
     uint32_t has_env;
     uint32_t estimator;       // MRT_EST_*
+    PhotonMapDev pm;          // caustic photon map, present when has_photons
+    uint32_t has_photons;
 };
 
 constexpr uint32_t TAG_ENV = 0x40000000u;   // shadow-ray tag of the HDRI ray
@@ -84,6 +113,8 @@ struct ShadeBufs {
     float4       *p_a;
     uint4        *p_b;
     float4       *p_c;
+    float4       *p_x;         // hit point of each pending record (photons)
+    float4       *p_k;         // caustic term of each pending record
     uint32_t      cap_paths;   // capacity of the path / pending arrays
     uint32_t      cap_shadow;  // capacity of the shadow queue
 };
@@ -117,5 +148,25 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                              const float *orig, const float *dir,
                              uint32_t *prim_id, float *t);
 int trace_blocks_per_cu(bool shadow);
+
+hipError_t launch_photon_emit(hipStream_t st, const FrameParams &fp,
+                              const SceneDev &S, const PhotonEmit &pe,
+                              const ShadeBufs &B, float4 *pos, float4 *pow,
+                              uint32_t *counter, uint32_t cap);
+hipError_t launch_photon_keys(hipStream_t st, const float4 *pos, uint32_t n,
+                              float cell, uint32_t table,
+                              unsigned long long *keys, uint32_t *vals);
+hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
+                                 const float4 *pow,
+                                 const unsigned long long *keys,
+                                 const uint32_t *vals, uint32_t n,
+                                 float4 *pos_out, float4 *pow_out,
+                                 uint32_t *cstart, uint32_t *cend);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
+hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
+                          const FrameParams &fp, const ShadeBufs &B,
+                          uint32_t *ctr);
+hipError_t launch_caustic_at(hipStream_t st, const PhotonMapDev &pm,
+                             uint32_t n, const float *pts, float *out);
 
 }  // namespace mrt

@@ -91,9 +91,10 @@ __device__ __forceinline__ void block_alloc3(uint32_t *ca, uint32_t *cb,
 }
 
 // Work distribution of the persistent trace kernels: the n items form
-// NPART contiguous parts, one per XCD (consecutive blocks go to consecutive
-// XCDs); a wave takes CHUNK items of its own part per atomic and moves on to
-// the next part once its own is used up.  Returns false when all are done.
+// NPART contiguous parts (NPART = 1: a single global cursor; NPART = 8: one
+// part per XCD, as consecutive blocks go to consecutive XCDs); a wave takes
+// CHUNK items of its own part per atomic and moves on to the next part once
+// its own is used up.  Returns false when all are done.
 __device__ __forceinline__ bool next_chunk(uint32_t *cursors, uint32_t n,
                                            uint32_t &part, uint32_t &tries,
                                            uint32_t &begin, uint32_t &end)
@@ -200,7 +201,7 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
         uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
     const uint32_t n = ctr[CTR_CUR];
     uint32_t *lc = s_code + threadIdx.x;
     float *lt = s_t + threadIdx.x;
@@ -235,7 +236,7 @@ k_shadow(SceneDev S, const float4 *__restrict__ s_o,
          uint32_t cap)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
     const uint32_t n = min(ctr[CTR_SHADOW], cap);
     uint32_t *lc = s_code + threadIdx.x;
     float *lt = s_t + threadIdx.x;
@@ -310,14 +311,13 @@ __device__ __forceinline__ V3 unit_sphere_pt(U4 u)
 }
 
 // metallic_color (src/material/metallic.c:25-48)
-__device__ __forceinline__ void metal(const FrameParams &fp, uint32_t pixel,
-                                      uint32_t sample, V3 n, const Mat &m,
+__device__ __forceinline__ void metal(const RngKey &rk, V3 n, const Mat &m,
                                       V3 &d, V3 &thr)
 {
     float cth = -dot(unit(d), n);
     V3 mn = n;
     if (m.b.x > 0.0f) {
-        U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+        U4 g = rk(2u);
         mn = ggx_sample(n, m.b.x, g.x, g.y);
     }
     d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
@@ -327,19 +327,19 @@ __device__ __forceinline__ void metal(const FrameParams &fp, uint32_t pixel,
 }
 
 // refracted_ray / refracted_value / refracted_v (src/material/refraction.c:16-76)
-__device__ __forceinline__ void refract(const FrameParams &fp, uint32_t pixel,
-                                        uint32_t sample, V3 n, bool front,
-                                        const Mat &m, V3 &d, V3 &thr)
+__device__ __forceinline__ void refract(const RngKey &rk, float ior_all, V3 n,
+                                        bool front, const Mat &m, V3 &d,
+                                        V3 &thr)
 {
     float ior = m.b.y;
-    float eta = front ? fp.ior_all / ior : ior / fp.ior_all;
+    float eta = front ? ior_all / ior : ior / ior_all;
     d = unit(d);
     float cth = -dot(d, n);
     if (cth > 1.0f)
         cth = 1.0f;
     float sth = sqrtf(1.0f - cth * cth);
     if (eta * sth > 1.0f) {
-        metal(fp, pixel, sample, n, m, d, thr);
+        metal(rk, n, m, d, thr);
         return;
     }
     float c2 = dot(neg(d), n);
@@ -349,11 +349,105 @@ __device__ __forceinline__ void refract(const FrameParams &fp, uint32_t pixel,
     V3 par = mul(n, -sqrtf(fabsf(1.0f - dot(perp, perp))));
     V3 out = unit(add(par, perp));
     if (m.b.x > 0.0f) {
-        U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+        U4 g = rk(2u);
         out = ggx_sample(out, m.b.x, g.x, g.y);
     }
     d = out;
     thr = had(thr, v3(m.a.x, m.a.y, m.a.z));
+}
+
+// The scattering part of material_manager_v3 (src/ray_tracing/
+// path_trace.c:15-35) and, for photons, material_manager_v3_phothon
+// (src/caustic/trace_photon_path.c:18-35): metal, or dielectric_mat
+// (src/material/dielectric.c:40-66).  Returns true when the material takes
+// the diffuse branch (default_mat / default_mat_photon); otherwise d and
+// thr are updated in place.
+__device__ __forceinline__ bool scatter(const RngKey &rk, float ior_all, V3 nrm,
+                                        bool front, const Mat &m, V3 &d,
+                                        V3 &thr)
+{
+    bool is_metal;
+    if (m.a.w == 1.0f) {
+        is_metal = true;
+    } else if (m.a.w == 0.0f) {
+        is_metal = false;
+    } else {
+        U4 u = rk(1u);
+        is_metal = m.a.w > u.x;
+    }
+    if (is_metal) {
+        metal(rk, nrm, m, d, thr);
+        return false;
+    }
+    if (!(m.b.y > 0.0f))
+        return true;
+    U4 u = rk(1u);
+    float cth = -dot(unit(d), nrm);
+    float q = (m.b.y - 1.0f) / (m.b.y + 1.0f);
+    float f0 = powf(q, 2.0f);
+    float refl = f0 + (1.0f - f0) * schlick_pow5(cth);
+    if (refl >= u.y && m.b.y != 1.0f) {
+        V3 mn = nrm;
+        if (m.b.x > 0.0f) {
+            U4 g = rk(2u);
+            mn = ggx_sample(nrm, m.b.x, g.x, g.y);
+        }
+        d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
+        return false;
+    }
+    if (m.b.z == 1.0f) {
+        refract(rk, ior_all, nrm, front, m, d, thr);
+        return false;
+    }
+    if (m.b.z == 0.0f || m.b.z < u.z)
+        return true;
+    refract(rk, ior_all, nrm, front, m, d, thr);
+    return false;
+}
+
+// Hit point, normal facing the ray, side and material of a hit record
+// h4 = (t, code, u, v): hit_plane (hit_plane.c:55-63), hit_sphere with
+// set_normal_face (hit_sphere.c:37-46, normal.c:16-29) and hit_triangle's
+// interpolated vertex normal (hit_triangle.c:15-27, 81-91).
+__device__ __forceinline__ void surface(const float4 *planes,
+                                        const float4 *prims, const float *nrm9,
+                                        bool one_sided, V3 o, V3 d, float4 h4,
+                                        V3 &p, V3 &nrm, bool &front,
+                                        uint32_t &mat)
+{
+    uint32_t code = __float_as_uint(h4.y);
+    p = add(o, mul(d, h4.x));
+    if (code & HIT_PLANE) {
+        uint32_t k = code & ~HIT_PLANE;
+        float4 P = planes[2 * k], N = planes[2 * k + 1];
+        mat = __float_as_uint(P.w);
+        front = (d.x * N.x + d.y * N.y + d.z * N.z) < 0.0f;
+        nrm = front ? xyz(N) : neg(xyz(N));
+        return;
+    }
+    const float4 *rec = prims + (size_t)code * 3;
+    float4 A = rec[0], Bv = rec[1], C = rec[2];
+    mat = __float_as_uint(C.w);
+    if (__float_as_uint(Bv.w) == MRT_PRIMKIND_SPHERE) {
+        float rad = Bv.x;
+        V3 ow = sub(p, xyz(A));
+        ow = v3(ow.x / rad, ow.y / rad, ow.z / rad);
+        front = dot(d, ow) < 0.0f;
+        nrm = front ? ow : neg(ow);
+        return;
+    }
+    const float *N = nrm9 + (size_t)code * 9;
+    float bu = h4.z, bv = h4.w, w = 1.0f - bu - bv;
+    V3 nn = v3(w * N[0] + bu * N[3] + bv * N[6], w * N[1] + bu * N[4] + bv * N[7],
+               w * N[2] + bu * N[5] + bv * N[8]);
+    nn = unit(nn);
+    if (!one_sided && dot(d, nn) > 0.0f) {
+        front = false;
+        nrm = neg(nn);
+    } else {
+        front = true;
+        nrm = nn;
+    }
 }
 
 enum : uint32_t { SC_NONE = 0, SC_DIFFUSE = 1 };
@@ -454,43 +548,10 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 a.z = a.z + bg.z * thr.z * wmis;
                 B.acc[slot] = a;
             } else {
-                float t = h4.x;
-                V3 o = xyz(o4);
-                p = add(o, mul(d, t));
                 uint32_t mat;
                 bool front;
-                if (code & HIT_PLANE) {
-                    uint32_t k = code & ~HIT_PLANE;
-                    float4 P = B.planes[2 * k], N = B.planes[2 * k + 1];
-                    mat = __float_as_uint(P.w);
-                    front = (d.x * N.x + d.y * N.y + d.z * N.z) < 0.0f;
-                    nrm = front ? xyz(N) : neg(xyz(N));
-                } else {
-                    const float4 *rec = B.prims + (size_t)code * 3;
-                    float4 A = rec[0], Bv = rec[1], C = rec[2];
-                    mat = __float_as_uint(C.w);
-                    if (__float_as_uint(Bv.w) == MRT_PRIMKIND_SPHERE) {
-                        float rad = Bv.x;
-                        V3 ow = sub(p, xyz(A));
-                        ow = v3(ow.x / rad, ow.y / rad, ow.z / rad);
-                        front = dot(d, ow) < 0.0f;
-                        nrm = front ? ow : neg(ow);
-                    } else {
-                        const float *N = B.nrm + (size_t)code * 9;
-                        float bu = h4.z, bv = h4.w, w = 1.0f - bu - bv;
-                        V3 nn = v3(w * N[0] + bu * N[3] + bv * N[6],
-                                   w * N[1] + bu * N[4] + bv * N[7],
-                                   w * N[2] + bu * N[5] + bv * N[8]);
-                        nn = unit(nn);
-                        if (!fp.one_sided && dot(d, nn) > 0.0f) {
-                            front = false;
-                            nrm = neg(nn);
-                        } else {
-                            front = true;
-                            nrm = nn;
-                        }
-                    }
-                }
+                surface(B.planes, B.prims, B.nrm, fp.one_sided != 0, xyz(o4), d,
+                        h4, p, nrm, front, mat);
                 Mat m;
                 m.a = B.mats[3 * mat];
                 m.b = B.mats[3 * mat + 1];
@@ -504,44 +565,8 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     a.z = a.z + (m.c.w * m.c.x) * thr.z;
                     B.acc[slot] = a;
                 }
-                bool is_metal;
-                if (m.a.w == 1.0f) {
-                    is_metal = true;
-                } else if (m.a.w == 0.0f) {
-                    is_metal = false;
-                } else {
-                    U4 u = rng4(fp.seed, pixel, sample, fp.bounce, 1u);
-                    is_metal = m.a.w > u.x;
-                }
-                bool diffuse = false;
-                if (is_metal) {
-                    metal(fp, pixel, sample, nrm, m, d, thr);
-                } else if (!(m.b.y > 0.0f)) {
-                    diffuse = true;
-                } else {
-                    // dielectric_mat (src/material/dielectric.c:40-66)
-                    U4 u = rng4(fp.seed, pixel, sample, fp.bounce, 1u);
-                    float cth = -dot(unit(d), nrm);
-                    float q = (m.b.y - 1.0f) / (m.b.y + 1.0f);
-                    float f0 = powf(q, 2.0f);
-                    float refl = f0 + (1.0f - f0) * schlick_pow5(cth);
-                    if (refl >= u.y && m.b.y != 1.0f) {
-                        V3 mn = nrm;
-                        if (m.b.x > 0.0f) {
-                            U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
-                            mn = ggx_sample(nrm, m.b.x, g.x, g.y);
-                        }
-                        d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
-                    } else if (m.b.z == 1.0f) {
-                        refract(fp, pixel, sample, nrm, front, m, d, thr);
-                    } else if (m.b.z == 0.0f) {
-                        diffuse = true;
-                    } else if (m.b.z < u.z) {
-                        diffuse = true;
-                    } else {
-                        refract(fp, pixel, sample, nrm, front, m, d, thr);
-                    }
-                }
+                const RngKey rk{fp.seed, pixel, sample, fp.bounce, 0u};
+                bool diffuse = scatter(rk, fp.ior_all, nrm, front, m, d, thr);
                 if (diffuse) {
                     // default_mat (src/material/material_default.c:71-92)
                     kind = SC_DIFFUSE;
@@ -611,7 +636,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
                     d = cos_hemi(nrm, g.z, g.w);
                     pb_next = dot(nrm, d) / 3.14159265358979324f;
-                    want_pend = n_rays ? 1u : 0u;
+                    want_pend = (n_rays || fp.has_photons) ? 1u : 0u;
                 }
                 want_next = fp.last_bounce ? 0u : 1u;
             }
@@ -689,6 +714,8 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             B.p_b[pr] = make_uint4(want_next ? qn : HIT_NONE, j0, n_rays,
                                    env_ray ? 1u : 0u);
             B.p_c[pr] = make_float4(env_rad.x, env_rad.y, env_rad.z, env_omc);
+            if (fp.has_photons)
+                B.p_x[pr] = make_float4(p.x, p.y, p.z, 0.0f);
         }
     }
 }
@@ -755,9 +782,13 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             }
         }
         float4 s = B.acc[slot];
-        s.x = s.x + a.x * (lr / 255.0f);
-        s.y = s.y + a.y * (lg / 255.0f);
-        s.z = s.z + a.z * (lb / 255.0f);
+        // default_mat: total lighting = direct + caustics
+        V3 dl = v3(lr / 255.0f, lg / 255.0f, lb / 255.0f);
+        if (fp.has_photons)
+            dl = add(dl, xyz(B.p_k[r]));
+        s.x = s.x + a.x * dl.x;
+        s.y = s.y + a.y * dl.y;
+        s.z = s.z + a.z * dl.z;
         if (b.w && B.vis[end]) {
             float4 c = B.p_c[r];
             s.x = s.x + c.x;
@@ -825,7 +856,7 @@ k_aov(FrameParams fp, SceneDev S, const float4 *__restrict__ prims,
       void *__restrict__ out)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
     uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
     if (px >= fp.WH)
         return;
@@ -894,7 +925,7 @@ k_trace_rays(SceneDev S, const float4 *__restrict__ prims, uint32_t n,
              uint32_t *__restrict__ prim_id, float *__restrict__ tout)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -913,7 +944,345 @@ k_trace_rays(SceneDev S, const float4 *__restrict__ prims, uint32_t n,
     tout[i] = hit ? t : -1.0f;
 }
 
+// ---- photons (src/caustic) -------------------------------------------------------
+// hash_cell (src/gpu/convert_caustic_utils.c:15-24) on integer cell coords
+__device__ __forceinline__ uint32_t hash_cell(int ix, int iy, int iz,
+                                              uint32_t table)
+{
+    uint32_t n = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u)
+                 ^ ((uint32_t)iz * 83492791u);
+    return n % table;
+}
+
+// One photon per thread: emission from a point light (isotropic, power
+// colour * brightness / N, caustic_manager.c:30-53) or from the environment
+// through the disk of the scene's bounding sphere, then at most 10 segments
+// of trace_single_photon_path (trace_photon_path.c:47-75); a photon is kept
+// where it first reaches a diffuse surface after a specular event.
+__global__ void __launch_bounds__(TRACE_BLOCK)
+k_photon_emit(FrameParams fp, SceneDev S, PhotonEmit pe, ShadeBufs B,
+              float4 *__restrict__ out_pos, float4 *__restrict__ out_pow,
+              uint32_t *__restrict__ counter, uint32_t cap)
+{
+    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
+    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
+    uint32_t nodes = 0, np = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < pe.n_total;
+         base += stride) {
+        const uint32_t gid = base + threadIdx.x;
+        bool stored = false;
+        V3 sp = v3(0.f, 0.f, 0.f), spow = v3(0.f, 0.f, 0.f);
+        if (gid < pe.n_total) {
+            RngKey rk{fp.seed, gid, 0u, 0u, 1u};
+            V3 o, d, pw;
+            bool ok = true;
+            const uint32_t npl = pe.n_point * pe.n_per;
+            if (gid < npl) {
+                uint32_t k = pe.point_idx[gid / pe.n_per];
+                float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1];
+                float nf = (float)pe.n_per;
+                pw = v3(l1.x * l0.w / nf, l1.y * l0.w / nf, l1.z * l0.w / nf);
+                // vec3_random_unit's law (uniform on the sphere)
+                U4 u = rk(0u);
+                float z = 1.0f - 2.0f * u.x;
+                float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+                float ph = 6.28318530717958648f * u.y;
+                d = v3(rr * cosf(ph), rr * sinf(ph), z);
+                o = xyz(l0);
+            } else {
+                const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
+                U4 e = rk(4u), u = rk(0u);
+                uint32_t y = cdf_search(fp.mis_marg, nh, e.x);
+                uint32_t x = cdf_search(fp.mis_cond + (size_t)y * fp.env_w, nw, e.y);
+                float pc = fp.mis_pcell[(size_t)y * nw + x];
+                float vv = ((float)y + e.z) / (float)nh;
+                float uu = ((float)x + e.w) / (float)nw;
+                float el = (vv - 0.5f) * 3.14159265358979324f;
+                float ph = (0.5f - uu) * 6.28318530717958648f;
+                float ce = cosf(el);
+                ok = pc > 0.0f && ce > 0.0f;
+                V3 w = v3(ce * cosf(ph), sinf(el), ce * sinf(ph));
+                float pl = pc * (float)nw * (float)nh / (19.7392088021787172f * ce);
+                V3 a = fabsf(w.y) >= 0.999f ? v3(1.f, 0.f, 0.f) : v3(0.f, 1.f, 0.f);
+                V3 c = cross(a, w);
+                float l = sqrtf((c.x * c.x + c.y * c.y) + c.z * c.z);
+                V3 t1 = v3(c.x / l, c.y / l, c.z / l);
+                V3 t2 = cross(w, t1);
+                float dx = 0.0f, dy = 0.0f;
+                float sx = 2.0f * u.x - 1.0f, sy = 2.0f * u.y - 1.0f;
+                if (sx != 0.0f || sy != 0.0f) {
+                    float r, phi;
+                    if (fabsf(sx) > fabsf(sy)) {
+                        r = sx;
+                        phi = 0.78539816339744831f * (sy / sx);
+                    } else {
+                        r = sy;
+                        phi = 1.57079632679489662f - 0.78539816339744831f * (sx / sy);
+                    }
+                    dx = r * cosf(phi);
+                    dy = r * sinf(phi);
+                }
+                const float R = pe.sr;
+                o = v3(pe.sc[0] + (w.x * (R * 1.01f) + (t1.x * dx + t2.x * dy) * R),
+                       pe.sc[1] + (w.y * (R * 1.01f) + (t1.y * dx + t2.y * dy) * R),
+                       pe.sc[2] + (w.z * (R * 1.01f) + (t1.z * dx + t2.z * dy) * R));
+                d = neg(w);
+                // planes are unbounded: one that the line meets on the
+                // environment side of the disk blocks the photon
+                for (uint32_t k = 0; ok && k < S.n_planes; k++) {
+                    float tb;
+                    if (isect_pln(o, w, S.planes[2 * k], S.planes[2 * k + 1], tb))
+                        ok = false;
+                }
+                float flux = 3.14159265358979324f * R * R / (pl * (float)pe.n_per);
+                pw = ok ? mul(env_texel(fp, w), flux) : v3(0.f, 0.f, 0.f);
+            }
+            for (uint32_t depth = 0; ok && depth < 10; depth++) {
+                rk.bounce = depth;
+                float t, bu, bv;
+                uint32_t code;
+                if (!closest_hit<false, TRACE_SS>(S, o, d, s_code + threadIdx.x,
+                                                  s_t + threadIdx.x, TRACE_BLOCK,
+                                                  t, code, bu, bv, nodes, np))
+                    break;
+                V3 p, nrm;
+                bool front;
+                uint32_t mat;
+                surface(B.planes, B.prims, B.nrm, fp.one_sided != 0, o, d,
+                        make_float4(t, __uint_as_float(code), bu, bv), p, nrm,
+                        front, mat);
+                o = p;
+                Mat m;
+                m.a = B.mats[3 * mat];
+                m.b = B.mats[3 * mat + 1];
+                m.c = B.mats[3 * mat + 2];
+                if (scatter(rk, fp.ior_all, nrm, front, m, d, pw)) {
+                    pw = had(pw, v3(m.a.x, m.a.y, m.a.z));
+                    if (depth != 0) {
+                        stored = true;
+                        sp = p;
+                        spow = pw;
+                    }
+                    break;
+                }
+            }
+        }
+        uint32_t slot = wave_alloc(counter, stored ? 1u : 0u);
+        if (stored && slot < cap) {
+            out_pos[slot] = make_float4(sp.x, sp.y, sp.z, __uint_as_float(gid));
+            out_pow[slot] = make_float4(spow.x, spow.y, spow.z, 0.0f);
+        }
+    }
+}
+
+// Sort key of a photon: (bucket << 32) | photon id, so the sort is
+// deterministic; value = its slot.
+__global__ void __launch_bounds__(256)
+k_photon_keys(const float4 *__restrict__ pos, uint32_t n, float cell,
+              uint32_t table, unsigned long long *__restrict__ keys,
+              uint32_t *__restrict__ vals)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float4 p = pos[i];
+    uint32_t b = hash_cell((int)floorf(p.x / cell), (int)floorf(p.y / cell),
+                           (int)floorf(p.z / cell), table);
+    keys[i] = ((unsigned long long)b << 32) | __float_as_uint(p.w);
+    vals[i] = i;
+}
+
+__global__ void __launch_bounds__(256)
+k_photon_reorder(const float4 *__restrict__ pos, const float4 *__restrict__ pow,
+                 const unsigned long long *__restrict__ keys,
+                 const uint32_t *__restrict__ vals, uint32_t n,
+                 float4 *__restrict__ pos_out, float4 *__restrict__ pow_out,
+                 uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t s = vals[i];
+    pos_out[i] = pos[s];
+    pow_out[i] = pow[s];
+    uint32_t b = (uint32_t)(keys[i] >> 32);
+    if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != b)
+        cstart[b] = i;
+    if (i + 1 == n || (uint32_t)(keys[i + 1] >> 32) != b)
+        cend[b] = i + 1;
+}
+
+__global__ void __launch_bounds__(256)
+k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// estimate_caustics_physically_based / calculate_caustic_radiance
+// (src/caustic/get_caustic.c:37-88) over the 27 cells around x: the k
+// nearest photons within squared distance `radius` (k > 0; nothing unless
+// k are found) or all photons within distance `radius` (k == 0), summed
+// and divided by pi r^2 and the norm factor.  A photon counts only from
+// its own cell, so a hash collision between neighbours cannot count it
+// twice.  kd / ki are this thread's k-NN lists in LDS (stride `stride`).
+constexpr int KNN_MAX = 32;
+__device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
+                                         float *kd, uint32_t *ki, int stride)
+{
+    const int cx = (int)floorf(x.x / pm.cell), cy = (int)floorf(x.y / pm.cell),
+              cz = (int)floorf(x.z / pm.cell);
+    const bool knn = pm.k != 0;
+    const uint32_t K = pm.k < (uint32_t)KNN_MAX ? pm.k : (uint32_t)KNN_MAX;
+    const float cut = knn ? pm.radius : pm.radius * pm.radius;
+    V3 sum = v3(0.f, 0.f, 0.f);
+    uint32_t cnt = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
+                uint32_t b = hash_cell(ix, iy, iz, pm.table);
+                for (uint32_t j = pm.cstart[b]; j < pm.cend[b]; j++) {
+                    float4 P = pm.pos[j];
+                    if ((int)floorf(P.x / pm.cell) != ix
+                        || (int)floorf(P.y / pm.cell) != iy
+                        || (int)floorf(P.z / pm.cell) != iz)
+                        continue;
+                    float ddx = P.x - x.x, ddy = P.y - x.y, ddz = P.z - x.z;
+                    float d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
+                    if (!(d2 <= cut))
+                        continue;
+                    if (!knn) {
+                        float4 w = pm.pow[j];
+                        sum = add(sum, xyz(w));
+                        continue;
+                    }
+                    if (cnt == K && !(d2 < kd[(K - 1) * stride]))
+                        continue;
+                    uint32_t pos = cnt < K ? cnt++ : K - 1;
+                    while (pos > 0 && d2 < kd[(pos - 1) * stride]) {
+                        kd[pos * stride] = kd[(pos - 1) * stride];
+                        ki[pos * stride] = ki[(pos - 1) * stride];
+                        pos--;
+                    }
+                    kd[pos * stride] = d2;
+                    ki[pos * stride] = j;
+                }
+            }
+    float area;
+    if (knn) {
+        if (cnt < K)
+            return v3(0.f, 0.f, 0.f);
+        area = 3.14159265358979324f * kd[(K - 1) * stride];
+        if (area < 1e-9f)
+            return v3(0.f, 0.f, 0.f);
+        for (uint32_t j = 0; j < K; j++)
+            sum = add(sum, xyz(pm.pow[ki[j * stride]]));
+    } else {
+        area = 3.14159265358979324f * cut;
+    }
+    return v3(sum.x / area / pm.norm, sum.y / area / pm.norm,
+              sum.z / area / pm.norm);
+}
+
+constexpr int CAUSTIC_BLOCK = 128;
+
+__global__ void __launch_bounds__(CAUSTIC_BLOCK)
+k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
+{
+    __shared__ float s_d[KNN_MAX * CAUSTIC_BLOCK];
+    __shared__ uint32_t s_i[KNN_MAX * CAUSTIC_BLOCK];
+    const uint32_t n = min(ctr[CTR_PEND], B.cap_paths);
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += gridDim.x * blockDim.x) {
+        V3 c = caustic_at(fp.pm, xyz(B.p_x[r]), s_d + threadIdx.x,
+                          s_i + threadIdx.x, CAUSTIC_BLOCK);
+        B.p_k[r] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(CAUSTIC_BLOCK)
+k_caustic_at(PhotonMapDev pm, uint32_t n, const float *__restrict__ pts,
+             float *__restrict__ out)
+{
+    __shared__ float s_d[KNN_MAX * CAUSTIC_BLOCK];
+    __shared__ uint32_t s_i[KNN_MAX * CAUSTIC_BLOCK];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    V3 c = caustic_at(pm, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),
+                      s_d + threadIdx.x, s_i + threadIdx.x, CAUSTIC_BLOCK);
+    out[3 * i] = c.x;
+    out[3 * i + 1] = c.y;
+    out[3 * i + 2] = c.z;
+}
+
 // ---- launchers -----------------------------------------------------------------
+hipError_t launch_photon_emit(hipStream_t st, const FrameParams &fp,
+                              const SceneDev &S, const PhotonEmit &pe,
+                              const ShadeBufs &B, float4 *pos, float4 *pow,
+                              uint32_t *counter, uint32_t cap)
+{
+    uint32_t blocks = (pe.n_total + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    if (blocks > 65535u)
+        blocks = 65535u;
+    hipLaunchKernelGGL(k_photon_emit, dim3(blocks), dim3(TRACE_BLOCK), 0, st, fp,
+                       S, pe, B, pos, pow, counter, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_photon_keys(hipStream_t st, const float4 *pos, uint32_t n,
+                              float cell, uint32_t table,
+                              unsigned long long *keys, uint32_t *vals)
+{
+    hipLaunchKernelGGL(k_photon_keys, dim3((n + 255) / 256), dim3(256), 0, st,
+                       pos, n, cell, table, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
+                                 const float4 *pow,
+                                 const unsigned long long *keys,
+                                 const uint32_t *vals, uint32_t n,
+                                 float4 *pos_out, float4 *pow_out,
+                                 uint32_t *cstart, uint32_t *cend)
+{
+    hipLaunchKernelGGL(k_photon_reorder, dim3((n + 255) / 256), dim3(256), 0,
+                       st, pos, pow, keys, vals, n, pos_out, pow_out, cstart,
+                       cend);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n)
+{
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65535)
+        blocks = 65535;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, st, p,
+                       v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
+                          const FrameParams &fp, const ShadeBufs &B,
+                          uint32_t *ctr)
+{
+    hipLaunchKernelGGL(k_caustic, dim3(blocks), dim3(CAUSTIC_BLOCK), 0, st, fp,
+                       B, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_caustic_at(hipStream_t st, const PhotonMapDev &pm,
+                             uint32_t n, const float *pts, float *out)
+{
+    hipLaunchKernelGGL(k_caustic_at, dim3((n + CAUSTIC_BLOCK - 1) / CAUSTIC_BLOCK),
+                       dim3(CAUSTIC_BLOCK), 0, st, pm, n, pts, out);
+    return hipGetLastError();
+}
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
                          float4 *q_d, float4 *q_t, float4 *acc, uint32_t P,
                          uint32_t *ctr)

@@ -55,6 +55,8 @@ OPT_ESTIMATOR = 2
 OPT_STATS = 3
 OPT_PATH_BUDGET = 4
 OPT_BLOCK = 5
+OPT_PHOTON_K = 6
+OPT_PHOTON_NORM = 7
 STATS_COUNT = 1
 STATS_TIME = 2
 
@@ -169,6 +171,9 @@ mrt_upload_hdri = _sig("mrt_upload_hdri", C.c_int, _ctx_p, _fp, C.c_uint32,
                        C.c_uint32, C.c_float)
 mrt_build_photons = _sig("mrt_build_photons", C.c_int, _ctx_p, C.c_uint32,
                          C.c_float)
+mrt_get_photons = _sig("mrt_get_photons", C.c_int, _ctx_p, C.c_uint32, _fp,
+                       _fp, _up, _up)
+mrt_caustic_at = _sig("mrt_caustic_at", C.c_int, _ctx_p, C.c_uint32, _fp, _fp)
 mrt_render = _sig("mrt_render", C.c_int, _ctx_p, C.c_uint32, C.c_uint32,
                   C.c_uint32)
 mrt_readback_add = _sig("mrt_readback_add", C.c_int, _ctx_p,
@@ -513,6 +518,39 @@ class Renderer:
 
     def set_seed(self, seed):
         self.set_option(OPT_SEED, seed)
+
+    def upload_hdri(self, rgb, exposure=0.0):
+        a = np.ascontiguousarray(rgb, dtype=np.float32)
+        h, w = a.shape[:2]
+        self._ck(mrt_upload_hdri(self._ctx, a.ctypes.data_as(_fp), w, h,
+                                 exposure))
+
+    def build_photons(self, n_per, radius, k=None, norm=None):
+        """Emit, sort and grid the caustic photon map (--caustic=N;R;NORM)."""
+        if k is not None:
+            self.set_option(OPT_PHOTON_K, k)
+        if norm is not None:
+            self.set_option(OPT_PHOTON_NORM, norm)
+        self._ck(mrt_build_photons(self._ctx, n_per, radius))
+
+    def photons(self):
+        """Stored photons in map order: (pos[n,3], power[n,3], ids[n])."""
+        n = C.c_uint32(0)
+        self._ck(mrt_get_photons(self._ctx, 0, None, None, None, C.byref(n)))
+        pos = np.zeros((n.value, 3), np.float32)
+        pw = np.zeros((n.value, 3), np.float32)
+        ids = np.zeros(n.value, np.uint32)
+        self._ck(mrt_get_photons(self._ctx, n.value, pos.ctypes.data_as(_fp),
+                                 pw.ctypes.data_as(_fp),
+                                 ids.ctypes.data_as(_up), C.byref(n)))
+        return pos, pw, ids
+
+    def caustic_at(self, pts):
+        p, pp = _f32(pts)
+        n = p.size // 3
+        out = np.zeros((n, 3), np.float32)
+        self._ck(mrt_caustic_at(self._ctx, n, pp, out.ctypes.data_as(_fp)))
+        return out
 
     def render(self, first_sample, n_samples, max_depth):
         self._ck(mrt_render(self._ctx, first_sample, n_samples, max_depth))

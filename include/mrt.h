@@ -118,6 +118,10 @@ typedef struct mrt_scene {
 #define MRT_OPT_STATS       3  /* MRT_STATS_* flags                          */
 #define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront batch        */
 #define MRT_OPT_BLOCK       5  /* threads per block for the trace kernels    */
+#define MRT_OPT_PHOTON_K    6  /* caustic gather: k nearest photons (default
+                                  15, get_caustic.c); 0 = every photon within
+                                  the radius                                */
+#define MRT_OPT_PHOTON_NORM 7  /* caustic norm factor (--caustic=..;..;NORM) */
 
 /* Flags of MRT_OPT_STATS. */
 #define MRT_STATS_COUNT     1  /* count BVH nodes / primitives per ray       */
@@ -175,8 +179,25 @@ int  mrt_upload_hdri(mrt_ctx *ctx, const float *rgb, uint32_t width,
                      uint32_t height, float exposure);
 
 /* Replaces caustic_manager (src/caustic/caustic_manager.c:83-102) and
- * build_caustic_grid (src/gpu/convert_caustic.c:85-102). */
+ * build_caustic_grid (src/gpu/convert_caustic.c:85-102), the handling of
+ * --caustic=N;RADIUS;NORM (src/options/option_caustic.c:63-88): traces
+ * n_photons photons from every point light (and from the HDRI when one is
+ * uploaded), keeps those that reach a diffuse surface through a specular
+ * event, and builds the hashed grid the caustic gather reads.  With k
+ * nearest photons (MRT_OPT_PHOTON_K > 0) `radius` bounds the squared
+ * distance of the k-th, as the reference compares it; with k = 0 every
+ * photon within distance `radius` is gathered.  n_photons = 0 turns
+ * caustics off. */
 int  mrt_build_photons(mrt_ctx *ctx, uint32_t n_photons, float radius);
+
+/* The stored photons in sorted (grid) order: positions and powers are 3
+ * floats each, ids the global photon index; *n_out gets the count. */
+int  mrt_get_photons(mrt_ctx *ctx, uint32_t max, float *pos, float *power,
+                     uint32_t *ids, uint32_t *n_out);
+
+/* The caustic term (estimate_caustics_physically_based,
+ * src/caustic/get_caustic.c:67-88) at n host points (3 floats each). */
+int  mrt_caustic_at(mrt_ctx *ctx, uint32_t n, const float *pts, float *rgb);
 
 /* Replaces compute_frame_gpu (src/gpu/use_shader.c:81-100): traces
  * n_samples paths per pixel, sample indices first_sample .. +n_samples-1,

@@ -52,6 +52,19 @@ typedef struct orc_hdri {
     const float *mis_pcell;        /* (h-1)*(w-1)                            */
 } orc_hdri;
 
+/* A photon map: positions and powers (3 floats each) and the gather rule:
+ * k > 0: the k nearest within squared distance `radius` (the reference's
+ * k-NN, get_caustic.c:37-88); k == 0: every photon within distance
+ * `radius`.  Estimate = sum of powers / (pi r^2) / norm. */
+typedef struct orc_photon_map {
+    const float *pos;
+    const float *pow;
+    uint32_t     n;
+    uint32_t     k;
+    float        radius;
+    float        norm;
+} orc_photon_map;
+
 typedef struct orc_params {
     uint32_t width, height;
     uint32_t first_sample, n_samples, max_depth;
@@ -111,9 +124,30 @@ void orc_occluded_f32(const mrt_scene *s, const orc_bvh *b, uint32_t n,
  * first_sample .. first_sample + n_samples - 1, added into acc
  * (width*height*3 doubles, like screen.float_render). */
 int  orc_render_f32(const mrt_scene *s, const orc_bvh *b, const orc_hdri *hd,
-                    const orc_params *p, double *acc, orc_counters *cnt);
+                    const orc_photon_map *pm, const orc_params *p,
+                    double *acc, orc_counters *cnt);
 int  orc_render_f64(const mrt_scene *s, const orc_bvh *b, const orc_hdri *hd,
-                    const orc_params *p, double *acc, orc_counters *cnt);
+                    const orc_photon_map *pm, const orc_params *p,
+                    double *acc, orc_counters *cnt);
+
+/* Photon emission (caustic_manager.c:20-102, trace_photon_path.c:18-75):
+ * n_per photons from every point light and, with an HDRI, n_per photons
+ * from the environment through the disk of the scene's bounding sphere.
+ * Stored photons go to pos / pow (3 floats each) and gid (global photon
+ * index) in increasing gid order; returns how many were stored.  The
+ * arrays hold (point lights + (hd != NULL)) * n_per entries. */
+uint32_t orc_emit_photons_f32(const mrt_scene *s, const orc_bvh *b,
+                              const orc_hdri *hd, uint32_t n_per,
+                              uint64_t seed, float *pos, float *pow,
+                              uint32_t *gid);
+uint32_t orc_emit_photons_f64(const mrt_scene *s, const orc_bvh *b,
+                              const orc_hdri *hd, uint32_t n_per,
+                              uint64_t seed, float *pos, float *pow,
+                              uint32_t *gid);
+
+/* The caustic estimate at n points (3 floats each) into out (3 floats). */
+void orc_caustic_f32(const orc_photon_map *pm, uint32_t n, const float *pts,
+                     float *out);
 
 /* HDRI importance tables (src/ray_tracing/importance_sampling/calc_pdf.c,
  * calc_cdf.c, importance_sampling_manager.c): pdf_joint w*h, marg_inv h,

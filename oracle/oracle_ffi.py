@@ -70,10 +70,15 @@ def lib():
                                        C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint32)]
         for name in ("orc_render_f32", "orc_render_f64"):
-            getattr(L, name).argtypes = [vp, vp, vp, C.POINTER(OrcParams),
+            getattr(L, name).argtypes = [vp, vp, vp, vp, C.POINTER(OrcParams),
                                          C.POINTER(C.c_double),
                                          C.POINTER(OrcCounters)]
             getattr(L, name).restype = C.c_int
+        for name in ("orc_emit_photons_f32", "orc_emit_photons_f64"):
+            getattr(L, name).argtypes = [vp, vp, vp, u32, C.c_uint64, fp, fp,
+                                         C.POINTER(C.c_uint32)]
+            getattr(L, name).restype = u32
+        L.orc_caustic_f32.argtypes = [vp, u32, fp, fp]
         L.orc_hdri_tables.argtypes = [fp, u32, u32, C.c_float, fp, fp, fp,
                                       C.POINTER(C.c_double)]
         L.orc_hdri_tables.restype = C.c_int
@@ -160,9 +165,43 @@ def occluded_f32(scene_view, bvh, orig, dirs, tmax):
     return occ, nodes, prims
 
 
+class OrcPhotonMap(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("pow", C.c_void_p), ("n", C.c_uint32),
+                ("k", C.c_uint32), ("radius", C.c_float), ("norm", C.c_float)]
+
+
+def photon_map(pos, pw, k=15, radius=0.01, norm=1.0):
+    pos = np.ascontiguousarray(pos, np.float32)
+    pw = np.ascontiguousarray(pw, np.float32)
+    pm = OrcPhotonMap(pos.ctypes.data, pw.ctypes.data, len(pos), k, radius,
+                      norm)
+    pm._keep = (pos, pw)
+    return pm
+
+
+def emit_photons(scene_view, bvh, n_per, seed, hdri=None, n_emitters=1,
+                 f64=False):
+    cap = max(1, n_emitters) * n_per
+    pos = np.zeros((cap, 3), np.float32)
+    pw = np.zeros((cap, 3), np.float32)
+    gid = np.zeros(cap, np.uint32)
+    fn = lib().orc_emit_photons_f64 if f64 else lib().orc_emit_photons_f32
+    n = fn(C.cast(scene_view, C.c_void_p), C.byref(bvh),
+           C.byref(hdri) if hdri is not None else None, n_per, seed,
+           _fp(pos), _fp(pw), gid.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return pos[:n].copy(), pw[:n].copy(), gid[:n].copy()
+
+
+def caustic(pm, pts):
+    p = np.ascontiguousarray(pts, np.float32)
+    out = np.zeros_like(p)
+    lib().orc_caustic_f32(C.byref(pm), len(p), _fp(p), _fp(out))
+    return out
+
+
 def render(scene_view, bvh, width, height, first_sample, n_samples,
            max_depth, seed, f64=False, traversal=TRAV_ORDERED,
-           estimator=0, threads=0, rows=None, hdri=None):
+           estimator=0, threads=0, rows=None, hdri=None, photons=None):
     acc = np.zeros((height, width, 3), np.float64)
     p = OrcParams(width, height, first_sample, n_samples, max_depth, seed,
                   traversal, estimator, threads,
@@ -171,7 +210,8 @@ def render(scene_view, bvh, width, height, first_sample, n_samples,
     fn = lib().orc_render_f64 if f64 else lib().orc_render_f32
     rc = fn(C.cast(scene_view, C.c_void_p),
             C.byref(bvh) if bvh is not None else None,
-            C.byref(hdri) if hdri is not None else None, C.byref(p),
+            C.byref(hdri) if hdri is not None else None,
+            C.byref(photons) if photons is not None else None, C.byref(p),
             acc.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cnt))
     if rc != 0:
         raise RuntimeError("oracle render failed")
