@@ -71,6 +71,33 @@ def cpu_baseline(hs, width, height, max_depth, seed, budget_s):
                               "sample": f"{rows1} rows, {t1:.1f} s"}}
 
 
+def copy_probe():
+    """Four launches of y = x * 1 over 512 MiB fp32 (512 MiB read, 512 MiB
+    written per launch): the known-size kernel that scales FETCH_SIZE and
+    WRITE_SIZE into HBM bytes."""
+    import torch
+    n = 128 << 20
+    x = torch.ones(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(4):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n,
+                      "copy_probe_bytes_written": 4 * n, "launches": 4}))
+
+
+def measured_traffic():
+    """HBM bytes per k_trace launch from the committed counter summary
+    (two rocprofv3 --pmc passes over this bench, scaled by the copy probe),
+    or None when it is absent."""
+    f = ROOT / "profiles" / "k_trace_mesh1080p_hbm.json"
+    try:
+        d = json.loads(f.read_text())
+        return d["hbm_bytes_per_launch"], f.relative_to(ROOT).as_posix()
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,12 +112,19 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 512 MiB elementwise copy (the "
+                         "calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(rendering shards by sample index; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import mrt_hip as M
     W, H = args.width, args.height
@@ -149,6 +183,7 @@ def main():
     achieved = per_launch / (trace_ms * 1e-3) / 1e9
     rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
         * value * 1e6
+    traffic, traffic_src = measured_traffic()
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -172,7 +207,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_trace (closest hit)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4)},
     }

@@ -73,6 +73,26 @@ def test_oracle_gather_needs_k_photons():
     np.testing.assert_allclose(got[0], 2.0 / (np.pi * 1e-4), rtol=1e-4)
 
 
+# ---- hash grid ------------------------------------------------------------
+
+def hash_cell_np(ix, iy, iz, table):
+    """hash_cell (src/gpu/convert_caustic_utils.c:15-24): int cell indices
+    converted to uint32, multiplied with wrap-around, xor-ed, mod table."""
+    m = np.uint64(0xFFFFFFFF)
+    u = lambda a, p: (np.asarray(a, np.int64).astype(np.uint64) * np.uint64(p)) & m
+    n = u(ix, 73856093) ^ u(iy, 19349663) ^ u(iz, 83492791)
+    return (n % np.uint64(table)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("cell,table,expect", [
+    ((0, 0, 0), 1000, 0), ((1, 0, 0), 1000, 93), ((0, 1, 0), 1000, 663),
+    ((0, 0, 1), 1000, 791), ((1, 1, 1), 1000, 157),
+    ((-1, 0, 0), 1 << 31, 4221111203 - (1 << 31)),
+    ((1, 0, 0), 1 << 31, 73856093), ((2, 0, 0), 7, (2 * 73856093) % 7)])
+def test_hash_cell_known_answers(cell, table, expect):
+    assert int(hash_cell_np(*cell, table)) == expect
+
+
 # ---- emission: point light above a mirror ---------------------------------
 
 LIGHT_Y, MIRROR_Y = 1.0, 2.0
@@ -209,7 +229,7 @@ def test_oracle_env_photons_carry_the_env_flux():
 # ---- HIP path -------------------------------------------------------------
 
 def _gpu_photons(hs, n, radius, env=None, k=15, norm=1.0):
-    r = M.Renderer(*hs.resolution())
+    r = M.Renderer(*hs.resolution)
     r.upload(hs)
     if env is not None:
         r.upload_hdri(env)
@@ -321,6 +341,22 @@ def test_gpu_image_with_caustics_matches_oracle():
     print(f"close {close.mean():.6f} mean diff {diff.mean():.3e}")
     assert close.mean() >= 0.995
     assert diff.mean() <= 1e-3 * max(ref.mean(), 1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,radius", [(15, 0.01), (0, 0.05)])
+def test_gpu_map_order_is_hash_bucket_then_id(k, radius):
+    """The map is sorted by (hash_cell of the photon's cell mod 2n, id)."""
+    hs = M.HostScene.plain(7, 64, 36)
+    r = _gpu_photons(hs, 100000, radius, k=k)
+    pos, _, ids = r.photons()
+    r.close()
+    n = len(pos)
+    cell = np.sqrt(np.float32(radius)) if k else np.float32(radius)
+    c = np.floor(pos / np.float32(cell)).astype(np.int64)
+    b = hash_cell_np(c[:, 0], c[:, 1], c[:, 2], 2 * n).astype(np.int64)
+    key = (b << 32) | ids.astype(np.int64)
+    assert n > 1000 and np.all(np.diff(key) > 0)
 
 
 @pytest.mark.gpu
