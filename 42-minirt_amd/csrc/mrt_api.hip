@@ -7,6 +7,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -262,10 +265,8 @@ static int upload(mrt_ctx *c, void *dst, const void *src, size_t bytes)
     return MRT_OK;
 }
 
-extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
+static int validate_scene(mrt_ctx *c, const mrt_scene *s)
 {
-    if (!c || !s)
-        return MRT_E_INVALID;
     if (s->n_materials == 0 || !s->materials)
         return fail(c, MRT_E_INVALID, "scene has no materials");
     if (s->n_lights > 64 || (s->n_lights && !s->lights))
@@ -284,6 +285,42 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     for (uint32_t k = 0; k < s->n_lights; k++)
         if (s->lights[k].shadow_samples > 4096)
             return fail(c, MRT_E_LIMIT, "light %u: more than 4096 shadow samples", k);
+    return MRT_OK;
+}
+
+// Planes (point | material, normal | 0), materials and lights as the
+// kernels read them; material indices are offset by mat0.
+static void pack_scene(const mrt_scene *s, uint32_t mat0, std::vector<float4> &planes,
+                       std::vector<float4> &mats, std::vector<float4> &lights)
+{
+    for (uint32_t k = 0; k < s->n_planes; k++) {
+        const float *q = s->pln + 6 * (size_t)k;
+        planes.push_back(make_float4(q[0], q[1], q[2], bits_f(mat0 + s->pln_mat[k])));
+        planes.push_back(make_float4(q[3], q[4], q[5], 0.0f));
+    }
+    for (uint32_t k = 0; k < s->n_materials; k++) {
+        const mrt_material &m = s->materials[k];
+        mats.push_back(make_float4(m.color[0], m.color[1], m.color[2], m.metallic));
+        mats.push_back(make_float4(m.roughness, m.ior, m.transmission, m.ao));
+        mats.push_back(make_float4(m.emission_strength, m.emission_color[0],
+                                   m.emission_color[1], m.emission_color[2]));
+    }
+    for (uint32_t k = 0; k < s->n_lights; k++) {
+        const mrt_light &l = s->lights[k];
+        lights.push_back(make_float4(l.vec[0], l.vec[1], l.vec[2], l.brightness));
+        lights.push_back(make_float4(l.color[0], l.color[1], l.color[2], l.radius));
+        lights.push_back(make_float4(bits_f((uint32_t)l.type),
+                                     bits_f((uint32_t)l.shadow_samples), 0.0f, 0.0f));
+    }
+}
+
+extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
+{
+    if (!c || !s)
+        return MRT_E_INVALID;
+    int vrc = validate_scene(c, s);
+    if (vrc != MRT_OK)
+        return vrc;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_scene(c);
@@ -297,28 +334,8 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
         return fail(c, MRT_E_LIMIT, "BVH depth %u exceeds %u", bvh.depth,
                     MRT_BVH_MAX_DEPTH);
     }
-    std::vector<float4> planes(2 * (size_t)s->n_planes);
-    for (uint32_t k = 0; k < s->n_planes; k++) {
-        const float *q = s->pln + 6 * (size_t)k;
-        planes[2 * k] = make_float4(q[0], q[1], q[2], bits_f(s->pln_mat[k]));
-        planes[2 * k + 1] = make_float4(q[3], q[4], q[5], 0.0f);
-    }
-    std::vector<float4> mats(3 * (size_t)s->n_materials);
-    for (uint32_t k = 0; k < s->n_materials; k++) {
-        const mrt_material &m = s->materials[k];
-        mats[3 * k] = make_float4(m.color[0], m.color[1], m.color[2], m.metallic);
-        mats[3 * k + 1] = make_float4(m.roughness, m.ior, m.transmission, m.ao);
-        mats[3 * k + 2] = make_float4(m.emission_strength, m.emission_color[0],
-                                      m.emission_color[1], m.emission_color[2]);
-    }
-    std::vector<float4> lights(3 * (size_t)(s->n_lights ? s->n_lights : 1));
-    for (uint32_t k = 0; k < s->n_lights; k++) {
-        const mrt_light &l = s->lights[k];
-        lights[3 * k] = make_float4(l.vec[0], l.vec[1], l.vec[2], l.brightness);
-        lights[3 * k + 1] = make_float4(l.color[0], l.color[1], l.color[2], l.radius);
-        lights[3 * k + 2] = make_float4(bits_f((uint32_t)l.type),
-                                        bits_f((uint32_t)l.shadow_samples), 0.0f, 0.0f);
-    }
+    std::vector<float4> planes, mats, lights;
+    pack_scene(s, 0, planes, mats, lights);
     rc = MRT_OK;
     if (dalloc(c->d_nodes, (size_t)bvh.n_nodes * 4) != hipSuccess
         || dalloc(c->d_prims, (size_t)bvh.n_prims * 3) != hipSuccess
@@ -683,18 +700,17 @@ extern "C" int mrt_caustic_at(mrt_ctx *c, uint32_t n, const float *pts,
 
 // init_viewport_values / init_viewport_vectors (src/ray_tracing/viewport.c:
 // 38-88) in double, rounded to float for the kernels.
-static FrameParams frame_params(const mrt_ctx *c)
+static void camera_frame(FrameParams &fp, const mrt_camera &cam, uint32_t W,
+                         uint32_t H, float ambient_ratio,
+                         const float ambient_color[3])
 {
-    FrameParams fp;
-    memset(&fp, 0, sizeof(fp));
-    const mrt_camera &cam = c->cam;
     double dir[3] = {cam.dir[0], cam.dir[1], cam.dir[2]};
     double l = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
     for (int k = 0; k < 3; k++)
         dir[k] = l > 0.0 ? dir[k] / l : 0.0;
     double focus = cam.focus_dist > 0.0f ? cam.focus_dist : 1.0;
     double width = 2.0 * tan((double)cam.fov_deg * (M_PI / 180.0) / 2.0) * focus;
-    double height = width / ((double)c->W / (double)c->H);
+    double height = width / ((double)W / (double)H);
     double up[3] = {0, 1, 0};
     if (fabs(dir[1]) > 0.999) {
         up[0] = 1;
@@ -711,7 +727,7 @@ static FrameParams frame_params(const mrt_ctx *c)
     double rad = focus * tan((double)cam.defocus_angle_deg * M_PI / 360.0);
     for (int k = 0; k < 3; k++) {
         double vu = u[k] * width, vv = cv[k] * height;
-        double du = vu / (double)c->W, dv = vv / (double)c->H;
+        double du = vu / (double)W, dv = vv / (double)H;
         double ul = ((double)cam.pos[k] + dir[k] * focus) - vu / 2 - vv / 2;
         fp.cam_o[k] = cam.pos[k];
         fp.p00[k] = (float)(ul + (du + dv) * 0.5);
@@ -720,15 +736,22 @@ static FrameParams frame_params(const mrt_ctx *c)
         fp.disk_u[k] = (float)(u[k] * rad);
         fp.disk_v[k] = (float)(cv[k] * rad);
         fp.cam_pos[k] = cam.pos[k];
-        double bg = c->ambient_color[k] / 255.0 * (double)c->ambient_ratio;
+        double bg = ambient_color[k] / 255.0 * (double)ambient_ratio;
         float bgf = (float)bg;
         fp.bg_clamped[k] = bgf < 0.0f ? 0.0f : (bgf > 16.0f ? 16.0f : bgf);
     }
     fp.lens = rad > 0.0;
+    fp.W = W;
+    fp.H = H;
+    fp.WH = W * H;
+}
+
+static FrameParams frame_params(const mrt_ctx *c)
+{
+    FrameParams fp;
+    memset(&fp, 0, sizeof(fp));
+    camera_frame(fp, c->cam, c->W, c->H, c->ambient_ratio, c->ambient_color);
     fp.ior_all = c->ior_all;
-    fp.W = c->W;
-    fp.H = c->H;
-    fp.WH = c->W * c->H;
     fp.n_lights = (uint32_t)c->lights.size();
     fp.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
     fp.flags = c->flags;
@@ -758,12 +781,14 @@ static FrameParams frame_params(const mrt_ctx *c)
 static SceneDev scene_dev(const mrt_ctx *c)
 {
     SceneDev S;
+    memset(&S, 0, sizeof(S));
     S.nodes = c->d_nodes;
     S.prims = c->d_prims;
     S.planes = c->d_planes;
     S.n_nodes = c->n_nodes;
     S.n_planes = c->n_planes;
     S.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
+    S.root = 0;
     return S;
 }
 
@@ -916,6 +941,7 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
             fp.bounce = b;
             fp.last_bounce = (b + 1 == max_depth) ? 1u : 0u;
             ShadeBufs B;
+            memset(&B, 0, sizeof(B));
             B.q_o = c->q_o[cur];
             B.q_d = c->q_d[cur];
             B.q_t = c->q_t[cur];
@@ -1167,4 +1193,664 @@ extern "C" int mrt_reset_stats(mrt_ctx *c)
     for (double &m : c->ms)
         m = 0.0;
     return MRT_OK;
+}
+
+// ---- batches ------------------------------------------------------------------
+namespace {
+// A scene of a batch, copied from the caller's arrays.
+struct BatchSrc {
+    std::vector<float>        tri_pos, tri_nrm, sph, pln;
+    std::vector<uint32_t>     tri_mat, sph_mat, pln_mat;
+    std::vector<mrt_material> mats;
+    std::vector<mrt_light>    lights;
+    mrt_scene                 view;
+    uint32_t                  W = 0, H = 0, spp = 0;
+    uint64_t                  seed = 0;
+};
+}  // namespace
+
+struct mrt_batch {
+    mrt_ctx                *c = nullptr;   // device, stream, queues, stats
+    std::vector<BatchSrc *> src;
+    bool                    committed = false;
+    float4                 *d_nodes = nullptr, *d_prims = nullptr;
+    float4                 *d_planes = nullptr, *d_mats = nullptr;
+    float4                 *d_lights = nullptr;
+    float                  *d_nrm = nullptr, *d_accum = nullptr;
+    BatchScene             *d_bs = nullptr;
+    BatchCtl               *d_ctl = nullptr;
+    uint2                  *d_slot = nullptr;
+    uint32_t                cap_slot = 0;
+    std::vector<BatchScene> hbs;
+    uint64_t                total = 0;
+    size_t                  accum_n = 0;
+    uint32_t                max_rays = 1;
+    bool                    has_lights = false;
+    mrt_batch_tuning        tune{};
+    hipGraph_t              graph = nullptr;
+    hipGraphExec_t          gexec = nullptr;
+    uint32_t                g_key[5] = {0, 0, 0, 0, 0};
+};
+
+template <typename T>
+static void vcopy(std::vector<T> &v, const T *p, size_t n)
+{
+    v.assign(p ? p : nullptr, p ? p + n : nullptr);
+}
+
+static void batch_free_device(mrt_batch *b)
+{
+    if (b->gexec)
+        (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    dfree(b->d_nodes);
+    dfree(b->d_prims);
+    dfree(b->d_planes);
+    dfree(b->d_mats);
+    dfree(b->d_lights);
+    dfree(b->d_nrm);
+    dfree(b->d_accum);
+    dfree(b->d_bs);
+    dfree(b->d_ctl);
+    dfree(b->d_slot);
+    b->cap_slot = 0;
+}
+
+extern "C" int mrt_batch_create(int device, mrt_batch **out)
+{
+    if (!out)
+        return MRT_E_INVALID;
+    *out = nullptr;
+    mrt_ctx *c = nullptr;
+    int rc = mrt_init(device, 1, 1, &c);
+    if (rc != MRT_OK)
+        return rc;
+    mrt_batch *b = new (std::nothrow) mrt_batch();
+    if (!b) {
+        mrt_destroy(c);
+        return MRT_E_NOMEM;
+    }
+    b->c = c;
+    b->tune.path_budget = 1u << 22;
+    b->tune.trace_blocks_per_cu = c->trace_blocks / (uint32_t)c->num_cu;
+    b->tune.shade_blocks_per_cu = 8;
+    b->tune.use_graph = 1;
+    *out = b;
+    return MRT_OK;
+}
+
+extern "C" void mrt_batch_destroy(mrt_batch *b)
+{
+    if (!b)
+        return;
+    if (b->c) {
+        (void)hipSetDevice(b->c->device);
+        (void)hipStreamSynchronize(b->c->stream);
+    }
+    batch_free_device(b);
+    for (BatchSrc *p : b->src)
+        delete p;
+    mrt_destroy(b->c);
+    delete b;
+}
+
+extern "C" const char *mrt_batch_last_error(const mrt_batch *b)
+{
+    return b ? b->c->err : "null batch";
+}
+
+extern "C" int mrt_batch_add_scene(mrt_batch *b, const mrt_scene *s,
+                                   uint32_t width, uint32_t height, uint32_t spp,
+                                   uint64_t seed, uint32_t *id)
+{
+    if (!b || !s)
+        return MRT_E_INVALID;
+    mrt_ctx *c = b->c;
+    if (b->committed)
+        return fail(c, MRT_E_STATE, "batch already committed");
+    if (width == 0 || height == 0 || spp == 0)
+        return fail(c, MRT_E_INVALID, "empty resolution or sample count");
+    if ((uint64_t)width * height * spp > 0xFFFFFFFFull)
+        return fail(c, MRT_E_LIMIT, "more than 2^32 paths in one scene");
+    if (b->src.size() >= 4096)
+        return fail(c, MRT_E_LIMIT, "at most 4096 scenes per batch");
+    int rc = validate_scene(c, s);
+    if (rc != MRT_OK)
+        return rc;
+    BatchSrc *p = new (std::nothrow) BatchSrc();
+    if (!p)
+        return fail(c, MRT_E_NOMEM, "out of host memory");
+    vcopy(p->tri_pos, s->tri_pos, (size_t)s->n_triangles * 9);
+    if (s->tri_nrm)
+        vcopy(p->tri_nrm, s->tri_nrm, (size_t)s->n_triangles * 9);
+    vcopy(p->tri_mat, s->tri_mat, s->n_triangles);
+    vcopy(p->sph, s->sph, (size_t)s->n_spheres * 4);
+    vcopy(p->sph_mat, s->sph_mat, s->n_spheres);
+    vcopy(p->pln, s->pln, (size_t)s->n_planes * 6);
+    vcopy(p->pln_mat, s->pln_mat, s->n_planes);
+    vcopy(p->mats, s->materials, s->n_materials);
+    vcopy(p->lights, s->lights, s->n_lights);
+    p->view = *s;
+    p->view.tri_pos = p->tri_pos.data();
+    p->view.tri_nrm = s->tri_nrm ? p->tri_nrm.data() : nullptr;
+    p->view.tri_mat = p->tri_mat.data();
+    p->view.sph = p->sph.data();
+    p->view.sph_mat = p->sph_mat.data();
+    p->view.pln = p->pln.data();
+    p->view.pln_mat = p->pln_mat.data();
+    p->view.materials = p->mats.data();
+    p->view.lights = p->lights.data();
+    p->W = width;
+    p->H = height;
+    p->spp = spp;
+    p->seed = seed;
+    if (id)
+        *id = (uint32_t)b->src.size();
+    b->src.push_back(p);
+    return MRT_OK;
+}
+
+extern "C" int mrt_batch_commit(mrt_batch *b)
+{
+    if (!b)
+        return MRT_E_INVALID;
+    mrt_ctx *c = b->c;
+    if (b->committed)
+        return MRT_OK;
+    const size_t n = b->src.size();
+    if (n == 0)
+        return fail(c, MRT_E_STATE, "batch has no scenes");
+    // BVHs, built in parallel
+    std::vector<mrt_bvh> bv(n);
+    std::vector<int> brc(n, MRT_OK);
+    std::atomic<size_t> next{0};
+    unsigned nth = std::thread::hardware_concurrency();
+    nth = std::max(1u, std::min({nth, 16u, (unsigned)n}));
+    auto work = [&]() {
+        for (size_t i; (i = next.fetch_add(1)) < n;)
+            brc[i] = mrt_bvh_build(&b->src[i]->view, MRT_BVH_MAX_LEAF, &bv[i]);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nth; t++)
+        pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool)
+        t.join();
+    int rc = MRT_OK;
+    for (size_t i = 0; i < n && rc == MRT_OK; i++) {
+        if (brc[i] != MRT_OK)
+            rc = fail(c, brc[i], "scene %zu: BVH build failed", i);
+        else if (bv[i].depth > MRT_BVH_MAX_DEPTH)
+            rc = fail(c, MRT_E_LIMIT, "scene %zu: BVH depth %u exceeds %u", i,
+                      bv[i].depth, MRT_BVH_MAX_DEPTH);
+    }
+    // shared arrays with rebased links, primitives and materials
+    std::vector<mrt_bvh_node> nodes;
+    std::vector<mrt_prim> prims;
+    std::vector<float> nrm;
+    std::vector<float4> planes, mats, lights;
+    b->hbs.assign(n, BatchScene{});
+    uint64_t path0 = 0;
+    size_t accum0 = 0;
+    b->max_rays = 1;
+    b->has_lights = false;
+    for (size_t i = 0; i < n && rc == MRT_OK; i++) {
+        const BatchSrc &S = *b->src[i];
+        const mrt_bvh &B = bv[i];
+        const uint32_t node0 = (uint32_t)nodes.size();
+        const uint32_t prim0 = (uint32_t)prims.size();
+        const uint32_t mat0 = (uint32_t)(mats.size() / 3);
+        const uint32_t plane0 = (uint32_t)(planes.size() / 2);
+        const uint32_t light0 = (uint32_t)(lights.size() / 3);
+        if ((uint64_t)prim0 + B.n_prims >= (1ull << 28)
+            || (uint64_t)node0 + B.n_nodes >= (1ull << 31)) {
+            rc = fail(c, MRT_E_LIMIT, "batch exceeds 2^28 primitives");
+            break;
+        }
+        for (uint32_t k = 0; k < B.n_nodes; k++) {
+            mrt_bvh_node nd = B.nodes[k];
+            for (int j = 0; j < 2; j++) {
+                int32_t ch = nd.child[j];
+                if (ch >= 0) {
+                    nd.child[j] = ch + (int32_t)node0;
+                } else {
+                    uint32_t v = ~(uint32_t)ch;
+                    v = (((v >> 3) + prim0) << 3) | (v & 7u);
+                    nd.child[j] = (int32_t)~v;
+                }
+            }
+            nodes.push_back(nd);
+        }
+        for (uint32_t k = 0; k < B.n_prims; k++) {
+            mrt_prim pr = B.prims[k];
+            uint32_t m;
+            memcpy(&m, &pr.c[3], 4);
+            m += mat0;
+            memcpy(&pr.c[3], &m, 4);
+            prims.push_back(pr);
+        }
+        nrm.insert(nrm.end(), B.shade_nrm, B.shade_nrm + (size_t)B.n_prims * 9);
+        pack_scene(&S.view, mat0, planes, mats, lights);
+        BatchScene &T = b->hbs[i];
+        memset(&T, 0, sizeof(T));
+        camera_frame(T.fp, S.view.camera, S.W, S.H, S.view.ambient_ratio,
+                     S.view.ambient_color);
+        T.fp.ior_all = S.view.ior_all > 0.0f ? S.view.ior_all : 1.0f;
+        T.fp.n_lights = S.view.n_lights;
+        T.fp.one_sided = (S.view.flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
+        T.fp.flags = S.view.flags;
+        T.fp.seed = S.seed;
+        T.fp.estimator = MRT_EST_REFERENCE;
+        T.fp.light0 = light0;
+        T.fp.plane0 = plane0;
+        T.root = B.n_nodes ? node0 : 0u;
+        T.n_nodes = B.n_nodes;
+        T.n_planes = S.view.n_planes;
+        T.accum0 = (uint32_t)accum0;
+        T.path0 = path0;
+        T.paths = (uint64_t)S.W * S.H * S.spp;
+        path0 += T.paths;
+        accum0 += (size_t)S.W * S.H * 3;
+        uint32_t rays = 0;
+        for (const mrt_light &l : S.lights) {
+            bool soft = l.type == MRT_LIGHT_POINT && l.radius != 0.0f
+                        && l.shadow_samples > 1;
+            rays += soft ? (uint32_t)l.shadow_samples : 1u;
+        }
+        b->max_rays = std::max(b->max_rays, rays);
+        b->has_lights = b->has_lights || !S.lights.empty();
+    }
+    for (mrt_bvh &B : bv)
+        mrt_bvh_free(&B);
+    if (rc == MRT_OK && accum0 >= (1ull << 32))
+        rc = fail(c, MRT_E_LIMIT, "batch images exceed 2^32 floats");
+    if (rc != MRT_OK)
+        return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    batch_free_device(b);
+    BatchCtl ctl{0, path0, (uint32_t)n, 0};
+    if (dalloc(b->d_nodes, nodes.size() * 4) != hipSuccess
+        || dalloc(b->d_prims, prims.size() * 3) != hipSuccess
+        || dalloc(b->d_nrm, nrm.size()) != hipSuccess
+        || dalloc(b->d_planes, planes.size()) != hipSuccess
+        || dalloc(b->d_mats, mats.size()) != hipSuccess
+        || dalloc(b->d_lights, lights.size()) != hipSuccess
+        || dalloc(b->d_accum, accum0) != hipSuccess
+        || dalloc(b->d_bs, n) != hipSuccess || dalloc(b->d_ctl, 1) != hipSuccess) {
+        batch_free_device(b);
+        return fail(c, MRT_E_NOMEM, "device allocation for the batch failed");
+    }
+    rc = upload(c, b->d_nodes, nodes.data(), nodes.size() * 64);
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_prims, prims.data(), prims.size() * 48);
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_nrm, nrm.data(), nrm.size() * 4);
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_planes, planes.data(), planes.size() * 16);
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_mats, mats.data(), mats.size() * 16);
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_lights, lights.data(), lights.size() * 16);
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_bs, b->hbs.data(), n * sizeof(BatchScene));
+    if (rc == MRT_OK)
+        rc = upload(c, b->d_ctl, &ctl, sizeof(ctl));
+    if (rc == MRT_OK
+        && (hipMemsetAsync(b->d_accum, 0, accum0 * 4, c->stream) != hipSuccess
+            || hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = fail(c, MRT_E_HIP, "batch upload failed");
+    if (rc != MRT_OK) {
+        batch_free_device(b);
+        return rc;
+    }
+    b->total = path0;
+    b->accum_n = accum0;
+    b->committed = true;
+    for (BatchSrc *p : b->src)
+        delete p;
+    b->src.clear();
+    return MRT_OK;
+}
+
+// Queues for P paths per pass.
+static int batch_buffers(mrt_batch *b, uint32_t P)
+{
+    mrt_ctx *c = b->c;
+    uint64_t S64 = (uint64_t)P * b->max_rays;
+    if (S64 > 0xFFFFFFF0ull)
+        return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
+    int rc = ensure_wavefront(c, P, (uint32_t)S64);
+    if (rc != MRT_OK)
+        return rc;
+    if (P > b->cap_slot) {
+        if (dalloc(b->d_slot, P) != hipSuccess) {
+            b->cap_slot = 0;
+            return fail(c, MRT_E_NOMEM, "slot table (%u) does not fit", P);
+        }
+        b->cap_slot = P;
+    }
+    return MRT_OK;
+}
+
+// One pass: the next P paths of the batch, all bounces, accumulation.
+static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
+                             bool count, Timer *tm)
+{
+    mrt_ctx *c = b->c;
+    SceneDev S;
+    memset(&S, 0, sizeof(S));
+    S.nodes = b->d_nodes;
+    S.prims = b->d_prims;
+    S.planes = b->d_planes;
+    FrameParams fp;
+    memset(&fp, 0, sizeof(fp));
+    ShadeBufs B;
+    memset(&B, 0, sizeof(B));
+    B.acc = c->acc;
+    B.planes = b->d_planes;
+    B.prims = b->d_prims;
+    B.mats = b->d_mats;
+    B.lights = b->d_lights;
+    B.nrm = b->d_nrm;
+    B.s_o = c->s_o;
+    B.s_d = c->s_d;
+    B.s_c = c->s_c;
+    B.vis = c->vis;
+    B.p_a = c->p_a;
+    B.p_b = c->p_b;
+    B.p_c = c->p_c;
+    B.p_x = c->p_x;
+    B.p_k = c->p_k;
+    B.cap_paths = c->cap_paths;
+    B.cap_shadow = c->cap_shadow;
+    B.bs = b->d_bs;
+    B.slot_info = b->d_slot;
+    const uint32_t tb = (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu;
+    uint32_t gb = (P + SHADE_BLOCK - 1) / SHADE_BLOCK;
+    gb = std::min(gb, (uint32_t)c->num_cu * b->tune.shade_blocks_per_cu);
+    hipError_t e;
+#define BSTEP(cls, expr)                                                     \
+    do {                                                                     \
+        if (tm)                                                              \
+            tm->begin(cls);                                                  \
+        e = (expr);                                                          \
+        if (tm)                                                              \
+            tm->end();                                                       \
+        if (e != hipSuccess)                                                 \
+            return e;                                                        \
+    } while (0)
+    BSTEP(K_RAYGEN, launch_braygen(c->stream, B, b->d_ctl, c->q_o[0], c->q_d[0],
+                                   c->q_t[0], P, c->d_ctr));
+    for (uint32_t k = 0; k < depth; k++) {
+        int cur = k & 1, nxt = cur ^ 1;
+        fp.bounce = k;
+        fp.last_bounce = (k + 1 == depth) ? 1u : 0u;
+        B.q_o = c->q_o[cur];
+        B.q_d = c->q_d[cur];
+        B.q_t = c->q_t[cur];
+        B.hit = c->hit;
+        B.qn_o = c->q_o[nxt];
+        B.qn_d = c->q_d[nxt];
+        B.qn_t = c->q_t[nxt];
+        BSTEP(K_TRACE, launch_btrace(c->stream, tb, count, S, B, c->q_o[cur],
+                                     c->q_d[cur], c->hit, c->d_ctr, c->d_stats));
+        BSTEP(K_SHADE, launch_bshade(c->stream, gb, fp, B, c->d_ctr));
+        if (b->has_lights) {
+            BSTEP(K_SHADOW, launch_bshadow(c->stream, tb, count, S, B, c->s_o,
+                                           c->s_d, c->vis, c->d_ctr, c->d_stats,
+                                           c->cap_shadow));
+            BSTEP(K_RESOLVE, launch_bresolve(c->stream, gb, fp, B, c->d_ctr));
+        }
+        BSTEP(K_SHADE, launch_advance(c->stream, c->d_ctr, c->d_stats, c->cap_paths));
+    }
+    BSTEP(K_ACCUM, launch_baccumulate(c->stream, B, b->d_ctl, b->d_accum, P,
+                                      c->d_ctr, c->d_stats));
+    BSTEP(K_ACCUM, launch_bnext(c->stream, b->d_ctl, P));
+#undef BSTEP
+    return hipSuccess;
+}
+
+// Renders global paths [0, total) of the batch in passes of P.
+static int batch_run(mrt_batch *b, uint32_t depth, uint64_t total)
+{
+    mrt_ctx *c = b->c;
+    uint32_t P = (uint32_t)std::min<uint64_t>(b->tune.path_budget, total);
+    if (P == 0)
+        return MRT_OK;
+    int rc = batch_buffers(b, P);
+    if (rc != MRT_OK)
+        return rc;
+    BatchCtl ctl{0, total, (uint32_t)b->hbs.size(), 0};
+    HIPCHK(c, hipMemcpyAsync(b->d_ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice,
+                             c->stream));
+    const uint64_t passes = (total + P - 1) / P;
+    const bool count = (c->stats_on & MRT_STATS_COUNT) != 0;
+    const bool timing = (c->stats_on & MRT_STATS_TIME) != 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timing) {
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        (void)hipEventRecord(t0, c->stream);
+    }
+    if (b->tune.use_graph && !count && !timing) {
+        const uint32_t key[5] = {depth, P, b->tune.trace_blocks_per_cu,
+                                 b->tune.shade_blocks_per_cu, c->cap_shadow};
+        if (!b->gexec || memcmp(key, b->g_key, sizeof(key)) != 0) {
+            if (b->gexec)
+                (void)hipGraphExecDestroy(b->gexec);
+            if (b->graph)
+                (void)hipGraphDestroy(b->graph);
+            b->gexec = nullptr;
+            b->graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = batch_pass(b, depth, P, false, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                return fail(c, MRT_E_HIP, "pass capture failed: %s",
+                            hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            b->graph = g;
+            HIPCHK(c, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+            memcpy(b->g_key, key, sizeof(key));
+        }
+        for (uint64_t k = 0; k < passes; k++)
+            HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
+    } else {
+        Timer tm{c, timing, {}};
+        for (uint64_t k = 0; k < passes; k++) {
+            hipError_t e = batch_pass(b, depth, P, count, &tm);
+            if (e != hipSuccess) {
+                tm.collect();
+                return fail(c, MRT_E_HIP, "kernel launch failed: %s",
+                            hipGetErrorString(e));
+            }
+        }
+        tm.collect();
+    }
+    c->paths += total;
+    if (timing) {
+        (void)hipEventRecord(t1, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess)
+            c->ms[5] += ms;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+    }
+    return MRT_OK;
+}
+
+extern "C" int mrt_batch_render(mrt_batch *b, uint32_t max_depth)
+{
+    if (!b)
+        return MRT_E_INVALID;
+    mrt_ctx *c = b->c;
+    if (!b->committed) {
+        int rc = mrt_batch_commit(b);
+        if (rc != MRT_OK)
+            return rc;
+    }
+    if (max_depth > 4095)
+        return fail(c, MRT_E_LIMIT, "max_depth above 4095");
+    if (max_depth == 0)
+        return MRT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    return batch_run(b, max_depth, b->total);
+}
+
+extern "C" int mrt_batch_readback(mrt_batch *b, uint32_t id, float *out)
+{
+    if (!b || !out)
+        return MRT_E_INVALID;
+    mrt_ctx *c = b->c;
+    if (!b->committed || id >= b->hbs.size())
+        return fail(c, MRT_E_INVALID, "no committed scene %u", id);
+    const BatchScene &T = b->hbs[id];
+    HIPCHK(c, hipMemcpyAsync(out, b->d_accum + T.accum0,
+                             (size_t)T.fp.WH * 3 * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_batch_clear(mrt_batch *b)
+{
+    if (!b)
+        return MRT_E_INVALID;
+    mrt_ctx *c = b->c;
+    if (b->committed)
+        HIPCHK(c, hipMemsetAsync(b->d_accum, 0, b->accum_n * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+static int check_tuning(mrt_batch *b, const mrt_batch_tuning *t)
+{
+    if (t->path_budget < 1024 || t->path_budget > (1u << 26))
+        return fail(b->c, MRT_E_LIMIT, "path budget must be in 2^10..2^26");
+    if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > 32
+        || t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 32)
+        return fail(b->c, MRT_E_LIMIT, "blocks per CU must be in 1..32");
+    return MRT_OK;
+}
+
+extern "C" int mrt_batch_set_tuning(mrt_batch *b, const mrt_batch_tuning *t)
+{
+    if (!b || !t)
+        return MRT_E_INVALID;
+    int rc = check_tuning(b, t);
+    if (rc != MRT_OK)
+        return rc;
+    b->tune = *t;
+    return MRT_OK;
+}
+
+extern "C" int mrt_batch_get_tuning(const mrt_batch *b, mrt_batch_tuning *t)
+{
+    if (!b || !t)
+        return MRT_E_INVALID;
+    *t = b->tune;
+    return MRT_OK;
+}
+
+// Times the first `trial` paths of the batch under each candidate shape and
+// keeps the fastest; the image sums are cleared afterwards.
+extern "C" int mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
+                                  mrt_batch_tuning *chosen)
+{
+    if (!b)
+        return MRT_E_INVALID;
+    mrt_ctx *c = b->c;
+    if (!b->committed) {
+        int rc = mrt_batch_commit(b);
+        if (rc != MRT_OK)
+            return rc;
+    }
+    if (max_depth == 0 || max_depth > 4095)
+        return fail(c, MRT_E_LIMIT, "max_depth must be in 1..4095");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int saved_stats = c->stats_on;
+    c->stats_on = 0;
+    const uint32_t occ = std::max(1u, c->trace_blocks / (uint32_t)c->num_cu);
+    const uint64_t trial = std::min<uint64_t>(b->total, 1ull << 24);
+    std::vector<mrt_batch_tuning> cand;
+    for (uint32_t pb : {1u << 20, 1u << 21, 1u << 22, 1u << 23})
+        for (uint32_t tbc : {occ, std::max(1u, occ / 2)})
+            for (uint32_t sbc : {4u, 8u}) {
+                if (pb > 2 * trial && pb != (1u << 20))
+                    continue;
+                mrt_batch_tuning t{pb, tbc, sbc, 1, 0.0};
+                cand.push_back(t);
+            }
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    mrt_batch_tuning best = b->tune;
+    double best_ms = 1e300;
+    int rc = MRT_OK;
+    for (mrt_batch_tuning &t : cand) {
+        b->tune = t;
+        rc = batch_run(b, max_depth, trial);          // warm-up and capture
+        if (rc != MRT_OK)
+            break;
+        (void)hipEventRecord(e0, c->stream);
+        rc = batch_run(b, max_depth, trial);
+        (void)hipEventRecord(e1, c->stream);
+        if (rc != MRT_OK)
+            break;
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        t.ms_per_mpath = ms / ((double)trial / 1e6);
+        if (t.ms_per_mpath < best_ms) {
+            best_ms = t.ms_per_mpath;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->stats_on = saved_stats;
+    b->tune = best;
+    c->paths = 0;
+    if (rc != MRT_OK)
+        return rc;
+    rc = mrt_batch_clear(b);
+    if (rc == MRT_OK)
+        rc = mrt_reset_stats(c);
+    if (chosen)
+        *chosen = best;
+    return rc;
+}
+
+extern "C" int mrt_batch_set_option(mrt_batch *b, int option, double value)
+{
+    if (!b)
+        return MRT_E_INVALID;
+    switch (option) {
+    case MRT_OPT_STATS:
+        return mrt_set_option(b->c, option, value);
+    case MRT_OPT_PATH_BUDGET: {
+        mrt_batch_tuning t = b->tune;
+        if (!(value >= 1024.0) || value > (double)(1u << 26))
+            return fail(b->c, MRT_E_LIMIT, "path budget must be in 2^10..2^26");
+        t.path_budget = (uint32_t)value;
+        return mrt_batch_set_tuning(b, &t);
+    }
+    default:
+        return fail(b->c, MRT_E_UNSUPPORTED, "option %d does not apply to batches",
+                    option);
+    }
+}
+
+extern "C" int mrt_batch_get_stats(mrt_batch *b, mrt_stats *out)
+{
+    if (!b)
+        return MRT_E_INVALID;
+    return mrt_get_stats(b->c, out);
 }

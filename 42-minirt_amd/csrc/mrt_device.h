@@ -191,9 +191,10 @@ struct SceneDev {
     const float4 *nodes;      // 4 per node
     const float4 *prims;      // 3 per primitive (leaf order)
     const float4 *planes;     // 2 per plane: point | mat, normal | 0
-    uint32_t      n_nodes;
+    uint32_t      n_nodes;    // 0: no BVH
     uint32_t      n_planes;
     uint32_t      one_sided;
+    uint32_t      root;       // root node (scenes of a batch share one array)
 };
 
 // Ordered traversal: near child first, far child on the stack with its
@@ -216,7 +217,7 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
     uint32_t ovf_code[STACK_MAX - SS];
     float ovf_t[STACK_MAX - SS];
     int sp = 0;
-    int32_t code = 0;
+    int32_t code = (int32_t)S.root;
     for (;;) {
         if (code >= 0) {
             const float4 *n = S.nodes + (size_t)code * 4;

@@ -21,6 +21,7 @@ enum : int {
     CTR_NEXT = 1,         // paths appended to the next queue
     CTR_SHADOW = 2,       // shadow rays appended this bounce
     CTR_PEND = 3,         // pending light sums this bounce
+    CTR_PASS = 4,         // paths of the current pass (batches)
     CTR_WORK_TRACE = 16,  // 8 work cursors of k_trace (one per XCD part)
     CTR_WORK_SHADOW = 24, // 8 work cursors of k_shadow
     CTR_COUNT = 32
@@ -97,6 +98,27 @@ struct FrameParams {
     uint32_t estimator;       // MRT_EST_*
     PhotonMapDev pm;          // caustic photon map, present when has_photons
     uint32_t has_photons;
+    uint32_t light0, plane0;  // first light / plane of the scene in the
+                              // shared arrays (0 for a single scene)
+};
+
+// One scene of a batch: its frame, where its BVH and planes sit in the
+// shared arrays, and its share of the global path index space
+// (W*H*spp paths, sample-major).
+struct BatchScene {
+    FrameParams fp;
+    uint32_t    root, n_nodes, n_planes;
+    uint32_t    accum0;       // first float of its W*H*3 sums
+    unsigned long long path0; // first global path index
+    unsigned long long paths; // W*H*spp
+};
+
+// Device-side cursor of a batch: the first global path of the current
+// wavefront pass and the total.  Kernels read it, so one captured graph
+// of a pass replays without any host-side argument change.
+struct BatchCtl {
+    unsigned long long base, total;
+    uint32_t           n_scenes, pad;
 };
 
 constexpr uint32_t TAG_ENV = 0x40000000u;   // shadow-ray tag of the HDRI ray
@@ -117,6 +139,9 @@ struct ShadeBufs {
     float4       *p_k;         // caustic term of each pending record
     uint32_t      cap_paths;   // capacity of the path / pending arrays
     uint32_t      cap_shadow;  // capacity of the shadow queue
+    // batches only: the scene table and (scene, local path index) per slot
+    const BatchScene *bs;
+    uint2            *slot_info;
 };
 
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
@@ -148,6 +173,30 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                              const float *orig, const float *dir,
                              uint32_t *prim_id, float *t);
 int trace_blocks_per_cu(bool shadow);
+
+// batched scenes (the same stages over the scene table of a batch)
+hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
+                          const BatchCtl *ctl, float4 *q_o, float4 *q_d,
+                          float4 *q_t, uint32_t P, uint32_t *ctr);
+hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
+                         const SceneDev &S, const ShadeBufs &B,
+                         const float4 *q_o, const float4 *q_d, float4 *hit,
+                         uint32_t *ctr, unsigned long long *stats);
+hipError_t launch_bshadow(hipStream_t st, uint32_t blocks, bool count,
+                          const SceneDev &S, const ShadeBufs &B,
+                          const float4 *s_o, const float4 *s_d, uint8_t *vis,
+                          uint32_t *ctr, unsigned long long *stats,
+                          uint32_t cap);
+hipError_t launch_bshade(hipStream_t st, uint32_t blocks,
+                         const FrameParams &fp, const ShadeBufs &B,
+                         uint32_t *ctr);
+hipError_t launch_bresolve(hipStream_t st, uint32_t blocks,
+                           const FrameParams &fp, const ShadeBufs &B,
+                           uint32_t *ctr);
+hipError_t launch_baccumulate(hipStream_t st, const ShadeBufs &B,
+                              const BatchCtl *ctl, float *accum, uint32_t P,
+                              uint32_t *ctr, unsigned long long *stats);
+hipError_t launch_bnext(hipStream_t st, BatchCtl *ctl, uint32_t P);
 
 hipError_t launch_photon_emit(hipStream_t st, const FrameParams &fp,
                               const SceneDev &S, const PhotonEmit &pe,

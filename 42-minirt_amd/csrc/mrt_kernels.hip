@@ -193,12 +193,33 @@ k_raygen(FrameParams fp, float4 *__restrict__ q_o, float4 *__restrict__ q_d,
     acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// The scene of a path of a batch: shared node / primitive arrays, its own
+// root, planes and flags.
+__device__ __forceinline__ SceneDev scene_of(const SceneDev &S,
+                                             const BatchScene *bs, uint32_t sid)
+{
+    const BatchScene &b = bs[sid];
+    SceneDev r = S;
+    r.root = b.root;
+    r.n_nodes = b.n_nodes;
+    r.planes = S.planes + 2 * (size_t)b.fp.plane0;
+    r.n_planes = b.n_planes;
+    r.one_sided = b.fp.one_sided;
+    return r;
+}
+
+// Shadow-ray tag: scene of a batch (bits 18..29), light (12..17), ray of a
+// soft light (0..11); TAG_ENV marks the environment ray.
+__device__ __forceinline__ uint32_t tag_scene(uint32_t tag) { return (tag >> 18) & 0xFFFu; }
+__device__ __forceinline__ uint32_t tag_light(uint32_t tag) { return (tag >> 12) & 63u; }
+
 // ---- closest-hit trace (persistent, wave-granular work stealing) ----------
-template <bool COUNT>
+template <bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_trace(SceneDev S, const float4 *__restrict__ q_o,
         const float4 *__restrict__ q_d, float4 *__restrict__ hit,
-        uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats)
+        uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats,
+        const BatchScene *__restrict__ bs, const uint2 *__restrict__ slot_info)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
     __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
@@ -214,9 +235,17 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
                 float4 o4 = q_o[i], d4 = q_d[i];
                 float t, bu, bv;
                 uint32_t code;
-                closest_hit<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), lc, lt,
-                                             TRACE_BLOCK, t, code, bu, bv,
-                                             nodes, prims);
+                if (BATCH) {
+                    const SceneDev Si =
+                        scene_of(S, bs, slot_info[__float_as_uint(d4.w)].x);
+                    closest_hit<COUNT, TRACE_SS>(Si, xyz(o4), xyz(d4), lc, lt,
+                                                 TRACE_BLOCK, t, code, bu, bv,
+                                                 nodes, prims);
+                } else {
+                    closest_hit<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), lc, lt,
+                                                 TRACE_BLOCK, t, code, bu, bv,
+                                                 nodes, prims);
+                }
                 hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
             }
         }
@@ -228,12 +257,12 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
 }
 
 // ---- shadow / visibility trace --------------------------------------------
-template <bool COUNT>
+template <bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_shadow(SceneDev S, const float4 *__restrict__ s_o,
          const float4 *__restrict__ s_d, uint8_t *__restrict__ vis,
          uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats,
-         uint32_t cap)
+         uint32_t cap, const BatchScene *__restrict__ bs)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
     __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
@@ -247,9 +276,18 @@ k_shadow(SceneDev S, const float4 *__restrict__ s_o,
             uint32_t i = base + lane_id();
             if (i < ce) {
                 float4 o4 = s_o[i], d4 = s_d[i];
-                bool occ = occluded<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), o4.w,
-                                                     lc, lt, TRACE_BLOCK, nodes,
-                                                     prims);
+                bool occ;
+                if (BATCH) {
+                    const SceneDev Si =
+                        scene_of(S, bs, tag_scene(__float_as_uint(d4.w)));
+                    occ = occluded<COUNT, TRACE_SS>(Si, xyz(o4), xyz(d4), o4.w,
+                                                    lc, lt, TRACE_BLOCK, nodes,
+                                                    prims);
+                } else {
+                    occ = occluded<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), o4.w,
+                                                    lc, lt, TRACE_BLOCK, nodes,
+                                                    prims);
+                }
                 vis[i] = occ ? 0 : 1;
             }
         }
@@ -500,6 +538,7 @@ __device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
     return pc * (float)nw * (float)nh / (19.7392088021787172f * cose);
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(SHADE_BLOCK)
 k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
@@ -516,11 +555,21 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         bool env_ray = false;
         V3 env_d = v3(0.f, 0.f, 0.f), env_rad = v3(0.f, 0.f, 0.f);
         float env_omc = 1.0f;
+        uint32_t sid = 0;
+        const FrameParams *Fp = &fp;
         if (active) {
             float4 o4 = B.q_o[i], d4 = B.q_d[i], t4 = B.q_t[i], h4 = B.hit[i];
             slot = __float_as_uint(d4.w);
-            pixel = slot % fp.WH;
-            sample = fp.sample0 + slot / fp.WH;
+            uint32_t local = slot;
+            if (BATCH) {
+                const uint2 si = B.slot_info[slot];
+                sid = si.x;
+                local = si.y;
+                Fp = &B.bs[sid].fp;
+            }
+            const FrameParams &F = *Fp;
+            pixel = local % F.WH;
+            sample = F.sample0 + local / F.WH;
             d = xyz(d4);
             thr = xyz(t4);
             uint32_t code = __float_as_uint(h4.y);
@@ -528,11 +577,11 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 // add_skybox (src/ray_tracing/path_trace.c:37-48)
                 float4 a = B.acc[slot];
                 float ao_last = o4.w;
-                V3 bg = v3(fp.bg_clamped[0], fp.bg_clamped[1], fp.bg_clamped[2]);
+                V3 bg = v3(F.bg_clamped[0], F.bg_clamped[1], F.bg_clamped[2]);
                 float wmis = 1.0f;
-                if (fp.has_env) {
+                if (F.has_env) {
                     bg = env_texel(fp, d);
-                    if (fp.estimator == MRT_EST_REFERENCE) {
+                    if (F.estimator == MRT_EST_REFERENCE) {
                         bg = v3(fminf(fmaxf(bg.x, 0.0f), 16.0f),
                                 fminf(fmaxf(bg.y, 0.0f), 16.0f),
                                 fminf(fmaxf(bg.z, 0.0f), 16.0f));
@@ -550,7 +599,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             } else {
                 uint32_t mat;
                 bool front;
-                surface(B.planes, B.prims, B.nrm, fp.one_sided != 0, xyz(o4), d,
+                surface(B.planes + 2 * (size_t)F.plane0, B.prims, B.nrm, F.one_sided != 0, xyz(o4), d,
                         h4, p, nrm, front, mat);
                 Mat m;
                 m.a = B.mats[3 * mat];
@@ -565,17 +614,17 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     a.z = a.z + (m.c.w * m.c.x) * thr.z;
                     B.acc[slot] = a;
                 }
-                const RngKey rk{fp.seed, pixel, sample, fp.bounce, 0u};
-                bool diffuse = scatter(rk, fp.ior_all, nrm, front, m, d, thr);
+                const RngKey rk{F.seed, pixel, sample, fp.bounce, 0u};
+                bool diffuse = scatter(rk, F.ior_all, nrm, front, m, d, thr);
                 if (diffuse) {
                     // default_mat (src/material/material_default.c:71-92)
                     kind = SC_DIFFUSE;
                     V3 col = v3(m.a.x, m.a.y, m.a.z);
                     cp = had(col, thr);
                     thr = had(thr, col);
-                    for (uint32_t k = 0; k < fp.n_lights; k++) {
-                        float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1],
-                               l2 = B.lights[3 * k + 2];
+                    const float4 *L = B.lights + 3 * (size_t)F.light0;
+                    for (uint32_t k = 0; k < F.n_lights; k++) {
+                        float4 l0 = L[3 * k], l1 = L[3 * k + 1], l2 = L[3 * k + 2];
                         uint32_t ltype = __float_as_uint(l2.x);
                         int32_t lss = __float_as_int(l2.y);
                         V3 ld = ltype == MRT_LIGHT_POINT ? unit(sub(xyz(l0), p))
@@ -585,15 +634,15 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                         bool soft = ltype == MRT_LIGHT_POINT && l1.w != 0.0f && lss > 1;
                         n_rays += soft ? (uint32_t)lss : 1u;
                     }
-                    if (fp.has_env) {
-                        U4 e = rng4(fp.seed, pixel, sample, fp.bounce, 3u);
-                        if (fp.estimator == MRT_EST_REFERENCE) {
+                    if (F.has_env) {
+                        U4 e = rng4(F.seed, pixel, sample, fp.bounce, 3u);
+                        if (F.estimator == MRT_EST_REFERENCE) {
                             // importance_sampling (material_default.c:41-69)
-                            const uint32_t ew = fp.env_w, eh = fp.env_h;
-                            double vy = fp.env_marg_inv[(int)((double)e.x * (eh - 1))];
-                            double vx = fp.env_cond_inv[(int)(vy * (eh - 1) * ew
+                            const uint32_t ew = F.env_w, eh = F.env_h;
+                            double vy = F.env_marg_inv[(int)((double)e.x * (eh - 1))];
+                            double vx = F.env_cond_inv[(int)(vy * (eh - 1) * ew
                                                               + (double)e.y * (ew - 1))];
-                            float pdf = fp.env_pdf[(int)(vy * (eh - 1) * ew
+                            float pdf = F.env_pdf[(int)(vy * (eh - 1) * ew
                                                          + vx * (ew - 1))];
                             if (pdf > 0.0f) {
                                 float th = (float)((vy - 0.5) * PI_D);
@@ -609,11 +658,11 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                             }
                         } else {
                             // light-sample half of the MIS pair
-                            const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
-                            uint32_t y = cdf_search(fp.mis_marg, nh, e.x);
-                            uint32_t x = cdf_search(fp.mis_cond + (size_t)y * fp.env_w,
+                            const uint32_t nw = F.env_w - 1, nh = F.env_h - 1;
+                            uint32_t y = cdf_search(F.mis_marg, nh, e.x);
+                            uint32_t x = cdf_search(F.mis_cond + (size_t)y * F.env_w,
                                                     nw, e.y);
-                            float pc = fp.mis_pcell[(size_t)y * nw + x];
+                            float pc = F.mis_pcell[(size_t)y * nw + x];
                             float vv = ((float)y + e.z) / (float)nh;
                             float uu = ((float)x + e.w) / (float)nw;
                             float el = (vv - 0.5f) * 3.14159265358979324f;
@@ -633,10 +682,10 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                         }
                         n_rays += env_ray ? 1u : 0u;
                     }
-                    U4 g = rng4(fp.seed, pixel, sample, fp.bounce, 2u);
+                    U4 g = rng4(F.seed, pixel, sample, fp.bounce, 2u);
                     d = cos_hemi(nrm, g.z, g.w);
                     pb_next = dot(nrm, d) / 3.14159265358979324f;
-                    want_pend = (n_rays || fp.has_photons) ? 1u : 0u;
+                    want_pend = (n_rays || F.has_photons) ? 1u : 0u;
                 }
                 want_next = fp.last_bounce ? 0u : 1u;
             }
@@ -647,6 +696,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                      want_next, n_rays, want_pend, qn, j0, pr);
         if (!active)
             continue;
+        const FrameParams &F = *Fp;
         // the host sizes every queue for the worst case; never write past it
         if (qn >= B.cap_paths)
             want_next = 0;
@@ -658,11 +708,11 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             B.qn_t[qn] = make_float4(thr.x, thr.y, thr.z, pb_next);
         }
         if (kind == SC_DIFFUSE && n_rays) {
-            V3 view = unit(sub(v3(fp.cam_pos[0], fp.cam_pos[1], fp.cam_pos[2]), p));
+            V3 view = unit(sub(v3(F.cam_pos[0], F.cam_pos[1], F.cam_pos[2]), p));
             uint32_t j = j0;
-            for (uint32_t k = 0; k < fp.n_lights; k++) {
-                float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1],
-                       l2 = B.lights[3 * k + 2];
+            const float4 *L = B.lights + 3 * (size_t)F.light0;
+            for (uint32_t k = 0; k < F.n_lights; k++) {
+                float4 l0 = L[3 * k], l1 = L[3 * k + 1], l2 = L[3 * k + 2];
                 uint32_t ltype = __float_as_uint(l2.x);
                 int32_t lss = __float_as_int(l2.y);
                 bool point = ltype == MRT_LIGHT_POINT;
@@ -688,7 +738,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     } else {
                         V3 tgt = lpos;
                         if (soft) {
-                            U4 u = rng4(fp.seed, pixel, sample, fp.bounce,
+                            U4 u = rng4(F.seed, pixel, sample, fp.bounce,
                                         16u + k * 4096u + s);
                             tgt = add(lpos, mul(unit_sphere_pt(u), l1.w));
                         }
@@ -698,7 +748,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     }
                     B.s_o[j] = make_float4(p.x, p.y, p.z, tmax);
                     B.s_d[j] = make_float4(sd.x, sd.y, sd.z,
-                                           __uint_as_float((k << 12) | s));
+                                           __uint_as_float((sid << 18) | (k << 12) | s));
                     B.s_c[j] = make_float2(ndl, sp64);
                 }
             }
@@ -714,7 +764,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             B.p_b[pr] = make_uint4(want_next ? qn : HIT_NONE, j0, n_rays,
                                    env_ray ? 1u : 0u);
             B.p_c[pr] = make_float4(env_rad.x, env_rad.y, env_rad.z, env_omc);
-            if (fp.has_photons)
+            if (F.has_photons)
                 B.p_x[pr] = make_float4(p.x, p.y, p.z, 0.0f);
         }
     }
@@ -728,25 +778,27 @@ __device__ __forceinline__ float lq(bool exact, float acc, float add)
     return exact ? s : floorf(s);
 }
 
+template <bool BATCH>
 __global__ void __launch_bounds__(256)
 k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
     const uint32_t n = min(ctr[CTR_PEND], B.cap_paths);
-    const bool exact = (fp.flags & MRT_SCENE_LIGHT_EXACT) != 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
         float4 a = B.p_a[r];
         uint4 b = B.p_b[r];
         uint32_t slot = __float_as_uint(a.w);
+        const FrameParams &F = BATCH ? B.bs[B.slot_info[slot].x].fp : fp;
+        const bool exact = (F.flags & MRT_SCENE_LIGHT_EXACT) != 0;
+        const float4 *L = B.lights + 3 * (size_t)F.light0;
         uint32_t j = b.y, end = b.y + b.z;
         float lr = 0.0f, lg = 0.0f, lb = 0.0f;
         if (b.w)
             end--;                      // the environment ray comes last
         while (j < end) {
             uint32_t tag = __float_as_uint(B.s_d[j].w);
-            uint32_t k = tag >> 12;
-            float4 l0 = B.lights[3 * k], l1 = B.lights[3 * k + 1],
-                   l2 = B.lights[3 * k + 2];
+            uint32_t k = tag_light(tag);
+            float4 l0 = L[3 * k], l1 = L[3 * k + 1], l2 = L[3 * k + 2];
             uint32_t ltype = __float_as_uint(l2.x);
             int32_t lss = __float_as_int(l2.y);
             bool point = ltype == MRT_LIGHT_POINT;
@@ -784,7 +836,7 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         float4 s = B.acc[slot];
         // default_mat: total lighting = direct + caustics
         V3 dl = v3(lr / 255.0f, lg / 255.0f, lb / 255.0f);
-        if (fp.has_photons)
+        if (F.has_photons)
             dl = add(dl, xyz(B.p_k[r]));
         s.x = s.x + a.x * dl.x;
         s.y = s.y + a.y * dl.y;
@@ -794,7 +846,7 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             s.x = s.x + c.x;
             s.y = s.y + c.y;
             s.z = s.z + c.z;
-            if (fp.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
+            if (F.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
                 float4 t = B.qn_t[b.x];
                 t.x = t.x * c.w;
                 t.y = t.y * c.w;
@@ -847,6 +899,112 @@ k_accumulate(const float4 *__restrict__ acc, float *__restrict__ accum,
         accum[(size_t)px * 3 + 2] += b;
     }
     wave_add_u64(&stats[STAT_NONFINITE], bad);
+}
+
+// ---- batches: many scenes in one wavefront -----------------------------------
+// The global path index space is the scenes' W*H*spp ranges one after the
+// other (sample-major inside a scene); a pass takes P consecutive indices.
+
+// Scene holding global path g: the last scene with path0 <= g.
+__device__ __forceinline__ uint32_t scene_at(const BatchScene *bs, uint32_t ns,
+                                             unsigned long long g)
+{
+    uint32_t lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (bs[mid].path0 <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_braygen(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
+          float4 *__restrict__ q_o, float4 *__restrict__ q_d,
+          float4 *__restrict__ q_t, float4 *__restrict__ acc,
+          uint2 *__restrict__ slot_info, uint32_t P,
+          uint32_t *__restrict__ ctr)
+{
+    const unsigned long long base = ctl->base, total = ctl->total;
+    const uint32_t n = base >= total ? 0u
+                       : (uint32_t)min((unsigned long long)P, total - base);
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ctr[CTR_CUR] = n;
+        ctr[CTR_PASS] = n;
+        ctr[CTR_NEXT] = 0;
+        ctr[CTR_SHADOW] = 0;
+        ctr[CTR_PEND] = 0;
+        for (int k = 0; k < NPART; k++) {
+            ctr[CTR_WORK_TRACE + k] = 0;
+            ctr[CTR_WORK_SHADOW + k] = 0;
+        }
+    }
+    if (i >= n)
+        return;
+    const unsigned long long g = base + i;
+    const uint32_t sid = scene_at(bs, ctl->n_scenes, g);
+    const FrameParams &F = bs[sid].fp;
+    const uint32_t local = (uint32_t)(g - bs[sid].path0);
+    V3 o, d;
+    primary_ray(F, local % F.WH, F.sample0 + local / F.WH, o, d);
+    q_o[i] = make_float4(o.x, o.y, o.z, 1.0f);
+    q_d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
+    q_t[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    slot_info[i] = make_uint2(sid, local);
+}
+
+// Per (scene, pixel) of the pass, in sample order: the thread of a pixel's
+// first slot in the pass adds that pixel's samples of the pass (slots
+// i, i + W*H, ... inside the scene's part of the pass).
+__global__ void __launch_bounds__(256)
+k_baccumulate(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
+              const float4 *__restrict__ acc, const uint2 *__restrict__ slot_info,
+              float *__restrict__ accum, const uint32_t *__restrict__ ctr,
+              unsigned long long *__restrict__ stats)
+{
+    const uint32_t n = ctr[CTR_PASS];
+    const unsigned long long base = ctl->base;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bad = 0;
+    if (i < n) {
+        const uint2 si = slot_info[i];
+        const BatchScene &b = bs[si.x];
+        const uint32_t WH = b.fp.WH;
+        // the scene's part of this pass, in local indices: [l0, l1)
+        const unsigned long long g0 = base > b.path0 ? base : b.path0;
+        const uint32_t l0 = (uint32_t)(g0 - b.path0);
+        const unsigned long long pend = base + n, send = b.path0 + b.paths;
+        const uint32_t l1 = (uint32_t)((pend < send ? pend : send) - b.path0);
+        if (si.y - l0 < WH) {
+            float r = 0.0f, g = 0.0f, bl = 0.0f;
+            uint32_t slot = i;
+            for (uint32_t l = si.y; l < l1; l += WH, slot += WH) {
+                float4 v = acc[slot];
+                if (isfinite(v.x) && isfinite(v.y) && isfinite(v.z)) {
+                    r += v.x;
+                    g += v.y;
+                    bl += v.z;
+                } else {
+                    bad++;
+                }
+            }
+            float *dst = accum + b.accum0 + (size_t)(si.y % WH) * 3;
+            dst[0] += r;
+            dst[1] += g;
+            dst[2] += bl;
+        }
+    }
+    wave_add_u64(&stats[STAT_NONFINITE], bad);
+}
+
+__global__ void k_bnext(BatchCtl *__restrict__ ctl, uint32_t P)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ctl->base += P;
 }
 
 // ---- primary-ray AOVs and explicit rays ------------------------------------
@@ -1298,12 +1456,14 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
                         const float4 *q_d, float4 *hit, uint32_t *ctr,
                         unsigned long long *stats)
 {
+    const BatchScene *nb = nullptr;
+    const uint2 *ni = nullptr;
     if (count)
-        hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(TRACE_BLOCK), 0,
-                           st, S, q_o, q_d, hit, ctr, stats);
+        hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(TRACE_BLOCK),
+                           0, st, S, q_o, q_d, hit, ctr, stats, nb, ni);
     else
-        hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(TRACE_BLOCK), 0,
-                           st, S, q_o, q_d, hit, ctr, stats);
+        hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(TRACE_BLOCK),
+                           0, st, S, q_o, q_d, hit, ctr, stats, nb, ni);
     return hipGetLastError();
 }
 
@@ -1312,20 +1472,22 @@ hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
                          const float4 *s_d, uint8_t *vis, uint32_t *ctr,
                          unsigned long long *stats, uint32_t cap)
 {
+    const BatchScene *nb = nullptr;
     if (count)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(blocks), dim3(TRACE_BLOCK), 0,
-                           st, S, s_o, s_d, vis, ctr, stats, cap);
+        hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(TRACE_BLOCK),
+                           0, st, S, s_o, s_d, vis, ctr, stats, cap, nb);
     else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(blocks), dim3(TRACE_BLOCK), 0,
-                           st, S, s_o, s_d, vis, ctr, stats, cap);
+        hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks),
+                           dim3(TRACE_BLOCK), 0, st, S, s_o, s_d, vis, ctr, stats,
+                           cap, nb);
     return hipGetLastError();
 }
 
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp, B,
-                       ctr);
+    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
+                       B, ctr);
     return hipGetLastError();
 }
 
@@ -1333,7 +1495,83 @@ hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
                           uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, st, fp, B, ctr);
+    hipLaunchKernelGGL(k_resolve<false>, dim3(blocks), dim3(256), 0, st, fp, B,
+                       ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
+                          const BatchCtl *ctl, float4 *q_o, float4 *q_d,
+                          float4 *q_t, uint32_t P, uint32_t *ctr)
+{
+    hipLaunchKernelGGL(k_braygen, dim3((P + 255) / 256), dim3(256), 0, st, B.bs,
+                       ctl, q_o, q_d, q_t, B.acc, B.slot_info, P, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
+                         const SceneDev &S, const ShadeBufs &B,
+                         const float4 *q_o, const float4 *q_d, float4 *hit,
+                         uint32_t *ctr, unsigned long long *stats)
+{
+    if (count)
+        hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(TRACE_BLOCK),
+                           0, st, S, q_o, q_d, hit, ctr, stats, B.bs,
+                           (const uint2 *)B.slot_info);
+    else
+        hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(TRACE_BLOCK),
+                           0, st, S, q_o, q_d, hit, ctr, stats, B.bs,
+                           (const uint2 *)B.slot_info);
+    return hipGetLastError();
+}
+
+hipError_t launch_bshadow(hipStream_t st, uint32_t blocks, bool count,
+                          const SceneDev &S, const ShadeBufs &B,
+                          const float4 *s_o, const float4 *s_d, uint8_t *vis,
+                          uint32_t *ctr, unsigned long long *stats,
+                          uint32_t cap)
+{
+    if (count)
+        hipLaunchKernelGGL((k_shadow<true, true>), dim3(blocks), dim3(TRACE_BLOCK),
+                           0, st, S, s_o, s_d, vis, ctr, stats, cap, B.bs);
+    else
+        hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocks),
+                           dim3(TRACE_BLOCK), 0, st, S, s_o, s_d, vis, ctr, stats,
+                           cap, B.bs);
+    return hipGetLastError();
+}
+
+hipError_t launch_bshade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
+                         const ShadeBufs &B, uint32_t *ctr)
+{
+    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
+                       B, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_bresolve(hipStream_t st, uint32_t blocks,
+                           const FrameParams &fp, const ShadeBufs &B,
+                           uint32_t *ctr)
+{
+    hipLaunchKernelGGL(k_resolve<true>, dim3(blocks), dim3(256), 0, st, fp, B,
+                       ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_baccumulate(hipStream_t st, const ShadeBufs &B,
+                              const BatchCtl *ctl, float *accum, uint32_t P,
+                              uint32_t *ctr, unsigned long long *stats)
+{
+    hipLaunchKernelGGL(k_baccumulate, dim3((P + 255) / 256), dim3(256), 0, st,
+                       B.bs, ctl, (const float4 *)B.acc,
+                       (const uint2 *)B.slot_info, accum, (const uint32_t *)ctr,
+                       stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_bnext(hipStream_t st, BatchCtl *ctl, uint32_t P)
+{
+    hipLaunchKernelGGL(k_bnext, dim3(1), dim3(64), 0, st, ctl, P);
     return hipGetLastError();
 }
 
@@ -1378,10 +1616,10 @@ int trace_blocks_per_cu(bool shadow)
 {
     int nb = 0;
     hipError_t e = shadow
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shadow<false>,
-                                                       TRACE_BLOCK, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false>,
-                                                       TRACE_BLOCK, 0);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, k_shadow<false, false>, TRACE_BLOCK, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, k_trace<false, false>, TRACE_BLOCK, 0);
     if (e != hipSuccess || nb < 1)
         nb = 4;
     return nb;

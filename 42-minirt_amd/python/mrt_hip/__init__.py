@@ -124,6 +124,15 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("path_budget", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32),
+                ("ms_per_mpath", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class BvhNode(C.Structure):
     _fields_ = [("lo0", C.c_float * 3), ("hi0", C.c_float * 3),
                 ("lo1", C.c_float * 3), ("hi1", C.c_float * 3),
@@ -191,6 +200,29 @@ mrt_synchronize = _sig("mrt_synchronize", C.c_int, _ctx_p)
 mrt_destroy = _sig("mrt_destroy", None, _ctx_p)
 mrt_last_error = _sig("mrt_last_error", C.c_char_p, _ctx_p)
 mrt_version = _sig("mrt_version", C.c_char_p)
+
+_bt_p = C.c_void_p
+mrt_batch_create = _sig("mrt_batch_create", C.c_int, C.c_int, C.POINTER(_bt_p))
+mrt_batch_add_scene = _sig("mrt_batch_add_scene", C.c_int, _bt_p,
+                           C.POINTER(Scene), C.c_uint32, C.c_uint32,
+                           C.c_uint32, C.c_uint64, _up)
+mrt_batch_commit = _sig("mrt_batch_commit", C.c_int, _bt_p)
+mrt_batch_autotune = _sig("mrt_batch_autotune", C.c_int, _bt_p, C.c_uint32,
+                          C.POINTER(BatchTuning))
+mrt_batch_set_tuning = _sig("mrt_batch_set_tuning", C.c_int, _bt_p,
+                            C.POINTER(BatchTuning))
+mrt_batch_get_tuning = _sig("mrt_batch_get_tuning", C.c_int, _bt_p,
+                            C.POINTER(BatchTuning))
+mrt_batch_render = _sig("mrt_batch_render", C.c_int, _bt_p, C.c_uint32)
+mrt_batch_readback = _sig("mrt_batch_readback", C.c_int, _bt_p, C.c_uint32,
+                          _fp)
+mrt_batch_clear = _sig("mrt_batch_clear", C.c_int, _bt_p)
+mrt_batch_set_option = _sig("mrt_batch_set_option", C.c_int, _bt_p, C.c_int,
+                            C.c_double)
+mrt_batch_get_stats = _sig("mrt_batch_get_stats", C.c_int, _bt_p,
+                           C.POINTER(Stats))
+mrt_batch_destroy = _sig("mrt_batch_destroy", None, _bt_p)
+mrt_batch_last_error = _sig("mrt_batch_last_error", C.c_char_p, _bt_p)
 
 mrt_bvh_build = _sig("mrt_bvh_build", C.c_int, C.POINTER(Scene), C.c_uint32,
                      C.POINTER(Bvh))
@@ -601,6 +633,88 @@ class Renderer:
 
     def reset_stats(self):
         self._ck(mrt_reset_stats(self._ctx))
+
+
+class Batch:
+    """Many scenes rendered through one set of wavefront queues."""
+
+    def __init__(self, device=0):
+        h = _bt_p()
+        rc = mrt_batch_create(device, C.byref(h))
+        if rc < 0:
+            raise MrtError(rc, "mrt_batch_create failed (no gfx950 device?)")
+        self._b = h
+        self.sizes = []
+
+    def close(self):
+        if getattr(self, "_b", None):
+            mrt_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc):
+        if rc < 0:
+            raise MrtError(rc, mrt_batch_last_error(self._b).decode())
+        return rc
+
+    def add(self, scene, width, height, spp, seed):
+        view = scene.view() if isinstance(scene, HostScene) else scene
+        sid = C.c_uint32(0)
+        self._ck(mrt_batch_add_scene(self._b, view, width, height, spp, seed,
+                                     C.byref(sid)))
+        self.sizes.append((width, height))
+        return sid.value
+
+    def commit(self):
+        self._ck(mrt_batch_commit(self._b))
+
+    def autotune(self, max_depth):
+        t = BatchTuning()
+        self._ck(mrt_batch_autotune(self._b, max_depth, C.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = BatchTuning()
+        self._ck(mrt_batch_get_tuning(self._b, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = BatchTuning()
+        self._ck(mrt_batch_get_tuning(self._b, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._ck(mrt_batch_set_tuning(self._b, C.byref(t)))
+
+    def set_option(self, opt, value):
+        self._ck(mrt_batch_set_option(self._b, opt, float(value)))
+
+    def render(self, max_depth):
+        self._ck(mrt_batch_render(self._b, max_depth))
+
+    def readback(self, sid):
+        w, h = self.sizes[sid]
+        out = np.zeros((h, w, 3), np.float32)
+        self._ck(mrt_batch_readback(self._b, sid, out.ctypes.data_as(_fp)))
+        return out
+
+    def clear(self):
+        self._ck(mrt_batch_clear(self._b))
+
+    def stats(self):
+        s = Stats()
+        self._ck(mrt_batch_get_stats(self._b, C.byref(s)))
+        return s.as_dict()
 
 
 def export_ppm(path, float_render, samples, gamma=1.0):

@@ -240,6 +240,50 @@ void mrt_destroy(mrt_ctx *ctx);
 const char *mrt_last_error(const mrt_ctx *ctx);
 const char *mrt_version(void);
 
+/* ---- batches: many independent scenes in one wavefront -----------------
+ *
+ * No counterpart in the reference, which renders one scene per process.
+ * A batch holds scenes of any size and resolution, each with its own
+ * sample count; mrt_batch_render traces all of their paths through one
+ * shared set of queues, so small scenes fill the GPU together.  Paths are
+ * numbered scene after scene (sample-major inside a scene) and every pass
+ * takes the next `path budget` of them; the pass is captured once in a HIP
+ * graph and replayed.  Samples use the same RNG keys as mrt_render with the
+ * scene's seed, so a batched scene equals its plain render up to the order
+ * of the fp32 sums.  Scenes must not use an HDRI or caustics. */
+typedef struct mrt_batch mrt_batch;
+
+/* Queue and launch shape of a batch (mrt_batch_autotune picks them). */
+typedef struct mrt_batch_tuning {
+    uint32_t path_budget;        /* paths per pass                           */
+    uint32_t trace_blocks_per_cu;/* persistent blocks of the trace kernels   */
+    uint32_t shade_blocks_per_cu;/* blocks of the shading kernels            */
+    uint32_t use_graph;          /* replay a captured pass (1) or launch (0) */
+    double   ms_per_mpath;       /* measured by the autotuner, else 0        */
+} mrt_batch_tuning;
+
+int  mrt_batch_create(int device, mrt_batch **out);
+/* Copies the scene; renders it at width x height with `spp` samples per
+ * pixel and RNG key `seed`.  *id receives the scene's index. */
+int  mrt_batch_add_scene(mrt_batch *b, const mrt_scene *scene, uint32_t width,
+                         uint32_t height, uint32_t spp, uint64_t seed,
+                         uint32_t *id);
+/* Builds every BVH (in parallel on the host) and uploads the batch. */
+int  mrt_batch_commit(mrt_batch *b);
+int  mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
+                        mrt_batch_tuning *chosen);
+int  mrt_batch_set_tuning(mrt_batch *b, const mrt_batch_tuning *t);
+int  mrt_batch_get_tuning(const mrt_batch *b, mrt_batch_tuning *t);
+/* Renders every scene's samples (accumulating onto earlier renders). */
+int  mrt_batch_render(mrt_batch *b, uint32_t max_depth);
+/* W*H*3 float sums of scene `id`. */
+int  mrt_batch_readback(mrt_batch *b, uint32_t id, float *out);
+int  mrt_batch_clear(mrt_batch *b);
+int  mrt_batch_set_option(mrt_batch *b, int option, double value);
+int  mrt_batch_get_stats(mrt_batch *b, mrt_stats *out);
+void mrt_batch_destroy(mrt_batch *b);
+const char *mrt_batch_last_error(const mrt_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
