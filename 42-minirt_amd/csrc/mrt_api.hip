@@ -1780,7 +1780,7 @@ extern "C" int mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
     const uint32_t occ = std::max(1u, c->trace_blocks / (uint32_t)c->num_cu);
     const uint64_t trial = std::min<uint64_t>(b->total, 1ull << 24);
     std::vector<mrt_batch_tuning> cand;
-    for (uint32_t pb : {1u << 20, 1u << 21, 1u << 22, 1u << 23})
+    for (uint32_t pb : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24})
         for (uint32_t tbc : {occ, std::max(1u, occ / 2)})
             for (uint32_t sbc : {4u, 8u}) {
                 if (pb > 2 * trial && pb != (1u << 20))

@@ -98,6 +98,120 @@ def measured_traffic():
         return None, None
 
 
+RESOLUTIONS = [(256, 256), (512, 512), (960, 540), (1920, 1080)]
+SPPS = [16, 64, 256]
+
+
+def batched_scenes(seed, n, tri_lo, tri_hi, spp_div):
+    """Scene list of the batched workload: displaced icospheres over a
+    heightfield floor whose triangle count is log-uniform in [tri_lo,
+    tri_hi] (subdivision level from the count, floor cells for the rest),
+    resolution and spp drawn from the fixed sets, all seeded."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        target = float(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        sub = 3
+        while 20 * 4 ** (sub + 1) <= target:
+            sub += 1
+        rest = max(target - 20 * 4 ** sub, 2.0)
+        cells = max(1, int(round((rest / 2) ** 0.5)))
+        w, h = RESOLUTIONS[int(rng.integers(len(RESOLUTIONS)))]
+        spp = max(1, SPPS[int(rng.integers(len(SPPS)))] // spp_div)
+        out.append({"seed": 1000 + k, "subdiv": sub, "floor": cells,
+                    "w": w, "h": h, "spp": spp,
+                    "rng_seed": int(rng.integers(1 << 62))})
+    return out
+
+
+def batched_leg(M, args):
+    """Batched vs sequential plain rendering of the same scenes and samples
+    (the images are equal up to fp32 summation order, so the error is equal
+    too)."""
+    from concurrent.futures import ThreadPoolExecutor
+    desc = batched_scenes(args.batch_seed, args.batch_scenes, 1e4, 2e6,
+                          args.batch_spp_div)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        scenes = list(ex.map(lambda d: M.HostScene.mesh(
+            d["seed"], d["w"], d["h"], subdiv=d["subdiv"],
+            floor_cells=d["floor"], amp=0.15), desc))
+    ntri = [s.view().contents.n_triangles for s in scenes]
+    log(f"batched: {len(scenes)} scenes, {sum(ntri)} triangles generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    samples = sum(d["w"] * d["h"] * d["spp"] for d in desc)
+    t0 = time.perf_counter()
+    b = M.Batch()
+    for hs, d in zip(scenes, desc):
+        b.add(hs, d["w"], d["h"], d["spp"], d["rng_seed"])
+    b.commit()
+    log(f"batched: commit (parallel BVH builds + upload) "
+        f"{time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    tuning = b.autotune(args.depth)
+    log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> {tuning}")
+    b.render(args.depth)                       # warm-up frame (graph capture)
+    b.clear()
+    t0 = time.perf_counter()
+    b.render(args.depth)
+    b.readback(0)                              # waits for the stream
+    t_batch = time.perf_counter() - t0
+    log(f"batched: {samples / t_batch / 1e6:.1f} Msamples/s")
+    b.close()
+    # 64 sequential plain launches: one context per scene, uploaded
+    # beforehand, each rendering all of its samples in one call
+    t0 = time.perf_counter()
+
+    def mk(args_):
+        hs, d = args_
+        r = M.Renderer(d["w"], d["h"])
+        r.upload(hs)
+        r.set_seed(d["rng_seed"])
+        return r
+    with ThreadPoolExecutor(16) as ex:
+        ctxs = list(ex.map(mk, zip(scenes, desc)))
+    log(f"plain: {len(ctxs)} contexts uploaded in "
+        f"{time.perf_counter() - t0:.1f} s")
+    ctxs[0].render(0, 1, args.depth)           # warm-up
+    ctxs[0].synchronize()
+    ctxs[0].clear()
+    t0 = time.perf_counter()
+    for r, d in zip(ctxs, desc):
+        r.render(0, d["spp"], args.depth)
+        r.synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"plain sequential: {samples / t_plain / 1e6:.1f} Msamples/s")
+    # the drop-in's own contract: one sample per render_frame tick
+    # (compute_frame_gpu), every scene in turn, the stream drained per tick
+    for r in ctxs:
+        r.clear()
+    t0 = time.perf_counter()
+    for r, d in zip(ctxs, desc):
+        for smp in range(d["spp"]):
+            r.render(smp, 1, args.depth)
+            r.synchronize()
+    t_tick = time.perf_counter() - t0
+    log(f"plain one sample per tick: {samples / t_tick / 1e6:.1f} Msamples/s")
+    for r in ctxs:
+        r.close()
+    return {
+        "workload": f"{len(desc)} scenes, triangles log-uniform in [1e4, 2e6] "
+                    f"({sum(ntri)} total), resolutions "
+                    f"{{256^2, 512^2, 960x540, 1920x1080}}, spp "
+                    f"{{16, 64, 256}}/{args.batch_spp_div}, depth {args.depth}",
+        "samples": samples, "triangles": sum(ntri),
+        "msamples_per_s": round(samples / t_batch / 1e6, 2),
+        "plain_sequential_msamples_per_s": round(samples / t_plain / 1e6, 2),
+        "plain_per_tick_msamples_per_s": round(samples / t_tick / 1e6, 2),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "speedup_vs_plain_per_tick": round(t_tick / t_batch, 3),
+        "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
+        "plain_per_tick_s": round(t_tick, 3),
+        "tuning": tuning,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +226,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-spp-div", type=int, default=1,
+                    help="divide the batched workload's spp set by this")
+    ap.add_argument("--batch-seed", type=int, default=4)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--only-batched", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 512 MiB elementwise copy (the "
                          "calibration kernel for HBM byte counters)")
@@ -127,6 +247,9 @@ def main():
         return
 
     import mrt_hip as M
+    if args.only_batched:
+        print(json.dumps({"batched": batched_leg(M, args)}), flush=True)
+        return
     W, H = args.width, args.height
     t0 = time.perf_counter()
     hs = M.HostScene.mesh(args.seed, W, H, subdiv=args.subdiv,
@@ -212,11 +335,15 @@ def main():
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4)},
     }
+    if not args.no_batched:
+        r.close()
+        out["batched"] = batched_leg(M, args)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(hs, W, H, args.depth, 0x5EED,
                                            args.cpu_budget)
-    r.close()
+    if args.no_batched:
+        r.close()
     print(json.dumps(out), flush=True)
 
 
