@@ -33,6 +33,7 @@ struct mrt_ctx {
     uint32_t     n_nodes = 0, n_prims = 0, n_planes = 0, n_mats = 0;
     uint32_t     bvh_depth = 0;
     std::vector<mrt_light> lights;
+    std::vector<mrt_material> mats_host;
     bool         has_scene = false;
     mrt_camera   cam{};
     float        ambient_ratio = 0.0f, ambient_color[3] = {0, 0, 0};
@@ -368,6 +369,7 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     }
     c->n_planes = s->n_planes;
     c->n_mats = s->n_materials;
+    c->mats_host.assign(s->materials, s->materials + s->n_materials);
     c->lights.assign(s->lights, s->lights + s->n_lights);
     c->cam = s->camera;
     c->ambient_ratio = s->ambient_ratio;
@@ -1193,6 +1195,155 @@ extern "C" int mrt_reset_stats(mrt_ctx *c)
     for (double &m : c->ms)
         m = 0.0;
     return MRT_OK;
+}
+
+// ---- differentiable rendering -------------------------------------------------
+extern "C" int mrt_diff_set_materials(mrt_ctx *c, uint32_t n, const float *albedo,
+                                      const float *roughness, const float *emission)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "no scene uploaded");
+    if (n != c->n_mats)
+        return fail(c, MRT_E_INVALID, "%u materials given, the scene has %u", n,
+                    c->n_mats);
+    for (uint32_t k = 0; k < n; k++) {
+        mrt_material &m = c->mats_host[k];
+        if (albedo)
+            for (int j = 0; j < 3; j++)
+                m.color[j] = albedo[3 * k + j];
+        if (roughness)
+            m.roughness = roughness[k];
+        if (emission) {
+            m.emission_strength = 1.0f;
+            for (int j = 0; j < 3; j++)
+                m.emission_color[j] = emission[3 * k + j];
+        }
+    }
+    std::vector<float4> mats;
+    for (const mrt_material &m : c->mats_host) {
+        mats.push_back(make_float4(m.color[0], m.color[1], m.color[2], m.metallic));
+        mats.push_back(make_float4(m.roughness, m.ior, m.transmission, m.ao));
+        mats.push_back(make_float4(m.emission_strength, m.emission_color[0],
+                                   m.emission_color[1], m.emission_color[2]));
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_mats, mats.data(), mats.size() * 16,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_diff_set_hdri(mrt_ctx *c, const float *rgb)
+{
+    if (!c || !rgb)
+        return MRT_E_INVALID;
+    if (!c->has_env)
+        return fail(c, MRT_E_STATE, "no HDRI uploaded");
+    const size_t n = (size_t)c->env_w * c->env_h;
+    std::vector<float4> tex(n);
+    for (size_t i = 0; i < n; i++)
+        tex[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_env_tex, tex.data(), n * 16,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MRT_OK;
+}
+
+extern "C" int mrt_adjoint(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
+                           uint32_t max_depth, const float *d_image, float *g_mat,
+                           float *g_hdri)
+{
+    if (!c || !d_image || !g_mat)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "mrt_adjoint before mrt_upload_scene");
+    if (max_depth == 0 || max_depth > (uint32_t)ADJ_MAXD)
+        return fail(c, MRT_E_LIMIT, "adjoint depth must be in 1..%d", ADJ_MAXD);
+    if (c->n_mats > (uint32_t)ADJ_MAXMAT)
+        return fail(c, MRT_E_LIMIT, "adjoint supports at most %d materials",
+                    ADJ_MAXMAT);
+    if (c->has_photons)
+        return fail(c, MRT_E_UNSUPPORTED, "no adjoint through the photon map");
+    if (!c->lights.empty() && !(c->flags & MRT_SCENE_LIGHT_EXACT))
+        return fail(c, MRT_E_UNSUPPORTED,
+                    "lights need MRT_SCENE_LIGHT_EXACT (truncated sums have no "
+                    "derivative)");
+    if (c->has_env && c->estimator != MRT_EST_MIS)
+        return fail(c, MRT_E_UNSUPPORTED, "the adjoint follows the MIS estimator");
+    if (g_hdri && !c->has_env)
+        return fail(c, MRT_E_STATE, "HDRI gradients without an HDRI");
+    const size_t ng = (size_t)c->n_mats * ADJ_NG;
+    std::vector<float> zero_mat(ng, 0.0f);
+    memcpy(g_mat, zero_mat.data(), ng * 4);
+    const size_t ntex = (size_t)c->env_w * c->env_h * 3;
+    if (g_hdri)
+        memset(g_hdri, 0, ntex * 4);
+    if (n_samples == 0)
+        return MRT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t WH = c->W * c->H;
+    float *d_img = nullptr, *d_gm = nullptr, *d_gt = nullptr;
+    hipError_t e = hipSuccess;
+    int rc = MRT_OK;
+    do {
+        if (dalloc(d_img, (size_t)WH * 3) != hipSuccess
+            || dalloc(d_gm, ng) != hipSuccess
+            || (g_hdri && dalloc(d_gt, ntex) != hipSuccess)) {
+            rc = fail(c, MRT_E_NOMEM, "adjoint buffers do not fit");
+            break;
+        }
+        if ((e = hipMemcpyAsync(d_img, d_image, (size_t)WH * 12,
+                                hipMemcpyHostToDevice, c->stream)) != hipSuccess
+            || (e = launch_fill_u32(c->stream, (uint32_t *)d_gm, 0u, ng)) != hipSuccess
+            || (g_hdri && (e = launch_fill_u32(c->stream, (uint32_t *)d_gt, 0u, ntex))
+                              != hipSuccess))
+            break;
+        FrameParams fp = frame_params(c);
+        SceneDev S = scene_dev(c);
+        ShadeBufs B;
+        memset(&B, 0, sizeof(B));
+        B.planes = c->d_planes;
+        B.prims = c->d_prims;
+        B.mats = c->d_mats;
+        B.lights = c->d_lights;
+        B.nrm = c->d_nrm;
+        AdjBufs A;
+        A.dimg = d_img;
+        A.g_mat = d_gm;
+        A.g_tex = d_gt;
+        A.n_mat = c->n_mats;
+        A.inv_spp = 1.0f / (float)n_samples;
+        uint32_t nb = c->path_budget / WH;
+        nb = nb < 1 ? 1 : (nb > n_samples ? n_samples : nb);
+        for (uint32_t s = 0; s < n_samples && e == hipSuccess; s += nb) {
+            uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
+            uint64_t P = (uint64_t)n * WH;
+            fp.sample0 = first_sample + s;
+            uint64_t blocks = (P + TRACE_BLOCK - 1) / TRACE_BLOCK;
+            uint64_t cap = (uint64_t)c->num_cu * 16;
+            e = launch_adjoint(c->stream, (uint32_t)(blocks < cap ? blocks : cap), fp,
+                               S, B, A, (uint32_t)P, max_depth);
+        }
+        if (e != hipSuccess)
+            break;
+        if ((e = hipMemcpyAsync(g_mat, d_gm, ng * 4, hipMemcpyDeviceToHost,
+                                c->stream)) != hipSuccess
+            || (g_hdri && (e = hipMemcpyAsync(g_hdri, d_gt, ntex * 4,
+                                              hipMemcpyDeviceToHost, c->stream))
+                              != hipSuccess)
+            || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+            break;
+    } while (0);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(d_img);
+    dfree(d_gm);
+    dfree(d_gt);
+    if (e != hipSuccess && rc == MRT_OK)
+        rc = fail(c, MRT_E_HIP, "adjoint failed: %s", hipGetErrorString(e));
+    return rc;
 }
 
 // ---- batches ------------------------------------------------------------------

@@ -174,6 +174,21 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                              uint32_t *prim_id, float *t);
 int trace_blocks_per_cu(bool shadow);
 
+// Path-replay adjoint: dL/dI per pixel in, gradients out.
+constexpr int ADJ_MAXD = 8;      // deepest path the adjoint replays
+constexpr int ADJ_MAXMAT = 64;   // materials with gradients (LDS sums)
+constexpr int ADJ_NG = 7;        // per material: albedo rgb, roughness, emission rgb
+struct AdjBufs {
+    const float *dimg;           // W*H*3, dL/dI of the mean image
+    float       *g_mat;          // n_mat * ADJ_NG
+    float       *g_tex;          // env_w*env_h*3, or null
+    uint32_t     n_mat;
+    float        inv_spp;
+};
+hipError_t launch_adjoint(hipStream_t st, uint32_t blocks, const FrameParams &fp,
+                          const SceneDev &S, const ShadeBufs &B,
+                          const AdjBufs &A, uint32_t P, uint32_t max_depth);
+
 // batched scenes (the same stages over the scene table of a batch)
 hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
                           const BatchCtl *ctl, float4 *q_o, float4 *q_d,

@@ -276,7 +276,11 @@ static void emit_face(ico_ctx *c, const double a[3], const double b[3],
     disp_vertex(c->d, a, v, n);
     disp_vertex(c->d, b, v + 3, n + 3);
     disp_vertex(c->d, e, v + 6, n + 6);
-    uint32_t m = c->mix ? draw_mix(&c->mst, c->base) : c->mat;
+    uint32_t m = c->mat;
+    if (c->mix == 1)
+        m = draw_mix(&c->mst, c->base);
+    else if (c->mix == 2)
+        m = (uint32_t)(mrt_u01(&c->mst) * c->mat) % c->mat;
     int r = mrt_hs_add_triangle(c->hs, v, n, m);
     if (r < 0)
         c->err = r;
@@ -333,10 +337,13 @@ int mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
     c.mix = material_mix;
     c.mat = mat;
     c.mst = seed ^ 0x5DEECE66Dull;
-    if (material_mix) {
+    if (material_mix == 1) {
         int r = add_mix_palette(hs, &c.base);
         if (r < 0)
             return r;
+    } else if (material_mix == 2) {
+        if (mat == 0 || mat > hs->n_mat)
+            return MRT_E_INVALID;
     } else if (mat >= hs->n_mat) {
         return MRT_E_INVALID;
     }

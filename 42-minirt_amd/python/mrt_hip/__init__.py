@@ -183,6 +183,11 @@ mrt_build_photons = _sig("mrt_build_photons", C.c_int, _ctx_p, C.c_uint32,
 mrt_get_photons = _sig("mrt_get_photons", C.c_int, _ctx_p, C.c_uint32, _fp,
                        _fp, _up, _up)
 mrt_caustic_at = _sig("mrt_caustic_at", C.c_int, _ctx_p, C.c_uint32, _fp, _fp)
+mrt_diff_set_materials = _sig("mrt_diff_set_materials", C.c_int, _ctx_p,
+                              C.c_uint32, _fp, _fp, _fp)
+mrt_diff_set_hdri = _sig("mrt_diff_set_hdri", C.c_int, _ctx_p, _fp)
+mrt_adjoint = _sig("mrt_adjoint", C.c_int, _ctx_p, C.c_uint32, C.c_uint32,
+                   C.c_uint32, _fp, _fp, _fp)
 mrt_render = _sig("mrt_render", C.c_int, _ctx_p, C.c_uint32, C.c_uint32,
                   C.c_uint32)
 mrt_readback_add = _sig("mrt_readback_add", C.c_int, _ctx_p,
@@ -428,7 +433,7 @@ class HostScene:
         c = np.asarray(centre, dtype=np.float32)
         return _check(mrt_gen_icosphere(self._h, c.ctypes.data_as(_fp),
                                         radius, subdiv, amp, seed,
-                                        1 if mix else 0, mat))
+                                        int(mix), mat))
 
     def set_hdri(self, rgb, exposure=0.0):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
@@ -583,6 +588,39 @@ class Renderer:
         out = np.zeros((n, 3), np.float32)
         self._ck(mrt_caustic_at(self._ctx, n, pp, out.ctypes.data_as(_fp)))
         return out
+
+    def diff_set_materials(self, albedo=None, roughness=None, emission=None):
+        n = None
+        arrs = []
+        for a, k in ((albedo, 3), (roughness, 1), (emission, 3)):
+            if a is None:
+                arrs.append((None, None))
+                continue
+            a = np.ascontiguousarray(a, np.float32).reshape(-1, k)
+            n = len(a)
+            arrs.append((a, a.ctypes.data_as(_fp)))
+        if n is None:
+            return
+        self._ck(mrt_diff_set_materials(self._ctx, n, arrs[0][1], arrs[1][1],
+                                        arrs[2][1]))
+
+    def diff_set_hdri(self, rgb):
+        a = np.ascontiguousarray(rgb, np.float32)
+        self._ck(mrt_diff_set_hdri(self._ctx, a.ctypes.data_as(_fp)))
+
+    def adjoint(self, first_sample, n_samples, max_depth, d_image, n_materials,
+                hdri_shape=None):
+        """dL/dparams for dL/dI of the mean image: (g_mat[n, 7] = albedo rgb,
+        roughness, emission rgb; g_hdri[h, w, 3] or None)."""
+        di = np.ascontiguousarray(d_image, np.float32)
+        gm = np.zeros((n_materials, 7), np.float32)
+        gt = None
+        if hdri_shape is not None:
+            gt = np.zeros((hdri_shape[0], hdri_shape[1], 3), np.float32)
+        self._ck(mrt_adjoint(self._ctx, first_sample, n_samples, max_depth,
+                             di.ctypes.data_as(_fp), gm.ctypes.data_as(_fp),
+                             gt.ctypes.data_as(_fp) if gt is not None else None))
+        return gm, gt
 
     def render(self, first_sample, n_samples, max_depth):
         self._ck(mrt_render(self._ctx, first_sample, n_samples, max_depth))

@@ -240,6 +240,32 @@ void mrt_destroy(mrt_ctx *ctx);
 const char *mrt_last_error(const mrt_ctx *ctx);
 const char *mrt_version(void);
 
+/* ---- differentiable rendering -------------------------------------------
+ *
+ * No counterpart in the reference.  Parameters: per material the albedo
+ * (rgb), the GGX roughness and the emission E (rgb, E = strength * colour;
+ * setting it stores strength 1 and colour E), and the HDRI texels.  The
+ * forward image is mrt_render's mean over n_samples; mrt_adjoint replays the
+ * same paths (same RNG keys) and returns dL/dparam for a given dL/dI.  The
+ * HDRI sampling tables stay those built by mrt_upload_hdri.  Requirements:
+ * the MIS estimator when an HDRI is present, MRT_SCENE_LIGHT_EXACT when
+ * lights are present, no photon map, depth <= 8, <= 64 materials. */
+
+/* Replace material parameters; any array may be NULL (kept).  n must equal
+ * the scene's material count. */
+int  mrt_diff_set_materials(mrt_ctx *ctx, uint32_t n, const float *albedo,
+                            const float *roughness, const float *emission);
+/* Replace the HDRI texels (same size as uploaded; tables unchanged). */
+int  mrt_diff_set_hdri(mrt_ctx *ctx, const float *rgb);
+/* d_image: W*H*3 dL/dI of the mean image.  g_mat: n_materials * 7 floats
+ * (albedo rgb, roughness, emission rgb); g_hdri: env_w*env_h*3 or NULL.
+ * Albedo, emission and texel gradients are exact derivatives of the
+ * replayed estimator; the roughness gradient is its score-function term
+ * (an unbiased estimate of the derivative of the expected image). */
+int  mrt_adjoint(mrt_ctx *ctx, uint32_t first_sample, uint32_t n_samples,
+                 uint32_t max_depth, const float *d_image, float *g_mat,
+                 float *g_hdri);
+
 /* ---- batches: many independent scenes in one wavefront -----------------
  *
  * No counterpart in the reference, which renders one scene per process.

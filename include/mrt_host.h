@@ -112,8 +112,11 @@ int  mrt_gen_mesh_scene(uint64_t seed, uint32_t width, uint32_t height,
 /* Adds a displaced icosphere to hs; returns the triangle count or < 0. */
 int  mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
                        uint32_t subdiv, float amp, uint64_t seed,
-                       int material_mix /* 1: per-triangle mix */,
-                       uint32_t mat /* used when material_mix == 0 */);
+                       int material_mix /* 0: every triangle uses `mat`;
+                                           1: per-triangle 60/25/15 mix;
+                                           2: uniform over materials
+                                              0 .. mat-1 of hs */,
+                       uint32_t mat);
 
 /* Equirectangular sky: analytic gradient plus a sun disc of the given
  * angular radius (degrees) and radiance at a seeded elevation in

@@ -145,6 +145,30 @@ uint32_t orc_emit_photons_f64(const mrt_scene *s, const orc_bvh *b,
                               uint64_t seed, float *pos, float *pow,
                               uint32_t *gid);
 
+/* Forward-mode derivative of the rendered image with respect to one
+ * parameter, on the MIS estimator (or no HDRI), exact light sums, no photon
+ * map: the independent reference for the HIP adjoint.  Albedo, emission
+ * (E = strength * colour, taken as the parameter) and texel derivatives are
+ * exact derivatives of the estimator; roughness contributes the
+ * score-function term d log p(m)/dr of every GGX micro-normal sample. */
+#define ORC_P_ALBEDO    0
+#define ORC_P_ROUGHNESS 1
+#define ORC_P_EMISSION  2
+#define ORC_P_TEXEL     3
+typedef struct orc_param {
+    int      kind;
+    uint32_t mat;      /* material (albedo, roughness, emission)            */
+    uint32_t chan;     /* colour channel (albedo, emission, texel)          */
+    uint32_t texel;    /* y * w + x (texel)                                 */
+} orc_param;
+/* img and dimg: width*height*3 doubles, sums over the samples (added to). */
+int  orc_render_grad_f32(const mrt_scene *s, const orc_bvh *b,
+                         const orc_hdri *hd, const orc_params *p,
+                         const orc_param *q, double *img, double *dimg);
+int  orc_render_grad_f64(const mrt_scene *s, const orc_bvh *b,
+                         const orc_hdri *hd, const orc_params *p,
+                         const orc_param *q, double *img, double *dimg);
+
 /* The caustic estimate at n points (3 floats each) into out (3 floats). */
 void orc_caustic_f32(const orc_photon_map *pm, uint32_t n, const float *pts,
                      float *out);

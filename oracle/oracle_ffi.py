@@ -31,6 +31,14 @@ class OrcHdri(C.Structure):
                 ("mis_pcell", C.c_void_p)]
 
 
+P_ALBEDO, P_ROUGHNESS, P_EMISSION, P_TEXEL = 0, 1, 2, 3
+
+
+class OrcParam(C.Structure):
+    _fields_ = [("kind", C.c_int), ("mat", C.c_uint32), ("chan", C.c_uint32),
+                ("texel", C.c_uint32)]
+
+
 class OrcParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
                 ("first_sample", C.c_uint32), ("n_samples", C.c_uint32),
@@ -79,6 +87,12 @@ def lib():
                                          C.POINTER(C.c_uint32)]
             getattr(L, name).restype = u32
         L.orc_caustic_f32.argtypes = [vp, u32, fp, fp]
+        for name in ("orc_render_grad_f32", "orc_render_grad_f64"):
+            getattr(L, name).argtypes = [vp, vp, vp, C.POINTER(OrcParams),
+                                         C.POINTER(OrcParam),
+                                         C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
+            getattr(L, name).restype = C.c_int
         L.orc_hdri_tables.argtypes = [fp, u32, u32, C.c_float, fp, fp, fp,
                                       C.POINTER(C.c_double)]
         L.orc_hdri_tables.restype = C.c_int
@@ -216,6 +230,27 @@ def render(scene_view, bvh, width, height, first_sample, n_samples,
     if rc != 0:
         raise RuntimeError("oracle render failed")
     return acc, cnt.as_dict()
+
+
+def render_grad(scene_view, bvh, width, height, first_sample, n_samples,
+                max_depth, seed, kind, mat=0, chan=0, texel=0, f64=False,
+                hdri=None, threads=0):
+    """Forward-mode derivative of the sum image with respect to one
+    parameter: returns (image sums, derivative sums)."""
+    img = np.zeros((height, width, 3), np.float64)
+    dimg = np.zeros((height, width, 3), np.float64)
+    p = OrcParams(width, height, first_sample, n_samples, max_depth, seed,
+                  TRAV_ORDERED, 1, threads, 0, 0)
+    q = OrcParam(kind, mat, chan, texel)
+    fn = lib().orc_render_grad_f64 if f64 else lib().orc_render_grad_f32
+    rc = fn(C.cast(scene_view, C.c_void_p),
+            C.byref(bvh) if bvh is not None else None,
+            C.byref(hdri) if hdri is not None else None, C.byref(p),
+            C.byref(q), img.ctypes.data_as(C.POINTER(C.c_double)),
+            dimg.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError("oracle render_grad failed")
+    return img, dimg
 
 
 def hdri_tables(rgb, exposure=0.0):
