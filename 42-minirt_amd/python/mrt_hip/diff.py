@@ -13,7 +13,7 @@ class RenderFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, albedo, roughness, emission, hdri, renderer, n_samples,
-                max_depth, first_sample):
+                max_depth, first_sample, grad_first_sample):
         renderer.diff_set_materials(albedo.detach().cpu().numpy(),
                                     roughness.detach().cpu().numpy(),
                                     emission.detach().cpu().numpy())
@@ -23,7 +23,8 @@ class RenderFunction(torch.autograd.Function):
         renderer.render(first_sample, n_samples, max_depth)
         img = renderer.readback() / np.float32(n_samples)
         ctx.renderer = renderer
-        ctx.args = (first_sample, n_samples, max_depth)
+        ctx.args = (first_sample if grad_first_sample is None
+                    else grad_first_sample, n_samples, max_depth)
         ctx.n_mat = albedo.shape[0]
         ctx.hshape = tuple(hdri.shape[:2]) if hdri is not None else None
         return torch.from_numpy(img)
@@ -38,13 +39,18 @@ class RenderFunction(torch.autograd.Function):
         g_rough = torch.from_numpy(np.ascontiguousarray(gm[:, 3]))
         g_emis = torch.from_numpy(np.ascontiguousarray(gm[:, 4:7]))
         g_hdri = torch.from_numpy(gt) if gt is not None else None
-        return g_alb, g_rough, g_emis, g_hdri, None, None, None, None
+        return g_alb, g_rough, g_emis, g_hdri, None, None, None, None, None
 
 
 def render(renderer, albedo, roughness, emission, hdri=None, n_samples=64,
-           max_depth=6, first_sample=0):
+           max_depth=6, first_sample=0, grad_first_sample=None):
+    """Mean image over samples first_sample .. +n_samples-1.  The backward
+    pass replays those samples, or the range starting at grad_first_sample:
+    with a disjoint range the gradient of an L2 loss is free of the
+    correlation between the residual and the derivative."""
     return RenderFunction.apply(albedo, roughness, emission, hdri, renderer,
-                                n_samples, max_depth, first_sample)
+                                n_samples, max_depth, first_sample,
+                                grad_first_sample)
 
 
 def inverse_scene(seed, width, height, n_materials=16, subdiv=6,

@@ -212,6 +212,148 @@ def batched_leg(M, args):
     }
 
 
+def adjoint_leg(M, args):
+    """Inverse rendering on the HIP path: forward (mean image over 64 spp) and
+    path-replay adjoint timed on the same parameters, 32 gradients checked
+    against central differences of the forward render, then Adam on the
+    L2 image loss from perturbed parameters."""
+    import numpy as np
+    import torch
+    from mrt_hip import diff
+    Wd, Hd, spp, depth = 512, 512, 64, 6
+    t0 = time.perf_counter()
+    hs, alb, rough, emis = diff.inverse_scene(args.adj_seed, Wd, Hd,
+                                              n_materials=16, subdiv=6,
+                                              floor_cells=240)
+    ntri = hs.view().contents.n_triangles
+    env = M.hdri_sky(args.adj_seed, 256, 128, sun_radius_deg=3.0,
+                     sun_radiance=50.0)
+    r = diff.make_renderer(hs, Wd, Hd, env)
+    r.set_seed(0xADD)
+    log(f"adjoint: scene {ntri} triangles, setup {time.perf_counter() - t0:.1f} s")
+    t_alb, t_rough = torch.from_numpy(alb), torch.from_numpy(rough)
+    t_emis, t_env = torch.from_numpy(emis), torch.from_numpy(env)
+    target = diff.render(r, t_alb, t_rough, t_emis, t_env, spp, depth).detach()
+    # forward / adjoint timing on the ground-truth parameters
+    dimg = np.random.default_rng(1).normal(size=(Hd, Wd, 3)).astype(np.float32)
+    fw, bw = [], []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        r.clear()
+        r.render(0, spp, depth)
+        img = r.readback()
+        fw.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        gm, gt = r.adjoint(0, spp, depth, dimg, 16, env.shape[:2])
+        bw.append(time.perf_counter() - t0)
+    t_f, t_b = min(fw[1:]), min(bw[1:])
+    log(f"adjoint: forward {t_f * 1e3:.1f} ms, adjoint {t_b * 1e3:.1f} ms")
+    # central differences (h = 1e-3) on 32 random parameters the image
+    # depends on
+    img0 = img.astype(np.float64) / spp
+    cand = [("a", k, c, gm[k, c]) for k in range(16) for c in range(3)]
+    cand += [("e", k, c, gm[k, 4 + c]) for k in range(16) for c in range(3)]
+    flat = gt.reshape(-1)
+    for i in np.argsort(-np.abs(flat))[:256]:
+        cand.append(("t", int(i), 0, flat[i]))
+    big = max(abs(c[3]) for c in cand)
+    cand = [c for c in cand if abs(c[3]) > 1e-2 * big]
+    pick = np.random.default_rng(2).choice(len(cand), 32, replace=False)
+
+    def loss_at(kind, a, b, hh):
+        al, em, tx = alb.copy(), emis.copy(), env.copy()
+        if kind == "a":
+            al[a, b] += hh
+        elif kind == "e":
+            em[a, b] += hh
+        else:
+            tx.reshape(-1)[a] += hh
+        r.diff_set_materials(al, rough, em)
+        r.diff_set_hdri(tx)
+        r.clear()
+        r.render(0, spp, depth)
+        return float(np.sum(dimg * (r.readback().astype(np.float64) / spp)))
+    errs = []
+    for i in pick:
+        kind, a, b, g = cand[i]
+        fd = (loss_at(kind, a, b, 1e-3) - loss_at(kind, a, b, -1e-3)) / 2e-3
+        errs.append(abs(g - fd) / max(abs(fd), 1e-12))
+    r.diff_set_materials(alb, rough, emis)
+    r.diff_set_hdri(env)
+    log(f"adjoint: FD check on 32 parameters, max rel err {max(errs):.2e}")
+    # inverse rendering from perturbed parameters
+    rng = np.random.default_rng(args.adj_seed + 1)
+    p_alb = torch.tensor(np.clip(alb + rng.normal(0, 0.15, alb.shape), 0.05,
+                                 0.95), dtype=torch.float32, requires_grad=True)
+    p_rough = torch.tensor(np.clip(rough * rng.uniform(0.6, 1.4, rough.shape),
+                                   0.03, 0.8), dtype=torch.float32,
+                           requires_grad=True)
+    p_emis = torch.tensor(np.clip(emis + (emis > 0) * rng.normal(0, 0.3,
+                                                                 emis.shape),
+                                  0.0, None), dtype=torch.float32,
+                          requires_grad=True)
+    # texels as log-scale factors over the perturbed map
+    env_start = torch.tensor(env * np.exp(rng.normal(0, 0.2, env.shape)),
+                             dtype=torch.float32)
+    p_log = torch.zeros(env.shape, dtype=torch.float32, requires_grad=True)
+    opt = torch.optim.Adam([
+        {"params": [p_alb, p_emis], "lr": args.adj_lr},
+        {"params": [p_rough], "lr": args.adj_lr * 0.5},
+        {"params": [p_log], "lr": args.adj_lr_env}])
+    alb_start = p_alb.detach().numpy().copy()
+    rough_start = p_rough.detach().numpy().copy()
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(args.adam_steps):
+        opt.zero_grad()
+        p_env = env_start * torch.exp(p_log)
+        f0 = 0 if args.adj_crn else spp * (2 * step + 1)
+        out = diff.render(r, p_alb, p_rough, p_emis, p_env, spp, depth,
+                          first_sample=f0,
+                          grad_first_sample=None if args.adj_crn else f0 + spp)
+        loss = ((out - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            p_alb.clamp_(0.0, 1.0)
+            p_rough.clamp_(0.02, 1.0)
+            p_emis.clamp_(min=0.0)
+        losses.append(loss.item())
+        if step % 25 == 0:
+            ae = float(np.sqrt(np.mean((p_alb.detach().numpy() - alb) ** 2)))
+            log(f"adam step {step}: loss {losses[-1]:.4e}, albedo rmse {ae:.4f}")
+    t_opt = time.perf_counter() - t0
+    r.close()
+    return {
+        "workload": f"inverse rendering 512x512, {spp} spp, depth {depth}, "
+                    f"{ntri} triangles, 16 materials (albedo, roughness, "
+                    f"emission) + 256x128 HDRI as parameters",
+        "forward_ms": round(t_f * 1e3, 2), "adjoint_ms": round(t_b * 1e3, 2),
+        "adjoint_over_forward": round(t_b / t_f, 3),
+        "fd_check_params": 32, "fd_max_rel_err": float(f"{max(errs):.3e}"),
+        "adam_steps": args.adam_steps,
+        "loss_first": round(losses[0], 5) if losses else None,
+        "loss_last": round(losses[-1], 5) if losses else None,
+        "albedo_rmse_start": round(float(np.sqrt(np.mean(
+            (alb_start - alb) ** 2))), 5),
+        "albedo_rmse_end": round(float(np.sqrt(np.mean(
+            (p_alb.detach().numpy() - alb) ** 2))), 5),
+        "roughness_rmse_start": round(float(np.sqrt(np.mean(
+            (rough_start - rough) ** 2))), 5),
+        "roughness_rmse_end": round(float(np.sqrt(np.mean(
+            (p_rough.detach().numpy() - rough) ** 2))), 5),
+        "hdri_log_rmse_start": round(float(np.sqrt(np.mean(np.log(
+            (env_start.numpy() + 1e-6) / (env + 1e-6)) ** 2))), 5),
+        "hdri_log_rmse_end": round(float(np.sqrt(np.mean(np.log(
+            ((env_start * torch.exp(p_log)).detach().numpy() + 1e-6)
+            / (env + 1e-6)) ** 2))), 5),
+        "samples": "common random numbers (the target's)" if args.adj_crn
+                   else "fresh samples every step, gradient from a disjoint "
+                        "sample range",
+        "optimisation_s": round(t_opt, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +373,14 @@ def main():
                     help="divide the batched workload's spp set by this")
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-adjoint", action="store_true")
+    ap.add_argument("--adam-steps", type=int, default=200)
+    ap.add_argument("--adj-seed", type=int, default=5)
+    ap.add_argument("--adj-lr", type=float, default=0.01)
+    ap.add_argument("--adj-lr-env", type=float, default=0.005)
+    ap.add_argument("--adj-crn", action="store_true",
+                    help="optimise on the target's own samples")
     ap.add_argument("--only-batched", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 512 MiB elementwise copy (the "
@@ -249,6 +399,9 @@ def main():
     import mrt_hip as M
     if args.only_batched:
         print(json.dumps({"batched": batched_leg(M, args)}), flush=True)
+        return
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_leg(M, args)}), flush=True)
         return
     W, H = args.width, args.height
     t0 = time.perf_counter()
@@ -335,14 +488,17 @@ def main():
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4)},
     }
-    if not args.no_batched:
+    if not args.no_batched or not args.no_adjoint:
         r.close()
+    if not args.no_batched:
         out["batched"] = batched_leg(M, args)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(M, args)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(hs, W, H, args.depth, 0x5EED,
                                            args.cpu_budget)
-    if args.no_batched:
+    if args.no_batched and args.no_adjoint:
         r.close()
     print(json.dumps(out), flush=True)
 
