@@ -335,10 +335,22 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
         return fail(c, MRT_E_LIMIT, "BVH depth %u exceeds %u", bvh.depth,
                     MRT_BVH_MAX_DEPTH);
     }
+    mrt_bvh4 bvh4;
+    rc = mrt_bvh4_build(&bvh, &bvh4);
+    if (rc != MRT_OK) {
+        mrt_bvh_free(&bvh);
+        return fail(c, rc, "4-wide BVH build failed (%d)", rc);
+    }
+    if (bvh4.depth > MRT_BVH4_MAX_DEPTH) {
+        mrt_bvh_free(&bvh);
+        mrt_bvh4_free(&bvh4);
+        return fail(c, MRT_E_LIMIT, "4-wide BVH depth %u exceeds %u", bvh4.depth,
+                    MRT_BVH4_MAX_DEPTH);
+    }
     std::vector<float4> planes, mats, lights;
     pack_scene(s, 0, planes, mats, lights);
     rc = MRT_OK;
-    if (dalloc(c->d_nodes, (size_t)bvh.n_nodes * 4) != hipSuccess
+    if (dalloc(c->d_nodes, (size_t)bvh4.n_nodes * 8) != hipSuccess
         || dalloc(c->d_prims, (size_t)bvh.n_prims * 3) != hipSuccess
         || dalloc(c->d_nrm, (size_t)bvh.n_prims * 9) != hipSuccess
         || dalloc(c->d_planes, planes.size()) != hipSuccess
@@ -346,7 +358,7 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
         || dalloc(c->d_lights, lights.size()) != hipSuccess)
         rc = fail(c, MRT_E_NOMEM, "device allocation for the scene failed");
     if (rc == MRT_OK)
-        rc = upload(c, c->d_nodes, bvh.nodes, (size_t)bvh.n_nodes * 64);
+        rc = upload(c, c->d_nodes, bvh4.nodes, (size_t)bvh4.n_nodes * 128);
     if (rc == MRT_OK)
         rc = upload(c, c->d_prims, bvh.prims, (size_t)bvh.n_prims * 48);
     if (rc == MRT_OK)
@@ -359,10 +371,11 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
         rc = upload(c, c->d_lights, lights.data(), lights.size() * 16);
     if (rc == MRT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
         rc = fail(c, MRT_E_HIP, "scene upload failed");
-    c->n_nodes = bvh.n_nodes;
+    c->n_nodes = bvh4.n_nodes;
     c->n_prims = bvh.n_prims;
-    c->bvh_depth = bvh.depth;
+    c->bvh_depth = bvh4.depth;
     mrt_bvh_free(&bvh);
+    mrt_bvh4_free(&bvh4);
     if (rc != MRT_OK) {
         free_scene(c);
         return rc;
@@ -1516,13 +1529,21 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         return fail(c, MRT_E_STATE, "batch has no scenes");
     // BVHs, built in parallel
     std::vector<mrt_bvh> bv(n);
+    std::vector<mrt_bvh4> bv4(n);
     std::vector<int> brc(n, MRT_OK);
     std::atomic<size_t> next{0};
     unsigned nth = std::thread::hardware_concurrency();
     nth = std::max(1u, std::min({nth, 16u, (unsigned)n}));
+    for (size_t i = 0; i < n; i++) {
+        memset(&bv[i], 0, sizeof(bv[i]));
+        memset(&bv4[i], 0, sizeof(bv4[i]));
+    }
     auto work = [&]() {
-        for (size_t i; (i = next.fetch_add(1)) < n;)
+        for (size_t i; (i = next.fetch_add(1)) < n;) {
             brc[i] = mrt_bvh_build(&b->src[i]->view, MRT_BVH_MAX_LEAF, &bv[i]);
+            if (brc[i] == MRT_OK)
+                brc[i] = mrt_bvh4_build(&bv[i], &bv4[i]);
+        }
     };
     std::vector<std::thread> pool;
     for (unsigned t = 1; t < nth; t++)
@@ -1534,12 +1555,13 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     for (size_t i = 0; i < n && rc == MRT_OK; i++) {
         if (brc[i] != MRT_OK)
             rc = fail(c, brc[i], "scene %zu: BVH build failed", i);
-        else if (bv[i].depth > MRT_BVH_MAX_DEPTH)
-            rc = fail(c, MRT_E_LIMIT, "scene %zu: BVH depth %u exceeds %u", i,
-                      bv[i].depth, MRT_BVH_MAX_DEPTH);
+        else if (bv[i].depth > MRT_BVH_MAX_DEPTH || bv4[i].depth > MRT_BVH4_MAX_DEPTH)
+            rc = fail(c, MRT_E_LIMIT, "scene %zu: BVH depth %u / %u exceeds %u / %u",
+                      i, bv[i].depth, bv4[i].depth, MRT_BVH_MAX_DEPTH,
+                      MRT_BVH4_MAX_DEPTH);
     }
     // shared arrays with rebased links, primitives and materials
-    std::vector<mrt_bvh_node> nodes;
+    std::vector<mrt_bvh4_node> nodes;
     std::vector<mrt_prim> prims;
     std::vector<float> nrm;
     std::vector<float4> planes, mats, lights;
@@ -1551,19 +1573,20 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     for (size_t i = 0; i < n && rc == MRT_OK; i++) {
         const BatchSrc &S = *b->src[i];
         const mrt_bvh &B = bv[i];
+        const mrt_bvh4 &B4 = bv4[i];
         const uint32_t node0 = (uint32_t)nodes.size();
         const uint32_t prim0 = (uint32_t)prims.size();
         const uint32_t mat0 = (uint32_t)(mats.size() / 3);
         const uint32_t plane0 = (uint32_t)(planes.size() / 2);
         const uint32_t light0 = (uint32_t)(lights.size() / 3);
         if ((uint64_t)prim0 + B.n_prims >= (1ull << 28)
-            || (uint64_t)node0 + B.n_nodes >= (1ull << 31)) {
+            || (uint64_t)node0 + B4.n_nodes >= (1ull << 31)) {
             rc = fail(c, MRT_E_LIMIT, "batch exceeds 2^28 primitives");
             break;
         }
-        for (uint32_t k = 0; k < B.n_nodes; k++) {
-            mrt_bvh_node nd = B.nodes[k];
-            for (int j = 0; j < 2; j++) {
+        for (uint32_t k = 0; k < B4.n_nodes; k++) {
+            mrt_bvh4_node nd = B4.nodes[k];
+            for (int j = 0; j < 4; j++) {
                 int32_t ch = nd.child[j];
                 if (ch >= 0) {
                     nd.child[j] = ch + (int32_t)node0;
@@ -1597,8 +1620,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         T.fp.estimator = MRT_EST_REFERENCE;
         T.fp.light0 = light0;
         T.fp.plane0 = plane0;
-        T.root = B.n_nodes ? node0 : 0u;
-        T.n_nodes = B.n_nodes;
+        T.root = B4.n_nodes ? node0 : 0u;
+        T.n_nodes = B4.n_nodes;
         T.n_planes = S.view.n_planes;
         T.accum0 = (uint32_t)accum0;
         T.path0 = path0;
@@ -1616,6 +1639,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     }
     for (mrt_bvh &B : bv)
         mrt_bvh_free(&B);
+    for (mrt_bvh4 &B : bv4)
+        mrt_bvh4_free(&B);
     if (rc == MRT_OK && accum0 >= (1ull << 32))
         rc = fail(c, MRT_E_LIMIT, "batch images exceed 2^32 floats");
     if (rc != MRT_OK)
@@ -1623,7 +1648,7 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     HIPCHK(c, hipSetDevice(c->device));
     batch_free_device(b);
     BatchCtl ctl{0, path0, (uint32_t)n, 0};
-    if (dalloc(b->d_nodes, nodes.size() * 4) != hipSuccess
+    if (dalloc(b->d_nodes, nodes.size() * 8) != hipSuccess
         || dalloc(b->d_prims, prims.size() * 3) != hipSuccess
         || dalloc(b->d_nrm, nrm.size()) != hipSuccess
         || dalloc(b->d_planes, planes.size()) != hipSuccess
@@ -1634,7 +1659,7 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         batch_free_device(b);
         return fail(c, MRT_E_NOMEM, "device allocation for the batch failed");
     }
-    rc = upload(c, b->d_nodes, nodes.data(), nodes.size() * 64);
+    rc = upload(c, b->d_nodes, nodes.data(), nodes.size() * 128);
     if (rc == MRT_OK)
         rc = upload(c, b->d_prims, prims.data(), prims.size() * 48);
     if (rc == MRT_OK)

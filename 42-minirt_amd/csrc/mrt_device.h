@@ -13,7 +13,6 @@ namespace mrt {
 constexpr float IT_MIN_F = 0.001f;    // IT_MIN, includes/minirt.h:44
 constexpr float IT_MAX_F = 1000.0f;   // IT_MAX, includes/minirt.h:43
 constexpr float BOX_WIDEN = 1.0000004f;  // 1 + 2 gamma_3 (conservative slab)
-constexpr int STACK_MAX = 64;         // >= MRT_BVH_MAX_DEPTH + 1
 #ifndef MRT_POPCULL
 #define MRT_POPCULL 0
 #endif
@@ -197,10 +196,27 @@ struct SceneDev {
     uint32_t      root;       // root node (scenes of a batch share one array)
 };
 
-// Ordered traversal: near child first, far child on the stack with its
-// entry distance, dropped at pop when beyond the closest hit.  The first
-// SS stack entries live in LDS (lds_code / lds_t, stride `stride`), deeper
-// ones in a private array.  ANY: stop at the first hit below tlimit.
+constexpr int STACK4_MAX = 3 * (int)MRT_BVH4_MAX_DEPTH + 1;
+
+// Compare-exchange of (entry distance, child code) pairs.
+__device__ __forceinline__ void cx(float &ka, int32_t &ca, float &kb, int32_t &cb)
+{
+    if (ka > kb) {
+        float t = ka;
+        ka = kb;
+        kb = t;
+        int32_t c = ca;
+        ca = cb;
+        cb = c;
+    }
+}
+
+// Ordered traversal of the 4-wide BVH: one 128-byte node gives the four
+// child boxes; the hits are sorted by entry distance (network (0,1) (2,3)
+// (0,2) (1,3) (1,2)), the nearest is entered and the others are pushed
+// farthest first.  The first SS stack entries live in LDS (lds_code,
+// stride `stride`), deeper ones in a private array.  ANY: stop at the first
+// hit below tlimit.
 template <bool ANY, bool COUNT, int SS>
 __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
@@ -209,48 +225,52 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                                          float &bu_out, float &bv_out,
                                          uint32_t &nodes, uint32_t &prims)
 {
+    (void)lds_t;
     if (S.n_nodes == 0)
         return false;
     V3 inv = v3(inv_dir(d.x), inv_dir(d.y), inv_dir(d.z));
     float tclose = tlimit;
     bool hit = false;
-    uint32_t ovf_code[STACK_MAX - SS];
-    float ovf_t[STACK_MAX - SS];
+    uint32_t ovf_code[STACK4_MAX - SS];
     int sp = 0;
     int32_t code = (int32_t)S.root;
+    const float INF = __int_as_float(0x7f800000);
     for (;;) {
         if (code >= 0) {
-            const float4 *n = S.nodes + (size_t)code * 4;
-            float4 a = n[0], b = n[1], c = n[2], e = n[3];
+            const float4 *n = S.nodes + (size_t)code * 8;
+            float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+            float4 cw = n[6];
             if (COUNT)
                 nodes++;
-            float t0, t1;
-            bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tclose, t0);
-            bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tclose, t1);
-            int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-            if (h0 && h1) {
-                bool near1 = t1 < t0;
-                int32_t farc = near1 ? c0 : c1;
-                float fart = near1 ? t0 : t1;
-                if (sp < SS) {
-                    lds_code[sp * stride] = (uint32_t)farc;
-                    if (POPCULL)
-                        lds_t[sp * stride] = fart;
-                } else if (sp < STACK_MAX) {
-                    ovf_code[sp - SS] = (uint32_t)farc;
-                    if (POPCULL)
-                        ovf_t[sp - SS] = fart;
-                }
-                sp = sp < STACK_MAX ? sp + 1 : sp;
-                code = near1 ? c1 : c0;
-                continue;
-            }
-            if (h0) {
+            float t0, t1, t2, t3;
+            float k0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, tclose, t0) ? t0 : INF;
+            float k1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, tclose, t1) ? t1 : INF;
+            float k2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, tclose, t2) ? t2 : INF;
+            float k3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, tclose, t3) ? t3 : INF;
+            int32_t c0 = __float_as_int(cw.x), c1 = __float_as_int(cw.y),
+                    c2 = __float_as_int(cw.z), c3 = __float_as_int(cw.w);
+            cx(k0, c0, k1, c1);
+            cx(k2, c2, k3, c3);
+            cx(k0, c0, k2, c2);
+            cx(k1, c1, k3, c3);
+            cx(k1, c1, k2, c2);
+            if (k0 != INF) {
+#define MRT_PUSH(cc)                                                         \
+    do {                                                                     \
+        if (sp < SS)                                                         \
+            lds_code[sp * stride] = (uint32_t)(cc);                          \
+        else if (sp < STACK4_MAX)                                            \
+            ovf_code[sp - SS] = (uint32_t)(cc);                              \
+        sp = sp < STACK4_MAX ? sp + 1 : sp;                                  \
+    } while (0)
+                if (k3 != INF)
+                    MRT_PUSH(c3);
+                if (k2 != INF)
+                    MRT_PUSH(c2);
+                if (k1 != INF)
+                    MRT_PUSH(c1);
+#undef MRT_PUSH
                 code = c0;
-                continue;
-            }
-            if (h1) {
-                code = c1;
                 continue;
             }
         } else {
@@ -281,29 +301,12 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                 }
             }
         }
-        // pop, skipping entries beyond the closest hit
-        for (;;) {
-            if (sp == 0) {
-                t_out = tclose;
-                return hit;
-            }
-            sp--;
-            float et = 0.0f;
-            uint32_t ec;
-            if (sp < SS) {
-                ec = lds_code[sp * stride];
-                if (POPCULL)
-                    et = lds_t[sp * stride];
-            } else {
-                ec = ovf_code[sp - SS];
-                if (POPCULL)
-                    et = ovf_t[sp - SS];
-            }
-            if (!POPCULL || et <= tclose * BOX_WIDEN) {
-                code = (int32_t)ec;
-                break;
-            }
+        if (sp == 0) {
+            t_out = tclose;
+            return hit;
         }
+        sp--;
+        code = (int32_t)(sp < SS ? lds_code[sp * stride] : ovf_code[sp - SS]);
     }
 }
 

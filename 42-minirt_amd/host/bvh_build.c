@@ -435,3 +435,131 @@ void mrt_bvh_free(mrt_bvh *b)
     free(b->shade_nrm);
     memset(b, 0, sizeof(*b));
 }
+
+/* ---- 4-wide collapse ----------------------------------------------------- */
+
+typedef struct {
+    int32_t code;
+    float   lo[3], hi[3];
+} child4;
+
+typedef struct {
+    const mrt_bvh *b;
+    mrt_bvh4_node *nodes;
+    uint32_t       n, cap, depth;
+    int            err;
+} col4;
+
+static double child_area(const child4 *c)
+{
+    double dx = (double)c->hi[0] - c->lo[0], dy = (double)c->hi[1] - c->lo[1],
+           dz = (double)c->hi[2] - c->lo[2];
+    if (dx < 0 || dy < 0 || dz < 0)
+        return 0.0;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+static void bin_children(const mrt_bvh_node *n, child4 *a, child4 *b)
+{
+    a->code = n->child[0];
+    b->code = n->child[1];
+    memcpy(a->lo, n->lo0, sizeof(a->lo));
+    memcpy(a->hi, n->hi0, sizeof(a->hi));
+    memcpy(b->lo, n->lo1, sizeof(b->lo));
+    memcpy(b->hi, n->hi1, sizeof(b->hi));
+}
+
+/* Builds the 4-wide node for binary inner node `code`; returns its index. */
+static int32_t collapse4(col4 *C, int32_t code, uint32_t level)
+{
+    if (C->err)
+        return 0;
+    if (level > C->depth)
+        C->depth = level;
+    child4 ch[4];
+    int nc = 2;
+    bin_children(&C->b->nodes[code], &ch[0], &ch[1]);
+    while (nc < 4) {
+        int best = -1;
+        double ba = -1.0;
+        for (int i = 0; i < nc; i++)
+            if (ch[i].code >= 0 && child_area(&ch[i]) > ba) {
+                ba = child_area(&ch[i]);
+                best = i;
+            }
+        if (best < 0)
+            break;
+        child4 a, b;
+        bin_children(&C->b->nodes[ch[best].code], &a, &b);
+        for (int i = nc; i > best + 1; i--)
+            ch[i] = ch[i - 1];
+        ch[best] = a;
+        ch[best + 1] = b;
+        nc++;
+    }
+    if (C->n == C->cap) {
+        uint32_t nc2 = C->cap ? C->cap * 2 : 1024;
+        mrt_bvh4_node *p = realloc(C->nodes, (size_t)nc2 * sizeof(*p));
+        if (!p) {
+            C->err = MRT_E_NOMEM;
+            return 0;
+        }
+        C->nodes = p;
+        C->cap = nc2;
+    }
+    const uint32_t k = C->n++;
+    mrt_bvh4_node node;
+    memset(&node, 0, sizeof(node));
+    for (int i = 0; i < 4; i++) {
+        if (i >= nc) {
+            node.lo_x[i] = node.hi_x[i] = 1e30f;
+            node.lo_y[i] = node.hi_y[i] = 1e30f;
+            node.lo_z[i] = node.hi_z[i] = 1e30f;
+            node.child[i] = ~(int32_t)0;       /* leaf (0, 1), never reached */
+            continue;
+        }
+        node.lo_x[i] = ch[i].lo[0];
+        node.hi_x[i] = ch[i].hi[0];
+        node.lo_y[i] = ch[i].lo[1];
+        node.hi_y[i] = ch[i].hi[1];
+        node.lo_z[i] = ch[i].lo[2];
+        node.hi_z[i] = ch[i].hi[2];
+        node.child[i] = ch[i].code;
+    }
+    C->nodes[k] = node;
+    for (int i = 0; i < nc; i++)
+        if (ch[i].code >= 0) {
+            int32_t sub = collapse4(C, ch[i].code, level + 1);
+            C->nodes[k].child[i] = sub;
+        }
+    return (int32_t)k;
+}
+
+int mrt_bvh4_build(const mrt_bvh *b, mrt_bvh4 *out)
+{
+    if (!b || !out)
+        return MRT_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    if (b->n_nodes == 0)
+        return MRT_OK;
+    col4 C;
+    memset(&C, 0, sizeof(C));
+    C.b = b;
+    (void)collapse4(&C, 0, 1);
+    if (C.err) {
+        free(C.nodes);
+        return C.err;
+    }
+    out->nodes = C.nodes;
+    out->n_nodes = C.n;
+    out->depth = C.depth;
+    return MRT_OK;
+}
+
+void mrt_bvh4_free(mrt_bvh4 *b)
+{
+    if (!b)
+        return;
+    free(b->nodes);
+    memset(b, 0, sizeof(*b));
+}

@@ -151,6 +151,11 @@ class Bvh(C.Structure):
                 ("n_leaves", C.c_uint32), ("sah_cost", C.c_double)]
 
 
+class Bvh4(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint32), ("nodes", C.c_void_p),
+                ("depth", C.c_uint32)]
+
+
 class HdriTables(C.Structure):
     _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("pdf_joint", _fp),
                 ("marg_inv", _fp), ("cond_inv", _fp),
@@ -232,6 +237,9 @@ mrt_batch_last_error = _sig("mrt_batch_last_error", C.c_char_p, _bt_p)
 mrt_bvh_build = _sig("mrt_bvh_build", C.c_int, C.POINTER(Scene), C.c_uint32,
                      C.POINTER(Bvh))
 mrt_bvh_free = _sig("mrt_bvh_free", None, C.POINTER(Bvh))
+mrt_bvh4_build = _sig("mrt_bvh4_build", C.c_int, C.POINTER(Bvh),
+                      C.POINTER(Bvh4))
+mrt_bvh4_free = _sig("mrt_bvh4_free", None, C.POINTER(Bvh4))
 mrt_hs_new = _sig("mrt_hs_new", _hs_p)
 mrt_hs_free = _sig("mrt_hs_free", None, _hs_p)
 mrt_hs_add_material = _sig("mrt_hs_add_material", C.c_int, _hs_p,
@@ -477,6 +485,17 @@ class BuiltBvh:
                 C.cast(b.prims, C.POINTER(C.c_uint8)),
                 shape=(48 * b.n_prims,)).copy() if b.n_prims else
                 np.zeros(0, np.uint8))
+            b4 = Bvh4()
+            _check(mrt_bvh4_build(C.byref(b), C.byref(b4)))
+            try:
+                self.n_nodes4 = b4.n_nodes
+                self.depth4 = b4.depth
+                self.nodes4 = (np.ctypeslib.as_array(
+                    C.cast(b4.nodes, C.POINTER(C.c_uint8)),
+                    shape=(128 * b4.n_nodes,)).copy() if b4.n_nodes else
+                    np.zeros(0, np.uint8))
+            finally:
+                mrt_bvh4_free(C.byref(b4))
         finally:
             mrt_bvh_free(C.byref(b))
 
@@ -485,6 +504,12 @@ class BuiltBvh:
 
     def node_children(self):
         return self.nodes.view(np.int32).reshape(-1, 16)[:, 12:14]
+
+    def node4_floats(self):
+        return self.nodes4.view(np.float32).reshape(-1, 32)
+
+    def node4_children(self):
+        return self.nodes4.view(np.int32).reshape(-1, 32)[:, 24:28]
 
     def prim_words(self):
         return self.prims.view(np.uint32).reshape(-1, 12)

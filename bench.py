@@ -24,7 +24,7 @@ METRIC = ("Msamples/s @1080p (batched vs plain GPU vs host CPU) at ≤1% "
           "image error; adjoint/fwd time")
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of the encodings the kernels read and write
-B_NODE = 64          # one inner node: both child boxes + child links
+B_NODE = 128         # one 4-wide node: four child boxes + child links
 B_PRIM = 48          # one primitive record
 Q_RAY = 32           # ray read by the trace kernel (origin, direction)
 Q_HIT = 16           # hit record written

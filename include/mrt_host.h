@@ -62,6 +62,31 @@ typedef struct mrt_bvh {
 int  mrt_bvh_build(const mrt_scene *scene, uint32_t max_leaf, mrt_bvh *out);
 void mrt_bvh_free(mrt_bvh *bvh);
 
+/* A 4-wide node, the layout the device traverses: the boxes of four
+ * children as structure-of-arrays (one 16-byte word per bound and axis) and
+ * their codes, in two 64-byte lines.  Child codes follow mrt_bvh_node (>= 0:
+ * inner node, < 0: leaf); an unused slot has a box at 1e30 that no ray
+ * reaches. */
+typedef struct mrt_bvh4_node {
+    float   lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int32_t child[4];
+    uint32_t pad[4];
+} mrt_bvh4_node;                      /* 128 bytes */
+
+#define MRT_BVH4_MAX_DEPTH 40u        /* stack: 3 entries per level + 1   */
+
+typedef struct mrt_bvh4 {
+    uint32_t       n_nodes;
+    mrt_bvh4_node *nodes;             /* node 0 is the root, pre-order      */
+    uint32_t       depth;             /* deepest node level (root = 1)      */
+} mrt_bvh4;
+
+/* Collapses a binary BVH into a 4-wide one over the same primitive order:
+ * every node takes the two children of its binary node and keeps opening
+ * the inner child of largest surface area until it holds four. */
+int  mrt_bvh4_build(const mrt_bvh *bvh2, mrt_bvh4 *out);
+void mrt_bvh4_free(mrt_bvh4 *bvh4);
+
 /* ---- scene container ---------------------------------------------------- */
 
 typedef struct mrt_host_scene mrt_host_scene;

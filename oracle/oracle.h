@@ -31,13 +31,18 @@ extern "C" {
 
 #define ORC_TRAV_BRUTE      0   /* every primitive, no BVH                 */
 #define ORC_TRAV_REFERENCE  1   /* hit_bvh: both children, no culling      */
-#define ORC_TRAV_ORDERED    2   /* near child first, t-culled (HIP order)  */
+#define ORC_TRAV_ORDERED    2   /* near child first, t-culled (HIP order):
+                                   4-wide when the BVH carries nodes4        */
 
 typedef struct orc_bvh {
     const mrt_bvh_node *nodes;
     uint32_t            n_nodes;
     const mrt_prim     *prims;     /* only a[3] (source id) is read          */
     uint32_t            n_prims;
+    /* the 4-wide collapse of `nodes` (mrt_bvh4_build); when present the
+     * ordered traversal walks it, in the device's order */
+    const mrt_bvh4_node *nodes4;
+    uint32_t             n_nodes4;
 } orc_bvh;
 
 typedef struct orc_hdri {

@@ -20,7 +20,8 @@ TRAV_ORDERED = 2
 
 class OrcBvh(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("n_nodes", C.c_uint32),
-                ("prims", C.c_void_p), ("n_prims", C.c_uint32)]
+                ("prims", C.c_void_p), ("n_prims", C.c_uint32),
+                ("nodes4", C.c_void_p), ("n_nodes4", C.c_uint32)]
 
 
 class OrcHdri(C.Structure):
@@ -141,10 +142,13 @@ def raygen_f32(scene_view, w, h, sample, seed):
     return o, d
 
 
-def make_bvh(built):
-    """OrcBvh over a mrt_hip.BuiltBvh (keeps a reference to its arrays)."""
+def make_bvh(built, wide=True):
+    """OrcBvh over a mrt_hip.BuiltBvh (keeps a reference to its arrays); the
+    ordered traversal walks the 4-wide nodes unless wide is False."""
     b = OrcBvh(built.nodes.ctypes.data, built.n_nodes,
-               built.prims.ctypes.data, built.n_prims)
+               built.prims.ctypes.data, built.n_prims,
+               built.nodes4.ctypes.data if wide else None,
+               built.n_nodes4 if wide else 0)
     b._keep = built
     return b
 

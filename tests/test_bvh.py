@@ -78,3 +78,61 @@ def test_sah_beats_median_depth_bound():
     n = b.n_prims
     assert b.depth <= 2 * int(np.ceil(np.log2(n)))
     assert b.n_leaves >= n // 4
+
+
+def collect4(b):
+    """Walk the 4-wide collapse: every primitive once, child boxes inside
+    their parent's slot box, depth as reported."""
+    nf = b.node4_floats()
+    ch = b.node4_children()
+    seen = np.zeros(b.n_prims, np.int32)
+    stack = [(0, 1)]
+    max_depth = 0
+    while stack:
+        node, depth = stack.pop()
+        max_depth = max(max_depth, depth)
+        lo = np.stack([nf[node, 0:4], nf[node, 8:12], nf[node, 16:20]], 1)
+        hi = np.stack([nf[node, 4:8], nf[node, 12:16], nf[node, 20:24]], 1)
+        for k in range(4):
+            c = int(ch[node, k])
+            if lo[k, 0] >= 1e29:            # unused slot
+                continue
+            if c >= 0:
+                clo = np.stack([nf[c, 0:4], nf[c, 8:12], nf[c, 16:20]], 1)
+                chi = np.stack([nf[c, 4:8], nf[c, 12:16], nf[c, 20:24]], 1)
+                used = clo[:, 0] < 1e29
+                assert np.all(clo[used] >= lo[k]) and np.all(chi[used] <= hi[k])
+                stack.append((c, depth + 1))
+            else:
+                v = ~c & 0xFFFFFFFF
+                first, cnt = v >> LEAF_BITS, (v & 7) + 1
+                seen[first:first + cnt] += 1
+    return seen, max_depth
+
+
+@pytest.mark.parametrize("subdiv,floor", [(3, 8), (5, 32)])
+def test_bvh4_collapse_invariants(subdiv, floor):
+    hs = M.HostScene.mesh(11, 64, 64, subdiv=subdiv, floor_cells=floor)
+    b = M.BuiltBvh(hs.view())
+    seen, depth = collect4(b)
+    assert np.all(seen == 1)
+    assert depth == b.depth4 <= b.depth
+    # bottom nodes over two leaves keep two children: about half the
+    # binary inner nodes, at about half the depth
+    assert b.n_nodes4 < 0.6 * b.n_nodes
+    assert b.depth4 <= (b.depth + 1) // 2 + 2
+
+
+def test_bvh4_traversal_visits_fewer_nodes_with_the_same_hits():
+    import oracle_ffi as O
+    hs = M.HostScene.mesh(5, 48, 27, subdiv=5, floor_cells=32)
+    view = hs.view()
+    b = M.BuiltBvh(view)
+    o, d = O.raygen_f32(view, 48, 27, 0, 9)
+    id2, t2, n2, _ = O.trace_rays(view, O.make_bvh(b, wide=False),
+                                  O.TRAV_ORDERED, o, d)
+    id4, t4, n4, _ = O.trace_rays(view, O.make_bvh(b), O.TRAV_ORDERED, o, d)
+    idb, tb, _, _ = O.trace_rays(view, None, O.TRAV_BRUTE, o, d)
+    assert np.array_equal(id4, idb) and np.array_equal(id2, idb)
+    np.testing.assert_array_equal(t4, tb)
+    assert n4.sum() < 0.7 * n2.sum()
