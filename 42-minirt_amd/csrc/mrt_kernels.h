@@ -13,6 +13,7 @@ namespace mrt {
 #define MRT_TRACE_SS 24
 #endif
 constexpr int TRACE_BLOCK = MRT_TRACE_BLOCK;  // threads per trace block
+static_assert(TRACE_BLOCK % 64 == 0, "trace blocks are whole waves");
 constexpr int TRACE_SS = MRT_TRACE_SS;        // stack entries kept in LDS
 
 // device counters (uint32)

@@ -13,8 +13,12 @@ def declared_functions():
     names = set()
     for h in (ROOT / "include").glob("*.h"):
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        # header-only helpers are not exported
+        inline = set(re.findall(r"static\s+inline\s+[\w\s\*]*?\b(mrt_[a-z0-9_]+)\s*\(",
+                                text))
         for m in re.finditer(r"\b(mrt_[a-z0-9_]+)\s*\(", text):
-            names.add(m.group(1))
+            if m.group(1) not in inline:
+                names.add(m.group(1))
     return sorted(names)
 
 
