@@ -81,7 +81,7 @@ struct mrt_ctx {
 
     // statistics
     uint64_t     paths = 0;
-    double       ms[6] = {0, 0, 0, 0, 0, 0};
+    double       ms[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 static int fail(mrt_ctx *c, int code, const char *fmt, ...)
@@ -853,7 +853,7 @@ static int ensure_wavefront(mrt_ctx *c, uint32_t paths, uint32_t shadow)
 }
 
 namespace {
-enum { K_RAYGEN = 0, K_TRACE, K_SHADE, K_SHADOW, K_ACCUM, K_RESOLVE };
+enum { K_RAYGEN = 0, K_TRACE, K_SHADE, K_SHADOW, K_ACCUM, K_RESOLVE, K_CAUSTIC };
 struct EvPair {
     int         cls;
     hipEvent_t  a, b;
@@ -885,7 +885,8 @@ struct Timer {
         for (EvPair &p : ev) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-                int slot = p.cls == K_RESOLVE ? K_SHADE : p.cls;
+                int slot = p.cls == K_RESOLVE ? K_SHADE
+                           : (p.cls == K_CAUSTIC ? 6 : p.cls);
                 c->ms[slot] += ms;
             }
             (void)hipEventDestroy(p.a);
@@ -990,7 +991,7 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
                                 c->d_stats));
             LAUNCH(c, tm, K_SHADE, launch_shade(c->stream, gb, fp, B, c->d_ctr));
             if (c->has_photons)
-                LAUNCH(c, tm, K_SHADE,
+                LAUNCH(c, tm, K_CAUSTIC,
                        launch_caustic(c->stream, c->grid_blocks, fp, B, c->d_ctr));
             if (!c->lights.empty() || c->has_env || c->has_photons) {
                 LAUNCH(c, tm, K_SHADOW,
@@ -1195,6 +1196,7 @@ extern "C" int mrt_get_stats(mrt_ctx *c, mrt_stats *out)
     out->ms_shadow = c->ms[K_SHADOW];
     out->ms_accum = c->ms[K_ACCUM];
     out->ms_total = c->ms[5];
+    out->ms_caustic = c->ms[6];
     return MRT_OK;
 }
 

@@ -1651,27 +1651,32 @@ k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
 }
 
 // estimate_caustics_physically_based / calculate_caustic_radiance
-// (src/caustic/get_caustic.c:37-88) over the 27 cells around x: the k
+// (src/caustic/get_caustic.c:37-88) over the cells around x: the k
 // nearest photons within squared distance `radius` (k > 0; nothing unless
 // k are found) or all photons within distance `radius` (k == 0), summed
 // and divided by pi r^2 and the norm factor.  A photon counts only from
 // its own cell, so a hash collision between neighbours cannot count it
 // twice.  kd / ki are this thread's k-NN lists in LDS (stride `stride`).
 constexpr int KNN_MAX = 32;
+template <bool KNN>
 __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                                          float *kd, uint32_t *ki, int stride)
 {
-    const int cx = (int)floorf(x.x / pm.cell), cy = (int)floorf(x.y / pm.cell),
-              cz = (int)floorf(x.z / pm.cell);
-    const bool knn = pm.k != 0;
+    // the cells meeting the cube [x - s, x + s]^3, s = the search distance
+    // (the cell edge, widened by 1e-4 against rounding): two or three per
+    // axis instead of 27 cells
+    const float sd = pm.cell * 1.0001f;
+    const int x0 = (int)floorf((x.x - sd) / pm.cell), x1 = (int)floorf((x.x + sd) / pm.cell);
+    const int y0 = (int)floorf((x.y - sd) / pm.cell), y1 = (int)floorf((x.y + sd) / pm.cell);
+    const int z0 = (int)floorf((x.z - sd) / pm.cell), z1 = (int)floorf((x.z + sd) / pm.cell);
+    const bool knn = KNN;
     const uint32_t K = pm.k < (uint32_t)KNN_MAX ? pm.k : (uint32_t)KNN_MAX;
     const float cut = knn ? pm.radius : pm.radius * pm.radius;
     V3 sum = v3(0.f, 0.f, 0.f);
     uint32_t cnt = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
+    for (int iz = z0; iz <= z1; iz++)
+        for (int iy = y0; iy <= y1; iy++)
+            for (int ix = x0; ix <= x1; ix++) {
                 uint32_t b = hash_cell(ix, iy, iz, pm.table);
                 for (uint32_t j = pm.cstart[b]; j < pm.cend[b]; j++) {
                     float4 P = pm.pos[j];
@@ -1718,31 +1723,36 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
 
 constexpr int CAUSTIC_BLOCK = 128;
 
+// The k-NN lists live in LDS; the fixed-radius gather needs none, so its
+// instantiation keeps the CU's LDS free for more resident waves.
+template <bool KNN>
 __global__ void __launch_bounds__(CAUSTIC_BLOCK)
 k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
-    __shared__ float s_d[KNN_MAX * CAUSTIC_BLOCK];
-    __shared__ uint32_t s_i[KNN_MAX * CAUSTIC_BLOCK];
+    __shared__ float s_d[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
+    __shared__ uint32_t s_i[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     const uint32_t n = min(ctr[CTR_PEND], B.cap_paths);
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
-        V3 c = caustic_at(fp.pm, xyz(B.p_x[r]), s_d + threadIdx.x,
-                          s_i + threadIdx.x, CAUSTIC_BLOCK);
+        V3 c = caustic_at<KNN>(fp.pm, xyz(B.p_x[r]), s_d + (KNN ? threadIdx.x : 0),
+                               s_i + (KNN ? threadIdx.x : 0), CAUSTIC_BLOCK);
         B.p_k[r] = make_float4(c.x, c.y, c.z, 0.0f);
     }
 }
 
+template <bool KNN>
 __global__ void __launch_bounds__(CAUSTIC_BLOCK)
 k_caustic_at(PhotonMapDev pm, uint32_t n, const float *__restrict__ pts,
              float *__restrict__ out)
 {
-    __shared__ float s_d[KNN_MAX * CAUSTIC_BLOCK];
-    __shared__ uint32_t s_i[KNN_MAX * CAUSTIC_BLOCK];
+    __shared__ float s_d[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
+    __shared__ uint32_t s_i[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    V3 c = caustic_at(pm, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),
-                      s_d + threadIdx.x, s_i + threadIdx.x, CAUSTIC_BLOCK);
+    V3 c = caustic_at<KNN>(pm, v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),
+                           s_d + (KNN ? threadIdx.x : 0),
+                           s_i + (KNN ? threadIdx.x : 0), CAUSTIC_BLOCK);
     out[3 * i] = c.x;
     out[3 * i + 1] = c.y;
     out[3 * i + 2] = c.z;
@@ -1800,16 +1810,25 @@ hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
                           uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_caustic, dim3(blocks), dim3(CAUSTIC_BLOCK), 0, st, fp,
-                       B, ctr);
+    if (fp.pm.k)
+        hipLaunchKernelGGL(k_caustic<true>, dim3(blocks), dim3(CAUSTIC_BLOCK), 0,
+                           st, fp, B, ctr);
+    else
+        hipLaunchKernelGGL(k_caustic<false>, dim3(blocks), dim3(CAUSTIC_BLOCK), 0,
+                           st, fp, B, ctr);
     return hipGetLastError();
 }
 
 hipError_t launch_caustic_at(hipStream_t st, const PhotonMapDev &pm,
                              uint32_t n, const float *pts, float *out)
 {
-    hipLaunchKernelGGL(k_caustic_at, dim3((n + CAUSTIC_BLOCK - 1) / CAUSTIC_BLOCK),
-                       dim3(CAUSTIC_BLOCK), 0, st, pm, n, pts, out);
+    dim3 g((n + CAUSTIC_BLOCK - 1) / CAUSTIC_BLOCK);
+    if (pm.k)
+        hipLaunchKernelGGL(k_caustic_at<true>, g, dim3(CAUSTIC_BLOCK), 0, st, pm, n,
+                           pts, out);
+    else
+        hipLaunchKernelGGL(k_caustic_at<false>, g, dim3(CAUSTIC_BLOCK), 0, st, pm, n,
+                           pts, out);
     return hipGetLastError();
 }
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,

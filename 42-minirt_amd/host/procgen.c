@@ -359,8 +359,39 @@ static double floor_height(double x, double z, double amp, uint64_t seed)
     return amp * fbm3(x * 0.8, 0.37, z * 0.8, seed);
 }
 
+/* Water: y0 plus a sum of eight seeded travelling sine ripples. */
+static double water_height(double x, double z, double amp, uint64_t seed)
+{
+    uint64_t st = seed;
+    double y = 0.0;
+    for (int k = 0; k < 8; k++) {
+        double ang = 2.0 * PI_D * mrt_u01(&st);
+        double f = 2.0 + 10.0 * mrt_u01(&st);
+        double ph = 2.0 * PI_D * mrt_u01(&st);
+        y += sin(f * (x * cos(ang) + z * sin(ang)) + ph) / (1.0 + 0.3 * k);
+    }
+    return amp * y / 3.0;
+}
+
+typedef double (*height_fn)(double, double, double, uint64_t);
+
+static int add_heightfield(mrt_host_scene *hs, uint32_t cells, double half,
+                           double y0, double amp, uint64_t seed, height_fn fn,
+                           int fixed_mat, uint32_t base);
+
 static int add_floor(mrt_host_scene *hs, uint32_t cells, double half,
                      double amp, uint64_t seed, uint32_t base)
+{
+    return add_heightfield(hs, cells, half, 0.0, amp, seed, floor_height, -1,
+                           base);
+}
+
+/* A cells x cells grid over [-half, half]^2 at height y0 + fn(x, z),
+ * two triangles per cell with finite-difference vertex normals; material
+ * fixed_mat, or drawn from the mix palette at base when fixed_mat < 0. */
+static int add_heightfield(mrt_host_scene *hs, uint32_t cells, double half,
+                           double y0, double amp, uint64_t seed, height_fn fn,
+                           int fixed_mat, uint32_t base)
 {
     uint32_t nv = cells + 1;
     float *P = malloc((size_t)nv * nv * 3 * sizeof(float));
@@ -374,11 +405,9 @@ static int add_floor(mrt_host_scene *hs, uint32_t cells, double half,
     for (uint32_t j = 0; j < nv; j++)
         for (uint32_t i = 0; i < nv; i++) {
             double x = -half + step * i, z = -half + step * j;
-            double y = floor_height(x, z, amp, seed);
-            double dx = (floor_height(x + h, z, amp, seed)
-                         - floor_height(x - h, z, amp, seed)) / (2 * h);
-            double dz = (floor_height(x, z + h, amp, seed)
-                         - floor_height(x, z - h, amp, seed)) / (2 * h);
+            double y = y0 + fn(x, z, amp, seed);
+            double dx = (fn(x + h, z, amp, seed) - fn(x - h, z, amp, seed)) / (2 * h);
+            double dz = (fn(x, z + h, amp, seed) - fn(x, z - h, amp, seed)) / (2 * h);
             double n[3] = {-dx, 1.0, -dz};
             normalize3(n);
             size_t o = ((size_t)j * nv + i) * 3;
@@ -403,7 +432,9 @@ static int add_floor(mrt_host_scene *hs, uint32_t cells, double half,
                     memcpy(v + 3 * k, P + q[tri[t][k]] * 3, 3 * sizeof(float));
                     memcpy(n + 3 * k, N + q[tri[t][k]] * 3, 3 * sizeof(float));
                 }
-                int r = mrt_hs_add_triangle(hs, v, n, draw_mix(&mst, base));
+                uint32_t m = fixed_mat >= 0 ? (uint32_t)fixed_mat
+                                            : draw_mix(&mst, base);
+                int r = mrt_hs_add_triangle(hs, v, n, m);
                 if (r < 0) {
                     err = r;
                     break;
@@ -461,6 +492,52 @@ int mrt_gen_mesh_scene(uint64_t seed, uint32_t width, uint32_t height,
     cam.fov_deg = 60.0f;
     cam.focus_dist = 1.0f;
     mrt_hs_set_camera(hs, &cam);
+    *out = hs;
+    return MRT_OK;
+}
+
+/* ---- HDRI + caustics scene ---------------------------------------------- */
+
+int mrt_gen_caustic_scene(uint64_t seed, uint32_t width, uint32_t height,
+                          uint32_t subdiv, uint32_t floor_cells,
+                          uint32_t water_cells, mrt_host_scene **out)
+{
+    if (!out)
+        return MRT_E_INVALID;
+    mrt_host_scene *hs = NULL;
+    int r = mrt_gen_mesh_scene(seed, width, height, subdiv, floor_cells, 0.15f,
+                               &hs);
+    if (r < 0)
+        return r;
+    /* lit by the environment only */
+    hs->n_light = 0;
+    float white[3] = {255, 255, 255};
+    mrt_hs_set_ambient(hs, 1.0f, white);
+    mrt_material water = mat_glass(1.33f);
+    water.color[0] = 0.92f;
+    water.color[1] = 0.97f;
+    water.color[2] = 1.0f;
+    int wm = mrt_hs_add_material(hs, &water, "water");
+    mrt_material glass = mat_glass(1.5f);
+    int gm = mrt_hs_add_material(hs, &glass, "caustic_glass");
+    if (wm < 0 || gm < 0) {
+        mrt_hs_free(hs);
+        return wm < 0 ? wm : gm;
+    }
+    if (water_cells) {
+        r = add_heightfield(hs, water_cells, 4.0, 0.16, 0.012, seed ^ 0x77A7E5ull,
+                            water_height, wm, 0);
+        if (r < 0) {
+            mrt_hs_free(hs);
+            return r;
+        }
+    }
+    float c[3] = {1.9f, 0.75f, -0.9f};
+    r = mrt_hs_add_sphere(hs, c, 0.5f, (uint32_t)gm);
+    if (r < 0) {
+        mrt_hs_free(hs);
+        return r;
+    }
     *out = hs;
     return MRT_OK;
 }

@@ -118,7 +118,7 @@ class Stats(C.Structure):
         "launches_shadow", "launches_shade", "nonfinite")] + \
         [(n, C.c_double) for n in (
             "ms_raygen", "ms_trace", "ms_shade", "ms_shadow", "ms_accum",
-            "ms_total")]
+            "ms_total", "ms_caustic")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -270,6 +270,9 @@ mrt_gen_plain_scene = _sig("mrt_gen_plain_scene", C.c_int, C.c_uint64,
 mrt_gen_mesh_scene = _sig("mrt_gen_mesh_scene", C.c_int, C.c_uint64,
                           C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                           C.c_float, C.POINTER(_hs_p))
+mrt_gen_caustic_scene = _sig("mrt_gen_caustic_scene", C.c_int, C.c_uint64,
+                             C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                             C.c_uint32, C.POINTER(_hs_p))
 mrt_gen_icosphere = _sig("mrt_gen_icosphere", C.c_int, _hs_p, _fp, C.c_float,
                          C.c_uint32, C.c_float, C.c_uint64, C.c_int,
                          C.c_uint32)
@@ -343,6 +346,14 @@ class HostScene:
         h = _hs_p()
         _check(mrt_gen_mesh_scene(seed, width, height, subdiv, floor_cells,
                                   amp, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def caustic(cls, seed, width=1920, height=1080, subdiv=8, floor_cells=256,
+                water_cells=512):
+        h = _hs_p()
+        _check(mrt_gen_caustic_scene(seed, width, height, subdiv, floor_cells,
+                                     water_cells, C.byref(h)))
         return cls(h)
 
     @classmethod
@@ -594,6 +605,11 @@ class Renderer:
         if norm is not None:
             self.set_option(OPT_PHOTON_NORM, norm)
         self._ck(mrt_build_photons(self._ctx, n_per, radius))
+
+    def photon_count(self):
+        n = C.c_uint32(0)
+        self._ck(mrt_get_photons(self._ctx, 0, None, None, None, C.byref(n)))
+        return n.value
 
     def photons(self):
         """Stored photons in map order: (pos[n,3], power[n,3], ids[n])."""

@@ -360,6 +360,62 @@ def adjoint_leg(M, args):
     }
 
 
+def caustics_leg(M, args):
+    """The mesh scene lit only by a 2048x1024 sky with a 0.5 degree sun of
+    radiance 5e4, plus a glass sphere and a 512x512 water heightfield: MIS
+    environment sampling and 4M environment photons gathered within 0.02."""
+    W, H, spp = args.width, args.height, args.spp_per_step
+    t0 = time.perf_counter()
+    hs = M.HostScene.caustic(args.seed, W, H, subdiv=args.subdiv,
+                             floor_cells=args.floor, water_cells=512)
+    ntri = hs.view().contents.n_triangles
+    sky = M.hdri_sky(args.seed, 2048, 1024, sun_radius_deg=0.5,
+                     sun_radiance=5e4)
+    r = M.Renderer(W, H)
+    r.upload(hs)
+    t1 = time.perf_counter()
+    r.upload_hdri(sky)
+    t_tab = time.perf_counter() - t1
+    r.set_option(M.OPT_ESTIMATOR, M.EST_MIS)
+    r.set_seed(0xCA05)
+    r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
+    log(f"caustics: {ntri} triangles, setup {time.perf_counter() - t0:.1f} s "
+        f"(HDRI tables {t_tab:.2f} s)")
+    r.build_photons(1000, 0.02, k=0, norm=1.0)      # warm-up
+    t1 = time.perf_counter()
+    r.build_photons(4_000_000, 0.02, k=0, norm=1.0)
+    n_ph = r.photon_count()
+    t_ph = time.perf_counter() - t1
+    log(f"caustics: 4M photons emitted, {n_ph} stored, {t_ph * 1e3:.1f} ms")
+    r.render(0, spp, args.depth)
+    r.synchronize()
+    steps = max(4, args.steps // 3)
+    t1 = time.perf_counter()
+    for s in range(steps):
+        r.render(spp * (s + 1), spp, args.depth)
+    r.synchronize()
+    dt = time.perf_counter() - t1
+    value = steps * spp * W * H / dt / 1e6
+    r.set_option(M.OPT_STATS, M.STATS_TIME)
+    r.reset_stats()
+    r.render(0, spp, args.depth)
+    tm = r.stats()
+    r.close()
+    log(f"caustics: {value:.1f} Msamples/s")
+    return {
+        "workload": f"mesh scene + glass sphere + 512^2 water heightfield "
+                    f"({ntri} triangles), {W}x{H}, 2048x1024 HDRI (0.5 deg "
+                    f"sun, 5e4), MIS, 4M photons gathered within 0.02, depth "
+                    f"{args.depth}, {spp} spp per step",
+        "msamples_per_s": round(value, 2), "steps": steps,
+        "photon_build_ms": round(t_ph * 1e3, 1), "photons_stored": n_ph,
+        "hdri_tables_ms": round(t_tab * 1e3, 1),
+        "kernel_ms_per_step": {k: round(tm[f"ms_{k}"], 3)
+                               for k in ("raygen", "trace", "shade", "caustic",
+                                         "shadow", "accum", "total")},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +436,8 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=4)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-caustics", action="store_true")
+    ap.add_argument("--only-caustics", action="store_true")
     ap.add_argument("--only-adjoint", action="store_true")
     ap.add_argument("--adam-steps", type=int, default=200)
     ap.add_argument("--adj-seed", type=int, default=5)
@@ -408,6 +466,9 @@ def main():
         return
     if args.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(M, args)}), flush=True)
+        return
+    if args.only_caustics:
+        print(json.dumps({"caustics": caustics_leg(M, args)}), flush=True)
         return
     W, H = args.width, args.height
     t0 = time.perf_counter()
@@ -494,8 +555,11 @@ def main():
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4)},
     }
-    if not args.no_batched or not args.no_adjoint:
+    extra = not (args.no_batched and args.no_adjoint and args.no_caustics)
+    if extra:
         r.close()
+    if not args.no_caustics:
+        out["caustics"] = caustics_leg(M, args)
     if not args.no_batched:
         out["batched"] = batched_leg(M, args)
     if not args.no_adjoint:
@@ -504,7 +568,7 @@ def main():
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(hs, W, H, args.depth, 0x5EED,
                                            args.cpu_budget)
-    if args.no_batched and args.no_adjoint:
+    if not extra:
         r.close()
     print(json.dumps(out), flush=True)
 

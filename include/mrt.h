@@ -153,6 +153,7 @@ typedef struct mrt_stats {
     double   ms_shadow;
     double   ms_accum;
     double   ms_total;         /* mrt_render wall time on the stream          */
+    double   ms_caustic;       /* photon-map gather                           */
 } mrt_stats;
 
 typedef struct mrt_ctx mrt_ctx;

@@ -134,6 +134,14 @@ int  mrt_gen_mesh_scene(uint64_t seed, uint32_t width, uint32_t height,
                         uint32_t subdiv, uint32_t floor_cells, float amp,
                         mrt_host_scene **out);
 
+/* The HDRI + caustics scene: the mesh scene without its point light
+ * (ambient ratio 1, so the HDRI is unscaled), a rippled water heightfield of
+ * water_cells^2 cells (IOR 1.33) just above the floor and a glass sphere
+ * (IOR 1.5) beside the icosphere. */
+int  mrt_gen_caustic_scene(uint64_t seed, uint32_t width, uint32_t height,
+                           uint32_t subdiv, uint32_t floor_cells,
+                           uint32_t water_cells, mrt_host_scene **out);
+
 /* Adds a displaced icosphere to hs; returns the triangle count or < 0. */
 int  mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
                        uint32_t subdiv, float amp, uint64_t seed,
