@@ -42,7 +42,8 @@ int main(int argc, char **argv)
     const char *load = NULL, *v;
     uint32_t spp = 64, bounces = 10, per_tick = 1;
     uint64_t seed = 42;
-    int auto_export = 0, auto_save = 0, one_sided = 0, device = 0;
+    int auto_export = 0, auto_save = 0, one_sided = 0, device = 0, mis = 0;
+    const char *caustic = NULL;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         if ((v = opt_val(a, "--max-samples")))
@@ -67,6 +68,10 @@ int main(int argc, char **argv)
             auto_save = 1;
         else if (!strcmp(a, "--triangle-one-sided"))
             one_sided = 1;
+        else if ((v = opt_val(a, "--caustic")))
+            caustic = v;             /* N;RADIUS;NORM (option_caustic.c) */
+        else if (!strcmp(a, "--mis"))
+            mis = 1;
         else if (!strcmp(a, "--no-display") || !strcmp(a, "--compute-gpu"))
             ;
         else if (a[0] != '-' && !scene_path)
@@ -83,6 +88,8 @@ int main(int argc, char **argv)
         rc = mrt_gen_plain_scene(seed, 640, 360, &hs);
     else if (procedural && !strcmp(procedural, "mesh"))
         rc = mrt_gen_mesh_scene(seed, 1920, 1080, 8, 256, 0.15f, &hs);
+    else if (procedural && !strcmp(procedural, "caustic"))
+        rc = mrt_gen_caustic_scene(seed, 1920, 1080, 8, 256, 512, &hs);
     else if (scene_path)
         rc = mrt_parse_rt_file(scene_path, &hs, err, sizeof(err));
     else {
@@ -109,6 +116,32 @@ int main(int argc, char **argv)
         mrt_destroy(ctx);
         mrt_hs_free(hs);
         return 1;
+    }
+    const float *sky = NULL;
+    uint32_t sw = 0, sh = 0;
+    float sex = 0.0f;
+    if (mrt_hs_get_hdri(hs, &sky, &sw, &sh, &sex) == MRT_OK
+        && (rc = mrt_upload_hdri(ctx, sky, sw, sh, sex)) != MRT_OK) {
+        fprintf(stderr, "Error
+%s
+", mrt_last_error(ctx));
+        mrt_destroy(ctx);
+        mrt_hs_free(hs);
+        return 1;
+    }
+    if (mis)
+        mrt_set_option(ctx, MRT_OPT_ESTIMATOR, MRT_EST_MIS);
+    if (caustic) {
+        unsigned long np = 0;
+        double rad = 0.0, norm = 1.0;
+        if (sscanf(caustic, "%lu;%lf;%lf", &np, &rad, &norm) < 2
+            || mrt_set_option(ctx, MRT_OPT_PHOTON_NORM, norm) != MRT_OK
+            || mrt_build_photons(ctx, (uint32_t)np, (float)rad) != MRT_OK) {
+            fprintf(stderr, "Warning
+caustics will be disabled: %s
+",
+                    mrt_last_error(ctx));
+        }
     }
     double *acc = calloc((size_t)w * h * 3, sizeof(double));
     mrt_frame f = {ctx, w, h, 0, spp, bounces, per_tick, acc};

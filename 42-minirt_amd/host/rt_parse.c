@@ -10,8 +10,13 @@
  *   mat name color metallic roughness ior transmission ao emission_strength
  *       emission_color [scale]          (parse_elements3.c:59-136)
  *   W win_w win_h [render_w render_h]   (parse_elements5.c:44-83)
- * Textures, meshes and quadrics (tex cu cy co hb) are reported as
- * unsupported.  Numbers go through strtod; the reference's ft_atod is not
+ *   tex name image file.hdr             (parse_elements5.c:85-142)
+ *   A ratio texname                     an HDRI skybox (get_hdr_pixel_skybox)
+ *   cu pos dir scale file.obj r,g,b|mat (parse_elements4.c:101-128)
+ * Image textures other than .hdr skyboxes, checker textures, textured
+ * materials and quadrics (cy co hb) are reported as unsupported.  Relative
+ * file paths are tried from the working directory, then from the .rt file's
+ * directory.  Numbers go through strtod; the reference's ft_atod is not
  * available, so the rounding of parsed decimals is not pinned to it.
  */
 #include "mrt_host_internal.h"
@@ -25,13 +30,60 @@
 
 #define MAXTOK 16
 
+#define MAXTEX 64
+
+typedef struct {
+    char name[64];
+    char path[1024];
+    int  is_hdr;              /* an image texture in Radiance format */
+} texdef;
+
 typedef struct {
     mrt_host_scene *hs;
     char           *err;
     size_t          errlen;
     int             line;
     int             n_amb, n_cam, n_L, n_W;
+    char            dir[1024];        /* directory of the .rt file, or "" */
+    texdef          tex[MAXTEX];
+    int             n_tex;
 } parser;
+
+/* path as given if it opens, else relative to the .rt file's directory */
+static void resolve(const parser *P, const char *path, char *out, size_t cap)
+{
+    snprintf(out, cap, "%s", path);
+    FILE *f = fopen(out, "rb");
+    if (f) {
+        fclose(f);
+        return;
+    }
+    size_t a = strlen(P->dir), b = strlen(path);
+    if (path[0] != '/' && a && a + 1 + b < cap) {
+        memcpy(out, P->dir, a);
+        out[a] = '/';
+        memcpy(out + a + 1, path, b + 1);
+    }
+}
+
+static int ends_with(const char *s, const char *suf)
+{
+    size_t a = strlen(s), b = strlen(suf);
+    if (a < b)
+        return 0;
+    for (size_t i = 0; i < b; i++)
+        if (tolower((unsigned char)s[a - b + i]) != suf[i])
+            return 0;
+    return 1;
+}
+
+static const texdef *find_tex(const parser *P, const char *name)
+{
+    for (int i = 0; i < P->n_tex; i++)
+        if (strcmp(P->tex[i].name, name) == 0)
+            return &P->tex[i];
+    return NULL;
+}
 
 static int perr(parser *P, int code, const char *msg, const char *tok)
 {
@@ -177,20 +229,102 @@ static int p_mat(parser *P, char **t, int n)
     return r < 0 ? r : MRT_OK;
 }
 
+/* tex name image path | tex name checker_local|checker_global ... */
+static int p_tex(parser *P, char **t, int n)
+{
+    if (n < 3)
+        return perr(P, MRT_E_INVALID, "bad texture", NULL);
+    if (P->n_tex == MAXTEX)
+        return perr(P, MRT_E_LIMIT, "too many textures", t[1]);
+    if (find_tex(P, t[1]))
+        return perr(P, MRT_E_INVALID, "texture defined twice", t[1]);
+    texdef *d = &P->tex[P->n_tex++];
+    memset(d, 0, sizeof(*d));
+    snprintf(d->name, sizeof(d->name), "%s", t[1]);
+    if (strcmp(t[2], "image") == 0) {
+        if (n != 4)
+            return perr(P, MRT_E_INVALID, "bad image texture", t[1]);
+        resolve(P, t[3], d->path, sizeof(d->path));
+        d->is_hdr = ends_with(t[3], ".hdr");
+    } else if (strcmp(t[2], "checker_local") != 0
+               && strcmp(t[2], "checker_global") != 0) {
+        return perr(P, MRT_E_INVALID, "unknown texture kind", t[2]);
+    }
+    return MRT_OK;
+}
+
+/* A ratio texname: the HDRI skybox.  Texels are c 2^(e-128)/255 scaled at
+ * lookup by 2^(exposure + ratio - 1); a GAMMA other than 1 is applied after
+ * that scale (get_hdr_pixel_skybox), so it is baked in here with an
+ * exposure that makes the lookup scale 1. */
+static int set_skybox(parser *P, const texdef *d, float ratio)
+{
+    if (!d->is_hdr)
+        return perr(P, MRT_E_UNSUPPORTED, "skybox textures must be .hdr images",
+                    d->name);
+    float *rgb = NULL, ex = 0.0f, ga = 1.0f;
+    uint32_t w = 0, h = 0;
+    int r = mrt_load_hdr(d->path, &rgb, &w, &h, &ex, &ga);
+    if (r < 0)
+        return perr(P, r, "cannot read the HDR image", d->path);
+    if (ga != 1.0f && ga > 0.0f) {
+        double sc = pow(2.0, (double)ex + ratio - 1.0);
+        for (size_t i = 0; i < (size_t)w * h * 3; i++)
+            rgb[i] = (float)pow(rgb[i] * sc, 1.0 / ga);
+        ex = 1.0f - ratio;
+    }
+    r = mrt_hs_set_hdri(P->hs, rgb, w, h, ex);
+    mrt_free(rgb);
+    return r < 0 ? perr(P, r, "HDRI", d->name) : MRT_OK;
+}
+
 static int p_line(parser *P, char **t, int n, int pass)
 {
     const char *id = t[0];
-    if (pass == 0)
-        return strcmp(id, "mat") == 0 ? p_mat(P, t, n) : MRT_OK;
-    if (strcmp(id, "mat") == 0)
+    if (pass == 0) {
+        if (strcmp(id, "mat") == 0)
+            return p_mat(P, t, n);
+        if (strcmp(id, "tex") == 0)
+            return p_tex(P, t, n);
+        return MRT_OK;
+    }
+    if (strcmp(id, "mat") == 0 || strcmp(id, "tex") == 0)
         return MRT_OK;
     if (strcmp(id, "A") == 0) {
         double r;
         float c[3];
-        if (n != 3 || !num(t[1], &r) || r < 0 || r > 1 || !color(t[2], c))
+        if (n != 3 || !num(t[1], &r) || r < 0 || r > 1)
             return perr(P, MRT_E_INVALID, "bad ambient light", NULL);
+        if (!strchr(t[2], ',')) {
+            const texdef *d = find_tex(P, t[2]);
+            if (!d)
+                return perr(P, MRT_E_INVALID, "unknown texture", t[2]);
+            int rc = set_skybox(P, d, (float)r);
+            if (rc != MRT_OK)
+                return rc;
+            c[0] = c[1] = c[2] = 255.0f;
+        } else if (!color(t[2], c)) {
+            return perr(P, MRT_E_INVALID, "bad ambient light", NULL);
+        }
         mrt_hs_set_ambient(P->hs, (float)r, c);
         P->n_amb++;
+        return MRT_OK;
+    }
+    if (strcmp(id, "cu") == 0) {
+        float pos[3], dir[3];
+        double sc;
+        uint32_t mat;
+        if (n != 6 || !pos3(t[1], pos) || !unit_vec(t[2], dir) || !num(t[3], &sc)
+            || !(sc > 0))
+            return perr(P, MRT_E_INVALID, "bad custom object", NULL);
+        int rc = color_or_mat(P, t[5], &mat);
+        if (rc != MRT_OK)
+            return rc;
+        char path[1024];
+        resolve(P, t[4], path, sizeof(path));
+        rc = mrt_hs_add_obj(P->hs, path, pos, dir, (float)sc, mat);
+        if (rc < 0)
+            return perr(P, rc, "cannot read the OBJ mesh", t[4]);
         return MRT_OK;
     }
     if (strcmp(id, "C") == 0) {
@@ -284,20 +418,21 @@ static int p_line(parser *P, char **t, int n, int pass)
         P->n_W++;
         return MRT_OK;
     }
-    if (!strcmp(id, "cy") || !strcmp(id, "co") || !strcmp(id, "hb")
-        || !strcmp(id, "cu") || !strcmp(id, "tex"))
+    if (!strcmp(id, "cy") || !strcmp(id, "co") || !strcmp(id, "hb"))
         return perr(P, MRT_E_UNSUPPORTED, "element not supported by this renderer", id);
     return perr(P, MRT_E_INVALID, "unknown element", id);
 }
 
-int mrt_parse_rt_string(const char *text, mrt_host_scene **out, char *err,
-                        size_t errlen)
+static int parse_text(const char *text, const char *dir, mrt_host_scene **out,
+                      char *err, size_t errlen)
 {
     if (!text || !out)
         return MRT_E_INVALID;
     *out = NULL;
     parser P;
     memset(&P, 0, sizeof(P));
+    if (dir)
+        snprintf(P.dir, sizeof(P.dir), "%s", dir);
     P.err = err;
     P.errlen = errlen;
     if (err && errlen)
@@ -354,6 +489,12 @@ int mrt_parse_rt_string(const char *text, mrt_host_scene **out, char *err,
     return MRT_OK;
 }
 
+int mrt_parse_rt_string(const char *text, mrt_host_scene **out, char *err,
+                        size_t errlen)
+{
+    return parse_text(text, NULL, out, err, errlen);
+}
+
 int mrt_parse_rt_file(const char *path, mrt_host_scene **out, char *err,
                       size_t errlen)
 {
@@ -380,7 +521,14 @@ int mrt_parse_rt_file(const char *path, mrt_host_scene **out, char *err,
     size_t rd = fread(text, 1, (size_t)sz, f);
     fclose(f);
     text[rd] = 0;
-    int rc = mrt_parse_rt_string(text, out, err, errlen);
+    char dir[1024];
+    snprintf(dir, sizeof(dir), "%s", path);
+    char *slash = strrchr(dir, '/');
+    if (slash)
+        *slash = 0;
+    else
+        dir[0] = 0;
+    int rc = parse_text(text, dir, out, err, errlen);
     free(text);
     return rc;
 }

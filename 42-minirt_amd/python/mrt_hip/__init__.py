@@ -287,6 +287,12 @@ mrt_parse_rt_file = _sig("mrt_parse_rt_file", C.c_int, C.c_char_p,
                          C.POINTER(_hs_p), C.c_char_p, C.c_size_t)
 mrt_parse_rt_string = _sig("mrt_parse_rt_string", C.c_int, C.c_char_p,
                            C.POINTER(_hs_p), C.c_char_p, C.c_size_t)
+mrt_decode_hdr = _sig("mrt_decode_hdr", C.c_int, C.c_char_p, C.c_size_t,
+                      C.POINTER(_fp), _up, _up, C.POINTER(C.c_float),
+                      C.POINTER(C.c_float))
+mrt_free = _sig("mrt_free", None, C.c_void_p)
+mrt_hs_add_obj_text = _sig("mrt_hs_add_obj_text", C.c_int, _hs_p, C.c_char_p,
+                           _fp, _fp, C.c_float, C.c_uint32)
 mrt_export_ppm = _sig("mrt_export_ppm", C.c_int, C.c_char_p,
                       C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
                       C.c_uint32, C.c_double)
@@ -453,6 +459,23 @@ class HostScene:
         return _check(mrt_gen_icosphere(self._h, c.ctypes.data_as(_fp),
                                         radius, subdiv, amp, seed,
                                         int(mix), mat))
+
+    def add_obj_text(self, text, pos, direction, scale, mat):
+        p = np.asarray(pos, np.float32)
+        d = np.asarray(direction, np.float32)
+        return _check(mrt_hs_add_obj_text(self._h, text.encode(),
+                                          p.ctypes.data_as(_fp),
+                                          d.ctypes.data_as(_fp), scale, mat))
+
+    def hdri(self):
+        """(rgb[h, w, 3], exposure) of the scene's HDRI, or None."""
+        p = _fp()
+        w, h, ex = C.c_uint32(0), C.c_uint32(0), C.c_float(0)
+        if mrt_hs_get_hdri(self._h, C.byref(p), C.byref(w), C.byref(h),
+                           C.byref(ex)) < 0:
+            return None
+        a = np.ctypeslib.as_array(p, shape=(h.value, w.value, 3)).copy()
+        return a, ex.value
 
     def set_hdri(self, rgb, exposure=0.0):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
@@ -794,6 +817,20 @@ class Batch:
         s = Stats()
         self._ck(mrt_batch_get_stats(self._b, C.byref(s)))
         return s.as_dict()
+
+
+def decode_hdr(data):
+    """Radiance .hdr bytes -> (rgb[h, w, 3] float32, exposure, gamma)."""
+    p = _fp()
+    w, h = C.c_uint32(0), C.c_uint32(0)
+    ex, ga = C.c_float(0), C.c_float(0)
+    _check(mrt_decode_hdr(data, len(data), C.byref(p), C.byref(w), C.byref(h),
+                          C.byref(ex), C.byref(ga)))
+    try:
+        a = np.ctypeslib.as_array(p, shape=(h.value, w.value, 3)).copy()
+    finally:
+        mrt_free(p)
+    return a, ex.value, ga.value
 
 
 def export_ppm(path, float_render, samples, gamma=1.0):

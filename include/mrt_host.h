@@ -157,6 +157,26 @@ int  mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
 int  mrt_gen_hdri_sky(uint64_t seed, uint32_t w, uint32_t h,
                       float sun_radius_deg, float sun_radiance, float *rgb);
 
+/* ---- scene files: Radiance .hdr and Wavefront .obj ---------------------- */
+
+/* Decodes an RGBE .hdr image (src/utils/hdr/parse_hdr*.c): *rgb receives
+ * w*h*3 floats c * 2^(e-128) / 255, row 0 at the bottom of the image; the
+ * header's EXPOSURE and GAMMA (defaults 0 and 1) are returned.  Free *rgb
+ * with mrt_free. */
+int  mrt_decode_hdr(const uint8_t *data, size_t size, float **rgb, uint32_t *w,
+                    uint32_t *h, float *exposure, float *gamma);
+int  mrt_load_hdr(const char *path, float **rgb, uint32_t *w, uint32_t *h,
+                  float *exposure, float *gamma);
+void mrt_free(void *p);
+
+/* Adds an OBJ mesh placed as the .rt "cu" element places it (scale, then
+ * the rotation taking +Z to dir, then pos); returns the triangle count. */
+int  mrt_hs_add_obj(mrt_host_scene *hs, const char *path, const float pos[3],
+                    const float dir[3], float scale, uint32_t mat);
+int  mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text,
+                         const float pos[3], const float dir[3], float scale,
+                         uint32_t mat);
+
 /* ---- HDRI tables (src/ray_tracing/importance_sampling/calc_*.c) --------- */
 
 typedef struct mrt_hdri_tables {
