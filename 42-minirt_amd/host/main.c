@@ -122,9 +122,7 @@ int main(int argc, char **argv)
     float sex = 0.0f;
     if (mrt_hs_get_hdri(hs, &sky, &sw, &sh, &sex) == MRT_OK
         && (rc = mrt_upload_hdri(ctx, sky, sw, sh, sex)) != MRT_OK) {
-        fprintf(stderr, "Error
-%s
-", mrt_last_error(ctx));
+        fprintf(stderr, "Error\n%s\n", mrt_last_error(ctx));
         mrt_destroy(ctx);
         mrt_hs_free(hs);
         return 1;
@@ -137,9 +135,7 @@ int main(int argc, char **argv)
         if (sscanf(caustic, "%lu;%lf;%lf", &np, &rad, &norm) < 2
             || mrt_set_option(ctx, MRT_OPT_PHOTON_NORM, norm) != MRT_OK
             || mrt_build_photons(ctx, (uint32_t)np, (float)rad) != MRT_OK) {
-            fprintf(stderr, "Warning
-caustics will be disabled: %s
-",
+            fprintf(stderr, "Warning\ncaustics will be disabled: %s\n",
                     mrt_last_error(ctx));
         }
     }
