@@ -21,14 +21,19 @@ typedef struct {
     float lo[3], hi[3];
 } box3;
 
+/* A primitive's bounds, centroid and source index.  The build partitions
+ * these records in place, so every pass over a range reads memory in order. */
+typedef struct {
+    box3     b;
+    float    c[3];
+    uint32_t id;
+} prec;
+
 typedef struct {
     const mrt_scene *sc;
     uint32_t  max_leaf;
     uint32_t  n;
-    box3     *pbox;      /* per source primitive */
-    float    *cent;      /* 3 per source primitive */
-    uint32_t *perm;      /* source index per leaf-order slot */
-    uint32_t *tmp;
+    prec     *rec;       /* leaf order once built */
     mrt_bvh_node *nodes;
     uint32_t  n_nodes, cap_nodes;
     uint32_t  max_depth, n_leaves;
@@ -36,6 +41,11 @@ typedef struct {
     double    root_area;
     int       err;
 } builder;
+
+/* min / max that compile to single instructions (bounds are never NaN:
+ * mrt_bvh_build rejects non-finite input) */
+static inline float mnf(float a, float b) { return b < a ? b : a; }
+static inline float mxf(float a, float b) { return b > a ? b : a; }
 
 static inline void box_empty(box3 *b)
 {
@@ -48,8 +58,8 @@ static inline void box_empty(box3 *b)
 static inline void box_grow(box3 *b, const box3 *o)
 {
     for (int k = 0; k < 3; k++) {
-        b->lo[k] = fminf(b->lo[k], o->lo[k]);
-        b->hi[k] = fmaxf(b->hi[k], o->hi[k]);
+        b->lo[k] = mnf(b->lo[k], o->lo[k]);
+        b->hi[k] = mxf(b->hi[k], o->hi[k]);
     }
 }
 
@@ -64,29 +74,38 @@ static inline double box_area(const box3 *b)
     return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
 }
 
-static void prim_bounds(builder *B)
+/* Fills the primitive records; MRT_E_INVALID for a non-finite vertex,
+ * centre or radius. */
+static int prim_bounds(builder *B)
 {
     const mrt_scene *s = B->sc;
+    int finite = 1;
     for (uint32_t i = 0; i < s->n_triangles; i++) {
         const float *v = s->tri_pos + (size_t)i * 9;
-        box3 *b = &B->pbox[i];
+        prec *r = &B->rec[i];
         for (int k = 0; k < 3; k++) {
-            b->lo[k] = fminf(v[k], fminf(v[3 + k], v[6 + k]));
-            b->hi[k] = fmaxf(v[k], fmaxf(v[3 + k], v[6 + k]));
-            B->cent[(size_t)i * 3 + k] = 0.5f * (b->lo[k] + b->hi[k]);
+            r->b.lo[k] = mnf(v[k], mnf(v[3 + k], v[6 + k]));
+            r->b.hi[k] = mxf(v[k], mxf(v[3 + k], v[6 + k]));
+            r->c[k] = 0.5f * (r->b.lo[k] + r->b.hi[k]);
+            finite &= isfinite(v[k]) && isfinite(v[3 + k]) && isfinite(v[6 + k]);
         }
+        r->id = i;
     }
     for (uint32_t j = 0; j < s->n_spheres; j++) {
         const float *q = s->sph + (size_t)j * 4;
         uint32_t i = s->n_triangles + j;
-        box3 *b = &B->pbox[i];
+        prec *r = &B->rec[i];
         for (int k = 0; k < 3; k++) {
             /* one ulp outward so the float box holds the exact sphere */
-            b->lo[k] = nextafterf(q[k] - q[3], -INFINITY);
-            b->hi[k] = nextafterf(q[k] + q[3], INFINITY);
-            B->cent[(size_t)i * 3 + k] = q[k];
+            r->b.lo[k] = nextafterf(q[k] - q[3], -INFINITY);
+            r->b.hi[k] = nextafterf(q[k] + q[3], INFINITY);
+            r->c[k] = q[k];
+            finite &= isfinite(r->b.lo[k]) && isfinite(r->b.hi[k]);
         }
+        finite &= q[3] >= 0.0f;
+        r->id = i;
     }
+    return finite ? MRT_OK : MRT_E_INVALID;
 }
 
 static int32_t leaf_code(uint32_t first, uint32_t count)
@@ -129,87 +148,110 @@ static uint32_t ceil_log2(uint32_t x)
     return l;
 }
 
-static int cmp_axis;
-static const float *cmp_cent;
+/* comparator state of the median fallback: per thread, since scenes of a
+ * batch are built concurrently */
+static _Thread_local int cmp_axis;
 static int cmp_prim(const void *a, const void *b)
 {
-    float ca = cmp_cent[(size_t)(*(const uint32_t *)a) * 3 + cmp_axis];
-    float cb = cmp_cent[(size_t)(*(const uint32_t *)b) * 3 + cmp_axis];
+    const prec *ra = a, *rb = b;
+    float ca = ra->c[cmp_axis], cb = rb->c[cmp_axis];
     if (ca < cb)
         return -1;
     if (ca > cb)
         return 1;
-    uint32_t ia = *(const uint32_t *)a, ib = *(const uint32_t *)b;
-    return (ia > ib) - (ia < ib);
+    return (ra->id > rb->id) - (ra->id < rb->id);
 }
 
-/* Chooses the split of perm[begin,end); returns the split point mid with
- * begin < mid < end after partitioning. */
+static inline void cbox_grow(box3 *b, const float *c)
+{
+    for (int k = 0; k < 3; k++) {
+        b->lo[k] = mnf(b->lo[k], c[k]);
+        b->hi[k] = mxf(b->hi[k], c[k]);
+    }
+}
+
+static inline int bin_of(float c, float lo, float scale)
+{
+    int bin = (int)((c - lo) * scale);
+    if (bin >= NBINS)
+        bin = NBINS - 1;
+    if (bin < 0)
+        bin = 0;
+    return bin;
+}
+
+typedef struct {
+    box3     bb, cb;          /* primitive and centroid bounds of the bin */
+    uint32_t cnt;
+} bin_t;
+
+/* Splits rec[begin,end) (centroid bounds cb): 32-bin SAH over the three
+ * axes binned in one pass, or the object median on the longest centroid
+ * axis near the depth limit or when no SAH split separates the range.
+ * Returns mid (begin < mid < end) and the primitive and centroid bounds of
+ * both halves. */
 static uint32_t split_range(builder *B, uint32_t begin, uint32_t end,
-                            uint32_t depth)
+                            uint32_t depth, const box3 *cb, box3 *lb,
+                            box3 *rb, box3 *lcb, box3 *rcb)
 {
     uint32_t count = end - begin;
-    box3 cb;
-    box_empty(&cb);
-    for (uint32_t i = begin; i < end; i++) {
-        const float *c = &B->cent[(size_t)B->perm[i] * 3];
-        for (int k = 0; k < 3; k++) {
-            cb.lo[k] = fminf(cb.lo[k], c[k]);
-            cb.hi[k] = fmaxf(cb.hi[k], c[k]);
-        }
-    }
     int axis_long = 0;
-    float ext_long = cb.hi[0] - cb.lo[0];
+    float ext_long = cb->hi[0] - cb->lo[0];
     for (int k = 1; k < 3; k++)
-        if (cb.hi[k] - cb.lo[k] > ext_long) {
-            ext_long = cb.hi[k] - cb.lo[k];
+        if (cb->hi[k] - cb->lo[k] > ext_long) {
+            ext_long = cb->hi[k] - cb->lo[k];
             axis_long = k;
         }
-
     uint32_t leaves_needed = (count + B->max_leaf - 1) / B->max_leaf;
     int median = (depth + 1 + ceil_log2(leaves_needed) >= MRT_BVH_MAX_DEPTH)
                  || !(ext_long > 0.0f);
     if (!median) {
+        bin_t bins[3][NBINS];
+        float scale[3];
+        int live[3];
+        for (int axis = 0; axis < 3; axis++) {
+            float ext = cb->hi[axis] - cb->lo[axis];
+            live[axis] = ext > 0.0f;
+            scale[axis] = live[axis] ? (float)NBINS / ext : 0.0f;
+            for (int k = 0; k < NBINS; k++) {
+                box_empty(&bins[axis][k].bb);
+                box_empty(&bins[axis][k].cb);
+                bins[axis][k].cnt = 0;
+            }
+        }
+        for (uint32_t i = begin; i < end; i++) {
+            const float *c = B->rec[i].c;
+            const box3 *pb = &B->rec[i].b;
+            for (int axis = 0; axis < 3; axis++) {
+                if (!live[axis])
+                    continue;
+                bin_t *bn = &bins[axis][bin_of(c[axis], cb->lo[axis], scale[axis])];
+                bn->cnt++;
+                box_grow(&bn->bb, pb);
+                cbox_grow(&bn->cb, c);
+            }
+        }
         double best = INFINITY;
         int best_axis = -1, best_bin = -1;
         for (int axis = 0; axis < 3; axis++) {
-            float ext = cb.hi[axis] - cb.lo[axis];
-            if (!(ext > 0.0f))
+            if (!live[axis])
                 continue;
-            box3 bb[NBINS];
-            uint32_t bc[NBINS];
-            for (int k = 0; k < NBINS; k++) {
-                box_empty(&bb[k]);
-                bc[k] = 0;
-            }
-            float scale = (float)NBINS / ext;
-            for (uint32_t i = begin; i < end; i++) {
-                uint32_t p = B->perm[i];
-                int bin = (int)((B->cent[(size_t)p * 3 + axis] - cb.lo[axis])
-                                * scale);
-                if (bin >= NBINS)
-                    bin = NBINS - 1;
-                if (bin < 0)
-                    bin = 0;
-                bc[bin]++;
-                box_grow(&bb[bin], &B->pbox[p]);
-            }
             double right_area[NBINS];
             uint32_t right_cnt[NBINS];
             box3 acc;
             box_empty(&acc);
             uint32_t cnt = 0;
             for (int k = NBINS - 1; k > 0; k--) {
-                box_grow(&acc, &bb[k]);
-                cnt += bc[k];
+                box_grow(&acc, &bins[axis][k].bb);
+                cnt += bins[axis][k].cnt;
                 right_area[k] = box_area(&acc);
                 right_cnt[k] = cnt;
             }
             box_empty(&acc);
             cnt = 0;
             for (int k = 0; k < NBINS - 1; k++) {
-                box_grow(&acc, &bb[k]);
-                cnt += bc[k];
+                box_grow(&acc, &bins[axis][k].bb);
+                cnt += bins[axis][k].cnt;
                 if (cnt == 0 || right_cnt[k + 1] == 0)
                     continue;
                 double cost = box_area(&acc) * cnt
@@ -222,47 +264,66 @@ static uint32_t split_range(builder *B, uint32_t begin, uint32_t end,
             }
         }
         if (best_axis >= 0) {
-            float ext = cb.hi[best_axis] - cb.lo[best_axis];
-            float scale = (float)NBINS / ext;
+            const float lo = cb->lo[best_axis], sc = scale[best_axis];
             uint32_t i = begin, j = end;
             while (i < j) {
-                uint32_t p = B->perm[i];
-                int bin = (int)((B->cent[(size_t)p * 3 + best_axis]
-                                 - cb.lo[best_axis]) * scale);
-                if (bin >= NBINS)
-                    bin = NBINS - 1;
-                if (bin < 0)
-                    bin = 0;
-                if (bin <= best_bin) {
+                if (bin_of(B->rec[i].c[best_axis], lo, sc) <= best_bin) {
                     i++;
                 } else {
                     j--;
-                    B->perm[i] = B->perm[j];
-                    B->perm[j] = p;
+                    prec t = B->rec[i];
+                    B->rec[i] = B->rec[j];
+                    B->rec[j] = t;
                 }
             }
-            if (i > begin && i < end)
+            if (i > begin && i < end) {
+                box_empty(lb);
+                box_empty(rb);
+                box_empty(lcb);
+                box_empty(rcb);
+                for (int k = 0; k < NBINS; k++) {
+                    const bin_t *bn = &bins[best_axis][k];
+                    box_grow(k <= best_bin ? lb : rb, &bn->bb);
+                    box_grow(k <= best_bin ? lcb : rcb, &bn->cb);
+                }
                 return i;
+            }
         }
     }
     /* object median on the longest centroid axis (ties by index) */
     cmp_axis = axis_long;
-    cmp_cent = B->cent;
-    qsort(B->perm + begin, count, sizeof(uint32_t), cmp_prim);
-    return begin + count / 2;
+    qsort(B->rec + begin, count, sizeof(prec), cmp_prim);
+    uint32_t mid = begin + count / 2;
+    box_empty(lb);
+    box_empty(rb);
+    box_empty(lcb);
+    box_empty(rcb);
+    for (uint32_t i = begin; i < end; i++) {
+        box_grow(i < mid ? lb : rb, &B->rec[i].b);
+        cbox_grow(i < mid ? lcb : rcb, B->rec[i].c);
+    }
+    return mid;
 }
 
 static void range_box(builder *B, uint32_t begin, uint32_t end, box3 *out)
 {
     box_empty(out);
     for (uint32_t i = begin; i < end; i++)
-        box_grow(out, &B->pbox[B->perm[i]]);
+        box_grow(out, &B->rec[i].b);
 }
 
-/* Builds the subtree of perm[begin,end) whose box is bx; returns its child
- * code (leaf or inner node index). depth = inner levels above it. */
+static void range_cbox(builder *B, uint32_t begin, uint32_t end, box3 *out)
+{
+    box_empty(out);
+    for (uint32_t i = begin; i < end; i++)
+        cbox_grow(out, B->rec[i].c);
+}
+
+/* Builds the subtree of rec[begin,end) whose box is bx and centroid box cb;
+ * returns its child code (leaf or inner node index). depth = inner levels
+ * above it. */
 static int32_t build_rec(builder *B, uint32_t begin, uint32_t end,
-                         const box3 *bx, uint32_t depth)
+                         const box3 *bx, const box3 *cb, uint32_t depth)
 {
     uint32_t count = end - begin;
     if (B->err)
@@ -278,13 +339,11 @@ static int32_t build_rec(builder *B, uint32_t begin, uint32_t end,
     if (depth + 1 > B->max_depth)
         B->max_depth = depth + 1;
     B->sah += box_area(bx) / B->root_area;
-    uint32_t mid = split_range(B, begin, end, depth);
-    box3 lb, rb;
-    range_box(B, begin, mid, &lb);
-    range_box(B, mid, end, &rb);
-    int32_t lc = build_rec(B, begin, mid, &lb, depth + 1);
+    box3 lb, rb, lcb, rcb;
+    uint32_t mid = split_range(B, begin, end, depth, cb, &lb, &rb, &lcb, &rcb);
+    int32_t lc = build_rec(B, begin, mid, &lb, &lcb, depth + 1);
     set_child(B, node, 0, &lb, lc);
-    int32_t rc = build_rec(B, mid, end, &rb, depth + 1);
+    int32_t rc = build_rec(B, mid, end, &rb, &rcb, depth + 1);
     set_child(B, node, 1, &rb, rc);
     return (int32_t)node;
 }
@@ -316,18 +375,13 @@ int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
     B.sc = s;
     B.max_leaf = max_leaf;
     B.n = n;
-    B.pbox = malloc((size_t)n * sizeof(box3));
-    B.cent = malloc((size_t)n * 3 * sizeof(float));
-    B.perm = malloc((size_t)n * sizeof(uint32_t));
-    if (!B.pbox || !B.cent || !B.perm) {
-        free(B.pbox);
-        free(B.cent);
-        free(B.perm);
+    B.rec = malloc((size_t)n * sizeof(prec));
+    if (!B.rec)
         return MRT_E_NOMEM;
+    if (prim_bounds(&B) != MRT_OK) {
+        free(B.rec);
+        return MRT_E_INVALID;
     }
-    prim_bounds(&B);
-    for (uint32_t i = 0; i < n; i++)
-        B.perm[i] = i;
     box3 root;
     range_box(&B, 0, n, &root);
     B.root_area = box_area(&root);
@@ -348,14 +402,14 @@ int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
         set_child(&B, node, 1, &far, leaf_code(0, 1));
         rc = 0;
     } else {
-        rc = build_rec(&B, 0, n, &root, 0);
+        box3 croot;
+        range_cbox(&B, 0, n, &croot);
+        rc = build_rec(&B, 0, n, &root, &croot, 0);
     }
     (void)rc;
-    free(B.pbox);
-    free(B.cent);
     if (B.err) {
         free(B.nodes);
-        free(B.perm);
+        free(B.rec);
         return B.err;
     }
 
@@ -365,12 +419,12 @@ int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
         free(out->prims);
         free(out->shade_nrm);
         free(B.nodes);
-        free(B.perm);
+        free(B.rec);
         memset(out, 0, sizeof(*out));
         return MRT_E_NOMEM;
     }
     for (uint32_t i = 0; i < n; i++) {
-        uint32_t p = B.perm[i];
+        uint32_t p = B.rec[i].id;
         mrt_prim *r = &out->prims[i];
         float *nrm = out->shade_nrm + (size_t)i * 9;
         if (p < s->n_triangles) {
@@ -416,7 +470,7 @@ int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
             memset(nrm, 0, 9 * sizeof(float));
         }
     }
-    free(B.perm);
+    free(B.rec);
     out->nodes = B.nodes;
     out->n_nodes = B.n_nodes;
     out->n_prims = n;

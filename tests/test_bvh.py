@@ -72,6 +72,28 @@ def test_spheres_and_single_leaf():
     assert np.all(seen == 1)
 
 
+@pytest.mark.parametrize("bad", ["nan_vertex", "inf_vertex", "inf_sphere",
+                                 "negative_radius"])
+def test_non_finite_primitives_are_rejected(bad):
+    hs = M.HostScene.empty()
+    m = hs.add_material()
+    hs.add_triangle(np.eye(3, dtype=np.float32), m)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    if bad == "nan_vertex":
+        v[1, 2] = np.nan
+    if bad == "inf_vertex":
+        v[2, 0] = np.inf
+    # the scene may refuse it already; the builder must refuse it otherwise
+    with pytest.raises(M.MrtError) as e:
+        hs.add_triangle(v, m)
+        if bad == "inf_sphere":
+            hs.add_sphere((0.0, np.inf, 0.0), 1.0, m)
+        if bad == "negative_radius":
+            hs.add_sphere((0.0, 0.0, 0.0), -1.0, m)
+        M.BuiltBvh(hs.view())
+    assert e.value.code == M.MRT_E_INVALID
+
+
 def test_sah_beats_median_depth_bound():
     hs = M.HostScene.mesh(5, 64, 36, subdiv=6, floor_cells=0)
     b = M.BuiltBvh(hs.view())
