@@ -30,6 +30,7 @@ struct mrt_ctx {
     float4      *d_nodes = nullptr, *d_prims = nullptr, *d_planes = nullptr;
     float4      *d_mats = nullptr, *d_lights = nullptr;
     float       *d_nrm = nullptr;
+    bool         has_quad = false;   // the scene has cylinders / cones / ...
     uint32_t     n_nodes = 0, n_prims = 0, n_planes = 0, n_mats = 0;
     uint32_t     bvh_depth = 0;
     std::vector<mrt_light> lights;
@@ -266,6 +267,19 @@ static int upload(mrt_ctx *c, void *dst, const void *src, size_t bytes)
     return MRT_OK;
 }
 
+// A distance from pos that no point of the quadric exceeds: height + radius
+// for cylinders and cones, height + max(a, b) sqrt(shape + (height/2c)^2)
+// for hyperboloids (their radius at |z| = height / 2).
+static float quadric_extent(const mrt_quadric *q)
+{
+    float r = q->radius;
+    if (q->kind == MRT_QUADRIC_HYPERBOLOID) {
+        float hc = 0.5f * q->height / q->c;
+        r = fmaxf(q->a, q->b) * sqrtf(fmaxf(0.0f, q->shape + hc * hc));
+    }
+    return q->height + r;
+}
+
 static int validate_scene(mrt_ctx *c, const mrt_scene *s)
 {
     if (s->n_materials == 0 || !s->materials)
@@ -278,6 +292,14 @@ static int validate_scene(mrt_ctx *c, const mrt_scene *s)
     for (uint32_t i = 0; i < s->n_spheres; i++)
         if (s->sph_mat[i] >= s->n_materials || !(s->sph[4 * i + 3] > 0.0f))
             return fail(c, MRT_E_INVALID, "sphere %u: bad material or radius", i);
+    if (s->n_quadrics && !s->quadrics)
+        return fail(c, MRT_E_INVALID, "quadrics without an array");
+    for (uint32_t i = 0; i < s->n_quadrics; i++) {
+        const mrt_quadric &q = s->quadrics[i];
+        if (q.kind > MRT_QUADRIC_HYPERBOLOID || q.mat >= s->n_materials
+            || q.mat_top >= s->n_materials || q.mat_bot >= s->n_materials)
+            return fail(c, MRT_E_INVALID, "quadric %u: bad kind or material", i);
+    }
     if (s->n_planes && (!s->pln || !s->pln_mat))
         return fail(c, MRT_E_INVALID, "planes without arrays");
     for (uint32_t i = 0; i < s->n_planes; i++)
@@ -373,6 +395,7 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
         rc = fail(c, MRT_E_HIP, "scene upload failed");
     c->n_nodes = bvh4.n_nodes;
     c->n_prims = bvh.n_prims;
+    c->has_quad = s->n_quadrics > 0;
     c->bvh_depth = bvh4.depth;
     mrt_bvh_free(&bvh);
     mrt_bvh4_free(&bvh4);
@@ -404,6 +427,13 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
             lo[k] = fminf(lo[k], s->sph[4 * i + k] - s->sph[4 * i + 3]);
             hi[k] = fmaxf(hi[k], s->sph[4 * i + k] + s->sph[4 * i + 3]);
         }
+    for (uint32_t i = 0; i < s->n_quadrics; i++) {
+        const float e = quadric_extent(&s->quadrics[i]);
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fminf(lo[k], s->quadrics[i].pos[k] - e);
+            hi[k] = fmaxf(hi[k], s->quadrics[i].pos[k] + e);
+        }
+    }
     float d2 = 0.0f;
     for (int k = 0; k < 3; k++) {
         c->scene_c[k] = 0.5f * (lo[k] + hi[k]);
@@ -799,11 +829,13 @@ static SceneDev scene_dev(const mrt_ctx *c)
     memset(&S, 0, sizeof(S));
     S.nodes = c->d_nodes;
     S.prims = c->d_prims;
+    S.aux = c->d_nrm;
     S.planes = c->d_planes;
     S.n_nodes = c->n_nodes;
     S.n_planes = c->n_planes;
     S.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
     S.root = 0;
+    S.quad = c->has_quad ? 1u : 0u;
     return S;
 }
 
@@ -1369,6 +1401,7 @@ struct BatchSrc {
     std::vector<uint32_t>     tri_mat, sph_mat, pln_mat;
     std::vector<mrt_material> mats;
     std::vector<mrt_light>    lights;
+    std::vector<mrt_quadric>  quad;
     mrt_scene                 view;
     uint32_t                  W = 0, H = 0, spp = 0;
     uint64_t                  seed = 0;
@@ -1392,6 +1425,7 @@ struct mrt_batch {
     size_t                  accum_n = 0;
     uint32_t                max_rays = 1;
     bool                    has_lights = false;
+    bool                    has_quad = false;
     mrt_batch_tuning        tune{};
     hipGraph_t              graph = nullptr;
     hipGraphExec_t          gexec = nullptr;
@@ -1499,7 +1533,9 @@ extern "C" int mrt_batch_add_scene(mrt_batch *b, const mrt_scene *s,
     vcopy(p->pln_mat, s->pln_mat, s->n_planes);
     vcopy(p->mats, s->materials, s->n_materials);
     vcopy(p->lights, s->lights, s->n_lights);
+    vcopy(p->quad, s->quadrics, s->n_quadrics);
     p->view = *s;
+    p->view.quadrics = p->quad.data();
     p->view.tri_pos = p->tri_pos.data();
     p->view.tri_nrm = s->tri_nrm ? p->tri_nrm.data() : nullptr;
     p->view.tri_mat = p->tri_mat.data();
@@ -1572,9 +1608,11 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     size_t accum0 = 0;
     b->max_rays = 1;
     b->has_lights = false;
+    b->has_quad = false;
     for (size_t i = 0; i < n && rc == MRT_OK; i++) {
         const BatchSrc &S = *b->src[i];
         const mrt_bvh &B = bv[i];
+        b->has_quad = b->has_quad || S.view.n_quadrics > 0;
         const mrt_bvh4 &B4 = bv4[i];
         const uint32_t node0 = (uint32_t)nodes.size();
         const uint32_t prim0 = (uint32_t)prims.size();
@@ -1600,15 +1638,27 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
             }
             nodes.push_back(nd);
         }
+        const size_t nrm0 = nrm.size();
+        nrm.insert(nrm.end(), B.shade_nrm, B.shade_nrm + (size_t)B.n_prims * 9);
         for (uint32_t k = 0; k < B.n_prims; k++) {
             mrt_prim pr = B.prims[k];
-            uint32_t m;
-            memcpy(&m, &pr.c[3], 4);
-            m += mat0;
-            memcpy(&pr.c[3], &m, 4);
+            uint32_t m, kind;
+            memcpy(&kind, &pr.b[3], 4);
+            if (kind >= MRT_PRIMKIND_QUADRIC) {
+                // materials of a quadric live in its shading record
+                for (int j = 1; j <= 3; j++) {
+                    float *f = &nrm[nrm0 + (size_t)k * 9 + j];
+                    memcpy(&m, f, 4);
+                    m += mat0;
+                    memcpy(f, &m, 4);
+                }
+            } else {
+                memcpy(&m, &pr.c[3], 4);
+                m += mat0;
+                memcpy(&pr.c[3], &m, 4);
+            }
             prims.push_back(pr);
         }
-        nrm.insert(nrm.end(), B.shade_nrm, B.shade_nrm + (size_t)B.n_prims * 9);
         pack_scene(&S.view, mat0, planes, mats, lights);
         BatchScene &T = b->hbs[i];
         memset(&T, 0, sizeof(T));
@@ -1722,7 +1772,9 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
     memset(&S, 0, sizeof(S));
     S.nodes = b->d_nodes;
     S.prims = b->d_prims;
+    S.aux = b->d_nrm;
     S.planes = b->d_planes;
+    S.quad = b->has_quad ? 1u : 0u;
     FrameParams fp;
     memset(&fp, 0, sizeof(fp));
     ShadeBufs B;

@@ -166,6 +166,235 @@ __device__ __forceinline__ bool isect_pln(V3 o, V3 d, float4 P, float4 N, float 
     return true;
 }
 
+// ---- cylinders, cones, hyperboloids ---------------------------------------
+// Record (mrt_host.h): A = pos | src id, B = axis | kind, C = (radius or a,
+// b, c, height); the shading record holds shape, mat, mat_top, mat_bot.
+// Hits report the part in bu: 0 body, 1 top cap, 2 bottom cap.
+enum : uint32_t { QPART_BODY = 0, QPART_TOP = 1, QPART_BOTTOM = 2 };
+
+// Roots of a t^2 + b t + c labelled as solve_quadratic labels them
+// (src/math/quadratique/quadratique.c:17-30): t0 = (-b - sqrt(disc)) / 2a,
+// t1 = (-b + sqrt(disc)) / 2a, also when a < 0, evaluated without
+// cancellation (q = -(b + sign(b) sqrt(disc)) / 2; roots q / a and c / q).
+// a == 0 (a ray along a cylinder's axis or a cone's side) has no root here;
+// the reference divides by zero there.
+__device__ __forceinline__ bool quad_roots(float a, float b, float c,
+                                           float &t0, float &t1)
+{
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f || a == 0.0f)
+        return false;
+    float sq = sqrtf(disc);
+    float q = -0.5f * (b + copysignf(sq, b));
+    if (q == 0.0f) {
+        t0 = t1 = 0.0f;
+        return true;
+    }
+    float r1 = q / a, r2 = c / q;
+    t0 = b >= 0.0f ? r1 : r2;
+    t1 = b >= 0.0f ? r2 : r1;
+    return true;
+}
+
+// valid_t (src/math/valid_t.c:16-26): t0 if in [IT_MIN, IT_MAX], else t1.
+__device__ __forceinline__ bool valid_root(float t0, float t1, float &t)
+{
+    t = t0;
+    if (t < IT_MIN_F || t > IT_MAX_F) {
+        t = t1;
+        if (t < IT_MIN_F || t > IT_MAX_F)
+            return false;
+    }
+    return true;
+}
+
+// A cap disk of centre pc and normal n (cylinder_cap_bottom / _top,
+// hit_cylinder.c:39-100; cone_cap, hit_cone.c:81-108).
+__device__ __forceinline__ bool cap_disk(V3 o, V3 d, V3 pc, V3 n, float rad,
+                                         float &t)
+{
+    float denom = dot(d, n);
+    if (fabsf(denom) < 1e-6f)
+        return false;
+    float tt = dot(sub(pc, o), n) / denom;
+    if (tt < IT_MIN_F || tt > IT_MAX_F)
+        return false;
+    V3 e = sub(add(o, mul(d, tt)), pc);
+    if (sqrtf(dot(e, e)) > rad)
+        return false;
+    t = tt;
+    return true;
+}
+
+// hit_cylinder (hit_cylinder.c:107-134): the body root valid_t picks, kept
+// when its projection on the axis from the bottom centre lies in [0, h]
+// (the other root is not tried), then each cap when nearer.
+__device__ __forceinline__ bool isect_cyl(V3 o, V3 d, float4 A, float4 B,
+                                          float4 C, float &t, float &part)
+{
+    V3 P = xyz(A), ax = xyz(B);
+    float rad = C.x, h = C.w;
+    V3 oc = sub(o, P);
+    float dd = dot(d, ax), oo = dot(oc, ax);
+    float qa = dot(d, d) - dd * dd;
+    float qb = 2.0f * (dot(d, oc) - dd * oo);
+    float qc = dot(oc, oc) - oo * oo - rad * rad;
+    bool hit = false;
+    float t0, t1, th;
+    V3 base = sub(P, mul(ax, h * 0.5f));
+    if (quad_roots(qa, qb, qc, t0, t1) && valid_root(t0, t1, th)) {
+        float proj = dot(sub(add(o, mul(d, th)), base), ax);
+        if (!(proj < 0.0f || proj > h)) {
+            t = th;
+            part = (float)QPART_BODY;
+            hit = true;
+        }
+    }
+    float tc;
+    if (cap_disk(o, d, base, ax, rad, tc) && (!hit || tc < t)) {
+        t = tc;
+        part = (float)QPART_BOTTOM;
+        hit = true;
+    }
+    if (cap_disk(o, d, add(P, mul(ax, h * 0.5f)), ax, rad, tc) && (!hit || tc < t)) {
+        t = tc;
+        part = (float)QPART_TOP;
+        hit = true;
+    }
+    return hit;
+}
+
+// hit_cone (hit_cone.c:111-129) with init_cone_quadratic (quadratique.c:
+// 55-66), k = (r / h)^2: the body root valid_t picks when its projection
+// from the apex lies in [0, h]; only otherwise the base cap.
+__device__ __forceinline__ bool isect_cone(V3 o, V3 d, float4 A, float4 B,
+                                           float4 C, float &t, float &part)
+{
+    V3 P = xyz(A), ax = xyz(B);
+    float rad = C.x, h = C.w;
+    float kr = rad / h, k1 = 1.0f + kr * kr;
+    V3 oc = sub(o, P);
+    float dd = dot(d, ax), oo = dot(oc, ax);
+    float qa = dot(d, d) - k1 * dd * dd;
+    float qb = 2.0f * (dot(d, oc) - k1 * dd * oo);
+    float qc = dot(oc, oc) - k1 * oo * oo;
+    float t0, t1, th;
+    if (!quad_roots(qa, qb, qc, t0, t1) || !valid_root(t0, t1, th))
+        return false;
+    float proj = dot(sub(add(o, mul(d, th)), P), ax);
+    if (!(proj < 0.0f || proj > h)) {
+        t = th;
+        part = (float)QPART_BODY;
+        return true;
+    }
+    float tc;
+    if (cap_disk(o, d, add(P, mul(ax, h)), ax, rad, tc)) {
+        t = tc;
+        part = (float)QPART_TOP;
+        return true;
+    }
+    return false;
+}
+
+// The hyperboloid's quadric frame (get_basis_vectors,
+// quadratique_hyperboloid.c:15-26): z = axis, x = unit(cross(t, z)) with
+// t = +Y when |z.x| > 0.9 else +X, y = cross(z, x).
+__device__ __forceinline__ void hyp_frame(V3 z, V3 &x, V3 &y)
+{
+    V3 tmp = fabsf(z.x) > 0.9f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+    x = unit(cross(tmp, z));
+    y = cross(z, x);
+}
+
+// hit_hyperboloid (hit_hyperboloid.c:79-104): roots in (IT_MIN, IT_MAX),
+// t0 first, then t1 when nearer, each kept when |proj on the axis| <= h/2.
+__device__ __forceinline__ bool isect_hyp(V3 o, V3 d, float4 A, float4 B,
+                                          float4 C, float shape, float &t)
+{
+    V3 P = xyz(A), z = xyz(B), x, y;
+    hyp_frame(z, x, y);
+    V3 oc = sub(o, P);
+    V3 dl = v3(dot(d, x), dot(d, y), dot(d, z));
+    V3 ol = v3(dot(oc, x), dot(oc, y), dot(oc, z));
+    float aa = C.x * C.x, bb = C.y * C.y, cc = C.z * C.z;
+    float qa = (dl.x * dl.x) / aa + (dl.y * dl.y) / bb - (dl.z * dl.z) / cc;
+    float qb = 2.0f * ((dl.x * ol.x) / aa + (dl.y * ol.y) / bb - (dl.z * ol.z) / cc);
+    float qc = (ol.x * ol.x) / aa + (ol.y * ol.y) / bb - (ol.z * ol.z) / cc - shape;
+    float t0, t1;
+    if (!quad_roots(qa, qb, qc, t0, t1))
+        return false;
+    float hh = C.w * 0.5f, tmax = IT_MAX_F;
+    bool hit = false;
+    if (t0 > IT_MIN_F && t0 < tmax
+        && fabsf(dot(sub(add(o, mul(d, t0)), P), z)) <= hh) {
+        t = t0;
+        tmax = t0;
+        hit = true;
+    }
+    if (t1 > IT_MIN_F && t1 < tmax
+        && fabsf(dot(sub(add(o, mul(d, t1)), P), z)) <= hh) {
+        t = t1;
+        hit = true;
+    }
+    return hit;
+}
+
+__device__ __forceinline__ bool isect_quadric(V3 o, V3 d, float4 A, float4 B,
+                                              float4 C, const float *aux,
+                                              float &t, float &part)
+{
+    uint32_t kind = __float_as_uint(B.w) - MRT_PRIMKIND_QUADRIC;
+    part = (float)QPART_BODY;
+    if (kind == MRT_QUADRIC_CYLINDER)
+        return isect_cyl(o, d, A, B, C, t, part);
+    if (kind == MRT_QUADRIC_CONE)
+        return isect_cone(o, d, A, B, C, t, part);
+    return isect_hyp(o, d, A, B, C, aux[0], t);
+}
+
+// Normal facing the ray, side and material at a quadric hit p:
+// handle_cylinder_hit (hit_cylinder.c:15-37; hit_cylinder sets front_face
+// again from the already flipped body normal, so the body is always the
+// front side, :128-131), handle_cone_hit (hit_cone.c:58-79), the caps, and
+// compute_normal (hit_hyperboloid.c:40-56, whose frame differs from the
+// quadric's: u = unit(cross(t, axis)), t = +Y when |axis.y| < 0.99 else
+// +X); materials per part as update_hit_cylinder / update_hit_cone
+// (hit_obj.c:17-53).
+__device__ __forceinline__ void quadric_surface(float4 A, float4 B, float4 C,
+                                                const float *aux, V3 d, V3 p,
+                                                uint32_t part, V3 &nrm,
+                                                bool &front, uint32_t &mat)
+{
+    uint32_t kind = __float_as_uint(B.w) - MRT_PRIMKIND_QUADRIC;
+    V3 P = xyz(A), ax = xyz(B);
+    float h = C.w;
+    mat = __float_as_uint(aux[part == QPART_TOP ? 2 : part == QPART_BOTTOM ? 3 : 1]);
+    if (part != QPART_BODY) {
+        V3 n = part == QPART_TOP ? ax : neg(ax);
+        front = dot(d, n) < 0.0f;
+        nrm = front ? n : neg(n);
+        return;
+    }
+    V3 ow;
+    if (kind == MRT_QUADRIC_CYLINDER) {
+        V3 rel = sub(p, sub(P, mul(ax, h * 0.5f)));
+        ow = unit(sub(rel, mul(ax, dot(rel, ax))));
+    } else if (kind == MRT_QUADRIC_CONE) {
+        V3 rel = sub(p, P);
+        float kr = C.x / h;
+        ow = unit(sub(rel, mul(ax, (1.0f + kr * kr) * dot(rel, ax))));
+    } else {
+        V3 tmp = fabsf(ax.y) < 0.99f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+        V3 u = unit(cross(tmp, ax)), v = cross(ax, u), rel = sub(p, P);
+        ow = unit(add(add(mul(u, 2.0f * dot(rel, u) / (C.x * C.x)),
+                          mul(v, 2.0f * dot(rel, v) / (C.y * C.y))),
+                      mul(ax, -2.0f * dot(rel, ax) / (C.z * C.z))));
+    }
+    bool f = dot(d, ow) < 0.0f;
+    nrm = f ? ow : neg(ow);
+    front = kind == MRT_QUADRIC_CYLINDER ? true : f;
+}
+
 __device__ __forceinline__ float inv_dir(float d)
 {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
@@ -187,13 +416,15 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx,
 }
 
 struct SceneDev {
-    const float4 *nodes;      // 4 per node
+    const float4 *nodes;      // 8 per node
     const float4 *prims;      // 3 per primitive (leaf order)
+    const float  *aux;        // 9 per primitive: shading record
     const float4 *planes;     // 2 per plane: point | mat, normal | 0
     uint32_t      n_nodes;    // 0: no BVH
     uint32_t      n_planes;
     uint32_t      one_sided;
     uint32_t      root;       // root node (scenes of a batch share one array)
+    uint32_t      quad;       // nonzero: the primitives include quadrics
 };
 
 constexpr int STACK4_MAX = 3 * (int)MRT_BVH4_MAX_DEPTH + 1;
@@ -217,7 +448,9 @@ __device__ __forceinline__ void cx(float &ka, int32_t &ca, float &kb, int32_t &c
 // farthest first.  The first SS stack entries live in LDS (lds_code,
 // stride `stride`), deeper ones in a private array.  ANY: stop at the first
 // hit below tlimit.
-template <bool ANY, bool COUNT, int SS>
+// QUAD = false drops the quadric tests from the leaf loop (scenes without
+// quadrics keep the traversal's register count and code size).
+template <bool ANY, bool COUNT, int SS, bool QUAD = true>
 __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
                                          float *lds_t, int stride,
@@ -282,11 +515,17 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                     prims++;
                 float tt, uu = 0.0f, vv = 0.0f;
                 bool h;
-                if (__float_as_uint(B.w) == MRT_PRIMKIND_SPHERE) {
+                uint32_t kind = __float_as_uint(B.w);
+                if (kind == MRT_PRIMKIND_SPHERE) {
                     h = isect_sph(o, d, A, B, tt);
-                } else {
+                } else if (kind == MRT_PRIMKIND_TRIANGLE) {
                     float4 C = p[2];
                     h = isect_tri(o, d, A, B, C, S.one_sided != 0, tt, uu, vv);
+                } else if (QUAD) {
+                    h = isect_quadric(o, d, A, B, p[2],
+                                      S.aux + (size_t)(first + i) * 9, tt, uu);
+                } else {
+                    h = false;
                 }
                 if (h && tt < tclose) {
                     tclose = tt;
@@ -312,7 +551,7 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
 
 // Closest hit over BVH + planes (hit_register_all, hit_register.c:109-123).
 // code: leaf-order slot, HIT_PLANE | k, or HIT_NONE.
-template <bool COUNT, int SS>
+template <bool COUNT, int SS, bool QUAD = true>
 __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
                                             uint32_t *lds_code, float *lds_t,
                                             int stride, float &t, uint32_t &code,
@@ -322,7 +561,7 @@ __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
     float tb = IT_MAX_F;
     uint32_t slot = HIT_NONE;
     float u = 0.0f, v = 0.0f;
-    bool hit = traverse<false, COUNT, SS>(S, o, d, IT_MAX_F, lds_code, lds_t,
+    bool hit = traverse<false, COUNT, SS, QUAD>(S, o, d, IT_MAX_F, lds_code, lds_t,
                                           stride, tb, slot, u, v, nodes, prims);
     for (uint32_t k = 0; k < S.n_planes; k++) {
         float tp;
@@ -342,7 +581,7 @@ __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
 
 // Occlusion below tlimit: BVH primitives below min(tlimit, 1000), planes
 // below tlimit (check_plight_hit, src/ray_tracing/light/p_light.c:48-64).
-template <bool COUNT, int SS>
+template <bool COUNT, int SS, bool QUAD = true>
 __device__ __forceinline__ bool occluded(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
                                          float *lds_t, int stride,
@@ -352,7 +591,7 @@ __device__ __forceinline__ bool occluded(const SceneDev &S, V3 o, V3 d,
     float t;
     uint32_t slot;
     float u, v;
-    if (traverse<true, COUNT, SS>(S, o, d, tl, lds_code, lds_t, stride, t, slot,
+    if (traverse<true, COUNT, SS, QUAD>(S, o, d, tl, lds_code, lds_t, stride, t, slot,
                                   u, v, nodes, prims))
         return true;
     for (uint32_t k = 0; k < S.n_planes; k++) {

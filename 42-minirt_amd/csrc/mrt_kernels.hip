@@ -214,7 +214,7 @@ __device__ __forceinline__ uint32_t tag_scene(uint32_t tag) { return (tag >> 18)
 __device__ __forceinline__ uint32_t tag_light(uint32_t tag) { return (tag >> 12) & 63u; }
 
 // ---- closest-hit trace (persistent, wave-granular work stealing) ----------
-template <bool COUNT, bool BATCH>
+template <bool COUNT, bool BATCH, bool QUAD>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_trace(SceneDev S, const float4 *__restrict__ q_o,
         const float4 *__restrict__ q_d, float4 *__restrict__ hit,
@@ -238,11 +238,11 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
                 if (BATCH) {
                     const SceneDev Si =
                         scene_of(S, bs, slot_info[__float_as_uint(d4.w)].x);
-                    closest_hit<COUNT, TRACE_SS>(Si, xyz(o4), xyz(d4), lc, lt,
+                    closest_hit<COUNT, TRACE_SS, QUAD>(Si, xyz(o4), xyz(d4), lc, lt,
                                                  TRACE_BLOCK, t, code, bu, bv,
                                                  nodes, prims);
                 } else {
-                    closest_hit<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), lc, lt,
+                    closest_hit<COUNT, TRACE_SS, QUAD>(S, xyz(o4), xyz(d4), lc, lt,
                                                  TRACE_BLOCK, t, code, bu, bv,
                                                  nodes, prims);
                 }
@@ -257,7 +257,7 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
 }
 
 // ---- shadow / visibility trace --------------------------------------------
-template <bool COUNT, bool BATCH>
+template <bool COUNT, bool BATCH, bool QUAD>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_shadow(SceneDev S, const float4 *__restrict__ s_o,
          const float4 *__restrict__ s_d, uint8_t *__restrict__ vis,
@@ -280,11 +280,11 @@ k_shadow(SceneDev S, const float4 *__restrict__ s_o,
                 if (BATCH) {
                     const SceneDev Si =
                         scene_of(S, bs, tag_scene(__float_as_uint(d4.w)));
-                    occ = occluded<COUNT, TRACE_SS>(Si, xyz(o4), xyz(d4), o4.w,
+                    occ = occluded<COUNT, TRACE_SS, QUAD>(Si, xyz(o4), xyz(d4), o4.w,
                                                     lc, lt, TRACE_BLOCK, nodes,
                                                     prims);
                 } else {
-                    occ = occluded<COUNT, TRACE_SS>(S, xyz(o4), xyz(d4), o4.w,
+                    occ = occluded<COUNT, TRACE_SS, QUAD>(S, xyz(o4), xyz(d4), o4.w,
                                                     lc, lt, TRACE_BLOCK, nodes,
                                                     prims);
                 }
@@ -445,8 +445,9 @@ __device__ __forceinline__ bool scatter(const RngKey &rk, float ior_all, V3 nrm,
 
 // Hit point, normal facing the ray, side and material of a hit record
 // h4 = (t, code, u, v): hit_plane (hit_plane.c:55-63), hit_sphere with
-// set_normal_face (hit_sphere.c:37-46, normal.c:16-29) and hit_triangle's
-// interpolated vertex normal (hit_triangle.c:15-27, 81-91).
+// set_normal_face (hit_sphere.c:37-46, normal.c:16-29), hit_triangle's
+// interpolated vertex normal (hit_triangle.c:15-27, 81-91) and the
+// quadrics' parts (u = part).
 __device__ __forceinline__ void surface(const float4 *planes,
                                         const float4 *prims, const float *nrm9,
                                         bool one_sided, V3 o, V3 d, float4 h4,
@@ -465,6 +466,11 @@ __device__ __forceinline__ void surface(const float4 *planes,
     }
     const float4 *rec = prims + (size_t)code * 3;
     float4 A = rec[0], Bv = rec[1], C = rec[2];
+    if (__float_as_uint(Bv.w) >= MRT_PRIMKIND_QUADRIC) {
+        quadric_surface(A, Bv, C, nrm9 + (size_t)code * 9, d, p,
+                        (uint32_t)h4.z, nrm, front, mat);
+        return;
+    }
     mat = __float_as_uint(C.w);
     if (__float_as_uint(Bv.w) == MRT_PRIMKIND_SPHERE) {
         float rad = Bv.x;
@@ -1417,25 +1423,12 @@ k_aov(FrameParams fp, SceneDev S, const float4 *__restrict__ prims,
     case MRT_AOV_NORMAL: {
         V3 n = v3(0.f, 0.f, 0.f);
         if (hit) {
-            V3 p = add(o, mul(d, t));
-            if (code & HIT_PLANE) {
-                float4 N = S.planes[2 * (code & ~HIT_PLANE) + 1];
-                n = (d.x * N.x + d.y * N.y + d.z * N.z) < 0.0f ? xyz(N) : neg(xyz(N));
-            } else {
-                float4 A = prims[(size_t)code * 3], Bv = prims[(size_t)code * 3 + 1];
-                if (__float_as_uint(Bv.w) == MRT_PRIMKIND_SPHERE) {
-                    V3 ow = sub(p, xyz(A));
-                    ow = v3(ow.x / Bv.x, ow.y / Bv.x, ow.z / Bv.x);
-                    n = dot(d, ow) < 0.0f ? ow : neg(ow);
-                } else {
-                    const float *N = nrm + (size_t)code * 9;
-                    float w = 1.0f - bu - bv;
-                    V3 nn = unit(v3(w * N[0] + bu * N[3] + bv * N[6],
-                                    w * N[1] + bu * N[4] + bv * N[7],
-                                    w * N[2] + bu * N[5] + bv * N[8]));
-                    n = (!fp.one_sided && dot(d, nn) > 0.0f) ? neg(nn) : nn;
-                }
-            }
+            V3 p;
+            bool front;
+            uint32_t mat;
+            surface(S.planes, prims, nrm, fp.one_sided != 0, o, d,
+                    make_float4(t, __uint_as_float(code), bu, bv), p, n, front,
+                    mat);
         }
         float *o3 = (float *)out + (size_t)px * 3;
         o3[0] = n.x;
@@ -1841,6 +1834,16 @@ hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
     return hipGetLastError();
 }
 
+// Launches MRT_GO(C, Q) with the template flags the two run-time flags pick.
+#define MRT_DISPATCH2(c, q)                                                   \
+    do {                                                                      \
+        if (c) {                                                              \
+            if (q) MRT_GO(true, true); else MRT_GO(true, false);              \
+        } else {                                                              \
+            if (q) MRT_GO(false, true); else MRT_GO(false, false);            \
+        }                                                                     \
+    } while (0)
+
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
                         const SceneDev &S, const float4 *q_o,
                         const float4 *q_d, float4 *hit, uint32_t *ctr,
@@ -1848,12 +1851,11 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
 {
     const BatchScene *nb = nullptr;
     const uint2 *ni = nullptr;
-    if (count)
-        hipLaunchKernelGGL((k_trace<true, false>), dim3(blocks), dim3(TRACE_BLOCK),
-                           0, st, S, q_o, q_d, hit, ctr, stats, nb, ni);
-    else
-        hipLaunchKernelGGL((k_trace<false, false>), dim3(blocks), dim3(TRACE_BLOCK),
-                           0, st, S, q_o, q_d, hit, ctr, stats, nb, ni);
+#define MRT_GO(C, Q)                                                          \
+    hipLaunchKernelGGL((k_trace<C, false, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
+                       0, st, S, q_o, q_d, hit, ctr, stats, nb, ni)
+    MRT_DISPATCH2(count, S.quad != 0);
+#undef MRT_GO
     return hipGetLastError();
 }
 
@@ -1863,13 +1865,11 @@ hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
                          unsigned long long *stats, uint32_t cap)
 {
     const BatchScene *nb = nullptr;
-    if (count)
-        hipLaunchKernelGGL((k_shadow<true, false>), dim3(blocks), dim3(TRACE_BLOCK),
-                           0, st, S, s_o, s_d, vis, ctr, stats, cap, nb);
-    else
-        hipLaunchKernelGGL((k_shadow<false, false>), dim3(blocks),
-                           dim3(TRACE_BLOCK), 0, st, S, s_o, s_d, vis, ctr, stats,
-                           cap, nb);
+#define MRT_GO(C, Q)                                                          \
+    hipLaunchKernelGGL((k_shadow<C, false, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
+                       0, st, S, s_o, s_d, vis, ctr, stats, cap, nb)
+    MRT_DISPATCH2(count, S.quad != 0);
+#undef MRT_GO
     return hipGetLastError();
 }
 
@@ -1904,14 +1904,12 @@ hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
                          const float4 *q_o, const float4 *q_d, float4 *hit,
                          uint32_t *ctr, unsigned long long *stats)
 {
-    if (count)
-        hipLaunchKernelGGL((k_trace<true, true>), dim3(blocks), dim3(TRACE_BLOCK),
-                           0, st, S, q_o, q_d, hit, ctr, stats, B.bs,
-                           (const uint2 *)B.slot_info);
-    else
-        hipLaunchKernelGGL((k_trace<false, true>), dim3(blocks), dim3(TRACE_BLOCK),
-                           0, st, S, q_o, q_d, hit, ctr, stats, B.bs,
-                           (const uint2 *)B.slot_info);
+#define MRT_GO(C, Q)                                                          \
+    hipLaunchKernelGGL((k_trace<C, true, Q>), dim3(blocks), dim3(TRACE_BLOCK),  \
+                       0, st, S, q_o, q_d, hit, ctr, stats, B.bs,              \
+                       (const uint2 *)B.slot_info)
+    MRT_DISPATCH2(count, S.quad != 0);
+#undef MRT_GO
     return hipGetLastError();
 }
 
@@ -1921,13 +1919,11 @@ hipError_t launch_bshadow(hipStream_t st, uint32_t blocks, bool count,
                           uint32_t *ctr, unsigned long long *stats,
                           uint32_t cap)
 {
-    if (count)
-        hipLaunchKernelGGL((k_shadow<true, true>), dim3(blocks), dim3(TRACE_BLOCK),
-                           0, st, S, s_o, s_d, vis, ctr, stats, cap, B.bs);
-    else
-        hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocks),
-                           dim3(TRACE_BLOCK), 0, st, S, s_o, s_d, vis, ctr, stats,
-                           cap, B.bs);
+#define MRT_GO(C, Q)                                                          \
+    hipLaunchKernelGGL((k_shadow<C, true, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
+                       0, st, S, s_o, s_d, vis, ctr, stats, cap, B.bs)
+    MRT_DISPATCH2(count, S.quad != 0);
+#undef MRT_GO
     return hipGetLastError();
 }
 
@@ -2007,9 +2003,9 @@ int trace_blocks_per_cu(bool shadow)
     int nb = 0;
     hipError_t e = shadow
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, k_shadow<false, false>, TRACE_BLOCK, 0)
+              &nb, k_shadow<false, false, false>, TRACE_BLOCK, 0)
         : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, k_trace<false, false>, TRACE_BLOCK, 0);
+              &nb, k_trace<false, false, false>, TRACE_BLOCK, 0);
     if (e != hipSuccess || nb < 1)
         nb = 4;
     return nb;

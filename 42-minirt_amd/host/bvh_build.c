@@ -74,6 +74,44 @@ static inline double box_area(const box3 *b)
     return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
 }
 
+/* Box of a cylinder, cone or hyperboloid: the finite solid that bounds its
+ * surface (the cylinder of radius r between its caps, the cone from apex
+ * to base disk, the hyperboloid's slab |z| <= height / 2 inside the
+ * circular cylinder of radius max(a, b) sqrt(shape + (height/2)^2 / c^2)),
+ * computed in double and widened by a relative 1e-5 so the float box holds
+ * every float hit point. */
+static void quadric_box(const mrt_quadric *q, box3 *b)
+{
+    double A[3] = {q->axis[0], q->axis[1], q->axis[2]};
+    double P[3] = {q->pos[0], q->pos[1], q->pos[2]};
+    double h = q->height, lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        double side = sqrt(fmax(0.0, 1.0 - A[k] * A[k]));
+        if (q->kind == MRT_QUADRIC_CONE) {
+            double base = P[k] + A[k] * h, rr = (double)q->radius * side;
+            lo[k] = fmin(P[k], base - rr);
+            hi[k] = fmax(P[k], base + rr);
+        } else {
+            double rad = q->radius;
+            if (q->kind == MRT_QUADRIC_HYPERBOLOID) {
+                double hc = 0.5 * h / q->c;
+                rad = fmax(q->a, q->b) * sqrt(fmax(0.0, q->shape + hc * hc));
+            }
+            double e = fabs(A[k]) * 0.5 * h + rad * side;
+            lo[k] = P[k] - e;
+            hi[k] = P[k] + e;
+        }
+    }
+    double scale = 0.0;
+    for (int k = 0; k < 3; k++)
+        scale = fmax(scale, fmax(fabs(lo[k]), fabs(hi[k])));
+    double pad = 1e-5 * (scale + h + q->radius + q->a + q->b + q->c);
+    for (int k = 0; k < 3; k++) {
+        b->lo[k] = nextafterf((float)(lo[k] - pad), -INFINITY);
+        b->hi[k] = nextafterf((float)(hi[k] + pad), INFINITY);
+    }
+}
+
 /* Fills the primitive records; MRT_E_INVALID for a non-finite vertex,
  * centre or radius. */
 static int prim_bounds(builder *B)
@@ -103,6 +141,16 @@ static int prim_bounds(builder *B)
             finite &= isfinite(r->b.lo[k]) && isfinite(r->b.hi[k]);
         }
         finite &= q[3] >= 0.0f;
+        r->id = i;
+    }
+    for (uint32_t j = 0; j < s->n_quadrics; j++) {
+        uint32_t i = s->n_triangles + s->n_spheres + j;
+        prec *r = &B->rec[i];
+        quadric_box(&s->quadrics[j], &r->b);
+        for (int k = 0; k < 3; k++) {
+            r->c[k] = 0.5f * (r->b.lo[k] + r->b.hi[k]);
+            finite &= isfinite(r->b.lo[k]) && isfinite(r->b.hi[k]);
+        }
         r->id = i;
     }
     return finite ? MRT_OK : MRT_E_INVALID;
@@ -363,7 +411,9 @@ int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
     if ((s->n_triangles && (!s->tri_pos || !s->tri_mat))
         || (s->n_spheres && (!s->sph || !s->sph_mat)))
         return MRT_E_INVALID;
-    uint64_t n64 = (uint64_t)s->n_triangles + s->n_spheres;
+    if (s->n_quadrics && !s->quadrics)
+        return MRT_E_INVALID;
+    uint64_t n64 = (uint64_t)s->n_triangles + s->n_spheres + s->n_quadrics;
     if (n64 >= (1u << 28))
         return MRT_E_LIMIT;
     uint32_t n = (uint32_t)n64;
@@ -454,6 +504,25 @@ int mrt_bvh_build(const mrt_scene *s, uint32_t max_leaf, mrt_bvh *out)
                     for (int j = 0; j < 3; j++)
                         nrm[k * 3 + j] = (float)(c[j] * l);
             }
+        } else if (p >= s->n_triangles + s->n_spheres) {
+            uint32_t j = p - s->n_triangles - s->n_spheres;
+            const mrt_quadric *q = &s->quadrics[j];
+            for (int k = 0; k < 3; k++) {
+                r->a[k] = q->pos[k];
+                r->b[k] = q->axis[k];
+            }
+            r->a[3] = bits_f(MRT_PRIM_QUADRIC | j);
+            r->b[3] = bits_f(MRT_PRIMKIND_QUADRIC + q->kind);
+            int hyp = q->kind == MRT_QUADRIC_HYPERBOLOID;
+            r->c[0] = hyp ? q->a : q->radius;
+            r->c[1] = hyp ? q->b : 0.0f;
+            r->c[2] = hyp ? q->c : 0.0f;
+            r->c[3] = q->height;
+            memset(nrm, 0, 9 * sizeof(float));
+            nrm[0] = hyp ? q->shape : 0.0f;
+            nrm[1] = bits_f(q->mat);
+            nrm[2] = bits_f(q->mat_top);
+            nrm[3] = bits_f(q->mat_bot);
         } else {
             uint32_t j = p - s->n_triangles;
             const float *q = s->sph + (size_t)j * 4;

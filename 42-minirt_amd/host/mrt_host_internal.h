@@ -16,6 +16,9 @@ struct mrt_host_scene {
     uint32_t     *sph_mat;
     size_t        cap_sph, cap_sph_mat;
     uint32_t      n_sph;
+    mrt_quadric  *quad;
+    size_t        cap_quad;
+    uint32_t      n_quad;
     float        *pln;
     uint32_t     *pln_mat;
     size_t        cap_pln, cap_pln_mat;

@@ -13,8 +13,12 @@
  *   tex name image file.hdr             (parse_elements5.c:85-142)
  *   A ratio texname                     an HDRI skybox (get_hdr_pixel_skybox)
  *   cu pos dir scale file.obj r,g,b|mat (parse_elements4.c:101-128)
- * Image textures other than .hdr skyboxes, checker textures, textured
- * materials and quadrics (cy co hb) are reported as unsupported.  Relative
+ *   cy pos dir diameter height r,g,b|mat [mat_top [mat_bot]]
+ *   co apex dir diameter height r,g,b|mat [mat_top]
+ *   hb pos dir height a b c shape r,g,b|mat   (parse_elements3.c:15-57,
+ *                                              parse_elements4.c:15-99)
+ * Image textures other than .hdr skyboxes, checker textures and textured
+ * materials are reported as unsupported.  Relative
  * file paths are tried from the working directory, then from the .rt file's
  * directory.  Numbers go through strtod; the reference's ft_atod is not
  * available, so the rounding of parsed decimals is not pinned to it.
@@ -278,6 +282,82 @@ static int set_skybox(parser *P, const texdef *d, float ratio)
     return r < 0 ? perr(P, r, "HDRI", d->name) : MRT_OK;
 }
 
+/* A material given by name only (parse_only_mat,
+ * parse_elements_utils5.c:55-71). */
+static int only_mat(parser *P, const char *s, uint32_t *mat)
+{
+    int r = mrt_hs_find_material(P->hs, s);
+    if (r < 0)
+        return perr(P, MRT_E_INVALID, "unknown material", s);
+    *mat = (uint32_t)r;
+    return MRT_OK;
+}
+
+/* "cy pos dir diameter height colour|mat [mat_top] [mat_bot]",
+ * "co apex dir diameter height colour|mat [mat_top]" and
+ * "hb pos dir height a b c shape colour|mat" (parse_cylinder / parse_cone /
+ * parse_hyperboloid, parse_elements3.c:15-57, parse_elements4.c:15-99).
+ * Sizes must be > 0 here: the reference also takes 0, where its
+ * intersections divide by zero. */
+static int p_quadric(parser *P, char **t, int n)
+{
+    const char *id = t[0];
+    mrt_quadric q;
+    memset(&q, 0, sizeof(q));
+    double v[5];
+    int nsz, imat;
+    if (!strcmp(id, "cy")) {
+        q.kind = MRT_QUADRIC_CYLINDER;
+        if (n < 6 || n > 8)
+            return perr(P, MRT_E_INVALID, "cy takes 5 to 7 fields", NULL);
+        nsz = 2;
+        imat = 5;
+    } else if (!strcmp(id, "co")) {
+        q.kind = MRT_QUADRIC_CONE;
+        if (n < 6 || n > 7)
+            return perr(P, MRT_E_INVALID, "co takes 5 or 6 fields", NULL);
+        nsz = 2;
+        imat = 5;
+    } else {
+        q.kind = MRT_QUADRIC_HYPERBOLOID;
+        if (n != 9)
+            return perr(P, MRT_E_INVALID, "hb takes 8 fields", NULL);
+        nsz = 5;
+        imat = 8;
+    }
+    if (!pos3(t[1], q.pos))
+        return perr(P, MRT_E_INVALID, "bad position", t[1]);
+    if (!unit_vec(t[2], q.axis))
+        return perr(P, MRT_E_INVALID, "bad orientation", t[2]);
+    for (int k = 0; k < nsz; k++) {
+        if (!num(t[3 + k], &v[k]))
+            return perr(P, MRT_E_INVALID, "bad size", t[3 + k]);
+        /* the hyperboloid's shape may take any sign */
+        if (!(q.kind == MRT_QUADRIC_HYPERBOLOID && k == 4) && !(v[k] > 0.0))
+            return perr(P, MRT_E_INVALID, "size must be > 0", t[3 + k]);
+    }
+    if (q.kind == MRT_QUADRIC_HYPERBOLOID) {
+        q.height = (float)v[0];
+        q.a = (float)v[1];
+        q.b = (float)v[2];
+        q.c = (float)v[3];
+        q.shape = (float)v[4];
+    } else {
+        q.radius = (float)(v[0] * 0.5);
+        q.height = (float)v[1];
+    }
+    int r = color_or_mat(P, t[imat], &q.mat);
+    if (r)
+        return r;
+    q.mat_top = q.mat_bot = q.mat;
+    if (q.kind != MRT_QUADRIC_HYPERBOLOID && n > 6 && (r = only_mat(P, t[6], &q.mat_top)))
+        return r;
+    if (q.kind == MRT_QUADRIC_CYLINDER && n > 7 && (r = only_mat(P, t[7], &q.mat_bot)))
+        return r;
+    r = mrt_hs_add_quadric(P->hs, &q);
+    return r < 0 ? perr(P, r, "quadric rejected", NULL) : MRT_OK;
+}
+
 static int p_line(parser *P, char **t, int n, int pass)
 {
     const char *id = t[0];
@@ -419,7 +499,7 @@ static int p_line(parser *P, char **t, int n, int pass)
         return MRT_OK;
     }
     if (!strcmp(id, "cy") || !strcmp(id, "co") || !strcmp(id, "hb"))
-        return perr(P, MRT_E_UNSUPPORTED, "element not supported by this renderer", id);
+        return p_quadric(P, t, n);
     return perr(P, MRT_E_INVALID, "unknown element", id);
 }
 

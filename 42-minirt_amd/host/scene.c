@@ -5,6 +5,7 @@
  */
 #include "mrt_host_internal.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -46,6 +47,7 @@ void mrt_hs_free(mrt_host_scene *hs)
     free(hs->tri_mat);
     free(hs->sph);
     free(hs->sph_mat);
+    free(hs->quad);
     free(hs->pln);
     free(hs->pln_mat);
     free(hs->materials);
@@ -101,6 +103,35 @@ int mrt_hs_add_sphere(mrt_host_scene *hs, const float c[3], float r,
     s[3] = r;
     hs->sph_mat[hs->n_sph] = mat;
     return (int)hs->n_sph++;
+}
+
+int mrt_hs_add_quadric(mrt_host_scene *hs, const mrt_quadric *q)
+{
+    if (!hs || !q || q->kind > MRT_QUADRIC_HYPERBOLOID || q->mat >= hs->n_mat
+        || q->mat_top >= hs->n_mat || q->mat_bot >= hs->n_mat)
+        return MRT_E_INVALID;
+    mrt_quadric c = *q;
+    double l = sqrt((double)c.axis[0] * c.axis[0] + (double)c.axis[1] * c.axis[1]
+                    + (double)c.axis[2] * c.axis[2]);
+    int ok = l > 0.0 && isfinite(l) && c.height > 0.0f && isfinite(c.height);
+    for (int k = 0; k < 3; k++)
+        ok = ok && isfinite(c.pos[k]);
+    if (c.kind == MRT_QUADRIC_HYPERBOLOID)
+        ok = ok && c.a > 0.0f && c.b > 0.0f && c.c > 0.0f && isfinite(c.a)
+             && isfinite(c.b) && isfinite(c.c) && isfinite(c.shape);
+    else
+        ok = ok && c.radius > 0.0f && isfinite(c.radius);
+    if (!ok)
+        return MRT_E_INVALID;
+    for (int k = 0; k < 3; k++)
+        c.axis[k] = (float)(c.axis[k] / l);
+    if (hs->n_quad >= (1u << 30))
+        return MRT_E_LIMIT;
+    if (grow((void **)&hs->quad, &hs->cap_quad, hs->n_quad + 1,
+             sizeof(mrt_quadric)))
+        return MRT_E_NOMEM;
+    hs->quad[hs->n_quad] = c;
+    return (int)hs->n_quad++;
 }
 
 int mrt_hs_add_plane(mrt_host_scene *hs, const float p[3], const float n[3],
@@ -254,6 +285,8 @@ const mrt_scene *mrt_hs_view(mrt_host_scene *hs)
     v->n_spheres = hs->n_sph;
     v->sph = hs->sph;
     v->sph_mat = hs->sph_mat;
+    v->n_quadrics = hs->n_quad;
+    v->quadrics = hs->quad;
     v->n_planes = hs->n_pln;
     v->pln = hs->pln;
     v->pln_mat = hs->pln_mat;

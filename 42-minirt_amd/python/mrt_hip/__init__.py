@@ -41,6 +41,7 @@ PRIM_NONE = 0xFFFFFFFF
 PRIM_TRIANGLE = 0x00000000
 PRIM_SPHERE = 0x40000000
 PRIM_PLANE = 0x80000000
+PRIM_QUADRIC = 0xC0000000
 PRIM_KIND_MASK = 0xC0000000
 
 LIGHT_POINT = 0
@@ -98,6 +99,22 @@ _fp = C.POINTER(C.c_float)
 _up = C.POINTER(C.c_uint32)
 
 
+QUADRIC_CYLINDER = 0
+QUADRIC_CONE = 1
+QUADRIC_HYPERBOLOID = 2
+
+
+class Quadric(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("pos", C.c_float * 3),
+                ("axis", C.c_float * 3), ("height", C.c_float),
+                ("radius", C.c_float), ("a", C.c_float), ("b", C.c_float),
+                ("c", C.c_float), ("shape", C.c_float), ("mat", C.c_uint32),
+                ("mat_top", C.c_uint32), ("mat_bot", C.c_uint32)]
+
+
+assert C.sizeof(Quadric) == 64
+
+
 class Scene(C.Structure):
     _fields_ = [("n_triangles", C.c_uint32), ("tri_pos", _fp),
                 ("tri_nrm", _fp), ("tri_mat", _up),
@@ -108,7 +125,8 @@ class Scene(C.Structure):
                 ("n_lights", C.c_uint32), ("lights", C.POINTER(Light)),
                 ("camera", Camera), ("ambient_ratio", C.c_float),
                 ("ambient_color", C.c_float * 3), ("ior_all", C.c_float),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("n_quadrics", C.c_uint32),
+                ("quadrics", C.POINTER(Quadric))]
 
 
 class Stats(C.Structure):
@@ -248,6 +266,8 @@ mrt_hs_find_material = _sig("mrt_hs_find_material", C.c_int, _hs_p,
                             C.c_char_p)
 mrt_hs_add_sphere = _sig("mrt_hs_add_sphere", C.c_int, _hs_p, _fp, C.c_float,
                          C.c_uint32)
+mrt_hs_add_quadric = _sig("mrt_hs_add_quadric", C.c_int, _hs_p,
+                          C.POINTER(Quadric))
 mrt_hs_add_plane = _sig("mrt_hs_add_plane", C.c_int, _hs_p, _fp, _fp,
                         C.c_uint32)
 mrt_hs_add_triangle = _sig("mrt_hs_add_triangle", C.c_int, _hs_p, _fp, _fp,
@@ -429,6 +449,38 @@ class HostScene:
         return _check(mrt_hs_add_material(
             self._h, C.byref(m), name.encode() if name else None))
 
+    def add_quadric(self, kind, pos, axis, height, mat, radius=0.0, a=0.0,
+                    b=0.0, c=0.0, shape=0.0, mat_top=None, mat_bot=None):
+        """A cylinder, cone or hyperboloid (mrt_quadric); cap materials
+        default to mat."""
+        q = Quadric()
+        q.kind = kind
+        q.pos[:] = [float(x) for x in pos]
+        q.axis[:] = [float(x) for x in axis]
+        q.height, q.radius = height, radius
+        q.a, q.b, q.c, q.shape = a, b, c, shape
+        q.mat = mat
+        q.mat_top = mat if mat_top is None else mat_top
+        q.mat_bot = mat if mat_bot is None else mat_bot
+        return _check(mrt_hs_add_quadric(self._h, C.byref(q)))
+
+    def add_cylinder(self, centre, axis, diameter, height, mat, mat_top=None,
+                     mat_bot=None):
+        """The .rt "cy" element: centre, axis, diameter, height."""
+        return self.add_quadric(QUADRIC_CYLINDER, centre, axis, height, mat,
+                                radius=0.5 * diameter, mat_top=mat_top,
+                                mat_bot=mat_bot)
+
+    def add_cone(self, apex, axis, diameter, height, mat, mat_top=None):
+        """The .rt "co" element: apex, axis, base diameter, height."""
+        return self.add_quadric(QUADRIC_CONE, apex, axis, height, mat,
+                                radius=0.5 * diameter, mat_top=mat_top)
+
+    def add_hyperboloid(self, centre, axis, height, a, b, c, shape, mat):
+        """The .rt "hb" element: centre, axis, height, a, b, c, shape."""
+        return self.add_quadric(QUADRIC_HYPERBOLOID, centre, axis, height, mat,
+                                a=a, b=b, c=c, shape=shape)
+
     def add_sphere(self, centre, radius, mat):
         c = np.asarray(centre, dtype=np.float32)
         return _check(mrt_hs_add_sphere(self._h, c.ctypes.data_as(_fp),
@@ -496,6 +548,11 @@ class HostScene:
             "sph_mat": _arr(s.sph_mat, s.n_spheres, np.uint32),
             "pln": _arr(s.pln, 6 * s.n_planes, np.float32).reshape(-1, 6),
             "pln_mat": _arr(s.pln_mat, s.n_planes, np.uint32),
+            "quadrics": [
+                {f: (list(getattr(s.quadrics[i], f))
+                     if f in ("pos", "axis") else getattr(s.quadrics[i], f))
+                 for f, _ in Quadric._fields_}
+                for i in range(s.n_quadrics)],
         }
 
 

@@ -77,6 +77,36 @@ typedef struct mrt_camera {
                                                integers (reference t_lcolor
                                                is uint64_t and truncates)   */
 
+/* Cylinders, cones and hyperboloids: t_cylinder / t_cone / t_hyperboloid
+ * (includes/scene_structs.h) as the .rt "cy", "co" and "hb" elements give
+ * them (parse_elements3.c:15-57, parse_elements4.c:15-99).
+ *   cylinder:    centre pos, |proj on axis| <= height / 2 of the centre,
+ *                radius = diameter / 2, closed by two caps
+ *                (hit_cylinder.c:107-134);
+ *   cone:        apex pos, body from the apex to apex + axis * height, base
+ *                radius = diameter / 2, closed by the base cap
+ *                (hit_cone.c:111-129);
+ *   hyperboloid: centre pos, x^2/a^2 + y^2/b^2 - z^2/c^2 = shape in a frame
+ *                whose z is the axis, |z| <= height / 2, no caps
+ *                (hit_hyperboloid.c:79-104).
+ * axis is a unit vector (the parser normalises it).  mat_top / mat_bot are
+ * the cap materials (material_top / material_bot, or mat when the file
+ * names none); the cone has no bottom cap. */
+#define MRT_QUADRIC_CYLINDER     0u
+#define MRT_QUADRIC_CONE         1u
+#define MRT_QUADRIC_HYPERBOLOID  2u
+
+typedef struct mrt_quadric {
+    uint32_t kind;            /* MRT_QUADRIC_*                               */
+    float    pos[3];
+    float    axis[3];
+    float    height;
+    float    radius;          /* cylinder, cone                              */
+    float    a, b, c;         /* hyperboloid semi-axes                       */
+    float    shape;           /* hyperboloid right-hand side                 */
+    uint32_t mat, mat_top, mat_bot;
+} mrt_quadric;                /* 64 bytes */
+
 typedef struct mrt_scene {
     uint32_t            n_triangles;
     const float        *tri_pos;      /* 9 floats / tri: v0 v1 v2             */
@@ -98,6 +128,8 @@ typedef struct mrt_scene {
     float               ambient_color[3];  /* A colour, 0..255                 */
     float               ior_all;           /* scene->ior_all (1.0)             */
     uint32_t            flags;             /* MRT_SCENE_*                      */
+    uint32_t            n_quadrics;
+    const mrt_quadric  *quadrics;
 } mrt_scene;
 
 /* Primitive ids reported by MRT_AOV_PRIM_ID: the index into the array the
@@ -106,6 +138,7 @@ typedef struct mrt_scene {
 #define MRT_PRIM_TRIANGLE  0x00000000u
 #define MRT_PRIM_SPHERE    0x40000000u
 #define MRT_PRIM_PLANE     0x80000000u
+#define MRT_PRIM_QUADRIC   0xC0000000u
 #define MRT_PRIM_KIND_MASK 0xC0000000u
 
 /* Estimator for the HDRI term (see mrt_upload_hdri). */

@@ -35,6 +35,10 @@ typedef struct mrt_bvh_node {
  *           c = (v2-v0).xyz | material
  * sphere:   a = centre.xyz | src id,  b = (radius, radius^2, 0) | kind 1,
  *           c = (0, 0, 0) | material
+ * quadric:  a = pos.xyz | src id,  b = axis.xyz | kind 2, 3 or 4
+ *           (cylinder, cone, hyperboloid), c = (radius or a, b, c, height);
+ *           its shading record (mrt_bvh.shade_nrm) holds shape, mat,
+ *           mat_top, mat_bot (the last three as uint32 bits).
  * "src id" is the MRT_PRIM_* tagged index into the scene arrays. */
 typedef struct mrt_prim {
     float a[4], b[4], c[4];
@@ -42,6 +46,7 @@ typedef struct mrt_prim {
 
 #define MRT_PRIMKIND_TRIANGLE 0u
 #define MRT_PRIMKIND_SPHERE   1u
+#define MRT_PRIMKIND_QUADRIC  2u      /* + MRT_QUADRIC_* */
 #define MRT_BVH_MAX_LEAF      4u
 #define MRT_BVH_MAX_DEPTH     60u     /* the device traversal stack holds 64 */
 
@@ -101,6 +106,9 @@ int  mrt_hs_add_sphere(mrt_host_scene *hs, const float centre[3], float radius,
                        uint32_t mat);
 int  mrt_hs_add_plane(mrt_host_scene *hs, const float point[3],
                       const float normal[3], uint32_t mat);
+/* Copies q with its axis normalised (vec3_unit); sizes must be > 0 and
+ * finite, the materials defined. */
+int  mrt_hs_add_quadric(mrt_host_scene *hs, const mrt_quadric *q);
 int  mrt_hs_add_triangle(mrt_host_scene *hs, const float v[9],
                          const float n[9] /* NULL: flat */, uint32_t mat);
 int  mrt_hs_add_light(mrt_host_scene *hs, const mrt_light *l);

@@ -118,6 +118,17 @@ void orc_trace_rays_f64(const mrt_scene *s, const orc_bvh *b, int traversal,
                         uint32_t *prim, double *t, uint32_t *nodes,
                         uint32_t *prims, int threads);
 
+/* Closest hits with the shading frame: normal facing the ray (3 per ray),
+ * front side flag and material of the part hit. */
+void orc_hit_info_f32(const mrt_scene *s, const orc_bvh *b, int traversal,
+                      uint32_t n, const float *orig, const float *dir,
+                      uint32_t *prim, float *t, float *nrm, uint8_t *front,
+                      uint32_t *mat);
+void orc_hit_info_f64(const mrt_scene *s, const orc_bvh *b, int traversal,
+                      uint32_t n, const float *orig, const float *dir,
+                      uint32_t *prim, double *t, double *nrm, uint8_t *front,
+                      uint32_t *mat);
+
 /* Any-hit: occluded[i] = 1 when a primitive lies at t in [0.001, tmax[i])
  * (tmax capped at 1000 inclusive), as check_plight_hit
  * (src/ray_tracing/light/p_light.c:48-64). */

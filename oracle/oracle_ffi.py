@@ -74,6 +74,11 @@ def lib():
                                          C.POINTER(C.c_uint32), vp,
                                          C.POINTER(C.c_uint32),
                                          C.POINTER(C.c_uint32), C.c_int]
+        for name in ("orc_hit_info_f32", "orc_hit_info_f64"):
+            getattr(L, name).argtypes = [vp, vp, C.c_int, u32, fp, fp,
+                                         C.POINTER(C.c_uint32), vp, vp,
+                                         C.POINTER(C.c_uint8),
+                                         C.POINTER(C.c_uint32)]
         L.orc_occluded_f32.argtypes = [vp, vp, u32, fp, fp, fp,
                                        C.POINTER(C.c_uint8),
                                        C.POINTER(C.c_uint32),
@@ -166,6 +171,25 @@ def trace_rays(scene_view, bvh, mode, orig, dirs, f64=False, threads=0):
        C.byref(bvh) if bvh is not None else None, mode, n, _fp(o), _fp(d),
        _up(prim), t.ctypes.data, _up(nodes), _up(prims), threads)
     return prim, t, nodes, prims
+
+
+def hit_info(scene_view, bvh, mode, orig, dirs, f64=False):
+    """Closest hits with normal (facing the ray), front flag and material."""
+    o = np.ascontiguousarray(orig, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    n = o.size // 3
+    ft = np.float64 if f64 else np.float32
+    prim = np.zeros(n, np.uint32)
+    t = np.zeros(n, ft)
+    nrm = np.zeros((n, 3), ft)
+    front = np.zeros(n, np.uint8)
+    mat = np.zeros(n, np.uint32)
+    fn = lib().orc_hit_info_f64 if f64 else lib().orc_hit_info_f32
+    fn(C.cast(scene_view, C.c_void_p),
+       C.byref(bvh) if bvh is not None else None, mode, n, _fp(o), _fp(d),
+       _up(prim), t.ctypes.data, nrm.ctypes.data,
+       front.ctypes.data_as(C.POINTER(C.c_uint8)), _up(mat))
+    return prim, t, nrm, front.astype(bool), mat
 
 
 def occluded_f32(scene_view, bvh, orig, dirs, tmax):

@@ -34,6 +34,24 @@ def planes_only_scene():
     return hs
 
 
+def quadric_scene():
+    """Cylinder, cone and hyperboloid with cap materials, after scenes with
+    materials of their own (their indices are rebased in the batch)."""
+    hs = M.HostScene.empty()
+    lam = hs.add_material(color=(0.7, 0.7, 0.7))
+    red = hs.add_material(color=(0.9, 0.2, 0.2))
+    metal = hs.add_material(color=(0.9, 0.9, 0.9), metallic=1.0, roughness=0.3)
+    glass = hs.add_material(color=(1.0, 1.0, 1.0), ior=1.5, transmission=1.0)
+    hs.add_plane((0, 0, 0), (0, 1, 0), lam)
+    hs.add_cylinder((-1.2, 0.8, 0), (0, 1, 0.2), 1.0, 1.6, red, metal, glass)
+    hs.add_cone((1.2, 1.6, 0), (0, -1, 0), 1.2, 1.6, glass, red)
+    hs.add_hyperboloid((0, 1.0, 1.5), (0, 1, 0), 2.0, 0.4, 0.4, 0.6, 1.0, metal)
+    hs.add_light((0, 4, -2), 0.9, (255, 255, 255))
+    hs.set_ambient(0.2, (200, 220, 255))
+    hs.set_camera((0, 1.5, -5), (0, -0.15, 1), fov=60.0)
+    return hs
+
+
 # (scene factory, width, height, spp, seed)
 CASES = [
     (lambda: M.HostScene.plain(7, 64, 36), 64, 36, 5, 0x1111),
@@ -42,6 +60,7 @@ CASES = [
      0x3333),
     (soft_light_scene, 33, 21, 6, 0x4444),
     (planes_only_scene, 20, 17, 2, 0x5555),
+    (quadric_scene, 40, 32, 4, 0x6666),
 ]
 
 

@@ -73,8 +73,8 @@ def test_light_radius_defaults_to_five_samples():
     ("A 0.2 255,255,255\nC 0,0,0 0,0,1 181\n", M.MRT_E_INVALID),  # fov
     ("A 0.2 255,255,255\nC 0,0,0 0,0,1 70\nsp 0,0,0 1 nomat\n",
      M.MRT_E_INVALID),
-    ("A 0.2 255,255,255\nC 0,0,0 0,0,1 70\ncy 0,0,0 0,1,0 1 2 255,0,0\n",
-     M.MRT_E_UNSUPPORTED),
+    ("A 0.2 255,255,255\nC 0,0,0 0,0,1 70\ncy 0,0,0 0,1,0 1 2 255,0,0 top\n",
+     M.MRT_E_INVALID),                                  # unknown cap material
     ("A 0.2 255,255,255\nC 0,0,0 0,0,1 70\nxx 1\n", M.MRT_E_INVALID),
 ])
 def test_parse_errors(text, code):
