@@ -2,13 +2,58 @@
  * Output formats of the accumulation buffer: the P6 PPM export
  * (src/utils/ppm/export_ppm.c:42-87 with render_float_to_int_gamma,
  * src/upscaling/utils.c:15-35) and the render_state.rts checkpoint
- * (src/utils/export_render_state.c:15-51).
+ * (src/utils/export_render_state.c:15-51), and the BVH heat map of render
+ * mode 1.
  */
 #include "mrt_host_internal.h"
 
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+
+/* get_heatmap_fcolor: blue -> cyan -> green -> yellow -> red in four float
+ * ramps of ratio = min(current / max, 1); max == 0 gives blue. */
+static void heat_color(int cur, int max, float c[3])
+{
+    c[0] = c[1] = c[2] = 0.0f;
+    if (max == 0) {
+        c[2] = 1.0f;
+        return;
+    }
+    float r = fminf((float)cur / (float)max, 1.0f);
+    if (r < 0.25f) {
+        c[2] = 4.0f * r;
+    } else if (r < 0.5f) {
+        c[1] = 1.0f;
+        c[2] = 1.0f - 4.0f * (r - 0.25f);
+    } else if (r < 0.75f) {
+        c[0] = 4.0f * (r - 0.5f);
+        c[1] = 1.0f;
+    } else {
+        c[0] = 1.0f;
+        c[1] = 1.0f - 4.0f * (r - 0.75f);
+    }
+}
+
+void mrt_heat_map_add(const int32_t *depth, uint32_t w, uint32_t h,
+                      double *acc)
+{
+    if (!depth || !acc)
+        return;
+    size_t n = (size_t)w * h;
+    int max = 0;
+    for (size_t i = 0; i < n; i++)
+        if (max < depth[i])
+            max = depth[i];
+    for (size_t i = 0; i < n; i++) {
+        if (depth[i] < 0)
+            continue;
+        float c[3];
+        heat_color(depth[i], max, c);
+        for (int k = 0; k < 3; k++)
+            acc[3 * i + k] += (double)c[k];
+    }
+}
 
 static unsigned char to_byte(double acc, uint32_t samples, double gamma)
 {

@@ -313,6 +313,8 @@ mrt_decode_hdr = _sig("mrt_decode_hdr", C.c_int, C.c_char_p, C.c_size_t,
 mrt_free = _sig("mrt_free", None, C.c_void_p)
 mrt_hs_add_obj_text = _sig("mrt_hs_add_obj_text", C.c_int, _hs_p, C.c_char_p,
                            _fp, _fp, C.c_float, C.c_uint32)
+mrt_heat_map_add = _sig("mrt_heat_map_add", None, C.POINTER(C.c_int32),
+                        C.c_uint32, C.c_uint32, C.POINTER(C.c_double))
 mrt_export_ppm = _sig("mrt_export_ppm", C.c_int, C.c_char_p,
                       C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
                       C.c_uint32, C.c_double)
@@ -888,6 +890,19 @@ def decode_hdr(data):
     finally:
         mrt_free(p)
     return a, ex.value, ga.value
+
+
+def heat_map_add(depth, float_render):
+    """set_heat_map_color: float_render (h, w, 3 doubles) += ramp colour of
+    depth / max(depth) where depth >= 0 (in place)."""
+    d = np.ascontiguousarray(depth, dtype=np.int32)
+    a = float_render
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    h, w = a.shape[:2]
+    assert d.size == w * h
+    mrt_heat_map_add(d.ctypes.data_as(C.POINTER(C.c_int32)), w, h,
+                     a.ctypes.data_as(C.POINTER(C.c_double)))
+    return a
 
 
 def export_ppm(path, float_render, samples, gamma=1.0):

@@ -215,7 +215,8 @@ void mrt_hdri_free_mis(mrt_hdri_mis *m);
 
 /* ---- .rt reader (src/parsing/parse_scene.c:99-125) ---------------------- */
 
-/* Reads a .rt scene: A C L l dl sp pl mat W (cu / cy / co / hb / tex are
+/* Reads a .rt scene: A C L l dl sp pl cy co hb cu mat tex W (image textures
+ * other than .hdr skyboxes, checker textures and textured materials are
  * rejected with MRT_E_UNSUPPORTED).  err receives a message on failure. */
 int  mrt_parse_rt_file(const char *path, mrt_host_scene **out, char *err,
                        size_t errlen);
@@ -232,6 +233,14 @@ int  mrt_export_rts(const char *path, const double *float_render,
                     uint32_t w, uint32_t h, uint32_t samples);
 int  mrt_load_rts(const char *path, double *float_render, uint32_t w,
                   uint32_t h, uint32_t *samples);
+
+/* BVH heat map, as set_heat_map_color (src/ray_tracing/calc_sample.c:16-42)
+ * with get_heatmap_fcolor (src/math/color/get_heat_map_color.c:16-34): adds
+ * the ramp colour of depth[i] / max(depth) to float_render[i] (3 doubles)
+ * for every pixel with depth[i] >= 0 (a miss is -1).  depth is the BVH work
+ * per pixel, e.g. MRT_AOV_NODES of the pixels MRT_AOV_PRIM_ID reports hit. */
+void mrt_heat_map_add(const int32_t *depth, uint32_t w, uint32_t h,
+                      double *float_render);
 
 /* ---- render-frame loop (src/rendering/loop.c:36-63) --------------------- */
 

@@ -105,6 +105,21 @@ def test_ppm_export(tmp_path):
     assert px[0] == int(np.sqrt(0.5) * 255) and px[5] == 127
 
 
+def test_heat_map_ramp():
+    """get_heatmap_fcolor's four ramps at hand-picked ratios, misses
+    untouched, values added to what the buffer holds."""
+    depth = np.array([[0, 10, 30, 50], [70, 90, 100, -1]], np.int32)
+    acc = np.full((2, 4, 3), 0.5)
+    M.heat_map_add(depth, acc)
+    want = np.array([[[0, 0, 0], [0, 0, 0.4], [0, 1, 0.8], [0, 1, 0]],
+                     [[0.8, 1, 0], [1, 0.4, 0], [1, 0, 0], [0, 0, 0]]])
+    np.testing.assert_allclose(acc - 0.5, want, atol=1e-6)
+    # a buffer whose largest depth is 0 is blue where hit
+    acc = np.zeros((1, 2, 3))
+    M.heat_map_add(np.array([0, -1], np.int32), acc)
+    assert acc.tolist() == [[[0, 0, 1], [0, 0, 0]]]
+
+
 def test_rts_round_trip(tmp_path):
     rng = np.random.default_rng(3)
     acc = rng.random((5, 7, 3))
