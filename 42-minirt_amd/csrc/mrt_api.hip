@@ -31,6 +31,12 @@ struct mrt_ctx {
     float4      *d_mats = nullptr, *d_lights = nullptr;
     float       *d_nrm = nullptr;
     bool         has_quad = false;   // the scene has cylinders / cones / ...
+    float4      *d_tex = nullptr;    // textures: 3 float4 each
+    uint32_t    *d_texels = nullptr;
+    uint4       *d_mtex = nullptr;   // material texture slots: 3 uint4 each
+    float       *d_tri_uv = nullptr;
+    bool         textured = false;   // some material has a texture slot
+    bool         alpha = false;      // ... a colour image with alpha < 255
     uint32_t     n_nodes = 0, n_prims = 0, n_planes = 0, n_mats = 0;
     uint32_t     bvh_depth = 0;
     std::vector<mrt_light> lights;
@@ -109,6 +115,13 @@ static inline float bits_f(uint32_t u)
     float f;
     memcpy(&f, &u, sizeof(f));
     return f;
+}
+
+static inline uint32_t bits_u(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, sizeof(u));
+    return u;
 }
 
 template <typename T>
@@ -225,6 +238,11 @@ static void free_scene(mrt_ctx *c)
     dfree(c->d_planes);
     dfree(c->d_mats);
     dfree(c->d_lights);
+    dfree(c->d_tex);
+    dfree(c->d_texels);
+    dfree(c->d_mtex);
+    dfree(c->d_tri_uv);
+    c->textured = c->alpha = false;
     c->has_scene = false;
 }
 
@@ -300,6 +318,23 @@ static int validate_scene(mrt_ctx *c, const mrt_scene *s)
             || q.mat_top >= s->n_materials || q.mat_bot >= s->n_materials)
             return fail(c, MRT_E_INVALID, "quadric %u: bad kind or material", i);
     }
+    if (s->n_textures && !s->textures)
+        return fail(c, MRT_E_INVALID, "textures without an array");
+    for (uint32_t i = 0; i < s->n_textures; i++) {
+        const mrt_texture &t = s->textures[i];
+        if (t.type > MRT_TEX_BLACK
+            || (t.type == MRT_TEX_IMAGE
+                && (t.width == 0 || t.height == 0 || !s->texels
+                    || (uint64_t)t.texel0 + (uint64_t)t.width * t.height > s->n_texels)))
+            return fail(c, MRT_E_INVALID, "texture %u: bad type or texels", i);
+    }
+    if (s->material_tex)
+        for (uint32_t k = 0; k < s->n_materials; k++) {
+            const uint32_t *sl = &s->material_tex[k].color;
+            for (int j = 0; j < 8; j++)
+                if (sl[j] != MRT_NO_TEX && sl[j] >= s->n_textures)
+                    return fail(c, MRT_E_INVALID, "material %u: texture out of range", k);
+        }
     if (s->n_planes && (!s->pln || !s->pln_mat))
         return fail(c, MRT_E_INVALID, "planes without arrays");
     for (uint32_t i = 0; i < s->n_planes; i++)
@@ -309,6 +344,44 @@ static int validate_scene(mrt_ctx *c, const mrt_scene *s)
         if (s->lights[k].shadow_samples > 4096)
             return fail(c, MRT_E_LIMIT, "light %u: more than 4096 shadow samples", k);
     return MRT_OK;
+}
+
+// Textures (type, w, h, texel0) (scale, c1) (c2, 0) and per material the
+// slots (color, metallic, roughness, transmission) (ao, emission strength,
+// emission colour, normal) (normal intensity, scale, 0, 0).  textured: a
+// material has a slot; alpha: a colour slot holds an image with a texel of
+// alpha < 255 (the traversal then runs the alpha test).
+static void pack_textures(const mrt_scene *s, std::vector<float4> &tex,
+                          std::vector<uint4> &mtex, bool &textured, bool &alpha)
+{
+    textured = alpha = false;
+    if (!s->material_tex || !s->n_textures)
+        return;
+    for (uint32_t k = 0; k < s->n_materials; k++) {
+        const mrt_material_tex &t = s->material_tex[k];
+        const uint32_t *sl = &t.color;
+        for (int j = 0; j < 8; j++)
+            textured |= sl[j] != MRT_NO_TEX;
+        mtex.push_back(make_uint4(t.color, t.metallic, t.roughness, t.transmission));
+        mtex.push_back(make_uint4(t.ao, t.emission_strength, t.emission_color, t.normal));
+        mtex.push_back(make_uint4(bits_u(t.normal_intensity), bits_u(t.scale), 0u, 0u));
+        if (t.color != MRT_NO_TEX && s->textures[t.color].type == MRT_TEX_IMAGE) {
+            const mrt_texture &x = s->textures[t.color];
+            for (size_t i = 0; !alpha && i < (size_t)x.width * x.height; i++)
+                alpha = s->texels[4 * ((size_t)x.texel0 + i) + 3] != 255;
+        }
+    }
+    if (!textured) {
+        mtex.clear();
+        return;
+    }
+    for (uint32_t k = 0; k < s->n_textures; k++) {
+        const mrt_texture &t = s->textures[k];
+        tex.push_back(make_float4(bits_f(t.type), bits_f(t.width), bits_f(t.height),
+                                  bits_f(t.texel0)));
+        tex.push_back(make_float4(t.scale, t.c1[0], t.c1[1], t.c1[2]));
+        tex.push_back(make_float4(t.c2[0], t.c2[1], t.c2[2], 0.0f));
+    }
 }
 
 // Planes (point | material, normal | 0), materials and lights as the
@@ -371,13 +444,23 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     }
     std::vector<float4> planes, mats, lights;
     pack_scene(s, 0, planes, mats, lights);
+    std::vector<float4> tex;
+    std::vector<uint4> mtex;
+    bool textured = false, alpha = false;
+    pack_textures(s, tex, mtex, textured, alpha);
     rc = MRT_OK;
     if (dalloc(c->d_nodes, (size_t)bvh4.n_nodes * 8) != hipSuccess
         || dalloc(c->d_prims, (size_t)bvh.n_prims * 3) != hipSuccess
         || dalloc(c->d_nrm, (size_t)bvh.n_prims * 9) != hipSuccess
         || dalloc(c->d_planes, planes.size()) != hipSuccess
         || dalloc(c->d_mats, mats.size()) != hipSuccess
-        || dalloc(c->d_lights, lights.size()) != hipSuccess)
+        || dalloc(c->d_lights, lights.size()) != hipSuccess
+        || (textured
+            && (dalloc(c->d_tex, tex.size()) != hipSuccess
+                || dalloc(c->d_texels, s->n_texels) != hipSuccess
+                || dalloc(c->d_mtex, mtex.size()) != hipSuccess
+                || (s->tri_uv
+                    && dalloc(c->d_tri_uv, (size_t)s->n_triangles * 6) != hipSuccess))))
         rc = fail(c, MRT_E_NOMEM, "device allocation for the scene failed");
     if (rc == MRT_OK)
         rc = upload(c, c->d_nodes, bvh4.nodes, (size_t)bvh4.n_nodes * 128);
@@ -391,11 +474,22 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
         rc = upload(c, c->d_mats, mats.data(), mats.size() * 16);
     if (rc == MRT_OK)
         rc = upload(c, c->d_lights, lights.data(), lights.size() * 16);
+    if (rc == MRT_OK && textured) {
+        rc = upload(c, c->d_tex, tex.data(), tex.size() * 16);
+        if (rc == MRT_OK)
+            rc = upload(c, c->d_texels, s->texels, (size_t)s->n_texels * 4);
+        if (rc == MRT_OK)
+            rc = upload(c, c->d_mtex, mtex.data(), mtex.size() * 16);
+        if (rc == MRT_OK && s->tri_uv)
+            rc = upload(c, c->d_tri_uv, s->tri_uv, (size_t)s->n_triangles * 24);
+    }
     if (rc == MRT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
         rc = fail(c, MRT_E_HIP, "scene upload failed");
     c->n_nodes = bvh4.n_nodes;
     c->n_prims = bvh.n_prims;
     c->has_quad = s->n_quadrics > 0;
+    c->textured = textured;
+    c->alpha = alpha;
     c->bvh_depth = bvh4.depth;
     mrt_bvh_free(&bvh);
     mrt_bvh4_free(&bvh4);
@@ -532,6 +626,16 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
 static FrameParams frame_params(const mrt_ctx *c);
 static SceneDev scene_dev(const mrt_ctx *c);
 
+static TexDev tex_dev(const mrt_ctx *c)
+{
+    TexDev T;
+    T.tex = c->d_tex;
+    T.texels = c->d_texels;
+    T.mtex = c->d_mtex;
+    T.tri_uv = c->d_tri_uv;
+    return T;
+}
+
 static ShadeBufs scene_bufs(const mrt_ctx *c)
 {
     ShadeBufs B;
@@ -541,6 +645,7 @@ static ShadeBufs scene_bufs(const mrt_ctx *c)
     B.mats = c->d_mats;
     B.lights = c->d_lights;
     B.nrm = c->d_nrm;
+    B.tx = tex_dev(c);
     return B;
 }
 
@@ -820,6 +925,7 @@ static FrameParams frame_params(const mrt_ctx *c)
         fp.has_photons = 1;
         fp.pm = photon_map_dev(c);
     }
+    fp.textured = c->textured ? 1u : 0u;
     return fp;
 }
 
@@ -836,6 +942,8 @@ static SceneDev scene_dev(const mrt_ctx *c)
     S.one_sided = (c->flags & MRT_SCENE_TRIANGLE_ONE_SIDED) ? 1u : 0u;
     S.root = 0;
     S.quad = c->has_quad ? 1u : 0u;
+    S.alpha = c->alpha ? 1u : 0u;
+    S.tx = tex_dev(c);
     return S;
 }
 
@@ -1003,6 +1111,7 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
             B.mats = c->d_mats;
             B.lights = c->d_lights;
             B.nrm = c->d_nrm;
+            B.tx = tex_dev(c);
             B.s_o = c->s_o;
             B.s_d = c->s_d;
             B.s_c = c->s_c;
@@ -1314,6 +1423,8 @@ extern "C" int mrt_adjoint(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples
                     ADJ_MAXMAT);
     if (c->has_photons)
         return fail(c, MRT_E_UNSUPPORTED, "no adjoint through the photon map");
+    if (c->textured)
+        return fail(c, MRT_E_UNSUPPORTED, "no adjoint through textured materials");
     if (!c->lights.empty() && !(c->flags & MRT_SCENE_LIGHT_EXACT))
         return fail(c, MRT_E_UNSUPPORTED,
                     "lights need MRT_SCENE_LIGHT_EXACT (truncated sums have no "
@@ -1357,6 +1468,7 @@ extern "C" int mrt_adjoint(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples
         B.mats = c->d_mats;
         B.lights = c->d_lights;
         B.nrm = c->d_nrm;
+        B.tx = tex_dev(c);
         AdjBufs A;
         A.dimg = d_img;
         A.g_mat = d_gm;
@@ -1520,6 +1632,14 @@ extern "C" int mrt_batch_add_scene(mrt_batch *b, const mrt_scene *s,
     int rc = validate_scene(c, s);
     if (rc != MRT_OK)
         return rc;
+    if (s->material_tex && s->n_textures)
+        for (uint32_t k = 0; k < s->n_materials; k++) {
+            const uint32_t *sl = &s->material_tex[k].color;
+            for (int j = 0; j < 8; j++)
+                if (sl[j] != MRT_NO_TEX)
+                    return fail(c, MRT_E_UNSUPPORTED,
+                                "textured materials are not batched");
+        }
     BatchSrc *p = new (std::nothrow) BatchSrc();
     if (!p)
         return fail(c, MRT_E_NOMEM, "out of host memory");

@@ -395,6 +395,190 @@ __device__ __forceinline__ void quadric_surface(float4 A, float4 B, float4 C,
     front = kind == MRT_QUADRIC_CYLINDER ? true : f;
 }
 
+// ---- textures ----------------------------------------------------------------
+// Per texture 3 float4: (type, width, height, texel0 as uint32 bits),
+// (scale, c1 rgb 0..255), (c2 rgb, 0); texels RGBA8; per material 3 uint4:
+// (color, metallic, roughness, transmission), (ao, emission_strength,
+// emission_color, normal) texture indices, (normal_intensity bits, scale
+// bits, 0, 0); tri_uv 6 floats per source triangle.
+struct TexDev {
+    const float4   *tex;
+    const uint32_t *texels;
+    const uint4    *mtex;
+    const float    *tri_uv;
+};
+
+__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+// modf's fractional part (keeps the sign)
+__device__ __forceinline__ float frac_part(float x) { return x - truncf(x); }
+
+// The reference's in-plane "right" vector of an axis: unit(cross(+Y, axis)),
+// or unit(cross(+X, axis)) when |axis.y| >= 0.99.
+__device__ __forceinline__ V3 axis_right(V3 ax)
+{
+    return unit(cross(fabsf(ax.y) < 0.99f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f), ax));
+}
+
+// Nearest texel of an image ((int)(u (w-1)), (int)(v (h-1))), kept inside
+// the image (the reference reads outside it for (u, v) outside [0, 1]).
+__device__ __forceinline__ uint32_t tex_texel(const TexDev &T, float4 T0, float u,
+                                              float v)
+{
+    uint32_t w = __float_as_uint(T0.y), h = __float_as_uint(T0.z);
+    int x = (int)(u * (float)(w - 1)), y = (int)(v * (float)(h - 1));
+    x = min(max(x, 0), (int)w - 1);
+    y = min(max(y, 0), (int)h - 1);
+    return T.texels[__float_as_uint(T0.w) + (uint32_t)y * w + (uint32_t)x];
+}
+
+// get_tex_color (get_tex_color.c:27-58) with get_solid_texture
+// (get_solid_texture.c:30-43): colour in 0..1.
+__device__ __forceinline__ V3 tex_rgb(const TexDev &T, uint32_t tid, float u,
+                                      float v, V3 p)
+{
+    float4 T0 = T.tex[3 * tid], T1 = T.tex[3 * tid + 1], T2 = T.tex[3 * tid + 2];
+    uint32_t type = __float_as_uint(T0.x);
+    if (type == MRT_TEX_IMAGE) {
+        uint32_t c = tex_texel(T, T0, u, v);
+        return v3((float)(c & 255u) / 255.0f, (float)((c >> 8) & 255u) / 255.0f,
+                  (float)((c >> 16) & 255u) / 255.0f);
+    }
+    if (type == MRT_TEX_BLACK)
+        return v3(0.f, 0.f, 0.f);
+    int parity;
+    float sc = T1.x;
+    if (type == MRT_TEX_CHECKER_LOCAL) {
+        parity = (int)floorf(u * sc) + (int)floorf(v * sc);
+    } else {
+        V3 q = mul(p, sc);
+        parity = (int)floorf(q.x) + (int)floorf(q.y + 0.5f) + (int)floorf(q.z);
+    }
+    return (parity & 1) ? v3(T2.x / 255.0f, T2.y / 255.0f, T2.z / 255.0f)
+                        : v3(T1.y / 255.0f, T1.z / 255.0f, T1.w / 255.0f);
+}
+
+// get_tex_transparency (get_tex_color.c:60-70): 1 - alpha of an image.
+__device__ __forceinline__ float tex_transparency(const TexDev &T, uint32_t tid,
+                                                  float u, float v)
+{
+    float4 T0 = T.tex[3 * tid];
+    if (__float_as_uint(T0.x) != MRT_TEX_IMAGE)
+        return 0.0f;
+    return 1.0f - (float)(tex_texel(T, T0, u, v) >> 24) / 255.0f;
+}
+
+// (u, v) at p on a BVH primitive (record A, B, C, shading record aux),
+// scaled by the material scale sc of the part hit: get_sphere_uv
+// (hit_sphere.c:15-29), hit_triangle's interpolation (hit_triangle.c:
+// 91-92), get_cylinder_lateral_uv / get_cylinder_cap_uv (cylinder_uv.c),
+// get_cone_lateral_uv / get_cone_cap_uv (hit_cone.c:14-56, whose cap
+// centre is taken at height h / 2) and get_hyperboloid_uv
+// (hit_hyperboloid.c:15-38).
+__device__ __forceinline__ void prim_uv(const TexDev &T, float4 A, float4 B,
+                                        float4 C, V3 p, float bu, float bv,
+                                        float sc, float &u, float &v)
+{
+    const float PI = 3.14159265358979323846f;
+    uint32_t kind = __float_as_uint(B.w);
+    if (kind == MRT_PRIMKIND_TRIANGLE) {
+        u = v = 0.0f;
+        if (T.tri_uv) {
+            const float *q = T.tri_uv + (size_t)(__float_as_uint(A.w) & ~MRT_PRIM_KIND_MASK) * 6;
+            float w = 1.0f - bu - bv;
+            u = w * q[0] + bu * q[2] + bv * q[4];
+            v = w * q[1] + bu * q[3] + bv * q[5];
+        }
+        return;
+    }
+    if (kind == MRT_PRIMKIND_SPHERE) {
+        float r = B.x;
+        u = clamp01((atan2f((-p.z + A.z) / r, (p.x - A.x) / r) + PI) / (2.0f * PI));
+        v = clamp01(acosf((-p.y + A.y) / r) / PI);
+    } else {
+        uint32_t q = kind - MRT_PRIMKIND_QUADRIC, part = (uint32_t)bu;
+        V3 P = xyz(A), ax = xyz(B), right = axis_right(ax);
+        float h = C.w;
+        if (part == QPART_BODY) {
+            V3 rel = sub(p, P);
+            u = clamp01((atan2f(dot(rel, cross(ax, right)), dot(rel, right)) + PI)
+                        / (2.0f * PI));
+            float z = dot(sub(p, P), ax);
+            v = q == MRT_QUADRIC_CYLINDER ? clamp01(z / h + 0.5f)
+                : q == MRT_QUADRIC_CONE   ? clamp01(z / h)
+                                          : clamp01(z / (h * 0.5f) * 0.5f + 0.5f);
+        } else {
+            float off = q == MRT_QUADRIC_CONE ? h * 0.5f
+                        : part == QPART_TOP  ? h * 0.5f : -(h * 0.5f);
+            V3 rel = sub(p, add(P, mul(ax, off)));
+            float dia = 2.0f * C.x;
+            u = clamp01(dot(rel, right) / dia + 0.5f);
+            v = clamp01(dot(rel, cross(ax, right)) / dia + 0.5f);
+        }
+    }
+    if (sc != 1.0f) {
+        u = frac_part(u * sc);
+        v = frac_part(v * sc);
+    }
+}
+
+// get_uv_plane (hit_plane.c:18-41): coordinates along the plane's tangent
+// frame, times 0.1 * scale, wrapped by v - floor(v).
+__device__ __forceinline__ void plane_uv(float4 P, float4 N, V3 p, float sc,
+                                         float &u, float &v)
+{
+    V3 n = xyz(N);
+    V3 tg = unit(cross(fabsf(n.y) < 0.99f ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f), n));
+    V3 local = sub(p, xyz(P));
+    u = dot(local, tg) * (sc * 0.1f);
+    v = dot(local, cross(n, tg)) * (sc * 0.1f);
+    u = u - floorf(u);
+    v = v - floorf(v);
+}
+
+// Counter of the alpha test's draw: the primitive, its (u, v) on a 2^-12
+// grid and the ray direction on a 2^-7 grid, so a decision depends on where
+// and from where a primitive is hit, not on the traversal order, and a
+// rounding difference in the ray changes it only at a grid line.
+__device__ __forceinline__ U4 alpha_draw(float u, float v, V3 d, uint32_t src)
+{
+    uint32_t qd = (uint32_t)((d.x + 1.0f) * 127.5f)
+                  | ((uint32_t)((d.y + 1.0f) * 127.5f) << 8)
+                  | ((uint32_t)((d.z + 1.0f) * 127.5f) << 16);
+    return philox_u4((uint32_t)(int32_t)floorf(u * 4096.0f),
+                     (uint32_t)(int32_t)floorf(v * 4096.0f), qd, src,
+                     0xA17A7E57u, 0x5EEDA1FAu);
+}
+
+// The stochastic alpha test of hit_register_bvh (hit_register.c:57-69): a
+// hit on a texel of transparency tr is skipped when tr == 1, or with
+// probability tr (alpha_draw).
+__device__ __forceinline__ bool alpha_skips(const TexDev &T, const float *aux,
+                                            float4 A, float4 B, float4 C, V3 o,
+                                            V3 d, float t, float bu, float bv)
+{
+    uint32_t kind = __float_as_uint(B.w), mat;
+    if (kind >= MRT_PRIMKIND_QUADRIC) {
+        uint32_t part = (uint32_t)bu;
+        mat = __float_as_uint(aux[part == QPART_TOP ? 2 : part == QPART_BOTTOM ? 3 : 1]);
+    } else {
+        mat = __float_as_uint(C.w);
+    }
+    uint4 M0 = T.mtex[3 * mat];
+    if (M0.x == MRT_NO_TEX)
+        return false;
+    if (__float_as_uint(T.tex[3 * M0.x].x) != MRT_TEX_IMAGE)
+        return false;
+    float sc = __uint_as_float(T.mtex[3 * mat + 2].y), u, v;
+    prim_uv(T, A, B, C, add(o, mul(d, t)), bu, bv, sc, u, v);
+    float tr = tex_transparency(T, M0.x, u, v);
+    if (tr == 1.0f)
+        return true;
+    if (tr == 0.0f)
+        return false;
+    return alpha_draw(u, v, d, __float_as_uint(A.w)).x < tr;
+}
+
 __device__ __forceinline__ float inv_dir(float d)
 {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
@@ -425,6 +609,9 @@ struct SceneDev {
     uint32_t      one_sided;
     uint32_t      root;       // root node (scenes of a batch share one array)
     uint32_t      quad;       // nonzero: the primitives include quadrics
+    uint32_t      alpha;      // nonzero: some material has an image colour
+                              // texture with transparent texels
+    TexDev        tx;
 };
 
 constexpr int STACK4_MAX = 3 * (int)MRT_BVH4_MAX_DEPTH + 1;
@@ -448,9 +635,9 @@ __device__ __forceinline__ void cx(float &ka, int32_t &ca, float &kb, int32_t &c
 // farthest first.  The first SS stack entries live in LDS (lds_code,
 // stride `stride`), deeper ones in a private array.  ANY: stop at the first
 // hit below tlimit.
-// QUAD = false drops the quadric tests from the leaf loop (scenes without
-// quadrics keep the traversal's register count and code size).
-template <bool ANY, bool COUNT, int SS, bool QUAD = true>
+// EXT = false drops the quadric and alpha tests from the leaf loop (scenes
+// without them keep the traversal's register count and code size).
+template <bool ANY, bool COUNT, int SS, bool EXT = true>
 __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
                                          float *lds_t, int stride,
@@ -518,15 +705,18 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                 uint32_t kind = __float_as_uint(B.w);
                 if (kind == MRT_PRIMKIND_SPHERE) {
                     h = isect_sph(o, d, A, B, tt);
-                } else if (kind == MRT_PRIMKIND_TRIANGLE) {
+                } else if (!EXT || kind == MRT_PRIMKIND_TRIANGLE) {
+                    // (without EXT every other record is a triangle)
                     float4 C = p[2];
                     h = isect_tri(o, d, A, B, C, S.one_sided != 0, tt, uu, vv);
-                } else if (QUAD) {
+                } else {
                     h = isect_quadric(o, d, A, B, p[2],
                                       S.aux + (size_t)(first + i) * 9, tt, uu);
-                } else {
-                    h = false;
                 }
+                if (EXT && h && tt < tclose && S.alpha
+                    && alpha_skips(S.tx, S.aux + (size_t)(first + i) * 9, A, B,
+                                   p[2], o, d, tt, uu, vv))
+                    h = false;
                 if (h && tt < tclose) {
                     tclose = tt;
                     hit = true;
@@ -551,7 +741,7 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
 
 // Closest hit over BVH + planes (hit_register_all, hit_register.c:109-123).
 // code: leaf-order slot, HIT_PLANE | k, or HIT_NONE.
-template <bool COUNT, int SS, bool QUAD = true>
+template <bool COUNT, int SS, bool EXT = true>
 __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
                                             uint32_t *lds_code, float *lds_t,
                                             int stride, float &t, uint32_t &code,
@@ -561,7 +751,7 @@ __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
     float tb = IT_MAX_F;
     uint32_t slot = HIT_NONE;
     float u = 0.0f, v = 0.0f;
-    bool hit = traverse<false, COUNT, SS, QUAD>(S, o, d, IT_MAX_F, lds_code, lds_t,
+    bool hit = traverse<false, COUNT, SS, EXT>(S, o, d, IT_MAX_F, lds_code, lds_t,
                                           stride, tb, slot, u, v, nodes, prims);
     for (uint32_t k = 0; k < S.n_planes; k++) {
         float tp;
@@ -581,7 +771,7 @@ __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
 
 // Occlusion below tlimit: BVH primitives below min(tlimit, 1000), planes
 // below tlimit (check_plight_hit, src/ray_tracing/light/p_light.c:48-64).
-template <bool COUNT, int SS, bool QUAD = true>
+template <bool COUNT, int SS, bool EXT = true>
 __device__ __forceinline__ bool occluded(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
                                          float *lds_t, int stride,
@@ -591,7 +781,7 @@ __device__ __forceinline__ bool occluded(const SceneDev &S, V3 o, V3 d,
     float t;
     uint32_t slot;
     float u, v;
-    if (traverse<true, COUNT, SS, QUAD>(S, o, d, tl, lds_code, lds_t, stride, t, slot,
+    if (traverse<true, COUNT, SS, EXT>(S, o, d, tl, lds_code, lds_t, stride, t, slot,
                                   u, v, nodes, prims))
         return true;
     for (uint32_t k = 0; k < S.n_planes; k++) {

@@ -101,6 +101,7 @@ struct FrameParams {
     uint32_t has_photons;
     uint32_t light0, plane0;  // first light / plane of the scene in the
                               // shared arrays (0 for a single scene)
+    uint32_t textured;        // some material has a texture slot
 };
 
 // One scene of a batch: its frame, where its BVH and planes sit in the
@@ -143,6 +144,7 @@ struct ShadeBufs {
     // batches only: the scene table and (scene, local path index) per slot
     const BatchScene *bs;
     uint2            *slot_info;
+    TexDev            tx;      // textures (when FrameParams::textured)
 };
 
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,

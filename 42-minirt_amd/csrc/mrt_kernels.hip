@@ -214,7 +214,7 @@ __device__ __forceinline__ uint32_t tag_scene(uint32_t tag) { return (tag >> 18)
 __device__ __forceinline__ uint32_t tag_light(uint32_t tag) { return (tag >> 12) & 63u; }
 
 // ---- closest-hit trace (persistent, wave-granular work stealing) ----------
-template <bool COUNT, bool BATCH, bool QUAD>
+template <bool COUNT, bool BATCH, bool EXT>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_trace(SceneDev S, const float4 *__restrict__ q_o,
         const float4 *__restrict__ q_d, float4 *__restrict__ hit,
@@ -238,11 +238,11 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
                 if (BATCH) {
                     const SceneDev Si =
                         scene_of(S, bs, slot_info[__float_as_uint(d4.w)].x);
-                    closest_hit<COUNT, TRACE_SS, QUAD>(Si, xyz(o4), xyz(d4), lc, lt,
+                    closest_hit<COUNT, TRACE_SS, EXT>(Si, xyz(o4), xyz(d4), lc, lt,
                                                  TRACE_BLOCK, t, code, bu, bv,
                                                  nodes, prims);
                 } else {
-                    closest_hit<COUNT, TRACE_SS, QUAD>(S, xyz(o4), xyz(d4), lc, lt,
+                    closest_hit<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), lc, lt,
                                                  TRACE_BLOCK, t, code, bu, bv,
                                                  nodes, prims);
                 }
@@ -257,7 +257,7 @@ k_trace(SceneDev S, const float4 *__restrict__ q_o,
 }
 
 // ---- shadow / visibility trace --------------------------------------------
-template <bool COUNT, bool BATCH, bool QUAD>
+template <bool COUNT, bool BATCH, bool EXT>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 k_shadow(SceneDev S, const float4 *__restrict__ s_o,
          const float4 *__restrict__ s_d, uint8_t *__restrict__ vis,
@@ -280,11 +280,11 @@ k_shadow(SceneDev S, const float4 *__restrict__ s_o,
                 if (BATCH) {
                     const SceneDev Si =
                         scene_of(S, bs, tag_scene(__float_as_uint(d4.w)));
-                    occ = occluded<COUNT, TRACE_SS, QUAD>(Si, xyz(o4), xyz(d4), o4.w,
+                    occ = occluded<COUNT, TRACE_SS, EXT>(Si, xyz(o4), xyz(d4), o4.w,
                                                     lc, lt, TRACE_BLOCK, nodes,
                                                     prims);
                 } else {
-                    occ = occluded<COUNT, TRACE_SS, QUAD>(S, xyz(o4), xyz(d4), o4.w,
+                    occ = occluded<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), o4.w,
                                                     lc, lt, TRACE_BLOCK, nodes,
                                                     prims);
                 }
@@ -494,6 +494,71 @@ __device__ __forceinline__ void surface(const float4 *planes,
     }
 }
 
+// apply_all_map (apply_map.c:60-86) and get_hit_register_color
+// (get_hit_color.c:17-36) at a hit with record h4 and point p: the normal
+// map (apply_normal_map, :37-58; only image textures act as one), then the
+// red channel of the transmission, roughness, metallic, ao and emission
+// strength textures, the emission colour and the colour texture.
+__device__ __forceinline__ void apply_textures(const TexDev &T,
+                                               const float4 *planes,
+                                               const float4 *prims, float4 h4,
+                                               V3 p, uint32_t mat, V3 &nrm,
+                                               Mat &m)
+{
+    uint4 M0 = T.mtex[3 * mat], M1 = T.mtex[3 * mat + 1], M2 = T.mtex[3 * mat + 2];
+    if ((M0.x & M0.y & M0.z & M0.w & M1.x & M1.y & M1.z & M1.w) == MRT_NO_TEX)
+        return;
+    float sc = __uint_as_float(M2.y), u, v;
+    uint32_t code = __float_as_uint(h4.y);
+    if (code & HIT_PLANE) {
+        uint32_t k = code & ~HIT_PLANE;
+        plane_uv(planes[2 * k], planes[2 * k + 1], p, sc, u, v);
+    } else {
+        const float4 *rec = prims + (size_t)code * 3;
+        prim_uv(T, rec[0], rec[1], rec[2], p, h4.z, h4.w, sc, u, v);
+    }
+    if (M1.w != MRT_NO_TEX) {
+        float4 T0 = T.tex[3 * M1.w];
+        if (__float_as_uint(T0.x) == MRT_TEX_IMAGE) {
+            uint32_t c = tex_texel(T, T0, u, v);
+            V3 nm = v3((float)(c & 255u) / 127.5f - 1.0f,
+                       (float)((c >> 8) & 255u) / 127.5f - 1.0f,
+                       (float)((c >> 16) & 255u) / 127.5f - 1.0f);
+            V3 tg = cross(nrm, v3(0.f, -1.f, 0.f));
+            if (dot(tg, tg) == 0.0f)
+                tg = cross(nrm, v3(0.f, 0.f, 1.f));
+            tg = unit(tg);
+            V3 bt = unit(cross(nrm, tg));
+            V3 mapped = unit(add(add(mul(tg, nm.x), mul(bt, nm.y)), mul(nrm, nm.z)));
+            float k = __uint_as_float(M2.x);
+            nrm = k == 1.0f ? mapped
+                            : unit(add(mul(nrm, 1.0f - k), mul(mapped, k)));
+        }
+    }
+    if (M0.w != MRT_NO_TEX)
+        m.b.z = tex_rgb(T, M0.w, u, v, p).x;
+    if (M0.z != MRT_NO_TEX)
+        m.b.x = tex_rgb(T, M0.z, u, v, p).x;
+    if (M0.y != MRT_NO_TEX)
+        m.a.w = tex_rgb(T, M0.y, u, v, p).x;
+    if (M1.x != MRT_NO_TEX)
+        m.b.w = tex_rgb(T, M1.x, u, v, p).x;
+    if (M1.y != MRT_NO_TEX)
+        m.c.x = tex_rgb(T, M1.y, u, v, p).x;
+    if (M1.z != MRT_NO_TEX) {
+        V3 e = tex_rgb(T, M1.z, u, v, p);
+        m.c.y = e.x;
+        m.c.z = e.y;
+        m.c.w = e.z;
+    }
+    if (M0.x != MRT_NO_TEX) {
+        V3 c = tex_rgb(T, M0.x, u, v, p);
+        m.a.x = c.x;
+        m.a.y = c.y;
+        m.a.z = c.z;
+    }
+}
+
 enum : uint32_t { SC_NONE = 0, SC_DIFFUSE = 1 };
 
 // ---- environment -------------------------------------------------------------
@@ -611,6 +676,9 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 m.a = B.mats[3 * mat];
                 m.b = B.mats[3 * mat + 1];
                 m.c = B.mats[3 * mat + 2];
+                if (F.textured)
+                    apply_textures(B.tx, B.planes + 2 * (size_t)F.plane0, B.prims,
+                                   h4, p, mat, nrm, m);
                 ao = m.b.w;
                 // material_manager_v3 (src/ray_tracing/path_trace.c:15-35)
                 if (m.c.x > 0.0f) {
@@ -1426,9 +1494,16 @@ k_aov(FrameParams fp, SceneDev S, const float4 *__restrict__ prims,
             V3 p;
             bool front;
             uint32_t mat;
-            surface(S.planes, prims, nrm, fp.one_sided != 0, o, d,
-                    make_float4(t, __uint_as_float(code), bu, bv), p, n, front,
-                    mat);
+            float4 h4 = make_float4(t, __uint_as_float(code), bu, bv);
+            surface(S.planes, prims, nrm, fp.one_sided != 0, o, d, h4, p, n,
+                    front, mat);
+            if (fp.textured) {
+                // the normal render mode shows the normal-mapped normal
+                // (get_hit_register_color after apply_all_map)
+                Mat m;
+                m.a = m.b = m.c = make_float4(0.f, 0.f, 0.f, 0.f);
+                apply_textures(S.tx, S.planes, prims, h4, p, mat, n, m);
+            }
         }
         float *o3 = (float *)out + (size_t)px * 3;
         o3[0] = n.x;
@@ -1579,6 +1654,10 @@ k_photon_emit(FrameParams fp, SceneDev S, PhotonEmit pe, ShadeBufs B,
                 m.a = B.mats[3 * mat];
                 m.b = B.mats[3 * mat + 1];
                 m.c = B.mats[3 * mat + 2];
+                if (fp.textured)
+                    apply_textures(B.tx, B.planes, B.prims,
+                                   make_float4(t, __uint_as_float(code), bu, bv),
+                                   p, mat, nrm, m);
                 if (scatter(rk, fp.ior_all, nrm, front, m, d, pw)) {
                     pw = had(pw, v3(m.a.x, m.a.y, m.a.z));
                     if (depth != 0) {
@@ -1854,7 +1933,7 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
 #define MRT_GO(C, Q)                                                          \
     hipLaunchKernelGGL((k_trace<C, false, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
                        0, st, S, q_o, q_d, hit, ctr, stats, nb, ni)
-    MRT_DISPATCH2(count, S.quad != 0);
+    MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
 #undef MRT_GO
     return hipGetLastError();
 }
@@ -1868,7 +1947,7 @@ hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
 #define MRT_GO(C, Q)                                                          \
     hipLaunchKernelGGL((k_shadow<C, false, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
                        0, st, S, s_o, s_d, vis, ctr, stats, cap, nb)
-    MRT_DISPATCH2(count, S.quad != 0);
+    MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
 #undef MRT_GO
     return hipGetLastError();
 }
@@ -1908,7 +1987,7 @@ hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
     hipLaunchKernelGGL((k_trace<C, true, Q>), dim3(blocks), dim3(TRACE_BLOCK),  \
                        0, st, S, q_o, q_d, hit, ctr, stats, B.bs,              \
                        (const uint2 *)B.slot_info)
-    MRT_DISPATCH2(count, S.quad != 0);
+    MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
 #undef MRT_GO
     return hipGetLastError();
 }
@@ -1922,7 +2001,7 @@ hipError_t launch_bshadow(hipStream_t st, uint32_t blocks, bool count,
 #define MRT_GO(C, Q)                                                          \
     hipLaunchKernelGGL((k_shadow<C, true, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
                        0, st, S, s_o, s_d, vis, ctr, stats, cap, B.bs)
-    MRT_DISPATCH2(count, S.quad != 0);
+    MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
 #undef MRT_GO
     return hipGetLastError();
 }

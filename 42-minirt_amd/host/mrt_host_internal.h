@@ -19,6 +19,17 @@ struct mrt_host_scene {
     mrt_quadric  *quad;
     size_t        cap_quad;
     uint32_t      n_quad;
+    mrt_texture  *tex;
+    char        (*tex_names)[32];
+    size_t        cap_tex, cap_tex_names;
+    uint32_t      n_tex;
+    uint8_t      *texels;
+    size_t        cap_texels;
+    uint32_t      n_texels;
+    mrt_material_tex *mtex;           /* NULL until a material has a slot */
+    size_t        cap_mtex;
+    float        *tri_uv;             /* NULL until a triangle has (u, v) */
+    size_t        cap_tri_uv;
     float        *pln;
     uint32_t     *pln_mat;
     size_t        cap_pln, cap_pln_mat;

@@ -1,8 +1,10 @@
 /*
  * Wavefront OBJ meshes for the .rt "cu pos dir scale file.obj colour|mat"
  * element (src/parsing/parse_elements4.c:101-128, the reference's OBJ reader in src/utils/obj):
- * "v x y z" positions, "vn x y z" normals, "f a[/t][/n] b c ..." faces with
- * 1-based or negative indices; other lines are ignored.  A face of n > 3
+ * "v x y z" positions, "vn x y z" normals, "vt u v" texture coordinates,
+ * "f a[/t][/n] b c ..." faces with 1-based or negative indices; other lines
+ * are ignored.  A triangle whose three corners name texture coordinates
+ * carries them (the (u, v) hit_triangle interpolates, hit_triangle.c:91-92).  A face of n > 3
  * vertices becomes the triangles the reference's ear clipping emits
  * (obj_ear_clipping.c:57-95): (v[n-1], v0, v1), then v0 is dropped and the
  * same is repeated on the remaining polygon.
@@ -105,11 +107,11 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
         return MRT_E_INVALID;
     rot_t R;
     rot_init(&R, dir);
-    fvec P = {0}, N = {0};
+    fvec P = {0}, N = {0}, TC = {0};
     int rc = MRT_OK, added = 0;
     const char *line = text;
     enum { MAXV = 256 };
-    long fv[MAXV], fn[MAXV];
+    long fv[MAXV], fn[MAXV], ft[MAXV];
     while (*line && rc == MRT_OK) {
         const char *end = strchr(line, '\n');
         size_t len = end ? (size_t)(end - line) : strlen(line);
@@ -142,6 +144,14 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
             double v[3] = {a, b, c}, o[3];
             rot_apply(&R, v, o);
             rc = fpush3(&N, o[0], o[1], o[2]);
+        } else if (s[0] == 'v' && s[1] == 't' && (s[2] == ' ' || s[2] == '\t')) {
+            double a, b = 0.0;
+            int got = sscanf(s + 3, "%lf %lf", &a, &b);
+            if (got < 1) {
+                rc = MRT_E_INVALID;
+                break;
+            }
+            rc = fpush3(&TC, a, b, 0.0);
         } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
             int nv = 0;
             char *p = s + 2;
@@ -155,7 +165,7 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
                     break;
                 }
                 char *q;
-                long vi = strtol(p, &q, 10), ni = 0;
+                long vi = strtol(p, &q, 10), ni = 0, ti = 0;
                 if (q == p) {
                     rc = MRT_E_INVALID;
                     break;
@@ -163,7 +173,7 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
                 p = q;
                 if (*p == '/') {
                     p++;
-                    (void)strtol(p, &q, 10);          /* texture index */
+                    ti = strtol(p, &q, 10);           /* texture index */
                     p = q;
                     if (*p == '/') {
                         p++;
@@ -175,6 +185,7 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
                     p++;
                 fv[nv] = obj_index(vi, P.n / 3);
                 fn[nv] = ni ? obj_index(ni, N.n / 3) : -1;
+                ft[nv] = ti ? obj_index(ti, TC.n / 3) : -1;
                 if (fv[nv] < 0) {
                     rc = MRT_E_INVALID;
                     break;
@@ -186,15 +197,18 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
             if (nv < 3)
                 continue;
             /* ear clipping at vertex 0 of the shrinking polygon */
-            long pv[MAXV], pn[MAXV];
+            long pv[MAXV], pn[MAXV], pt[MAXV];
             memcpy(pv, fv, nv * sizeof(long));
             memcpy(pn, fn, nv * sizeof(long));
+            memcpy(pt, ft, nv * sizeof(long));
             int m = nv;
             while (m >= 3 && rc == MRT_OK) {
                 long tri[3] = {m > 3 ? pv[m - 1] : pv[0], m > 3 ? pv[0] : pv[1],
                                m > 3 ? pv[1] : pv[2]};
                 long trn[3] = {m > 3 ? pn[m - 1] : pn[0], m > 3 ? pn[0] : pn[1],
                                m > 3 ? pn[1] : pn[2]};
+                long trt[3] = {m > 3 ? pt[m - 1] : pt[0], m > 3 ? pt[0] : pt[1],
+                               m > 3 ? pt[1] : pt[2]};
                 float v9[9], n9[9];
                 int have_n = 1;
                 for (int k = 0; k < 3; k++) {
@@ -205,6 +219,15 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
                         memcpy(n9 + 3 * k, N.v + 3 * trn[k], 3 * sizeof(float));
                 }
                 int r = mrt_hs_add_triangle(hs, v9, have_n ? n9 : NULL, mat);
+                if (r >= 0 && trt[0] >= 0 && trt[1] >= 0 && trt[2] >= 0) {
+                    float uv[6];
+                    for (int k = 0; k < 3; k++) {
+                        uv[2 * k] = TC.v[3 * trt[k]];
+                        uv[2 * k + 1] = TC.v[3 * trt[k] + 1];
+                    }
+                    int e = mrt_hs_set_triangle_uv(hs, (uint32_t)r, uv);
+                    r = e < 0 ? e : r;
+                }
                 if (r < 0)
                     rc = r;
                 else
@@ -213,12 +236,14 @@ int mrt_hs_add_obj_text(mrt_host_scene *hs, const char *text, const float pos[3]
                     break;
                 memmove(pv, pv + 1, (m - 1) * sizeof(long));
                 memmove(pn, pn + 1, (m - 1) * sizeof(long));
+                memmove(pt, pt + 1, (m - 1) * sizeof(long));
                 m--;
             }
         }
     }
     free(P.v);
     free(N.v);
+    free(TC.v);
     return rc < 0 ? rc : added;
 }
 

@@ -17,8 +17,9 @@
  *   co apex dir diameter height r,g,b|mat [mat_top]
  *   hb pos dir height a b c shape r,g,b|mat   (parse_elements3.c:15-57,
  *                                              parse_elements4.c:15-99)
- * Image textures other than .hdr skyboxes, checker textures and textured
- * materials are reported as unsupported.  Relative
+ *   tex name image file.png|.bmp        image textures of materials
+ *   tex name checker_local|checker_global scale c1 c2
+ * Every material slot but ior and scale may name a texture.  Relative
  * file paths are tried from the working directory, then from the .rt file's
  * directory.  Numbers go through strtod; the reference's ft_atod is not
  * available, so the rounding of parsed decimals is not pinned to it.
@@ -37,9 +38,12 @@
 #define MAXTEX 64
 
 typedef struct {
-    char name[64];
-    char path[1024];
-    int  is_hdr;              /* an image texture in Radiance format */
+    char  name[64];
+    char  path[1024];
+    int   is_hdr;             /* an image texture in Radiance format */
+    int   kind;               /* MRT_TEX_IMAGE or a checker            */
+    float scale, c1[3], c2[3];
+    int   scene_index;        /* texture in the scene once a material uses it */
 } texdef;
 
 typedef struct {
@@ -196,6 +200,86 @@ static int color_or_mat(parser *P, const char *s, uint32_t *mat)
     return MRT_OK;
 }
 
+static int is_name(const char *s)
+{
+    for (; *s; s++)
+        if (isalpha((unsigned char)*s) || *s == '_')
+            return 1;
+    return 0;
+}
+
+/* The scene texture of a "tex" definition, added on first use.  An image
+ * that does not decode becomes the reference's fallback pattern,
+ * get_solid_texture_default (get_solid_texture.c:15-28): a global checker
+ * of scale 2 between black and (243, 0, 243); a material's .hdr texture
+ * reads as black (get_tex_color drops get_hdr_value's result,
+ * get_tex_color.c:56-57). */
+static int scene_tex(parser *P, texdef *d, uint32_t *out)
+{
+    if (d->scene_index >= 0) {
+        *out = (uint32_t)d->scene_index;
+        return MRT_OK;
+    }
+    mrt_texture t;
+    memset(&t, 0, sizeof(t));
+    uint8_t *rgba = NULL;
+    if (d->kind != MRT_TEX_IMAGE) {
+        t.type = (uint32_t)d->kind;
+        t.scale = d->scale;
+        memcpy(t.c1, d->c1, sizeof(t.c1));
+        memcpy(t.c2, d->c2, sizeof(t.c2));
+    } else if (d->is_hdr) {
+        t.type = MRT_TEX_BLACK;
+    } else if (mrt_load_image(d->path, &rgba, &t.width, &t.height) == MRT_OK) {
+        t.type = MRT_TEX_IMAGE;
+    } else {
+        t.type = MRT_TEX_CHECKER_GLOBAL;
+        t.scale = 2.0f;
+        t.c2[0] = 243.0f;
+        t.c2[2] = 243.0f;
+    }
+    int r = mrt_hs_add_texture(P->hs, &t, rgba, d->name);
+    mrt_free(rgba);
+    if (r < 0)
+        return perr(P, r, "texture rejected", d->name);
+    d->scene_index = r;
+    *out = (uint32_t)r;
+    return MRT_OK;
+}
+
+/* A material slot: a number in [lo, hi] or a texture name
+ * (parse_double_b_or_tex / parse_double_or_tex / parse_color_or_tex,
+ * parse_elements_utils*.c).  is_color: an r,g,b colour instead of a number. */
+static int slot(parser *P, const char *s, int is_color, double lo, double hi,
+                float *val, uint32_t *tex)
+{
+    *tex = MRT_NO_TEX;
+    double v;
+    int named = is_color ? !strchr(s, ',') : (!num(s, &v) && is_name(s));
+    if (named) {
+        texdef *d = (texdef *)find_tex(P, s);
+        if (!d)
+            return perr(P, MRT_E_INVALID, "unknown texture", s);
+        return scene_tex(P, d, tex);
+    }
+    if (is_color) {
+        float c[3];
+        if (!color(s, c))
+            return perr(P, MRT_E_INVALID, "bad colour", s);
+        for (int k = 0; k < 3; k++)
+            val[k] = (float)(c[k] / 255.0);
+        return MRT_OK;
+    }
+    if (!num(s, &v) || v < lo || v > hi)
+        return perr(P, MRT_E_INVALID, "material value out of range", s);
+    *val = (float)v;
+    return MRT_OK;
+}
+
+/* mat name colour metallic roughness ior transmission ao emission_strength
+ * emission_colour [scale [normal_map [normal_intensity]]] (parse_material,
+ * parse_elements3.c:59-136); every slot but ior and scale may name a
+ * texture. */
 static int p_mat(parser *P, char **t, int n)
 {
     if (n < 10 || n > 13)
@@ -204,33 +288,55 @@ static int p_mat(parser *P, char **t, int n)
         return perr(P, MRT_E_INVALID, "material defined twice", t[1]);
     mrt_material m;
     memset(&m, 0, sizeof(m));
-    float c[3], e[3];
-    double v[6];
-    if (!color(t[2], c))
-        return perr(P, MRT_E_UNSUPPORTED, "material colour (textures unsupported)", t[2]);
-    for (int k = 0; k < 6; k++)
-        if (!num(t[3 + k], &v[k]))
-            return perr(P, MRT_E_UNSUPPORTED, "material value (textures unsupported)",
-                        t[3 + k]);
-    if (!color(t[9], e))
-        return perr(P, MRT_E_INVALID, "emission colour", t[9]);
-    if (v[0] < 0 || v[0] > 1 || v[1] < 0 || v[1] > 1 || v[2] < 0 || v[3] < 0
-        || v[3] > 1 || v[4] < 0 || v[4] > 1 || v[5] < 0)
-        return perr(P, MRT_E_INVALID, "material value out of range", t[1]);
-    if (n >= 12)
-        return perr(P, MRT_E_UNSUPPORTED, "normal maps", t[11]);
-    for (int k = 0; k < 3; k++) {
-        m.color[k] = (float)(c[k] / 255.0);
-        m.emission_color[k] = (float)(e[k] / 255.0);
+    m.ao = 1.0f;
+    mrt_material_tex mt;
+    mt.normal = MRT_NO_TEX;
+    mt.normal_intensity = 1.0f;
+    mt.scale = 1.0f;
+    double ior;
+    int r;
+    if ((r = slot(P, t[2], 1, 0, 0, m.color, &mt.color))
+        || (r = slot(P, t[3], 0, 0.0, 1.0, &m.metallic, &mt.metallic))
+        || (r = slot(P, t[4], 0, 0.0, 1.0, &m.roughness, &mt.roughness)))
+        return r;
+    if (!num(t[5], &ior) || ior < 0)
+        return perr(P, MRT_E_INVALID, "bad ior", t[5]);
+    m.ior = (float)ior;
+    if ((r = slot(P, t[6], 0, 0.0, 1.0, &m.transmission, &mt.transmission))
+        || (r = slot(P, t[7], 0, 0.0, 1.0, &m.ao, &mt.ao))
+        || (r = slot(P, t[8], 0, 0.0, INFINITY, &m.emission_strength,
+                     &mt.emission_strength))
+        || (r = slot(P, t[9], 1, 0, 0, m.emission_color, &mt.emission_color)))
+        return r;
+    if (n >= 11) {
+        double sc;
+        if (!num(t[10], &sc))
+            return perr(P, MRT_E_INVALID, "bad texture scale", t[10]);
+        mt.scale = (float)sc;
     }
-    m.metallic = (float)v[0];
-    m.roughness = (float)v[1];
-    m.ior = (float)v[2];
-    m.transmission = (float)v[3];
-    m.ao = (float)v[4];
-    m.emission_strength = (float)v[5];
-    int r = mrt_hs_add_material(P->hs, &m, t[1]);
-    return r < 0 ? r : MRT_OK;
+    if (n >= 12) {
+        texdef *d = (texdef *)find_tex(P, t[11]);
+        if (!d)
+            return perr(P, MRT_E_INVALID, "unknown normal map", t[11]);
+        if ((r = scene_tex(P, d, &mt.normal)))
+            return r;
+    }
+    if (n >= 13) {
+        double ni;
+        if (!num(t[12], &ni))
+            return perr(P, MRT_E_INVALID, "bad normal intensity", t[12]);
+        mt.normal_intensity = (float)ni;
+    }
+    r = mrt_hs_add_material(P->hs, &m, t[1]);
+    if (r < 0)
+        return r;
+    const uint32_t *sl = &mt.color;
+    int textured = 0;
+    for (int k = 0; k < 8; k++)
+        textured |= sl[k] != MRT_NO_TEX;
+    if (textured && (r = mrt_hs_set_material_tex(P->hs, (uint32_t)r, &mt)) < 0)
+        return perr(P, r, "material textures", t[1]);
+    return MRT_OK;
 }
 
 /* tex name image path | tex name checker_local|checker_global ... */
@@ -244,14 +350,26 @@ static int p_tex(parser *P, char **t, int n)
         return perr(P, MRT_E_INVALID, "texture defined twice", t[1]);
     texdef *d = &P->tex[P->n_tex++];
     memset(d, 0, sizeof(*d));
+    d->scene_index = -1;
     snprintf(d->name, sizeof(d->name), "%s", t[1]);
     if (strcmp(t[2], "image") == 0) {
         if (n != 4)
             return perr(P, MRT_E_INVALID, "bad image texture", t[1]);
         resolve(P, t[3], d->path, sizeof(d->path));
         d->is_hdr = ends_with(t[3], ".hdr");
-    } else if (strcmp(t[2], "checker_local") != 0
-               && strcmp(t[2], "checker_global") != 0) {
+        d->kind = MRT_TEX_IMAGE;
+    } else if (strcmp(t[2], "checker_local") == 0
+               || strcmp(t[2], "checker_global") == 0) {
+        double sc;
+        /* tex name checker_* scale c1 c2 (parse_texture_type,
+         * parse_elements5.c:85-112) */
+        if (n != 6 || !num(t[3], &sc) || sc < 0 || !color(t[4], d->c1)
+            || !color(t[5], d->c2))
+            return perr(P, MRT_E_INVALID, "bad checker texture", t[1]);
+        d->scale = (float)sc;
+        d->kind = strcmp(t[2], "checker_local") == 0 ? MRT_TEX_CHECKER_LOCAL
+                                                      : MRT_TEX_CHECKER_GLOBAL;
+    } else {
         return perr(P, MRT_E_INVALID, "unknown texture kind", t[2]);
     }
     return MRT_OK;
@@ -361,13 +479,10 @@ static int p_quadric(parser *P, char **t, int n)
 static int p_line(parser *P, char **t, int n, int pass)
 {
     const char *id = t[0];
-    if (pass == 0) {
-        if (strcmp(id, "mat") == 0)
-            return p_mat(P, t, n);
-        if (strcmp(id, "tex") == 0)
-            return p_tex(P, t, n);
-        return MRT_OK;
-    }
+    if (pass == 0)
+        return strcmp(id, "tex") == 0 ? p_tex(P, t, n) : MRT_OK;
+    if (pass == 1)
+        return strcmp(id, "mat") == 0 ? p_mat(P, t, n) : MRT_OK;
     if (strcmp(id, "mat") == 0 || strcmp(id, "tex") == 0)
         return MRT_OK;
     if (strcmp(id, "A") == 0) {
@@ -527,7 +642,7 @@ static int parse_text(const char *text, const char *dir, mrt_host_scene **out,
         return MRT_E_NOMEM;
     }
     int rc = MRT_OK;
-    for (int pass = 0; pass < 2 && rc == MRT_OK; pass++) {
+    for (int pass = 0; pass < 3 && rc == MRT_OK; pass++) {
         memcpy(buf, text, len + 1);
         P.line = 0;
         char *save = NULL;

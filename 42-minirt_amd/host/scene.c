@@ -48,6 +48,11 @@ void mrt_hs_free(mrt_host_scene *hs)
     free(hs->sph);
     free(hs->sph_mat);
     free(hs->quad);
+    free(hs->tex);
+    free(hs->tex_names);
+    free(hs->texels);
+    free(hs->mtex);
+    free(hs->tri_uv);
     free(hs->pln);
     free(hs->pln_mat);
     free(hs->materials);
@@ -55,6 +60,92 @@ void mrt_hs_free(mrt_host_scene *hs)
     free(hs->lights);
     free(hs->hdri_rgb);
     free(hs);
+}
+
+static void mtex_default(mrt_material_tex *t)
+{
+    t->color = t->metallic = t->roughness = t->transmission = t->ao = MRT_NO_TEX;
+    t->emission_strength = t->emission_color = t->normal = MRT_NO_TEX;
+    t->normal_intensity = 1.0f;
+    t->scale = 1.0f;
+}
+
+int mrt_hs_add_texture(mrt_host_scene *hs, const mrt_texture *t,
+                       const uint8_t *rgba, const char *name)
+{
+    if (!hs || !t || t->type > MRT_TEX_BLACK)
+        return MRT_E_INVALID;
+    mrt_texture c = *t;
+    c.texel0 = 0;
+    c.pad = 0;
+    if (c.type == MRT_TEX_IMAGE) {
+        uint64_t n = (uint64_t)c.width * c.height;
+        if (!rgba || c.width == 0 || c.height == 0)
+            return MRT_E_INVALID;
+        if ((uint64_t)hs->n_texels + n >= (1ull << 31))
+            return MRT_E_LIMIT;
+        if (grow((void **)&hs->texels, &hs->cap_texels,
+                 (size_t)(hs->n_texels + n) * 4, 1))
+            return MRT_E_NOMEM;
+        memcpy(hs->texels + (size_t)hs->n_texels * 4, rgba, (size_t)n * 4);
+        c.texel0 = hs->n_texels;
+        hs->n_texels += (uint32_t)n;
+    } else {
+        c.width = c.height = 0;
+    }
+    if (grow((void **)&hs->tex, &hs->cap_tex, hs->n_tex + 1, sizeof(mrt_texture))
+        || grow((void **)&hs->tex_names, &hs->cap_tex_names, hs->n_tex + 1,
+                sizeof(*hs->tex_names)))
+        return MRT_E_NOMEM;
+    hs->tex[hs->n_tex] = c;
+    memset(hs->tex_names[hs->n_tex], 0, sizeof(*hs->tex_names));
+    if (name)
+        strncpy(hs->tex_names[hs->n_tex], name, sizeof(*hs->tex_names) - 1);
+    return (int)hs->n_tex++;
+}
+
+int mrt_hs_find_texture(const mrt_host_scene *hs, const char *name)
+{
+    if (!hs || !name)
+        return MRT_E_INVALID;
+    for (uint32_t i = 0; i < hs->n_tex; i++)
+        if (strcmp(hs->tex_names[i], name) == 0)
+            return (int)i;
+    return MRT_E_INVALID;
+}
+
+int mrt_hs_set_material_tex(mrt_host_scene *hs, uint32_t mat,
+                            const mrt_material_tex *mt)
+{
+    if (!hs || !mt || mat >= hs->n_mat)
+        return MRT_E_INVALID;
+    const uint32_t *slot = &mt->color;
+    for (int k = 0; k < 8; k++)
+        if (slot[k] != MRT_NO_TEX && slot[k] >= hs->n_tex)
+            return MRT_E_INVALID;
+    if (!hs->mtex) {
+        if (grow((void **)&hs->mtex, &hs->cap_mtex, hs->n_mat,
+                 sizeof(mrt_material_tex)))
+            return MRT_E_NOMEM;
+        for (uint32_t i = 0; i < hs->n_mat; i++)
+            mtex_default(&hs->mtex[i]);
+    }
+    hs->mtex[mat] = *mt;
+    return MRT_OK;
+}
+
+int mrt_hs_set_triangle_uv(mrt_host_scene *hs, uint32_t tri, const float uv[6])
+{
+    if (!hs || !uv || tri >= hs->n_tri)
+        return MRT_E_INVALID;
+    if (!hs->tri_uv) {
+        if (grow((void **)&hs->tri_uv, &hs->cap_tri_uv, (size_t)hs->n_tri * 6,
+                 sizeof(float)))
+            return MRT_E_NOMEM;
+        memset(hs->tri_uv, 0, (size_t)hs->n_tri * 6 * sizeof(float));
+    }
+    memcpy(hs->tri_uv + (size_t)tri * 6, uv, 6 * sizeof(float));
+    return MRT_OK;
 }
 
 int mrt_hs_add_material(mrt_host_scene *hs, const mrt_material *m,
@@ -67,6 +158,12 @@ int mrt_hs_add_material(mrt_host_scene *hs, const mrt_material *m,
         || grow((void **)&hs->mat_names, &hs->cap_mat_names, hs->n_mat + 1,
                 sizeof(*hs->mat_names)))
         return MRT_E_NOMEM;
+    if (hs->mtex) {
+        if (grow((void **)&hs->mtex, &hs->cap_mtex, hs->n_mat + 1,
+                 sizeof(mrt_material_tex)))
+            return MRT_E_NOMEM;
+        mtex_default(&hs->mtex[hs->n_mat]);
+    }
     hs->materials[hs->n_mat] = *m;
     memset(hs->mat_names[hs->n_mat], 0, sizeof(*hs->mat_names));
     if (name)
@@ -183,6 +280,12 @@ int mrt_hs_add_triangle(mrt_host_scene *hs, const float v[9], const float n[9],
             for (int j = 0; j < 3; j++)
                 q[k * 3 + j] = (float)(c[j] * l);
     }
+    if (hs->tri_uv) {
+        if (grow((void **)&hs->tri_uv, &hs->cap_tri_uv, need / 9 * 6,
+                 sizeof(float)))
+            return MRT_E_NOMEM;
+        memset(hs->tri_uv + (size_t)hs->n_tri * 6, 0, 6 * sizeof(float));
+    }
     hs->tri_mat[hs->n_tri] = mat;
     return (int)hs->n_tri++;
 }
@@ -287,6 +390,12 @@ const mrt_scene *mrt_hs_view(mrt_host_scene *hs)
     v->sph_mat = hs->sph_mat;
     v->n_quadrics = hs->n_quad;
     v->quadrics = hs->quad;
+    v->n_textures = hs->n_tex;
+    v->textures = hs->tex;
+    v->n_texels = hs->n_texels;
+    v->texels = hs->texels;
+    v->material_tex = hs->mtex;
+    v->tri_uv = hs->tri_uv;
     v->n_planes = hs->n_pln;
     v->pln = hs->pln;
     v->pln_mat = hs->pln_mat;

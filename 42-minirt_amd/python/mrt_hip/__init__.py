@@ -115,6 +115,30 @@ class Quadric(C.Structure):
 assert C.sizeof(Quadric) == 64
 
 
+TEX_IMAGE = 0
+TEX_CHECKER_LOCAL = 1
+TEX_CHECKER_GLOBAL = 2
+TEX_BLACK = 3
+NO_TEX = 0xFFFFFFFF
+
+
+class Texture(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("texel0", C.c_uint32),
+                ("scale", C.c_float), ("c1", C.c_float * 3),
+                ("c2", C.c_float * 3), ("pad", C.c_uint32)]
+
+
+class MaterialTex(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "color", "metallic", "roughness", "transmission", "ao",
+        "emission_strength", "emission_color", "normal")] + \
+        [("normal_intensity", C.c_float), ("scale", C.c_float)]
+
+
+assert C.sizeof(Texture) == 48 and C.sizeof(MaterialTex) == 40
+
+
 class Scene(C.Structure):
     _fields_ = [("n_triangles", C.c_uint32), ("tri_pos", _fp),
                 ("tri_nrm", _fp), ("tri_mat", _up),
@@ -126,7 +150,11 @@ class Scene(C.Structure):
                 ("camera", Camera), ("ambient_ratio", C.c_float),
                 ("ambient_color", C.c_float * 3), ("ior_all", C.c_float),
                 ("flags", C.c_uint32), ("n_quadrics", C.c_uint32),
-                ("quadrics", C.POINTER(Quadric))]
+                ("quadrics", C.POINTER(Quadric)),
+                ("n_textures", C.c_uint32), ("textures", C.POINTER(Texture)),
+                ("n_texels", C.c_uint32), ("texels", C.POINTER(C.c_uint8)),
+                ("material_tex", C.POINTER(MaterialTex)),
+                ("tri_uv", _fp)]
 
 
 class Stats(C.Structure):
@@ -268,6 +296,15 @@ mrt_hs_add_sphere = _sig("mrt_hs_add_sphere", C.c_int, _hs_p, _fp, C.c_float,
                          C.c_uint32)
 mrt_hs_add_quadric = _sig("mrt_hs_add_quadric", C.c_int, _hs_p,
                           C.POINTER(Quadric))
+mrt_hs_add_texture = _sig("mrt_hs_add_texture", C.c_int, _hs_p,
+                          C.POINTER(Texture), C.POINTER(C.c_uint8), C.c_char_p)
+mrt_hs_set_material_tex = _sig("mrt_hs_set_material_tex", C.c_int, _hs_p,
+                               C.c_uint32, C.POINTER(MaterialTex))
+mrt_hs_set_triangle_uv = _sig("mrt_hs_set_triangle_uv", C.c_int, _hs_p,
+                              C.c_uint32, _fp)
+mrt_decode_image = _sig("mrt_decode_image", C.c_int, C.c_char_p, C.c_size_t,
+                        C.POINTER(C.POINTER(C.c_uint8)),
+                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32))
 mrt_hs_add_plane = _sig("mrt_hs_add_plane", C.c_int, _hs_p, _fp, _fp,
                         C.c_uint32)
 mrt_hs_add_triangle = _sig("mrt_hs_add_triangle", C.c_int, _hs_p, _fp, _fp,
@@ -482,6 +519,43 @@ class HostScene:
         """The .rt "hb" element: centre, axis, height, a, b, c, shape."""
         return self.add_quadric(QUADRIC_HYPERBOLOID, centre, axis, height, mat,
                                 a=a, b=b, c=c, shape=shape)
+
+    def add_image_texture(self, rgba, name=None):
+        """An IMAGE texture from an (h, w, 4) uint8 array (row 0 first)."""
+        a = np.ascontiguousarray(rgba, dtype=np.uint8)
+        t = Texture()
+        t.type = TEX_IMAGE
+        t.height, t.width = a.shape[0], a.shape[1]
+        return _check(mrt_hs_add_texture(
+            self._h, C.byref(t), a.ctypes.data_as(C.POINTER(C.c_uint8)),
+            name.encode() if name else None))
+
+    def add_checker_texture(self, kind, scale, c1, c2, name=None):
+        """A CHECKER_LOCAL / CHECKER_GLOBAL texture (colours 0..255)."""
+        t = Texture()
+        t.type = kind
+        t.scale = scale
+        t.c1[:] = [float(x) for x in c1]
+        t.c2[:] = [float(x) for x in c2]
+        return _check(mrt_hs_add_texture(self._h, C.byref(t), None,
+                                         name.encode() if name else None))
+
+    def set_material_tex(self, mat, normal_intensity=1.0, scale=1.0,
+                         **slots):
+        """Texture slots of a material: color, metallic, roughness,
+        transmission, ao, emission_strength, emission_color, normal."""
+        mt = MaterialTex()
+        for f, _ in MaterialTex._fields_[:8]:
+            setattr(mt, f, slots.pop(f, NO_TEX))
+        if slots:
+            raise TypeError(f"unknown texture slots {sorted(slots)}")
+        mt.normal_intensity = normal_intensity
+        mt.scale = scale
+        _check(mrt_hs_set_material_tex(self._h, mat, C.byref(mt)))
+
+    def set_triangle_uv(self, tri, uv):
+        a = np.ascontiguousarray(uv, dtype=np.float32).reshape(6)
+        _check(mrt_hs_set_triangle_uv(self._h, tri, a.ctypes.data_as(_fp)))
 
     def add_sphere(self, centre, radius, mat):
         c = np.asarray(centre, dtype=np.float32)
@@ -890,6 +964,19 @@ def decode_hdr(data):
     finally:
         mrt_free(p)
     return a, ex.value, ga.value
+
+
+def decode_image(data):
+    """BMP / PNG bytes -> (h, w, 4) uint8, rows as the reference's readers
+    order them."""
+    p = C.POINTER(C.c_uint8)()
+    w, h = C.c_uint32(), C.c_uint32()
+    _check(mrt_decode_image(data, len(data), C.byref(p), C.byref(w),
+                            C.byref(h)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 4)).copy()
+    finally:
+        mrt_free(C.cast(p, C.c_void_p))
 
 
 def heat_map_add(depth, float_render):

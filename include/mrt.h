@@ -107,6 +107,48 @@ typedef struct mrt_quadric {
     uint32_t mat, mat_top, mat_bot;
 } mrt_quadric;                /* 64 bytes */
 
+/* Textures: the .rt "tex" element (parse_texture, parse_elements5.c:
+ * 85-142) and get_tex_color / get_tex_transparency
+ * (hit_register/get_tex_color.c:27-67).
+ *   IMAGE           nearest texel ((int)(u (w-1)), (int)(v (h-1))) of an
+ *                   RGBA8 image, rows in file order; colour rgb / 255,
+ *                   transparency 1 - a / 255;
+ *   CHECKER_LOCAL   c1 where floor(u scale) + floor(v scale) is even, else c2;
+ *   CHECKER_GLOBAL  c1 where floor(s x) + floor(s y + 1/2) + floor(s z) of
+ *                   the hit point is even, else c2 (get_solid_texture,
+ *                   textures/get_solid_texture.c:30-43).
+ * Checker colours are 0..255 as in the file. */
+#define MRT_TEX_IMAGE           0u
+#define MRT_TEX_CHECKER_LOCAL   1u
+#define MRT_TEX_CHECKER_GLOBAL  2u
+#define MRT_TEX_BLACK           3u   /* a material's .hdr texture: the
+                                        reference's get_tex_color returns 0 */
+#define MRT_NO_TEX              0xFFFFFFFFu
+
+typedef struct mrt_texture {
+    uint32_t type;            /* MRT_TEX_*                                   */
+    uint32_t width, height;   /* IMAGE                                       */
+    uint32_t texel0;          /* IMAGE: first texel in mrt_scene.texels      */
+    float    scale;           /* checkers                                    */
+    float    c1[3], c2[3];
+    uint32_t pad;
+} mrt_texture;                /* 48 bytes */
+
+/* The texture slots of one material (t_mat, includes/scene_structs.h:
+ * 84-106): a texture index or MRT_NO_TEX per slot.  metallic, roughness,
+ * transmission, ao and emission_strength take the texture's red channel,
+ * color and emission_color its colour (apply_all_map, apply_map.c:60-86);
+ * normal is a tangent-space normal map mixed in by normal_intensity
+ * (apply_normal_map, apply_map.c:37-58).  scale multiplies the (u, v) of
+ * spheres and quadrics (then modf) and 0.1 * scale those of planes (then
+ * v - floor(v)). */
+typedef struct mrt_material_tex {
+    uint32_t color, metallic, roughness, transmission, ao;
+    uint32_t emission_strength, emission_color, normal;
+    float    normal_intensity;
+    float    scale;
+} mrt_material_tex;           /* 40 bytes */
+
 typedef struct mrt_scene {
     uint32_t            n_triangles;
     const float        *tri_pos;      /* 9 floats / tri: v0 v1 v2             */
@@ -130,6 +172,14 @@ typedef struct mrt_scene {
     uint32_t            flags;             /* MRT_SCENE_*                      */
     uint32_t            n_quadrics;
     const mrt_quadric  *quadrics;
+    /* textures (all may be 0 / NULL) */
+    uint32_t            n_textures;
+    const mrt_texture  *textures;
+    uint32_t            n_texels;
+    const uint8_t      *texels;        /* RGBA8, 4 bytes / texel           */
+    const mrt_material_tex *material_tex;  /* NULL, or one per material    */
+    const float        *tri_uv;        /* NULL, or 6 floats / tri: (u, v)
+                                          at v0, v1, v2 (OBJ "vt")          */
 } mrt_scene;
 
 /* Primitive ids reported by MRT_AOV_PRIM_ID: the index into the array the

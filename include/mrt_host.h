@@ -109,6 +109,18 @@ int  mrt_hs_add_plane(mrt_host_scene *hs, const float point[3],
 /* Copies q with its axis normalised (vec3_unit); sizes must be > 0 and
  * finite, the materials defined. */
 int  mrt_hs_add_quadric(mrt_host_scene *hs, const mrt_quadric *q);
+/* A texture; an IMAGE copies width * height RGBA8 texels from rgba (its
+ * texel0 is assigned here).  name may be NULL. */
+int  mrt_hs_add_texture(mrt_host_scene *hs, const mrt_texture *t,
+                        const uint8_t *rgba, const char *name);
+int  mrt_hs_find_texture(const mrt_host_scene *hs, const char *name);
+/* Texture slots of material mat (materials without slots get all
+ * MRT_NO_TEX, normal_intensity 1, scale 1). */
+int  mrt_hs_set_material_tex(mrt_host_scene *hs, uint32_t mat,
+                             const mrt_material_tex *mt);
+/* (u, v) at the three vertices of triangle tri (others default to 0). */
+int  mrt_hs_set_triangle_uv(mrt_host_scene *hs, uint32_t tri,
+                            const float uv[6]);
 int  mrt_hs_add_triangle(mrt_host_scene *hs, const float v[9],
                          const float n[9] /* NULL: flat */, uint32_t mat);
 int  mrt_hs_add_light(mrt_host_scene *hs, const mrt_light *l);
@@ -176,6 +188,14 @@ int  mrt_decode_hdr(const uint8_t *data, size_t size, float **rgb, uint32_t *w,
 int  mrt_load_hdr(const char *path, float **rgb, uint32_t *w, uint32_t *h,
                   float *exposure, float *gamma);
 void mrt_free(void *p);
+
+/* BMP / PNG texture images as RGBA8 (w * h * 4 bytes, free with mrt_free),
+ * rows as the reference's readers order them (src/utils/bmp, src/utils/png):
+ * the file's rows for BMP, the last scanline first for PNG. */
+int  mrt_decode_image(const uint8_t *data, size_t size, uint8_t **rgba,
+                      uint32_t *w, uint32_t *h);
+int  mrt_load_image(const char *path, uint8_t **rgba, uint32_t *w,
+                    uint32_t *h);
 
 /* Adds an OBJ mesh placed as the .rt "cu" element places it (scale, then
  * the rotation taking +Z to dir, then pos); returns the triangle count. */

@@ -118,16 +118,17 @@ void orc_trace_rays_f64(const mrt_scene *s, const orc_bvh *b, int traversal,
                         uint32_t *prim, double *t, uint32_t *nodes,
                         uint32_t *prims, int threads);
 
-/* Closest hits with the shading frame: normal facing the ray (3 per ray),
- * front side flag and material of the part hit. */
+/* Closest hits with the shading frame: normal facing the ray (3 per ray,
+ * normal-mapped), front side flag, material of the part hit and its
+ * (textured) colour (3 per ray; albedo may be NULL). */
 void orc_hit_info_f32(const mrt_scene *s, const orc_bvh *b, int traversal,
                       uint32_t n, const float *orig, const float *dir,
                       uint32_t *prim, float *t, float *nrm, uint8_t *front,
-                      uint32_t *mat);
+                      uint32_t *mat, float *albedo);
 void orc_hit_info_f64(const mrt_scene *s, const orc_bvh *b, int traversal,
                       uint32_t n, const float *orig, const float *dir,
                       uint32_t *prim, double *t, double *nrm, uint8_t *front,
-                      uint32_t *mat);
+                      uint32_t *mat, float *albedo);
 
 /* Any-hit: occluded[i] = 1 when a primitive lies at t in [0.001, tmax[i])
  * (tmax capped at 1000 inclusive), as check_plight_hit

@@ -78,7 +78,7 @@ def lib():
             getattr(L, name).argtypes = [vp, vp, C.c_int, u32, fp, fp,
                                          C.POINTER(C.c_uint32), vp, vp,
                                          C.POINTER(C.c_uint8),
-                                         C.POINTER(C.c_uint32)]
+                                         C.POINTER(C.c_uint32), fp]
         L.orc_occluded_f32.argtypes = [vp, vp, u32, fp, fp, fp,
                                        C.POINTER(C.c_uint8),
                                        C.POINTER(C.c_uint32),
@@ -173,8 +173,9 @@ def trace_rays(scene_view, bvh, mode, orig, dirs, f64=False, threads=0):
     return prim, t, nodes, prims
 
 
-def hit_info(scene_view, bvh, mode, orig, dirs, f64=False):
-    """Closest hits with normal (facing the ray), front flag and material."""
+def hit_info(scene_view, bvh, mode, orig, dirs, f64=False, albedo=False):
+    """Closest hits with normal (facing the ray), front flag and material
+    (and the textured colour when albedo)."""
     o = np.ascontiguousarray(orig, np.float32)
     d = np.ascontiguousarray(dirs, np.float32)
     n = o.size // 3
@@ -184,11 +185,14 @@ def hit_info(scene_view, bvh, mode, orig, dirs, f64=False):
     nrm = np.zeros((n, 3), ft)
     front = np.zeros(n, np.uint8)
     mat = np.zeros(n, np.uint32)
+    alb = np.zeros((n, 3), np.float32)
     fn = lib().orc_hit_info_f64 if f64 else lib().orc_hit_info_f32
     fn(C.cast(scene_view, C.c_void_p),
        C.byref(bvh) if bvh is not None else None, mode, n, _fp(o), _fp(d),
        _up(prim), t.ctypes.data, nrm.ctypes.data,
-       front.ctypes.data_as(C.POINTER(C.c_uint8)), _up(mat))
+       front.ctypes.data_as(C.POINTER(C.c_uint8)), _up(mat), _fp(alb))
+    if albedo:
+        return prim, t, nrm, front.astype(bool), mat, alb
     return prim, t, nrm, front.astype(bool), mat
 
 

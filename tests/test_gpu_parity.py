@@ -65,6 +65,83 @@ def quadric_scene(w, h):
     return hs
 
 
+def textured_scene(w, h):
+    """Every texture kind in every slot: checker floor, image-mapped
+    spheres (one normal-mapped, one emissive), a cylinder with checker caps,
+    a metal cone with a roughness texture and an alpha-cut quad with
+    texture coordinates."""
+    rng = np.random.default_rng(21)
+    hs = M.HostScene.empty()
+    hs.set_resolution(w, h)
+    hs.set_camera((0.0, 2.0, -6.5), (0.0, -0.2, 1.0), 60.0)
+    hs.set_ambient(0.2, (255, 255, 255))
+    img = rng.integers(0, 256, (16, 24, 4), dtype=np.uint8)
+    img[..., 3] = 255
+    cut = img.copy()
+    cut[:, :8, 3] = 0
+    cut[:, 8:16, 3] = 100
+    nmap = np.zeros((8, 8, 4), np.uint8)
+    nmap[..., 0] = rng.integers(60, 200, (8, 8))
+    nmap[..., 1] = rng.integers(60, 200, (8, 8))
+    nmap[..., 2] = 230
+    nmap[..., 3] = 255
+    t_img = hs.add_image_texture(img)
+    t_cut = hs.add_image_texture(cut)
+    t_nrm = hs.add_image_texture(nmap)
+    t_cg = hs.add_checker_texture(M.TEX_CHECKER_GLOBAL, 1.5, (230, 230, 230),
+                                  (40, 60, 90))
+    t_cl = hs.add_checker_texture(M.TEX_CHECKER_LOCAL, 6.0, (255, 200, 0),
+                                  (20, 20, 20))
+    floor = hs.add_material(color=(0.5, 0.5, 0.5))
+    hs.set_material_tex(floor, color=t_cg)
+    sph = hs.add_material(color=(0.5, 0.5, 0.5))
+    hs.set_material_tex(sph, color=t_img, normal=t_nrm, scale=2.0)
+    glow = hs.add_material(color=(0.5, 0.5, 0.5), emission=2.0)
+    hs.set_material_tex(glow, emission_color=t_img, emission_strength=t_cl)
+    body = hs.add_material(color=(0.7, 0.2, 0.2))
+    cap = hs.add_material(color=(0.5, 0.5, 0.5))
+    hs.set_material_tex(cap, color=t_cl, normal_intensity=0.5, normal=t_nrm)
+    metal = hs.add_material(color=(0.9, 0.9, 0.9), metallic=1.0, roughness=0.3)
+    hs.set_material_tex(metal, roughness=t_cl, ao=t_cg)
+    quad = hs.add_material(color=(1.0, 1.0, 1.0))
+    hs.set_material_tex(quad, color=t_cut, metallic=t_cg)
+    hs.add_plane((0, 0, 0), (0, 1, 0), floor)
+    hs.add_sphere((-1.8, 1.0, 0.5), 1.0, sph)
+    hs.add_sphere((1.6, 0.5, -1.5), 0.5, glow)
+    hs.add_cylinder((0.3, 0.8, 1.8), (0.2, 1, 0.1), 1.4, 1.6, body, cap, cap)
+    hs.add_cone((2.2, 2.0, 1.0), (0, -1, 0), 1.6, 2.0, metal, cap)
+    obj = ("v -1 0 0\nv 1 0 0\nv 1 2 0\nv -1 2 0\n"
+           "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nf 1/1 2/2 3/3 4/4\n")
+    hs.add_obj_text(obj, (-0.2, 0.0, -1.0), (0.3, 0.0, -1.0), 0.9, quad)
+    hs.add_light((1.0, 5.0, -3.0), 0.8, (255, 255, 255))
+    return hs
+
+
+@pytest.fixture(scope="module")
+def textured():
+    hs = textured_scene(160, 96)
+    view, bvh = _setup(hs)
+    r = M.Renderer(160, 96)
+    r.upload(hs)
+    r.set_seed(SEED)
+    yield hs, view, bvh, r
+    r.close()
+
+
+def test_textured_normals_match_oracle(textured):
+    hs, view, bvh, r = textured
+    w, h = r.width, r.height
+    ids = r.aov(M.AOV_PRIM_ID, 1)
+    nrm = r.aov(M.AOV_NORMAL, 1).reshape(-1, 3)
+    o, d = O.raygen_f32(view, w, h, 1, SEED)
+    rid, _, rn, _, _ = O.hit_info(view, bvh, O.TRAV_ORDERED, o, d)
+    same = ids == rid
+    assert same.mean() >= 0.999
+    hit = same & (rid != M.PRIM_NONE)
+    err = np.abs(nrm[hit] - rn[hit]).max(1)
+    assert np.mean(err <= 1e-5) >= 0.999
+
+
 @pytest.fixture(scope="module")
 def quadrics():
     hs = quadric_scene(160, 96)
@@ -91,7 +168,7 @@ def test_quadric_normals_and_sides_match_oracle(quadrics):
     assert np.mean(err <= 1e-5) >= 0.999
 
 
-@pytest.mark.parametrize("which", ["plain", "mesh", "quadrics"])
+@pytest.mark.parametrize("which", ["plain", "mesh", "quadrics", "textured"])
 @pytest.mark.parametrize("sample", [0, 5])
 def test_primary_hits_match_fp32_oracle(request, which, sample):
     hs, view, bvh, r = request.getfixturevalue(which)
@@ -113,7 +190,7 @@ def test_primary_hits_match_fp32_oracle(request, which, sample):
     assert np.mean(prims == rp) >= 0.999
 
 
-@pytest.mark.parametrize("which", ["plain", "mesh", "quadrics"])
+@pytest.mark.parametrize("which", ["plain", "mesh", "quadrics", "textured"])
 def test_primary_hits_match_fp64_reference_semantics(request, which):
     """IDs match the reference-precision oracle (fp64 vectors, fp32 MT
     scalars, recursive both-children hit_bvh) on >= 99.99% of pixels and t
@@ -135,7 +212,7 @@ def test_primary_hits_match_fp64_reference_semantics(request, which):
     assert good.mean() >= 0.9999
 
 
-@pytest.mark.parametrize("which", ["plain", "mesh", "quadrics"])
+@pytest.mark.parametrize("which", ["plain", "mesh", "quadrics", "textured"])
 def test_image_matches_oracle_same_stream(request, which):
     hs, view, bvh, r = request.getfixturevalue(which)
     w, h = r.width, r.height
