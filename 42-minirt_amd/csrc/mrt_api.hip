@@ -572,46 +572,46 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     free_env(c);
-    mrt_hdri_tables ref;
-    mrt_hdri_mis mis;
-    int rc = mrt_hdri_build_tables(rgb, w, h, exposure, &ref);
-    if (rc != MRT_OK)
-        return fail(c, rc, "HDRI table build failed");
-    rc = mrt_hdri_build_mis(rgb, w, h, exposure, &mis);
-    if (rc != MRT_OK) {
-        mrt_hdri_free_tables(&ref);
-        return fail(c, rc, "HDRI MIS table build failed");
-    }
+    // the tables are built on the device from the uploaded texels (the
+    // host builders in hdri_tables.c give the same bits)
     size_t n = (size_t)w * h;
     std::vector<float4> tex(n);
     for (size_t i = 0; i < n; i++)
         tex[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+    std::vector<double> band(h - 1);
+    const double pi = 3.14159265358979323846;
+    for (uint32_t y = 0; y + 1 < h; y++) {
+        double elo = ((double)y / (h - 1) - 0.5) * pi;
+        double ehi = ((double)(y + 1) / (h - 1) - 0.5) * pi;
+        band[y] = sin(ehi) - sin(elo);
+    }
+    const double sc = exposure == 0.0f ? 1.0 : (double)powf(2.0f, exposure);
+    double *d_band = nullptr, *d_scratch = nullptr;
+    int rc = MRT_OK;
     if (dalloc(c->d_env_tex, n) != hipSuccess
         || dalloc(c->d_env_pdf, n) != hipSuccess
         || dalloc(c->d_env_marg_inv, h) != hipSuccess
         || dalloc(c->d_env_cond_inv, n) != hipSuccess
         || dalloc(c->d_mis_marg, h) != hipSuccess
         || dalloc(c->d_mis_cond, (size_t)(h - 1) * w) != hipSuccess
-        || dalloc(c->d_mis_pcell, (size_t)(h - 1) * (w - 1)) != hipSuccess)
+        || dalloc(c->d_mis_pcell, (size_t)(h - 1) * (w - 1)) != hipSuccess
+        || dalloc(d_band, h - 1) != hipSuccess
+        || dalloc(d_scratch, 3 * (size_t)h + 2 + n) != hipSuccess)
         rc = fail(c, MRT_E_NOMEM, "device allocation for the HDRI failed");
     if (rc == MRT_OK)
         rc = upload(c, c->d_env_tex, tex.data(), n * 16);
     if (rc == MRT_OK)
-        rc = upload(c, c->d_env_pdf, ref.pdf_joint, n * 4);
-    if (rc == MRT_OK)
-        rc = upload(c, c->d_env_marg_inv, ref.marg_inv, (size_t)h * 4);
-    if (rc == MRT_OK)
-        rc = upload(c, c->d_env_cond_inv, ref.cond_inv, n * 4);
-    if (rc == MRT_OK)
-        rc = upload(c, c->d_mis_marg, mis.marg, (size_t)h * 4);
-    if (rc == MRT_OK)
-        rc = upload(c, c->d_mis_cond, mis.cond, (size_t)(h - 1) * w * 4);
-    if (rc == MRT_OK)
-        rc = upload(c, c->d_mis_pcell, mis.pcell, (size_t)(h - 1) * (w - 1) * 4);
-    if (rc == MRT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
-        rc = fail(c, MRT_E_HIP, "HDRI upload failed");
-    mrt_hdri_free_tables(&ref);
-    mrt_hdri_free_mis(&mis);
+        rc = upload(c, d_band, band.data(), (h - 1) * sizeof(double));
+    if (rc == MRT_OK) {
+        hipError_t e = launch_env_tables(c->stream, c->d_env_tex, w, h, sc, d_band,
+                                         d_scratch, c->d_env_pdf, c->d_env_marg_inv,
+                                         c->d_env_cond_inv, c->d_mis_marg,
+                                         c->d_mis_cond, c->d_mis_pcell);
+        if (e != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(c, MRT_E_HIP, "HDRI table build failed");
+    }
+    dfree(d_band);
+    dfree(d_scratch);
     if (rc != MRT_OK) {
         free_env(c);
         return rc;
@@ -620,6 +620,34 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     c->env_h = h;
     c->env_exposure = exposure;
     c->has_env = true;
+    return MRT_OK;
+}
+
+extern "C" int mrt_get_hdri_tables(mrt_ctx *c, float *pdf_joint, float *marg_inv,
+                                   float *cond_inv, float *mis_marg,
+                                   float *mis_cond, float *mis_pcell)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!c->has_env)
+        return fail(c, MRT_E_STATE, "no HDRI uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t w = c->env_w, h = c->env_h, n = w * h;
+    struct {
+        float       *dst;
+        const float *src;
+        size_t       count;
+    } cp[6] = {{pdf_joint, c->d_env_pdf, n},
+               {marg_inv, c->d_env_marg_inv, h},
+               {cond_inv, c->d_env_cond_inv, n},
+               {mis_marg, c->d_mis_marg, h},
+               {mis_cond, c->d_mis_cond, (h - 1) * w},
+               {mis_pcell, c->d_mis_pcell, (h - 1) * (w - 1)}};
+    for (int k = 0; k < 6; k++)
+        if (cp[k].dst)
+            HIPCHK(c, hipMemcpyAsync(cp[k].dst, cp[k].src, cp[k].count * 4,
+                                     hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MRT_OK;
 }
 

@@ -147,6 +147,15 @@ struct ShadeBufs {
     TexDev            tx;      // textures (when FrameParams::textured)
 };
 
+// Builds the HDRI's reference and MIS tables from its texels (tex, w * h)
+// on the device, bit for bit as mrt_hdri_build_tables / _mis do on the
+// host.  sc = 2^exposure (or 1); band[y] = sin(e_hi) - sin(e_lo) of the h-1
+// MIS rows; scratch holds 3 h + 2 + w h doubles.
+hipError_t launch_env_tables(hipStream_t st, const float4 *tex, uint32_t w,
+                             uint32_t h, double sc, const double *band,
+                             double *scratch, float *pdf, float *marg_inv,
+                             float *cond_inv, float *mis_marg, float *mis_cond,
+                             float *mis_pcell);
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
                          float4 *q_d, float4 *q_t, float4 *acc, uint32_t P,
                          uint32_t *ctr);

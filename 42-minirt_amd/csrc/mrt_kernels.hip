@@ -1903,6 +1903,181 @@ hipError_t launch_caustic_at(hipStream_t st, const PhotonMapDev &pm,
                            pts, out);
     return hipGetLastError();
 }
+// ---- HDRI importance tables on the device -----------------------------------
+// The same double-precision operations, in the same order, as the host
+// builders (hdri_tables.c: make_importance_sampling_map's tables and the MIS
+// tables), so the results are bit for bit the host's: every sequential sum
+// stays one thread's loop, divisions are IEEE, and the inverse-CDF entries
+// come from a binary search for the first CDF value >= i / (n-1), which is
+// the index the host's forward scan stops at (the CDFs never decrease).
+
+// get_grey_value (calc_pdf.c:16-35) of texel i; also the MIS luminance.
+__device__ __forceinline__ double env_gray(const float4 *tex, size_t i, double sc)
+{
+    float4 t = tex[i];
+    return 0.2126 * ((double)t.x * sc) + 0.7152 * ((double)t.y * sc)
+           + 0.0722 * ((double)t.z * sc);
+}
+
+// One thread per row: the row average (calc_raw_average) and, for the
+// h - 1 MIS rows, the sum of the cell weights lum * band[y].
+__global__ void __launch_bounds__(64)
+k_env_rows(const float4 *__restrict__ tex, uint32_t w, uint32_t h, double sc,
+           const double *__restrict__ band, double *__restrict__ rowavg,
+           double *__restrict__ mrow)
+{
+    uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h)
+        return;
+    double s = 0.0;
+    for (uint32_t x = 0; x < w; x++)
+        s += env_gray(tex, (size_t)y * w + x, sc);
+    rowavg[y] = s / w;
+    if (y + 1 < h) {
+        double b = band[y], m = 0.0;
+        for (uint32_t x = 0; x + 1 < w; x++) {
+            double lum = env_gray(tex, (size_t)y * w + x, sc);
+            m += lum > 0.0 ? lum * b : 0.0;
+        }
+        mrow[y] = m;
+    }
+}
+
+// One thread: the column average, the normalised marginal CDF of the
+// reference tables, and the MIS row CDF.  scal = {colavg, total}.
+__global__ void k_env_marginal(uint32_t h, const double *__restrict__ rowavg,
+                               const double *__restrict__ mrow,
+                               double *__restrict__ mcdf,
+                               float *__restrict__ mis_marg,
+                               double *__restrict__ scal)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    double colavg = 0.0;
+    for (uint32_t y = 0; y < h; y++)
+        colavg += rowavg[y];
+    colavg = colavg / h;
+    double sum = 0.0;
+    for (uint32_t y = 0; y < h; y++) {
+        sum += rowavg[y] / (colavg / 255);
+        mcdf[y] = sum;
+    }
+    if (sum > 0.0)
+        for (uint32_t y = 0; y < h; y++)
+            mcdf[y] /= sum;
+    const uint32_t nh = h - 1;
+    double total = 0.0;
+    for (uint32_t y = 0; y < nh; y++)
+        total += mrow[y];
+    double acc = 0.0;
+    mis_marg[0] = 0.0f;
+    for (uint32_t y = 0; y < nh; y++) {
+        acc += mrow[y];
+        mis_marg[y + 1] = total > 0.0 ? (float)(acc / total) : (float)(y + 1) / nh;
+    }
+    mis_marg[nh] = 1.0f;
+    scal[0] = colavg;
+    scal[1] = total;
+}
+
+// One thread per row: the normalised conditional CDF of the reference
+// tables (process_conditional_row) and the MIS column CDF of the row.
+__global__ void __launch_bounds__(64)
+k_env_cond(const float4 *__restrict__ tex, uint32_t w, uint32_t h, double sc,
+           const double *__restrict__ band, const double *__restrict__ rowavg,
+           const double *__restrict__ mrow, double *__restrict__ ccdf,
+           float *__restrict__ mis_cond)
+{
+    uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= h)
+        return;
+    double *c = ccdf + (size_t)y * w;
+    double sum = 0.0;
+    for (uint32_t x = 0; x < w; x++) {
+        sum += env_gray(tex, (size_t)y * w + x, sc) / (rowavg[y] / 255);
+        c[x] = sum;
+    }
+    if (sum > 0.0)
+        for (uint32_t x = 0; x < w; x++)
+            c[x] /= sum;
+    if (y + 1 < h) {
+        const uint32_t nw = w - 1;
+        float *mc = mis_cond + (size_t)y * w;
+        double b = band[y], r = mrow[y], ra = 0.0;
+        mc[0] = 0.0f;
+        for (uint32_t x = 0; x < nw; x++) {
+            double lum = env_gray(tex, (size_t)y * w + x, sc);
+            ra += lum > 0.0 ? lum * b : 0.0;
+            mc[x + 1] = r > 0.0 ? (float)(ra / r) : (float)(x + 1) / nw;
+        }
+        mc[nw] = 1.0f;
+    }
+}
+
+// First index x with cdf[x] >= u, or n when none does.
+__device__ __forceinline__ uint32_t cdf_first_ge(const double *cdf, uint32_t n,
+                                                 double u)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] >= u)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// One thread per texel: pdf_joint, the inverse conditional CDF entry, the
+// MIS cell probability, and (first h threads) the inverse marginal entry.
+// Entries whose target no CDF value reaches stay 0, as the host's calloc.
+__global__ void __launch_bounds__(256)
+k_env_point(const float4 *__restrict__ tex, uint32_t w, uint32_t h, double sc,
+            const double *__restrict__ band, const double *__restrict__ mcdf,
+            const double *__restrict__ ccdf, const double *__restrict__ scal,
+            float *__restrict__ pdf, float *__restrict__ marg_inv,
+            float *__restrict__ cond_inv, float *__restrict__ pcell)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)w * h)
+        return;
+    uint32_t y = (uint32_t)(i / w), x = (uint32_t)(i % w);
+    double g = env_gray(tex, i, sc);
+    pdf[i] = (float)(g / scal[0]);
+    uint32_t k = cdf_first_ge(ccdf + (size_t)y * w, w, (double)x / (double)(w - 1));
+    cond_inv[i] = k < w ? (float)((double)k / (double)(w - 1)) : 0.0f;
+    if (y + 1 < h && x + 1 < w) {
+        double v = g > 0.0 ? g * band[y] : 0.0;
+        pcell[(size_t)y * (w - 1) + x] = scal[1] > 0.0 ? (float)(v / scal[1]) : 0.0f;
+    }
+    if (i < h) {
+        uint32_t m = cdf_first_ge(mcdf, h, (double)i / (double)(h - 1));
+        marg_inv[i] = m < h ? (float)((double)m / (double)(h - 1)) : 0.0f;
+    }
+}
+
+hipError_t launch_env_tables(hipStream_t st, const float4 *tex, uint32_t w,
+                             uint32_t h, double sc, const double *band,
+                             double *scratch, float *pdf, float *marg_inv,
+                             float *cond_inv, float *mis_marg, float *mis_cond,
+                             float *mis_pcell)
+{
+    double *rowavg = scratch, *mrow = rowavg + h, *mcdf = mrow + h,
+           *scal = mcdf + h, *ccdf = scal + 2;
+    hipLaunchKernelGGL(k_env_rows, dim3((h + 63) / 64), dim3(64), 0, st, tex, w,
+                       h, sc, band, rowavg, mrow);
+    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(1), 0, st, h, rowavg, mrow,
+                       mcdf, mis_marg, scal);
+    hipLaunchKernelGGL(k_env_cond, dim3((h + 63) / 64), dim3(64), 0, st, tex, w,
+                       h, sc, band, rowavg, mrow, ccdf, mis_cond);
+    size_t n = (size_t)w * h;
+    hipLaunchKernelGGL(k_env_point, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                       0, st, tex, w, h, sc, band, mcdf, ccdf, scal, pdf,
+                       marg_inv, cond_inv, mis_pcell);
+    return hipGetLastError();
+}
+
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
                          float4 *q_d, float4 *q_t, float4 *acc, uint32_t P,
                          uint32_t *ctr)

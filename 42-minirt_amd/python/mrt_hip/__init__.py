@@ -302,6 +302,8 @@ mrt_hs_set_material_tex = _sig("mrt_hs_set_material_tex", C.c_int, _hs_p,
                                C.c_uint32, C.POINTER(MaterialTex))
 mrt_hs_set_triangle_uv = _sig("mrt_hs_set_triangle_uv", C.c_int, _hs_p,
                               C.c_uint32, _fp)
+mrt_get_hdri_tables = _sig("mrt_get_hdri_tables", C.c_int, C.c_void_p,
+                           *([_fp] * 6))
 mrt_decode_image = _sig("mrt_decode_image", C.c_int, C.c_char_p, C.c_size_t,
                         C.POINTER(C.POINTER(C.c_uint8)),
                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32))
@@ -753,6 +755,19 @@ class Renderer:
         h, w = a.shape[:2]
         self._ck(mrt_upload_hdri(self._ctx, a.ctypes.data_as(_fp), w, h,
                                  exposure))
+        self._env_wh = (w, h)
+
+    def hdri_tables(self):
+        """The device's HDRI tables: (pdf_joint, marg_inv, cond_inv,
+        mis_marg, mis_cond, mis_pcell)."""
+        w, h = self._env_wh
+        out = [np.zeros((h, w), np.float32), np.zeros(h, np.float32),
+               np.zeros((h, w), np.float32), np.zeros(h, np.float32),
+               np.zeros((h - 1, w), np.float32),
+               np.zeros((h - 1, w - 1), np.float32)]
+        self._ck(mrt_get_hdri_tables(self._ctx, *[a.ctypes.data_as(_fp)
+                                                  for a in out]))
+        return tuple(out)
 
     def build_photons(self, n_per, radius, k=None, norm=None):
         """Emit, sort and grid the caustic photon map (--caustic=N;R;NORM)."""

@@ -258,9 +258,20 @@ int  mrt_set_camera(mrt_ctx *ctx, const mrt_camera *camera);
  * make_importance_sampling_map (src/ray_tracing/importance_sampling/
  * importance_sampling_manager.c:41-61): rgb is width*height linear texels,
  * row 0 at the nadir (the reference's hdr.pixels order), exposure as
- * t_hdr.exposure.  Builds the inverse-CDF tables and uploads them. */
+ * t_hdr.exposure.  Uploads the texels and builds the inverse-CDF tables of
+ * the reference estimator and the MIS tables on the device (bit for bit
+ * the tables mrt_hdri_build_tables / mrt_hdri_build_mis give on the host,
+ * where the reference builds them in O(W^2 H)). */
 int  mrt_upload_hdri(mrt_ctx *ctx, const float *rgb, uint32_t width,
                      uint32_t height, float exposure);
+
+/* No counterpart in the reference: copies the device's HDRI tables back
+ * (for inspection and tests): pdf_joint, cond_inv (w*h), marg_inv (h) as
+ * mrt_hdri_tables; mis_marg (h), mis_cond ((h-1)*w), mis_pcell
+ * ((h-1)*(w-1)) as mrt_hdri_mis.  Any pointer may be NULL. */
+int  mrt_get_hdri_tables(mrt_ctx *ctx, float *pdf_joint, float *marg_inv,
+                         float *cond_inv, float *mis_marg, float *mis_cond,
+                         float *mis_pcell);
 
 /* Replaces caustic_manager (src/caustic/caustic_manager.c:83-102) and
  * build_caustic_grid (src/gpu/convert_caustic.c:85-102), the handling of

@@ -220,3 +220,34 @@ def test_gpu_mis_white_furnace():
         img = r.readback() / 256
     _, inner = sphere_mask(hs)
     assert abs(img[inner].mean() - 1.0) < 0.005
+
+
+def _env_maps():
+    rng = np.random.default_rng(9)
+    rand = (rng.random((37, 53, 3)) ** 4 * 30).astype(np.float32)
+    rand[5] = 0.0                                   # a black row: sum 0
+    rand[10:12, 7] = 4e4                            # a bright spot
+    return [(M.hdri_sky(5, 2048, 1024), 0.0),       # the HDRI workload's size
+            (rand, 0.0), (rand, 1.5), (M.hdri_sky(2, 64, 2), 0.0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(4))
+def test_gpu_hdri_tables_equal_host_tables_bitwise(case):
+    """The device build of both table sets is bit for bit the host's
+    (mrt_hdri_build_tables) and the oracle's (reference and MIS tables)."""
+    rgb, ex = _env_maps()[case]
+    h, w = rgb.shape[:2]
+    with M.Renderer(8, 8) as r:
+        r.upload_hdri(rgb, ex)
+        pdf, marg, cond, mm, mc, mp = r.hdri_tables()
+    hp, hm, hc, _ = M.hdri_tables(rgb, ex)
+    np.testing.assert_array_equal(pdf, hp)
+    np.testing.assert_array_equal(marg, hm)
+    np.testing.assert_array_equal(cond, hc)
+    op, om, oc, _ = O.hdri_tables(rgb, ex)
+    np.testing.assert_array_equal(cond, oc)
+    om_, oc_, opc, _ = O.hdri_mis_tables(rgb, ex)
+    np.testing.assert_array_equal(mm, om_)
+    np.testing.assert_array_equal(mc, oc_)
+    np.testing.assert_array_equal(mp, opc)
