@@ -14,6 +14,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "mrt_build.h"
+
 #include "mrt_kernels.h"
 
 using namespace mrt;
@@ -36,6 +38,8 @@ struct mrt_ctx {
     uint4       *d_mtex = nullptr;   // material texture slots: 3 uint4 each
     float       *d_tri_uv = nullptr;
     bool         textured = false;   // some material has a texture slot
+    int          builder = MRT_BUILDER_HOST;
+    bool         built_on_device = false;  // the last upload's BVH
     bool         alpha = false;      // ... a colour image with alpha < 255
     uint32_t     n_nodes = 0, n_prims = 0, n_planes = 0, n_mats = 0;
     uint32_t     bvh_depth = 0;
@@ -422,26 +426,48 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     free_scene(c);
 
     mrt_bvh bvh;
-    int rc = mrt_bvh_build(s, MRT_BVH_MAX_LEAF, &bvh);
-    if (rc != MRT_OK)
-        return fail(c, rc, "BVH build failed (%d)", rc);
-    if (bvh.depth > MRT_BVH_MAX_DEPTH) {
-        mrt_bvh_free(&bvh);
-        return fail(c, MRT_E_LIMIT, "BVH depth %u exceeds %u", bvh.depth,
-                    MRT_BVH_MAX_DEPTH);
-    }
     mrt_bvh4 bvh4;
-    rc = mrt_bvh4_build(&bvh, &bvh4);
-    if (rc != MRT_OK) {
-        mrt_bvh_free(&bvh);
-        return fail(c, rc, "4-wide BVH build failed (%d)", rc);
+    memset(&bvh, 0, sizeof(bvh));
+    memset(&bvh4, 0, sizeof(bvh4));
+    DeviceBvh D;
+    memset(&D, 0, sizeof(D));
+    bool on_dev = false;
+    int rc = MRT_OK;
+    if (c->builder == MRT_BUILDER_DEVICE) {
+        const char *why = "";
+        rc = device_bvh_build(c->stream, s, &D, &why);
+        if (rc == MRT_OK)
+            on_dev = true;
+        else if (rc != MRT_E_UNSUPPORTED)
+            return fail(c, rc, "device BVH build failed: %s", why);
     }
-    if (bvh4.depth > MRT_BVH4_MAX_DEPTH) {
-        mrt_bvh_free(&bvh);
-        mrt_bvh4_free(&bvh4);
-        return fail(c, MRT_E_LIMIT, "4-wide BVH depth %u exceeds %u", bvh4.depth,
-                    MRT_BVH4_MAX_DEPTH);
+    if (!on_dev) {
+        rc = mrt_bvh_build(s, MRT_BVH_MAX_LEAF, &bvh);
+        if (rc != MRT_OK)
+            return fail(c, rc, "BVH build failed (%d)", rc);
+        if (bvh.depth > MRT_BVH_MAX_DEPTH) {
+            mrt_bvh_free(&bvh);
+            return fail(c, MRT_E_LIMIT, "BVH depth %u exceeds %u", bvh.depth,
+                        MRT_BVH_MAX_DEPTH);
+        }
+        rc = mrt_bvh4_build(&bvh, &bvh4);
+        if (rc != MRT_OK) {
+            mrt_bvh_free(&bvh);
+            return fail(c, rc, "4-wide BVH build failed (%d)", rc);
+        }
+        if (bvh4.depth > MRT_BVH4_MAX_DEPTH) {
+            mrt_bvh_free(&bvh);
+            mrt_bvh4_free(&bvh4);
+            return fail(c, MRT_E_LIMIT, "4-wide BVH depth %u exceeds %u", bvh4.depth,
+                        MRT_BVH4_MAX_DEPTH);
+        }
+    } else {
+        c->d_nodes = D.nodes;
+        c->d_prims = D.prims;
+        c->d_nrm = D.nrm;
     }
+    const uint32_t n_nodes4 = on_dev ? D.n_nodes : bvh4.n_nodes;
+    const uint32_t n_prims = on_dev ? D.n_prims : bvh.n_prims;
     std::vector<float4> planes, mats, lights;
     pack_scene(s, 0, planes, mats, lights);
     std::vector<float4> tex;
@@ -449,9 +475,10 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     bool textured = false, alpha = false;
     pack_textures(s, tex, mtex, textured, alpha);
     rc = MRT_OK;
-    if (dalloc(c->d_nodes, (size_t)bvh4.n_nodes * 8) != hipSuccess
-        || dalloc(c->d_prims, (size_t)bvh.n_prims * 3) != hipSuccess
-        || dalloc(c->d_nrm, (size_t)bvh.n_prims * 9) != hipSuccess
+    if ((!on_dev
+         && (dalloc(c->d_nodes, (size_t)bvh4.n_nodes * 8) != hipSuccess
+             || dalloc(c->d_prims, (size_t)bvh.n_prims * 3) != hipSuccess
+             || dalloc(c->d_nrm, (size_t)bvh.n_prims * 9) != hipSuccess))
         || dalloc(c->d_planes, planes.size()) != hipSuccess
         || dalloc(c->d_mats, mats.size()) != hipSuccess
         || dalloc(c->d_lights, lights.size()) != hipSuccess
@@ -462,11 +489,11 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
                 || (s->tri_uv
                     && dalloc(c->d_tri_uv, (size_t)s->n_triangles * 6) != hipSuccess))))
         rc = fail(c, MRT_E_NOMEM, "device allocation for the scene failed");
-    if (rc == MRT_OK)
+    if (rc == MRT_OK && !on_dev)
         rc = upload(c, c->d_nodes, bvh4.nodes, (size_t)bvh4.n_nodes * 128);
-    if (rc == MRT_OK)
+    if (rc == MRT_OK && !on_dev)
         rc = upload(c, c->d_prims, bvh.prims, (size_t)bvh.n_prims * 48);
-    if (rc == MRT_OK)
+    if (rc == MRT_OK && !on_dev)
         rc = upload(c, c->d_nrm, bvh.shade_nrm, (size_t)bvh.n_prims * 36);
     if (rc == MRT_OK)
         rc = upload(c, c->d_planes, planes.data(), planes.size() * 16);
@@ -485,12 +512,13 @@ extern "C" int mrt_upload_scene(mrt_ctx *c, const mrt_scene *s)
     }
     if (rc == MRT_OK && hipStreamSynchronize(c->stream) != hipSuccess)
         rc = fail(c, MRT_E_HIP, "scene upload failed");
-    c->n_nodes = bvh4.n_nodes;
-    c->n_prims = bvh.n_prims;
+    c->n_nodes = n_nodes4;
+    c->n_prims = n_prims;
+    c->built_on_device = on_dev;
     c->has_quad = s->n_quadrics > 0;
     c->textured = textured;
     c->alpha = alpha;
-    c->bvh_depth = bvh4.depth;
+    c->bvh_depth = on_dev ? D.depth : bvh4.depth;
     mrt_bvh_free(&bvh);
     mrt_bvh4_free(&bvh4);
     if (rc != MRT_OK) {
@@ -620,6 +648,36 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     c->env_h = h;
     c->env_exposure = exposure;
     c->has_env = true;
+    return MRT_OK;
+}
+
+extern "C" int mrt_get_scene_bvh(mrt_ctx *c, uint32_t *n_nodes, uint32_t *n_prims,
+                                 uint32_t *depth, int *on_device, void *nodes,
+                                 void *prims, float *shade_nrm)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "no scene uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (n_nodes)
+        *n_nodes = c->n_nodes;
+    if (n_prims)
+        *n_prims = c->n_prims;
+    if (depth)
+        *depth = c->bvh_depth;
+    if (on_device)
+        *on_device = c->built_on_device ? 1 : 0;
+    if (nodes)
+        HIPCHK(c, hipMemcpyAsync(nodes, c->d_nodes, (size_t)c->n_nodes * 128,
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (prims)
+        HIPCHK(c, hipMemcpyAsync(prims, c->d_prims, (size_t)c->n_prims * 48,
+                                 hipMemcpyDeviceToHost, c->stream));
+    if (shade_nrm)
+        HIPCHK(c, hipMemcpyAsync(shade_nrm, c->d_nrm, (size_t)c->n_prims * 36,
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MRT_OK;
 }
 
@@ -1336,6 +1394,11 @@ extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
         if (!(value > 0.0))
             return fail(c, MRT_E_INVALID, "photon norm must be positive");
         c->ph_norm = (float)value;
+        return MRT_OK;
+    case MRT_OPT_BUILDER:
+        if (value != MRT_BUILDER_HOST && value != MRT_BUILDER_DEVICE)
+            return fail(c, MRT_E_INVALID, "unknown BVH builder");
+        c->builder = (int)value;
         return MRT_OK;
     default:
         return fail(c, MRT_E_INVALID, "unknown option %d", option);

@@ -58,6 +58,9 @@ OPT_PATH_BUDGET = 4
 OPT_BLOCK = 5
 OPT_PHOTON_K = 6
 OPT_PHOTON_NORM = 7
+OPT_BUILDER = 8
+BUILDER_HOST = 0
+BUILDER_DEVICE = 1
 STATS_COUNT = 1
 STATS_TIME = 2
 
@@ -302,6 +305,10 @@ mrt_hs_set_material_tex = _sig("mrt_hs_set_material_tex", C.c_int, _hs_p,
                                C.c_uint32, C.POINTER(MaterialTex))
 mrt_hs_set_triangle_uv = _sig("mrt_hs_set_triangle_uv", C.c_int, _hs_p,
                               C.c_uint32, _fp)
+mrt_get_scene_bvh = _sig("mrt_get_scene_bvh", C.c_int, C.c_void_p,
+                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                         C.POINTER(C.c_uint32), C.POINTER(C.c_int),
+                         C.c_void_p, C.c_void_p, _fp)
 mrt_get_hdri_tables = _sig("mrt_get_hdri_tables", C.c_int, C.c_void_p,
                            *([_fp] * 6))
 mrt_decode_image = _sig("mrt_decode_image", C.c_int, C.c_char_p, C.c_size_t,
@@ -634,6 +641,22 @@ class HostScene:
         }
 
 
+class SceneBvh:
+    """A scene's BVH as read back from a Renderer: 4-wide nodes and records
+    only (no binary tree)."""
+    nodes = np.zeros(0, np.uint8)
+    n_nodes = 0
+
+    def node4_floats(self):
+        return self.nodes4.view(np.float32).reshape(-1, 32)
+
+    def node4_children(self):
+        return self.nodes4.view(np.int32).reshape(-1, 32)[:, 24:28]
+
+    def prim_words(self):
+        return self.prims.view(np.uint32).reshape(-1, 12)
+
+
 class BuiltBvh:
     """mrt_bvh_build result as numpy arrays (host, no GPU)."""
 
@@ -654,6 +677,9 @@ class BuiltBvh:
                 C.cast(b.prims, C.POINTER(C.c_uint8)),
                 shape=(48 * b.n_prims,)).copy() if b.n_prims else
                 np.zeros(0, np.uint8))
+            self.shade_nrm = (np.ctypeslib.as_array(
+                b.shade_nrm, shape=(9 * b.n_prims,)).copy() if b.n_prims else
+                np.zeros(0, np.float32))
             b4 = Bvh4()
             _check(mrt_bvh4_build(C.byref(b), C.byref(b4)))
             try:
@@ -756,6 +782,24 @@ class Renderer:
         self._ck(mrt_upload_hdri(self._ctx, a.ctypes.data_as(_fp), w, h,
                                  exposure))
         self._env_wh = (w, h)
+
+    def scene_bvh(self):
+        """The uploaded scene's BVH (SceneBvh: 4-wide nodes, primitive and
+        shading records as the kernels read them)."""
+        nn, npr, dep, dev = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_int()
+        self._ck(mrt_get_scene_bvh(self._ctx, C.byref(nn), C.byref(npr),
+                                   C.byref(dep), C.byref(dev), None, None,
+                                   None))
+        b = SceneBvh()
+        b.n_nodes4, b.n_prims, b.depth4 = nn.value, npr.value, dep.value
+        b.on_device = bool(dev.value)
+        b.nodes4 = np.zeros(128 * b.n_nodes4, np.uint8)
+        b.prims = np.zeros(48 * b.n_prims, np.uint8)
+        b.shade_nrm = np.zeros(9 * b.n_prims, np.float32)
+        self._ck(mrt_get_scene_bvh(self._ctx, None, None, None, None,
+                                   b.nodes4.ctypes.data, b.prims.ctypes.data,
+                                   b.shade_nrm.ctypes.data_as(_fp)))
+        return b
 
     def hdri_tables(self):
         """The device's HDRI tables: (pdf_joint, marg_inv, cond_inv,

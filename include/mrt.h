@@ -206,6 +206,14 @@ typedef struct mrt_scene {
                                   the radius                                */
 #define MRT_OPT_PHOTON_NORM 7  /* caustic norm factor (--caustic=..;..;NORM) */
 
+#define MRT_OPT_BUILDER     8  /* MRT_BUILDER_*                              */
+
+/* BVH builders (MRT_OPT_BUILDER). */
+#define MRT_BUILDER_HOST    0  /* binned SAH on the host (default)            */
+#define MRT_BUILDER_DEVICE  1  /* PLOC clustering on the device; falls back to
+                                  the host below 64 primitives or when the
+                                  tree would be too deep for the traversal  */
+
 /* Flags of MRT_OPT_STATS. */
 #define MRT_STATS_COUNT     1  /* count BVH nodes / primitives per ray       */
 #define MRT_STATS_TIME      2  /* time every kernel launch with HIP events   */
@@ -264,6 +272,15 @@ int  mrt_set_camera(mrt_ctx *ctx, const mrt_camera *camera);
  * where the reference builds them in O(W^2 H)). */
 int  mrt_upload_hdri(mrt_ctx *ctx, const float *rgb, uint32_t width,
                      uint32_t height, float exposure);
+
+/* No counterpart in the reference: the uploaded scene's BVH as the kernels
+ * read it: counts, 4-wide depth, whether the device built it, and (when the
+ * pointers are not NULL) copies of the 4-wide nodes (n_nodes x 128 bytes,
+ * mrt_bvh4_node), the primitive records (n_prims x 48, mrt_prim) and the
+ * shading records (n_prims x 9 floats). */
+int  mrt_get_scene_bvh(mrt_ctx *ctx, uint32_t *n_nodes, uint32_t *n_prims,
+                       uint32_t *depth, int *on_device, void *nodes,
+                       void *prims, float *shade_nrm);
 
 /* No counterpart in the reference: copies the device's HDRI tables back
  * (for inspection and tests): pdf_joint, cond_inv (w*h), marg_inv (h) as
