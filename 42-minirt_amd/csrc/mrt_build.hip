@@ -29,8 +29,11 @@ namespace mrt {
 
 namespace {
 
-constexpr int PLOC_R = 16;             // search radius (positions)
 constexpr int PLOC_BLOCK = 256;
+// Search radius in positions: on the 1080p mesh workload 16 traces fastest
+// (404.6 Msamples/s against 392.6 for 8, 386.8 for 32, 378.3 for 64; wider
+// windows merge into fewer but looser nodes).
+constexpr int PLOC_RADIUS = 16;
 
 __device__ __forceinline__ float box_area(float4 lo, float4 hi)
 {
@@ -210,6 +213,7 @@ k_leaves(uint32_t n, const uint32_t *__restrict__ src, const float4 *__restrict_
 
 // Nearest neighbour of every cluster in [i-R, i+R] by the area of the union
 // box; ties to the lower position.  Boxes of the block's window in LDS.
+template <int PLOC_R>
 __global__ void __launch_bounds__(PLOC_BLOCK)
 k_ploc_nn(uint32_t m, const uint32_t *__restrict__ clus,
           const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
@@ -695,8 +699,8 @@ int device_bvh_build(hipStream_t st, const mrt_scene *s, DeviceBvh *out,
                 soft = true;
                 goto fail;
             }
-            hipLaunchKernelGGL(k_ploc_nn, dim3(grid(m, PLOC_BLOCK)), dim3(PLOC_BLOCK), 0, st, m,
-                               clus.p, nlo.p, nhi.p, nn.p);
+            hipLaunchKernelGGL(k_ploc_nn<PLOC_RADIUS>, dim3(grid(m, PLOC_BLOCK)),
+                               dim3(PLOC_BLOCK), 0, st, m, clus.p, nlo.p, nhi.p, nn.p);
             hipLaunchKernelGGL(k_ploc_flags, dim3(grid(m, 256)), dim3(256), 0, st, m, nn.p,
                                flags.p);
             HE(hipGetLastError());
