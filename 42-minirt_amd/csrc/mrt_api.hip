@@ -1787,8 +1787,40 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         memset(&bv[i], 0, sizeof(bv[i]));
         memset(&bv4[i], 0, sizeof(bv4[i]));
     }
+    // with the device builder, the scenes it takes are built one after
+    // the other on the stream and stay on the device; the host threads
+    // build the rest
+    std::vector<DeviceBvh> dv(n);
+    std::vector<char> on_dev(n, 0);
+    for (size_t i = 0; i < n; i++)
+        memset(&dv[i], 0, sizeof(dv[i]));
+    auto free_built = [&]() {
+        for (size_t i = 0; i < n; i++) {
+            mrt_bvh_free(&bv[i]);
+            mrt_bvh4_free(&bv4[i]);
+            if (on_dev[i]) {
+                (void)hipFree(dv[i].nodes);
+                (void)hipFree(dv[i].prims);
+                (void)hipFree(dv[i].nrm);
+                on_dev[i] = 0;
+            }
+        }
+    };
+    if (c->builder == MRT_BUILDER_DEVICE) {
+        HIPCHK(c, hipSetDevice(c->device));
+        for (size_t i = 0; i < n; i++) {
+            const char *why = "";
+            int r = device_bvh_build(c->stream, &b->src[i]->view, &dv[i], &why);
+            if (r == MRT_OK)
+                on_dev[i] = 1;
+            else if (r != MRT_E_UNSUPPORTED)
+                brc[i] = r;
+        }
+    }
     auto work = [&]() {
         for (size_t i; (i = next.fetch_add(1)) < n;) {
+            if (on_dev[i] || brc[i] != MRT_OK)
+                continue;
             brc[i] = mrt_bvh_build(&b->src[i]->view, MRT_BVH_MAX_LEAF, &bv[i]);
             if (brc[i] == MRT_OK)
                 brc[i] = mrt_bvh4_build(&bv[i], &bv4[i]);
@@ -1804,73 +1836,39 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     for (size_t i = 0; i < n && rc == MRT_OK; i++) {
         if (brc[i] != MRT_OK)
             rc = fail(c, brc[i], "scene %zu: BVH build failed", i);
-        else if (bv[i].depth > MRT_BVH_MAX_DEPTH || bv4[i].depth > MRT_BVH4_MAX_DEPTH)
+        else if (!on_dev[i]
+                 && (bv[i].depth > MRT_BVH_MAX_DEPTH || bv4[i].depth > MRT_BVH4_MAX_DEPTH))
             rc = fail(c, MRT_E_LIMIT, "scene %zu: BVH depth %u / %u exceeds %u / %u",
                       i, bv[i].depth, bv4[i].depth, MRT_BVH_MAX_DEPTH,
                       MRT_BVH4_MAX_DEPTH);
     }
-    // shared arrays with rebased links, primitives and materials
-    std::vector<mrt_bvh4_node> nodes;
-    std::vector<mrt_prim> prims;
-    std::vector<float> nrm;
+    // offsets of every scene in the shared arrays; planes, materials and
+    // lights packed on the host (small)
+    std::vector<uint32_t> node0s(n), prim0s(n), mat0s(n);
     std::vector<float4> planes, mats, lights;
     b->hbs.assign(n, BatchScene{});
-    uint64_t path0 = 0;
+    uint64_t path0 = 0, n_nodes_all = 0, n_prims_all = 0;
     size_t accum0 = 0;
     b->max_rays = 1;
     b->has_lights = false;
     b->has_quad = false;
     for (size_t i = 0; i < n && rc == MRT_OK; i++) {
         const BatchSrc &S = *b->src[i];
-        const mrt_bvh &B = bv[i];
+        const uint32_t nn4 = on_dev[i] ? dv[i].n_nodes : bv4[i].n_nodes;
+        const uint32_t np = on_dev[i] ? dv[i].n_prims : bv[i].n_prims;
         b->has_quad = b->has_quad || S.view.n_quadrics > 0;
-        const mrt_bvh4 &B4 = bv4[i];
-        const uint32_t node0 = (uint32_t)nodes.size();
-        const uint32_t prim0 = (uint32_t)prims.size();
-        const uint32_t mat0 = (uint32_t)(mats.size() / 3);
-        const uint32_t plane0 = (uint32_t)(planes.size() / 2);
-        const uint32_t light0 = (uint32_t)(lights.size() / 3);
-        if ((uint64_t)prim0 + B.n_prims >= (1ull << 28)
-            || (uint64_t)node0 + B4.n_nodes >= (1ull << 31)) {
+        if (n_prims_all + np >= (1ull << 28) || n_nodes_all + nn4 >= (1ull << 31)) {
             rc = fail(c, MRT_E_LIMIT, "batch exceeds 2^28 primitives");
             break;
         }
-        for (uint32_t k = 0; k < B4.n_nodes; k++) {
-            mrt_bvh4_node nd = B4.nodes[k];
-            for (int j = 0; j < 4; j++) {
-                int32_t ch = nd.child[j];
-                if (ch >= 0) {
-                    nd.child[j] = ch + (int32_t)node0;
-                } else {
-                    uint32_t v = ~(uint32_t)ch;
-                    v = (((v >> 3) + prim0) << 3) | (v & 7u);
-                    nd.child[j] = (int32_t)~v;
-                }
-            }
-            nodes.push_back(nd);
-        }
-        const size_t nrm0 = nrm.size();
-        nrm.insert(nrm.end(), B.shade_nrm, B.shade_nrm + (size_t)B.n_prims * 9);
-        for (uint32_t k = 0; k < B.n_prims; k++) {
-            mrt_prim pr = B.prims[k];
-            uint32_t m, kind;
-            memcpy(&kind, &pr.b[3], 4);
-            if (kind >= MRT_PRIMKIND_QUADRIC) {
-                // materials of a quadric live in its shading record
-                for (int j = 1; j <= 3; j++) {
-                    float *f = &nrm[nrm0 + (size_t)k * 9 + j];
-                    memcpy(&m, f, 4);
-                    m += mat0;
-                    memcpy(f, &m, 4);
-                }
-            } else {
-                memcpy(&m, &pr.c[3], 4);
-                m += mat0;
-                memcpy(&pr.c[3], &m, 4);
-            }
-            prims.push_back(pr);
-        }
-        pack_scene(&S.view, mat0, planes, mats, lights);
+        node0s[i] = (uint32_t)n_nodes_all;
+        prim0s[i] = (uint32_t)n_prims_all;
+        mat0s[i] = (uint32_t)(mats.size() / 3);
+        const uint32_t plane0 = (uint32_t)(planes.size() / 2);
+        const uint32_t light0 = (uint32_t)(lights.size() / 3);
+        n_nodes_all += nn4;
+        n_prims_all += np;
+        pack_scene(&S.view, mat0s[i], planes, mats, lights);
         BatchScene &T = b->hbs[i];
         memset(&T, 0, sizeof(T));
         camera_frame(T.fp, S.view.camera, S.W, S.H, S.view.ambient_ratio,
@@ -1883,8 +1881,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         T.fp.estimator = MRT_EST_REFERENCE;
         T.fp.light0 = light0;
         T.fp.plane0 = plane0;
-        T.root = B4.n_nodes ? node0 : 0u;
-        T.n_nodes = B4.n_nodes;
+        T.root = nn4 ? node0s[i] : 0u;
+        T.n_nodes = nn4;
         T.n_planes = S.view.n_planes;
         T.accum0 = (uint32_t)accum0;
         T.path0 = path0;
@@ -1900,33 +1898,45 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         b->max_rays = std::max(b->max_rays, rays);
         b->has_lights = b->has_lights || !S.lights.empty();
     }
-    for (mrt_bvh &B : bv)
-        mrt_bvh_free(&B);
-    for (mrt_bvh4 &B : bv4)
-        mrt_bvh4_free(&B);
     if (rc == MRT_OK && accum0 >= (1ull << 32))
         rc = fail(c, MRT_E_LIMIT, "batch images exceed 2^32 floats");
-    if (rc != MRT_OK)
+    if (rc != MRT_OK) {
+        free_built();
         return rc;
+    }
     HIPCHK(c, hipSetDevice(c->device));
     batch_free_device(b);
     BatchCtl ctl{0, path0, (uint32_t)n, 0};
-    if (dalloc(b->d_nodes, nodes.size() * 8) != hipSuccess
-        || dalloc(b->d_prims, prims.size() * 3) != hipSuccess
-        || dalloc(b->d_nrm, nrm.size()) != hipSuccess
+    if (dalloc(b->d_nodes, n_nodes_all * 8) != hipSuccess
+        || dalloc(b->d_prims, n_prims_all * 3) != hipSuccess
+        || dalloc(b->d_nrm, n_prims_all * 9) != hipSuccess
         || dalloc(b->d_planes, planes.size()) != hipSuccess
         || dalloc(b->d_mats, mats.size()) != hipSuccess
         || dalloc(b->d_lights, lights.size()) != hipSuccess
         || dalloc(b->d_accum, accum0) != hipSuccess
         || dalloc(b->d_bs, n) != hipSuccess || dalloc(b->d_ctl, 1) != hipSuccess) {
         batch_free_device(b);
+        free_built();
         return fail(c, MRT_E_NOMEM, "device allocation for the batch failed");
     }
-    rc = upload(c, b->d_nodes, nodes.data(), nodes.size() * 128);
-    if (rc == MRT_OK)
-        rc = upload(c, b->d_prims, prims.data(), prims.size() * 48);
-    if (rc == MRT_OK)
-        rc = upload(c, b->d_nrm, nrm.data(), nrm.size() * 4);
+    // every scene's arrays into place, then moved by its offsets
+    for (size_t i = 0; i < n && rc == MRT_OK; i++) {
+        const uint32_t nn4 = on_dev[i] ? dv[i].n_nodes : bv4[i].n_nodes;
+        const uint32_t np = on_dev[i] ? dv[i].n_prims : bv[i].n_prims;
+        float4 *dn = b->d_nodes + (size_t)node0s[i] * 8;
+        float4 *dp = b->d_prims + (size_t)prim0s[i] * 3;
+        float *dr = b->d_nrm + (size_t)prim0s[i] * 9;
+        hipMemcpyKind k = on_dev[i] ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        const void *sn = on_dev[i] ? (const void *)dv[i].nodes : (const void *)bv4[i].nodes;
+        const void *sp = on_dev[i] ? (const void *)dv[i].prims : (const void *)bv[i].prims;
+        const void *sr = on_dev[i] ? (const void *)dv[i].nrm : (const void *)bv[i].shade_nrm;
+        if ((nn4 && hipMemcpyAsync(dn, sn, (size_t)nn4 * 128, k, c->stream) != hipSuccess)
+            || (np && hipMemcpyAsync(dp, sp, (size_t)np * 48, k, c->stream) != hipSuccess)
+            || (np && hipMemcpyAsync(dr, sr, (size_t)np * 36, k, c->stream) != hipSuccess)
+            || launch_rebase(c->stream, dn, nn4, dp, dr, np, node0s[i], prim0s[i],
+                             mat0s[i]) != hipSuccess)
+            rc = fail(c, MRT_E_HIP, "scene %zu: copy into the batch failed", i);
+    }
     if (rc == MRT_OK)
         rc = upload(c, b->d_planes, planes.data(), planes.size() * 16);
     if (rc == MRT_OK)
@@ -1941,6 +1951,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         && (hipMemsetAsync(b->d_accum, 0, accum0 * 4, c->stream) != hipSuccess
             || hipStreamSynchronize(c->stream) != hipSuccess))
         rc = fail(c, MRT_E_HIP, "batch upload failed");
+    (void)hipStreamSynchronize(c->stream);
+    free_built();
     if (rc != MRT_OK) {
         batch_free_device(b);
         return rc;
@@ -2275,6 +2287,9 @@ extern "C" int mrt_batch_set_option(mrt_batch *b, int option, double value)
         return MRT_E_INVALID;
     switch (option) {
     case MRT_OPT_STATS:
+    case MRT_OPT_BUILDER:
+        if (option == MRT_OPT_BUILDER && b->committed)
+            return fail(b->c, MRT_E_STATE, "batch already committed");
         return mrt_set_option(b->c, option, value);
     case MRT_OPT_PATH_BUDGET: {
         mrt_batch_tuning t = b->tune;

@@ -147,6 +147,12 @@ struct ShadeBufs {
     TexDev            tx;      // textures (when FrameParams::textured)
 };
 
+// Moves one scene's nodes / records, already copied into a batch's shared
+// arrays, to their place there: links by node0, leaf ranges by prim0,
+// materials by mat0.
+hipError_t launch_rebase(hipStream_t st, float4 *nodes, uint32_t n_nodes,
+                         float4 *prims, float *nrm, uint32_t n_prims,
+                         uint32_t node0, uint32_t prim0, uint32_t mat0);
 // Builds the HDRI's reference and MIS tables from its texels (tex, w * h)
 // on the device, bit for bit as mrt_hdri_build_tables / _mis do on the
 // host.  sc = 2^exposure (or 1); band[y] = sin(e_hi) - sin(e_lo) of the h-1

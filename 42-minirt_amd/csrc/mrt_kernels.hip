@@ -1903,6 +1903,57 @@ hipError_t launch_caustic_at(hipStream_t st, const PhotonMapDev &pm,
                            pts, out);
     return hipGetLastError();
 }
+// ---- batch merge ----------------------------------------------------------------
+// One scene's 4-wide nodes and records copied into the shared arrays at
+// node0 / prim0: inner links move by node0, leaf ranges by prim0, material
+// indices (record word 11, or a quadric's shading words 1-3) by mat0.
+__global__ void __launch_bounds__(256)
+k_rebase(float4 *__restrict__ nodes, uint32_t n_nodes, float4 *__restrict__ prims,
+         float *__restrict__ nrm, uint32_t n_prims, uint32_t node0, uint32_t prim0,
+         uint32_t mat0)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_nodes) {
+        float4 c = nodes[(size_t)i * 8 + 6];
+        float *cf = &c.x;
+        for (int j = 0; j < 4; j++) {
+            int32_t ch = __float_as_int(cf[j]);
+            if (ch >= 0) {
+                ch += (int32_t)node0;
+            } else {
+                uint32_t v = ~(uint32_t)ch;
+                v = (((v >> 3) + prim0) << 3) | (v & 7u);
+                ch = (int32_t)~v;
+            }
+            cf[j] = __int_as_float(ch);
+        }
+        nodes[(size_t)i * 8 + 6] = c;
+    }
+    if (i < n_prims) {
+        uint32_t kind = __float_as_uint(prims[(size_t)i * 3 + 1].w);
+        if (kind >= MRT_PRIMKIND_QUADRIC) {
+            float *a = nrm + (size_t)i * 9;
+            for (int j = 1; j <= 3; j++)
+                a[j] = __uint_as_float(__float_as_uint(a[j]) + mat0);
+        } else {
+            float4 &cw = prims[(size_t)i * 3 + 2];
+            cw.w = __uint_as_float(__float_as_uint(cw.w) + mat0);
+        }
+    }
+}
+
+hipError_t launch_rebase(hipStream_t st, float4 *nodes, uint32_t n_nodes,
+                         float4 *prims, float *nrm, uint32_t n_prims,
+                         uint32_t node0, uint32_t prim0, uint32_t mat0)
+{
+    uint32_t n = n_nodes > n_prims ? n_nodes : n_prims;
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_rebase, dim3((n + 255) / 256), dim3(256), 0, st, nodes,
+                       n_nodes, prims, nrm, n_prims, node0, prim0, mat0);
+    return hipGetLastError();
+}
+
 // ---- HDRI importance tables on the device -----------------------------------
 // The same double-precision operations, in the same order, as the host
 // builders (hdri_tables.c: make_importance_sampling_map's tables and the MIS

@@ -147,13 +147,24 @@ def batched_leg(M, args):
     log(f"batched: {len(scenes)} scenes, {sum(ntri)} triangles generated in "
         f"{time.perf_counter() - t0:.1f} s")
     samples = sum(d["w"] * d["h"] * d["spp"] for d in desc)
+    # commit with the device builder (timed, then dropped), then with the
+    # host SAH builder, whose trees the render below uses
+    t0 = time.perf_counter()
+    bd = M.Batch()
+    bd.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
+    for hs, d in zip(scenes, desc):
+        bd.add(hs, d["w"], d["h"], d["spp"], d["rng_seed"])
+    bd.commit()
+    t_commit_dev = time.perf_counter() - t0
+    bd.close()
+    log(f"batched: commit with device BVH builds {t_commit_dev:.2f} s")
     t0 = time.perf_counter()
     b = M.Batch()
     for hs, d in zip(scenes, desc):
         b.add(hs, d["w"], d["h"], d["spp"], d["rng_seed"])
     b.commit()
-    log(f"batched: commit (parallel BVH builds + upload) "
-        f"{time.perf_counter() - t0:.1f} s")
+    t_commit = time.perf_counter() - t0
+    log(f"batched: commit (parallel host BVH builds + upload) {t_commit:.2f} s")
     t0 = time.perf_counter()
     tuning = b.autotune(args.depth)
     log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> {tuning}")
@@ -214,6 +225,8 @@ def batched_leg(M, args):
         "speedup_vs_plain_per_tick": round(t_tick / t_batch, 3),
         "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
         "plain_per_tick_s": round(t_tick, 3),
+        "commit_s": round(t_commit, 2),
+        "commit_device_builder_s": round(t_commit_dev, 2),
         "tuning": tuning,
     }
 
@@ -479,7 +492,16 @@ def main():
     r = M.Renderer(W, H)
     t0 = time.perf_counter()
     r.upload(hs)
-    log(f"upload (BVH build + copy): {time.perf_counter() - t0:.1f} s")
+    t_up_host = time.perf_counter() - t0
+    log(f"upload (host BVH build + copy): {t_up_host:.2f} s")
+    with M.Renderer(W, H) as rd:
+        rd.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
+        rd.upload(hs)                          # warm-up (code objects)
+        t0 = time.perf_counter()
+        rd.upload(hs)
+        t_up_dev = time.perf_counter() - t0
+        dev_nodes = rd.scene_bvh().n_nodes4
+    log(f"upload (device BVH build): {t_up_dev:.3f} s, {dev_nodes} 4-wide nodes")
     r.set_seed(0x5EED)
     spp = args.spp_per_step
     r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
@@ -554,6 +576,9 @@ def main():
                      "traffic_source": traffic_src,
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4)},
+        "bvh_build": {"host_sah_upload_s": round(t_up_host, 3),
+                      "device_ploc_upload_s": round(t_up_dev, 4),
+                      "device_nodes4": dev_nodes},
     }
     extra = not (args.no_batched and args.no_adjoint and args.no_caustics)
     if extra:

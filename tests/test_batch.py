@@ -153,3 +153,24 @@ def test_batch_counting_mode(scenes):
     assert st["rays_closest"] >= st["paths"]
     assert st["nodes_closest"] > 0 and st["ms_total"] > 0
     b.close()
+
+
+@pytest.mark.gpu
+def test_device_built_batch_matches_device_built_plain_renders(scenes):
+    """With MRT_BUILDER_DEVICE every scene of the batch gets the tree a plain
+    context builds for it on the device (the build is deterministic)."""
+    b = M.Batch()
+    b.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
+    for hs, w, h, spp, seed in scenes:
+        b.add(hs, w, h, spp, seed)
+    b.commit()
+    b.render(DEPTH)
+    for k, (hs, w, h, spp, seed) in enumerate(scenes):
+        with M.Renderer(w, h) as r:
+            r.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
+            r.upload(hs)
+            r.set_seed(seed)
+            r.render(0, spp, DEPTH)
+            ref = r.readback()
+        np.testing.assert_allclose(b.readback(k), ref, rtol=2e-5, atol=1e-6)
+    b.close()
