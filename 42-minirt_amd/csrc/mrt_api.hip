@@ -26,7 +26,9 @@ struct mrt_ctx {
     uint32_t     W = 0, H = 0;
     char         err[512] = {0};
     int          num_cu = 0;
-    uint32_t     trace_blocks = 0, shadow_blocks = 0, grid_blocks = 0;
+    uint32_t     trace_blocks = 0, grid_blocks = 0;
+    uint32_t    *d_ovf = nullptr;    // trace-stack overflow columns
+    uint32_t     ovf_blocks = 0;     // trace grid size d_ovf covers
 
     // scene
     float4      *d_nodes = nullptr, *d_prims = nullptr, *d_planes = nullptr;
@@ -93,6 +95,7 @@ struct mrt_ctx {
     // statistics
     uint64_t     paths = 0;
     double       ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint64_t     launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per launch class
 };
 
 static int fail(mrt_ctx *c, int code, const char *fmt, ...)
@@ -144,6 +147,23 @@ static hipError_t dalloc(T *&p, size_t count)
         count = 1;
     return hipMalloc((void **)&p, count * sizeof(T));
 }
+
+// Overflow columns for persistent trace grids of up to `blocks` blocks.
+static hipError_t ensure_ovf(mrt_ctx *c, uint32_t blocks)
+{
+    if (blocks <= c->ovf_blocks && c->d_ovf)
+        return hipSuccess;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess)
+        return e;
+    c->ovf_blocks = 0;
+    e = dalloc(c->d_ovf, (size_t)TRACE_OVF * blocks * TRACE_BLOCK);
+    if (e == hipSuccess)
+        c->ovf_blocks = blocks;
+    return e;
+}
+
+static StackOvf stack_ovf(const mrt_ctx *c) { return StackOvf{c->d_ovf, c->ovf_blocks}; }
 
 extern "C" const char *mrt_version(void) { return "mrt_hip 0.1 gfx950"; }
 
@@ -201,9 +221,13 @@ extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
             rc = MRT_E_HIP;
             break;
         }
-        c->trace_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu(false));
-        c->shadow_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu(true));
+        c->trace_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu());
         c->grid_blocks = (uint32_t)c->num_cu * 8u;
+        if (ensure_ovf(c, c->trace_blocks)
+                != hipSuccess) {
+            rc = MRT_E_NOMEM;
+            break;
+        }
     } while (0);
     if (rc != MRT_OK) {
         mrt_destroy(c);
@@ -276,6 +300,7 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     dfree(c->d_accum);
     dfree(c->d_ctr);
     dfree(c->d_stats);
+    dfree(c->d_ovf);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1079,7 +1104,8 @@ static int ensure_wavefront(mrt_ctx *c, uint32_t paths, uint32_t shadow)
 }
 
 namespace {
-enum { K_RAYGEN = 0, K_TRACE, K_SHADE, K_SHADOW, K_ACCUM, K_RESOLVE, K_CAUSTIC };
+enum { K_RAYGEN = 0, K_TRACE, K_SHADE, K_SHADOW, K_ACCUM, K_RESOLVE, K_CAUSTIC,
+       K_ADVANCE };
 struct EvPair {
     int         cls;
     hipEvent_t  a, b;
@@ -1111,7 +1137,7 @@ struct Timer {
         for (EvPair &p : ev) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-                int slot = p.cls == K_RESOLVE ? K_SHADE
+                int slot = (p.cls == K_RESOLVE || p.cls == K_ADVANCE) ? K_SHADE
                            : (p.cls == K_CAUSTIC ? 6 : p.cls);
                 c->ms[slot] += ms;
             }
@@ -1125,6 +1151,7 @@ struct Timer {
 
 #define LAUNCH(ctx, tm, cls, expr)                                           \
     do {                                                                     \
+        (ctx)->launches[cls]++;                                              \
         (tm).begin(cls);                                                     \
         hipError_t e_ = (expr);                                              \
         (tm).end();                                                          \
@@ -1178,8 +1205,41 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
         LAUNCH(c, tm, K_RAYGEN,
                launch_raygen(c->stream, fp, c->q_o[0], c->q_d[0], c->q_t[0],
                              c->acc, P, c->d_ctr));
-        for (uint32_t b = 0; b < max_depth; b++) {
-            int cur = b & 1, nxt = cur ^ 1;
+        // bounce b: its closest-hit rays and the shadow rays of bounce b - 1
+        // in one launch, then the light sums of bounce b - 1, then the
+        // shade of bounce b; a last launch takes the shadow rays of the
+        // last bounce.
+        const bool lit = !c->lights.empty() || c->has_env || c->has_photons;
+        const uint32_t gb = std::min((P + SHADE_BLOCK - 1) / SHADE_BLOCK,
+                                     c->grid_blocks);
+        ShadeBufs Bp;
+        FrameParams fpp = fp;
+        memset(&Bp, 0, sizeof(Bp));
+        for (uint32_t b = 0; b <= max_depth; b++) {
+            const bool last = b == max_depth;
+            const int cur = b & 1, nxt = cur ^ 1;
+            TraceWork w;
+            w.q_o = c->q_o[cur];
+            w.q_d = c->q_d[cur];
+            w.hit = c->hit;
+            w.s_o = c->s_o;
+            w.s_d = c->s_d;
+            w.vis = c->vis;
+            w.cap_shadow = c->cap_shadow;
+            w.closest = !last;
+            w.shadow_ctr = (b > 0 && lit) ? ctr_shadow(b - 1) : -1;
+            if (w.closest || w.shadow_ctr >= 0)
+                LAUNCH(c, tm, K_TRACE,
+                       launch_trace(c->stream, c->trace_blocks, count, S,
+                                    stack_ovf(c), w, c->d_ctr, c->d_stats));
+            if (b > 0 && lit)
+                LAUNCH(c, tm, K_RESOLVE,
+                       launch_resolve(c->stream, gb, fpp, Bp, c->d_ctr));
+            if (last) {
+                LAUNCH(c, tm, K_ADVANCE, launch_advance(c->stream, c->d_ctr,
+                                                      c->d_stats, c->cap_paths, b));
+                break;
+            }
             fp.bounce = b;
             fp.last_bounce = (b + 1 == max_depth) ? 1u : 0u;
             ShadeBufs B;
@@ -1209,27 +1269,14 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
             B.p_k = c->p_k;
             B.cap_paths = c->cap_paths;
             B.cap_shadow = c->cap_shadow;
-            uint32_t gb = (P + SHADE_BLOCK - 1) / SHADE_BLOCK;
-            if (gb > c->grid_blocks)
-                gb = c->grid_blocks;
-            LAUNCH(c, tm, K_TRACE,
-                   launch_trace(c->stream, c->trace_blocks, count, S,
-                                c->q_o[cur], c->q_d[cur], c->hit, c->d_ctr,
-                                c->d_stats));
             LAUNCH(c, tm, K_SHADE, launch_shade(c->stream, gb, fp, B, c->d_ctr));
             if (c->has_photons)
                 LAUNCH(c, tm, K_CAUSTIC,
                        launch_caustic(c->stream, c->grid_blocks, fp, B, c->d_ctr));
-            if (!c->lights.empty() || c->has_env || c->has_photons) {
-                LAUNCH(c, tm, K_SHADOW,
-                       launch_shadow(c->stream, c->shadow_blocks, count, S,
-                                     c->s_o, c->s_d, c->vis, c->d_ctr,
-                                     c->d_stats, c->cap_shadow));
-                LAUNCH(c, tm, K_RESOLVE,
-                       launch_resolve(c->stream, gb, fp, B, c->d_ctr));
-            }
-            LAUNCH(c, tm, K_SHADE, launch_advance(c->stream, c->d_ctr, c->d_stats,
-                                  c->cap_paths));
+            LAUNCH(c, tm, K_ADVANCE, launch_advance(c->stream, c->d_ctr, c->d_stats,
+                                                  c->cap_paths, b));
+            Bp = B;
+            fpp = fp;
         }
         LAUNCH(c, tm, K_ACCUM,
                launch_accumulate(c->stream, c->acc, c->d_accum, WH, n,
@@ -1422,6 +1469,9 @@ extern "C" int mrt_get_stats(mrt_ctx *c, mrt_stats *out)
     out->nodes_shadow = st[STAT_NODES_SHADOW];
     out->prims_shadow = st[STAT_PRIMS_SHADOW];
     out->nonfinite = st[STAT_NONFINITE];
+    out->launches_trace = c->launches[K_TRACE];
+    out->launches_shadow = c->launches[K_SHADOW];
+    out->launches_shade = c->launches[K_SHADE];
     out->ms_raygen = c->ms[K_RAYGEN];
     out->ms_trace = c->ms[K_TRACE];
     out->ms_shade = c->ms[K_SHADE];
@@ -1441,6 +1491,8 @@ extern "C" int mrt_reset_stats(mrt_ctx *c)
     c->paths = 0;
     for (double &m : c->ms)
         m = 0.0;
+    for (uint64_t &l : c->launches)
+        l = 0;
     return MRT_OK;
 }
 
@@ -1976,6 +2028,9 @@ static int batch_buffers(mrt_batch *b, uint32_t P)
     int rc = ensure_wavefront(c, P, (uint32_t)S64);
     if (rc != MRT_OK)
         return rc;
+    if (ensure_ovf(c, (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu)
+        != hipSuccess)
+        return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
     if (P > b->cap_slot) {
         if (dalloc(b->d_slot, P) != hipSuccess) {
             b->cap_slot = 0;
@@ -2027,6 +2082,7 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
     hipError_t e;
 #define BSTEP(cls, expr)                                                     \
     do {                                                                     \
+        c->launches[cls]++;                                                  \
         if (tm)                                                              \
             tm->begin(cls);                                                  \
         e = (expr);                                                          \
@@ -2037,8 +2093,33 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
     } while (0)
     BSTEP(K_RAYGEN, launch_braygen(c->stream, B, b->d_ctl, c->q_o[0], c->q_d[0],
                                    c->q_t[0], P, c->d_ctr));
-    for (uint32_t k = 0; k < depth; k++) {
-        int cur = k & 1, nxt = cur ^ 1;
+    // the order of the plain render: the closest-hit rays of bounce k with
+    // the shadow rays of bounce k - 1, the light sums of k - 1, the shade of k
+    ShadeBufs Bp = B;
+    FrameParams fpp = fp;
+    for (uint32_t k = 0; k <= depth; k++) {
+        const bool last = k == depth;
+        const int cur = k & 1, nxt = cur ^ 1;
+        TraceWork w;
+        w.q_o = c->q_o[cur];
+        w.q_d = c->q_d[cur];
+        w.hit = c->hit;
+        w.s_o = c->s_o;
+        w.s_d = c->s_d;
+        w.vis = c->vis;
+        w.cap_shadow = c->cap_shadow;
+        w.closest = !last;
+        w.shadow_ctr = (k > 0 && b->has_lights) ? ctr_shadow(k - 1) : -1;
+        if (w.closest || w.shadow_ctr >= 0)
+            BSTEP(K_TRACE, launch_btrace(c->stream, tb, count, S, stack_ovf(c), B,
+                                         w, c->d_ctr, c->d_stats));
+        if (k > 0 && b->has_lights)
+            BSTEP(K_RESOLVE, launch_bresolve(c->stream, gb, fpp, Bp, c->d_ctr));
+        if (last) {
+            BSTEP(K_ADVANCE, launch_advance(c->stream, c->d_ctr, c->d_stats,
+                                            c->cap_paths, k));
+            break;
+        }
         fp.bounce = k;
         fp.last_bounce = (k + 1 == depth) ? 1u : 0u;
         B.q_o = c->q_o[cur];
@@ -2048,16 +2129,11 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
         B.qn_o = c->q_o[nxt];
         B.qn_d = c->q_d[nxt];
         B.qn_t = c->q_t[nxt];
-        BSTEP(K_TRACE, launch_btrace(c->stream, tb, count, S, B, c->q_o[cur],
-                                     c->q_d[cur], c->hit, c->d_ctr, c->d_stats));
         BSTEP(K_SHADE, launch_bshade(c->stream, gb, fp, B, c->d_ctr));
-        if (b->has_lights) {
-            BSTEP(K_SHADOW, launch_bshadow(c->stream, tb, count, S, B, c->s_o,
-                                           c->s_d, c->vis, c->d_ctr, c->d_stats,
-                                           c->cap_shadow));
-            BSTEP(K_RESOLVE, launch_bresolve(c->stream, gb, fp, B, c->d_ctr));
-        }
-        BSTEP(K_SHADE, launch_advance(c->stream, c->d_ctr, c->d_stats, c->cap_paths));
+        BSTEP(K_ADVANCE, launch_advance(c->stream, c->d_ctr, c->d_stats,
+                                        c->cap_paths, k));
+        Bp = B;
+        fpp = fp;
     }
     BSTEP(K_ACCUM, launch_baccumulate(c->stream, B, b->d_ctl, b->d_accum, P,
                                       c->d_ctr, c->d_stats));

@@ -15,27 +15,56 @@ namespace mrt {
 constexpr int TRACE_BLOCK = MRT_TRACE_BLOCK;  // threads per trace block
 static_assert(TRACE_BLOCK % 64 == 0, "trace blocks are whole waves");
 constexpr int TRACE_SS = MRT_TRACE_SS;        // stack entries kept in LDS
+constexpr int TRACE_OVF = STACK4_MAX - TRACE_SS;  // the rest, per lane
 
-// device counters (uint32)
-enum : int {
-    CTR_CUR = 0,          // paths in the current queue
-    CTR_NEXT = 1,         // paths appended to the next queue
-    CTR_SHADOW = 2,       // shadow rays appended this bounce
-    CTR_PEND = 3,         // pending light sums this bounce
-    CTR_PASS = 4,         // paths of the current pass (batches)
-    CTR_WORK_TRACE = 16,  // 8 work cursors of k_trace (one per XCD part)
-    CTR_WORK_SHADOW = 24, // 8 work cursors of k_shadow
-    CTR_COUNT = 32
+// Device buffer for the traversal stack entries beyond the LDS part, one
+// column per lane of a persistent trace grid of up to `blocks` blocks
+// (TRACE_OVF * blocks * TRACE_BLOCK uint32).  With it the trace kernels use
+// no scratch memory, whose set-up the runtime repeats per launch when the
+// whole grid's scratch is large.
+struct StackOvf {
+    uint32_t *buf;
+    uint32_t  blocks;
 };
 
-#ifndef MRT_NPART
-#define MRT_NPART 1
+// Work distribution of the persistent trace kernels.  The items form
+// chunks of CHUNK; wave w of the grid takes chunk w first, without an
+// atomic, and the chunks beyond the grid's waves form NSET interleaved sets
+// (the k-th chunk of set s is waves + s + NSET k), each claimed through its
+// own cursor.  All waves thus stay on one narrow, coherent window of the
+// queue (neighbouring pixels share BVH nodes in every L2), while atomics on
+// one address, which the memory system serialises (~12 ns each on MI355X,
+// 79 us for one per wave of a 6656-wave grid), are split 8 ways and skipped
+// where the static chunks cover the queue.
+constexpr uint32_t CHUNK = 64;     // items per chunk: one per lane
+#ifndef MRT_NSET
+#define MRT_NSET 8
 #endif
-// Work parts of the persistent trace kernels.  One global cursor keeps the
-// whole GPU on a narrow, coherent window of the queue (neighbouring pixels
-// share BVH nodes in every L2); per-XCD parts measured slower.
-constexpr int NPART = MRT_NPART;
-constexpr uint32_t CHUNK = 64;     // items a wave takes per atomic
+constexpr int NSET = MRT_NSET;
+
+// device counters (uint32), each on its own 128-byte line so that atomics
+// on one never queue behind atomics on another.  The shadow-ray and
+// pending-sum counts come in two copies indexed by the bounce's parity: the
+// shadow rays of bounce b are traced and resolved while bounce b + 1 runs.
+constexpr int CTR_LINE = 32;
+enum : int {
+    CTR_CUR = 0,                   // paths in the current queue
+    CTR_NEXT = 1 * CTR_LINE,       // paths appended to the next queue
+    CTR_SHADOW = 2 * CTR_LINE,     // shadow rays of an even / odd bounce
+    CTR_PEND = 4 * CTR_LINE,       // pending light sums of an even / odd bounce
+    CTR_PASS = 6 * CTR_LINE,       // paths of the current pass (batches)
+    CTR_SETS = 7 * CTR_LINE,       // NSET work cursors of k_trace
+    CTR_COUNT = (7 + NSET) * CTR_LINE
+};
+__host__ __device__ constexpr int ctr_shadow(uint32_t bounce)
+{
+    return CTR_SHADOW + (int)(bounce & 1u) * CTR_LINE;
+}
+__host__ __device__ constexpr int ctr_pend(uint32_t bounce)
+{
+    return CTR_PEND + (int)(bounce & 1u) * CTR_LINE;
+}
+
 constexpr int SHADE_BLOCK = 512;   // threads per block of k_shade
 
 // device statistics (uint64)
@@ -165,21 +194,32 @@ hipError_t launch_env_tables(hipStream_t st, const float4 *tex, uint32_t w,
 hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
                          float4 *q_d, float4 *q_t, float4 *acc, uint32_t P,
                          uint32_t *ctr);
+// One launch of k_trace: the closest-hit rays of the current queue (when
+// closest) and the shadow rays counted at ctr[shadow_ctr] (when >= 0).
+struct TraceWork {
+    const float4 *q_o, *q_d;      // closest-hit rays: origin | ao, dir | slot
+    float4       *hit;            // (t, code, u, v) per closest-hit ray
+    const float4 *s_o, *s_d;      // shadow rays: origin | tmax, dir | tag
+    uint8_t      *vis;            // 1: the shadow ray reaches its light
+    uint32_t      cap_shadow;     // shadow queue capacity
+    int           shadow_ctr;     // counter of the shadow rays, or -1
+    bool          closest;        // trace ctr[CTR_CUR] closest-hit rays
+};
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
-                        const SceneDev &S, const float4 *q_o,
-                        const float4 *q_d, float4 *hit, uint32_t *ctr,
+                        const SceneDev &S, const StackOvf &ovf,
+                        const TraceWork &w, uint32_t *ctr,
                         unsigned long long *stats);
-hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
-                         const SceneDev &S, const float4 *s_o,
-                         const float4 *s_d, uint8_t *vis, uint32_t *ctr,
-                         unsigned long long *stats, uint32_t cap);
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr);
 hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
                           uint32_t *ctr);
+// After the shade of bounce b: the next queue becomes current, and the
+// shadow / pending counts of bounce b - 1 (resolved by now) are added to
+// the statistics and cleared for bounce b + 1.
 hipError_t launch_advance(hipStream_t st, uint32_t *ctr,
-                          unsigned long long *stats, uint32_t cap);
+                          unsigned long long *stats, uint32_t cap,
+                          uint32_t bounce);
 hipError_t launch_accumulate(hipStream_t st, const float4 *acc, float *accum,
                              uint32_t WH, uint32_t nb,
                              unsigned long long *stats);
@@ -190,7 +230,7 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                              const float4 *prims, uint32_t n,
                              const float *orig, const float *dir,
                              uint32_t *prim_id, float *t);
-int trace_blocks_per_cu(bool shadow);
+int trace_blocks_per_cu();
 
 // Path-replay adjoint: dL/dI per pixel in, gradients out.
 constexpr int ADJ_MAXD = 8;      // deepest path the adjoint replays
@@ -212,14 +252,9 @@ hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
                           const BatchCtl *ctl, float4 *q_o, float4 *q_d,
                           float4 *q_t, uint32_t P, uint32_t *ctr);
 hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
-                         const SceneDev &S, const ShadeBufs &B,
-                         const float4 *q_o, const float4 *q_d, float4 *hit,
+                         const SceneDev &S, const StackOvf &ovf,
+                         const ShadeBufs &B, const TraceWork &w,
                          uint32_t *ctr, unsigned long long *stats);
-hipError_t launch_bshadow(hipStream_t st, uint32_t blocks, bool count,
-                          const SceneDev &S, const ShadeBufs &B,
-                          const float4 *s_o, const float4 *s_d, uint8_t *vis,
-                          uint32_t *ctr, unsigned long long *stats,
-                          uint32_t cap);
 hipError_t launch_bshade(hipStream_t st, uint32_t blocks,
                          const FrameParams &fp, const ShadeBufs &B,
                          uint32_t *ctr);

@@ -90,33 +90,56 @@ __device__ __forceinline__ void block_alloc3(uint32_t *ca, uint32_t *cb,
     __syncthreads();
 }
 
-// Work distribution of the persistent trace kernels: the n items form
-// NPART contiguous parts (NPART = 1: a single global cursor; NPART = 8: one
-// part per XCD, as consecutive blocks go to consecutive XCDs); a wave takes
-// CHUNK items of its own part per atomic and moves on to the next part once
-// its own is used up.  Returns false when all are done.
-__device__ __forceinline__ bool next_chunk(uint32_t *cursors, uint32_t n,
-                                           uint32_t &part, uint32_t &tries,
-                                           uint32_t &begin, uint32_t &end)
+// Next chunk of a persistent trace kernel (the scheme is described with
+// CHUNK): nch chunks in all, W = waves in the grid; set / tries keep the
+// wave's place among the NSET cursors at `sets` (one per CTR_LINE).  The
+// wave's own set is claimed from directly; the others are read first, so a
+// used-up set costs a load, not an atomic.  Returns false when all chunks
+// are taken.  Wave-uniform: every lane gets lane 0's answer.
+__device__ __forceinline__ bool next_chunk(uint32_t *sets, uint32_t nch,
+                                           uint32_t W, uint32_t &set,
+                                           uint32_t &tries, uint32_t &chunk)
 {
-    while (tries < (uint32_t)NPART) {
-        uint32_t p0 = (uint32_t)((uint64_t)n * part / NPART);
-        uint32_t p1 = (uint32_t)((uint64_t)n * (part + 1) / NPART);
-        uint32_t b = 0;
-        if (p0 < p1) {
-            if (lane_id() == 0)
-                b = atomicAdd(&cursors[part], CHUNK);
-            b = __shfl(b, 0, 64);
-            if (b < p1 - p0) {
-                begin = p0 + b;
-                end = min(begin + CHUNK, p1);
-                return true;
-            }
+    if (nch <= W)
+        return false;
+    const uint32_t nd = nch - W;
+    while (tries < (uint32_t)NSET) {
+        const uint32_t cnt = set < nd ? (nd - set + NSET - 1) / NSET : 0u;
+        uint32_t *cur = sets + set * CTR_LINE;
+        uint32_t k = cnt;
+        if (lane_id() == 0 && cnt) {
+            uint32_t seen = tries == 0 ? 0u
+                : __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            k = seen < cnt ? atomicAdd(cur, 1u) : seen;
         }
-        part = part + 1 == (uint32_t)NPART ? 0 : part + 1;
+        k = __shfl(k, 0, 64);
+        if (k < cnt) {
+            chunk = W + set + NSET * k;
+            return true;
+        }
+        set = set + 1 == (uint32_t)NSET ? 0u : set + 1;
         tries++;
     }
     return false;
+}
+
+// Resets the work cursors of k_trace (one thread).
+__device__ __forceinline__ void reset_sets(uint32_t *ctr)
+{
+    for (int k = 0; k < NSET; k++)
+        ctr[CTR_SETS + k * CTR_LINE] = 0;
+}
+
+// Clears every queue counter of a new pass (one thread).
+__device__ __forceinline__ void reset_queues(uint32_t *ctr, uint32_t paths)
+{
+    ctr[CTR_CUR] = paths;
+    ctr[CTR_NEXT] = 0;
+    for (uint32_t p = 0; p < 2; p++) {
+        ctr[ctr_shadow(p)] = 0;
+        ctr[ctr_pend(p)] = 0;
+    }
+    reset_sets(ctr);
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long *dst, uint32_t v)
@@ -172,16 +195,8 @@ k_raygen(FrameParams fp, float4 *__restrict__ q_o, float4 *__restrict__ q_d,
          uint32_t *__restrict__ ctr)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        ctr[CTR_CUR] = P;
-        ctr[CTR_NEXT] = 0;
-        ctr[CTR_SHADOW] = 0;
-        ctr[CTR_PEND] = 0;
-        for (int k = 0; k < NPART; k++) {
-            ctr[CTR_WORK_TRACE + k] = 0;
-            ctr[CTR_WORK_SHADOW + k] = 0;
-        }
-    }
+    if (i == 0)
+        reset_queues(ctr, P);
     if (i >= P)
         return;
     uint32_t pixel = i % fp.WH, sample = fp.sample0 + i / fp.WH;
@@ -213,88 +228,81 @@ __device__ __forceinline__ SceneDev scene_of(const SceneDev &S,
 __device__ __forceinline__ uint32_t tag_scene(uint32_t tag) { return (tag >> 18) & 0xFFFu; }
 __device__ __forceinline__ uint32_t tag_light(uint32_t tag) { return (tag >> 12) & 63u; }
 
-// ---- closest-hit trace (persistent, wave-granular work stealing) ----------
+// ---- closest-hit and shadow rays (persistent, wave-granular) ----------------
+// One launch traces the closest-hit rays of the current bounce and the
+// shadow rays of the previous one: the two sets are independent, and in one
+// grid the longest rays of one overlap the work of the other (a launch of
+// its own waits for its longest ray: ~300 traversal steps late in a path,
+// whatever the number of rays).  Chunks [0, ncc) hold closest-hit rays,
+// [ncc, ncc + ncs) shadow rays; a chunk is one kind, so the branch is
+// wave-uniform.
 template <bool COUNT, bool BATCH, bool EXT>
 __global__ void __launch_bounds__(TRACE_BLOCK)
-k_trace(SceneDev S, const float4 *__restrict__ q_o,
-        const float4 *__restrict__ q_d, float4 *__restrict__ hit,
-        uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats,
-        const BatchScene *__restrict__ bs, const uint2 *__restrict__ slot_info)
+k_trace(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
+        unsigned long long *__restrict__ stats,
+        const BatchScene *__restrict__ bs, const uint2 *__restrict__ slot_info,
+        uint32_t *__restrict__ ovf_buf)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
-    const uint32_t n = ctr[CTR_CUR];
+    const uint32_t nc = w.closest ? ctr[CTR_CUR] : 0u;
+    const uint32_t ns = w.shadow_ctr >= 0 ? min(ctr[w.shadow_ctr], w.cap_shadow) : 0u;
     uint32_t *lc = s_code + threadIdx.x;
-    float *lt = s_t + threadIdx.x;
-    uint32_t nodes = 0, prims = 0;
-    uint32_t part = blockIdx.x % NPART, tries = 0, cb, ce;
-    while (next_chunk(&ctr[CTR_WORK_TRACE], n, part, tries, cb, ce)) {
-        for (uint32_t base = cb; base < ce; base += 64) {
-            uint32_t i = base + lane_id();
-            if (i < ce) {
-                float4 o4 = q_o[i], d4 = q_d[i];
+    uint32_t *ovf = ovf_buf + blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint32_t os = gridDim.x * TRACE_BLOCK;
+    uint32_t nodes_c = 0, prims_c = 0, nodes_s = 0, prims_s = 0;
+    const uint32_t W = gridDim.x * (TRACE_BLOCK / 64);
+    const uint32_t ncc = nc / CHUNK + (nc % CHUNK != 0);
+    const uint32_t nch = ncc + ns / CHUNK + (ns % CHUNK != 0);
+    uint32_t chunk = blockIdx.x * (TRACE_BLOCK / 64) + (threadIdx.x >> 6);
+    uint32_t set = blockIdx.x % NSET, tries = 0;
+    for (bool more = chunk < nch; more;
+         more = next_chunk(&ctr[CTR_SETS], nch, W, set, tries, chunk)) {
+        if (chunk < ncc) {
+            const uint32_t i = chunk * CHUNK + lane_id();
+            if (i < nc) {
+                float4 o4 = w.q_o[i], d4 = w.q_d[i];
                 float t, bu, bv;
                 uint32_t code;
                 if (BATCH) {
                     const SceneDev Si =
                         scene_of(S, bs, slot_info[__float_as_uint(d4.w)].x);
-                    closest_hit<COUNT, TRACE_SS, EXT>(Si, xyz(o4), xyz(d4), lc, lt,
-                                                 TRACE_BLOCK, t, code, bu, bv,
-                                                 nodes, prims);
+                    closest_hit<COUNT, TRACE_SS, EXT>(Si, xyz(o4), xyz(d4), lc,
+                                                      TRACE_BLOCK, ovf, os, t,
+                                                      code, bu, bv, nodes_c,
+                                                      prims_c);
                 } else {
-                    closest_hit<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), lc, lt,
-                                                 TRACE_BLOCK, t, code, bu, bv,
-                                                 nodes, prims);
+                    closest_hit<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), lc,
+                                                      TRACE_BLOCK, ovf, os, t,
+                                                      code, bu, bv, nodes_c,
+                                                      prims_c);
                 }
-                hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
+                w.hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
             }
-        }
-    }
-    if (COUNT) {
-        wave_add_u64(&stats[STAT_NODES_CLOSEST], nodes);
-        wave_add_u64(&stats[STAT_PRIMS_CLOSEST], prims);
-    }
-}
-
-// ---- shadow / visibility trace --------------------------------------------
-template <bool COUNT, bool BATCH, bool EXT>
-__global__ void __launch_bounds__(TRACE_BLOCK)
-k_shadow(SceneDev S, const float4 *__restrict__ s_o,
-         const float4 *__restrict__ s_d, uint8_t *__restrict__ vis,
-         uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats,
-         uint32_t cap, const BatchScene *__restrict__ bs)
-{
-    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
-    const uint32_t n = min(ctr[CTR_SHADOW], cap);
-    uint32_t *lc = s_code + threadIdx.x;
-    float *lt = s_t + threadIdx.x;
-    uint32_t nodes = 0, prims = 0;
-    uint32_t part = blockIdx.x % NPART, tries = 0, cb, ce;
-    while (next_chunk(&ctr[CTR_WORK_SHADOW], n, part, tries, cb, ce)) {
-        for (uint32_t base = cb; base < ce; base += 64) {
-            uint32_t i = base + lane_id();
-            if (i < ce) {
-                float4 o4 = s_o[i], d4 = s_d[i];
+        } else {
+            const uint32_t i = (chunk - ncc) * CHUNK + lane_id();
+            if (i < ns) {
+                float4 o4 = w.s_o[i], d4 = w.s_d[i];
                 bool occ;
                 if (BATCH) {
                     const SceneDev Si =
                         scene_of(S, bs, tag_scene(__float_as_uint(d4.w)));
                     occ = occluded<COUNT, TRACE_SS, EXT>(Si, xyz(o4), xyz(d4), o4.w,
-                                                    lc, lt, TRACE_BLOCK, nodes,
-                                                    prims);
+                                                         lc, TRACE_BLOCK, ovf, os,
+                                                         nodes_s, prims_s);
                 } else {
                     occ = occluded<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), o4.w,
-                                                    lc, lt, TRACE_BLOCK, nodes,
-                                                    prims);
+                                                         lc, TRACE_BLOCK, ovf, os,
+                                                         nodes_s, prims_s);
                 }
-                vis[i] = occ ? 0 : 1;
+                w.vis[i] = occ ? 0 : 1;
             }
         }
     }
     if (COUNT) {
-        wave_add_u64(&stats[STAT_NODES_SHADOW], nodes);
-        wave_add_u64(&stats[STAT_PRIMS_SHADOW], prims);
+        wave_add_u64(&stats[STAT_NODES_CLOSEST], nodes_c);
+        wave_add_u64(&stats[STAT_PRIMS_CLOSEST], prims_c);
+        wave_add_u64(&stats[STAT_NODES_SHADOW], nodes_s);
+        wave_add_u64(&stats[STAT_PRIMS_SHADOW], prims_s);
     }
 }
 
@@ -766,7 +774,8 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         }
         // allocations: every lane of the wave takes part
         uint32_t qn, j0, pr;
-        block_alloc3(&ctr[CTR_NEXT], &ctr[CTR_SHADOW], &ctr[CTR_PEND],
+        block_alloc3(&ctr[CTR_NEXT], &ctr[ctr_shadow(fp.bounce)],
+                     &ctr[ctr_pend(fp.bounce)],
                      want_next, n_rays, want_pend, qn, j0, pr);
         if (!active)
             continue;
@@ -856,7 +865,7 @@ template <bool BATCH>
 __global__ void __launch_bounds__(256)
 k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
-    const uint32_t n = min(ctr[CTR_PEND], B.cap_paths);
+    const uint32_t n = min(ctr[ctr_pend(fp.bounce)], B.cap_paths);
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
         float4 a = B.p_a[r];
@@ -933,19 +942,18 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 }
 
 __global__ void k_advance(uint32_t *__restrict__ ctr,
-                          unsigned long long *__restrict__ stats, uint32_t cap)
+                          unsigned long long *__restrict__ stats, uint32_t cap,
+                          uint32_t bounce)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t prev = bounce + 1;      // parity of bounce - 1
         stats[STAT_RAYS_CLOSEST] += ctr[CTR_CUR];
-        stats[STAT_RAYS_SHADOW] += ctr[CTR_SHADOW];
+        stats[STAT_RAYS_SHADOW] += ctr[ctr_shadow(prev)];
         ctr[CTR_CUR] = min(ctr[CTR_NEXT], cap);
         ctr[CTR_NEXT] = 0;
-        ctr[CTR_SHADOW] = 0;
-        ctr[CTR_PEND] = 0;
-        for (int k = 0; k < NPART; k++) {
-            ctr[CTR_WORK_TRACE + k] = 0;
-            ctr[CTR_WORK_SHADOW + k] = 0;
-        }
+        ctr[ctr_shadow(prev)] = 0;
+        ctr[ctr_pend(prev)] = 0;
+        reset_sets(ctr);
     }
 }
 
@@ -1112,7 +1120,7 @@ __device__ __forceinline__ bool scatter_adj(const RngKey &rk, float ior_all, V3 
 __device__ __forceinline__ V3 lights_exact(const FrameParams &fp, const SceneDev &S,
                                            const float4 *L, V3 p, V3 nrm,
                                            uint32_t pixel, uint32_t sample,
-                                           uint32_t bounce, uint32_t *lc, float *lt,
+                                           uint32_t bounce, uint32_t *lc, uint32_t *ovf,
                                            uint32_t &nodes, uint32_t &np)
 {
     float lr = 0.0f, lg = 0.0f, lb = 0.0f;
@@ -1149,8 +1157,8 @@ __device__ __forceinline__ V3 lights_exact(const FrameParams &fp, const SceneDev
                 sd = unit(dl);
                 tmax = sqrtf(dot(dl, dl));
             }
-            if (!occluded<false, TRACE_SS>(S, p, sd, tmax, lc, lt, TRACE_BLOCK, nodes,
-                                           np))
+            if (!occluded<false, TRACE_SS>(S, p, sd, tmax, lc, TRACE_BLOCK, ovf, 1u,
+                                           nodes, np))
                 nvis++;
         }
         if (!nvis)
@@ -1180,13 +1188,12 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
           uint32_t max_depth)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
     __shared__ float s_g[ADJ_MAXMAT * ADJ_NG];
     for (uint32_t k = threadIdx.x; k < ADJ_MAXMAT * ADJ_NG; k += blockDim.x)
         s_g[k] = 0.0f;
     __syncthreads();
     uint32_t *lc = s_code + threadIdx.x;
-    float *lt = s_t + threadIdx.x;
+    uint32_t ovf_l[TRACE_OVF];
     uint32_t nodes = 0, np = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P;
          i += gridDim.x * blockDim.x) {
@@ -1209,7 +1216,7 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
         for (uint32_t b = 0; b < max_depth; b++) {
             float t, bu, bv;
             uint32_t code;
-            if (!closest_hit<false, TRACE_SS>(S, o, d, lc, lt, TRACE_BLOCK, t, code,
+            if (!closest_hit<false, TRACE_SS>(S, o, d, lc, TRACE_BLOCK, ovf_l, 1u, t, code,
                                               bu, bv, nodes, np)) {
                 // add_skybox; the MIS weight after a diffuse bounce
                 if (fp.has_env) {
@@ -1248,7 +1255,7 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
                 df = v3(1.f, 1.f, 1.f);
                 if (fp.n_lights)
                     N = lights_exact(fp, S, B.lights, p, nrm, pixel, sample, b, lc,
-                                     lt, nodes, np);
+                                     ovf_l, nodes, np);
                 if (fp.has_env) {
                     U4 e = rng4(fp.seed, pixel, sample, b, 3u);
                     const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
@@ -1264,7 +1271,8 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
                     float cn = dot(nrm, ed);
                     if (pc > 0.0f && ce > 0.0f && cn > 0.0f
                         && !occluded<false, TRACE_SS>(S, p, ed, 2.0f * IT_MAX_F, lc,
-                                                      lt, TRACE_BLOCK, nodes, np)) {
+                                                      TRACE_BLOCK, ovf_l, 1u, nodes,
+                                                      np)) {
                         float pl = pc * (float)nw * (float)nh
                                    / (19.7392088021787172f * ce);
                         float wl = pl / (pl + cn / 3.14159265358979324f);
@@ -1377,15 +1385,8 @@ k_braygen(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
                        : (uint32_t)min((unsigned long long)P, total - base);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        ctr[CTR_CUR] = n;
+        reset_queues(ctr, n);
         ctr[CTR_PASS] = n;
-        ctr[CTR_NEXT] = 0;
-        ctr[CTR_SHADOW] = 0;
-        ctr[CTR_PEND] = 0;
-        for (int k = 0; k < NPART; k++) {
-            ctr[CTR_WORK_TRACE + k] = 0;
-            ctr[CTR_WORK_SHADOW + k] = 0;
-        }
     }
     if (i >= n)
         return;
@@ -1459,7 +1460,7 @@ k_aov(FrameParams fp, SceneDev S, const float4 *__restrict__ prims,
       void *__restrict__ out)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
+    uint32_t ovf_l[TRACE_OVF];
     uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
     if (px >= fp.WH)
         return;
@@ -1468,7 +1469,7 @@ k_aov(FrameParams fp, SceneDev S, const float4 *__restrict__ prims,
     float t, bu, bv;
     uint32_t code, nodes = 0, np = 0;
     bool hit = closest_hit<true, TRACE_SS>(S, o, d, s_code + threadIdx.x,
-                                           s_t + threadIdx.x, TRACE_BLOCK, t,
+                                           TRACE_BLOCK, ovf_l, 1u, t,
                                            code, bu, bv, nodes, np);
     switch (aov) {
     case MRT_AOV_PRIM_ID: {
@@ -1522,7 +1523,7 @@ k_trace_rays(SceneDev S, const float4 *__restrict__ prims, uint32_t n,
              uint32_t *__restrict__ prim_id, float *__restrict__ tout)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
+    uint32_t ovf_l[TRACE_OVF];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
@@ -1531,7 +1532,7 @@ k_trace_rays(SceneDev S, const float4 *__restrict__ prims, uint32_t n,
     float t, bu, bv;
     uint32_t code, nodes = 0, np = 0;
     bool hit = closest_hit<false, TRACE_SS>(S, o, d, s_code + threadIdx.x,
-                                            s_t + threadIdx.x, TRACE_BLOCK, t,
+                                            TRACE_BLOCK, ovf_l, 1u, t,
                                             code, bu, bv, nodes, np);
     uint32_t id = MRT_PRIM_NONE;
     if (hit)
@@ -1562,7 +1563,7 @@ k_photon_emit(FrameParams fp, SceneDev S, PhotonEmit pe, ShadeBufs B,
               uint32_t *__restrict__ counter, uint32_t cap)
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
-    __shared__ float s_t[POPCULL ? TRACE_SS * TRACE_BLOCK : 1];
+    uint32_t ovf_l[TRACE_OVF];
     uint32_t nodes = 0, np = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < pe.n_total;
@@ -1640,7 +1641,7 @@ k_photon_emit(FrameParams fp, SceneDev S, PhotonEmit pe, ShadeBufs B,
                 float t, bu, bv;
                 uint32_t code;
                 if (!closest_hit<false, TRACE_SS>(S, o, d, s_code + threadIdx.x,
-                                                  s_t + threadIdx.x, TRACE_BLOCK,
+                                                  TRACE_BLOCK, ovf_l, 1u,
                                                   t, code, bu, bv, nodes, np))
                     break;
                 V3 p, nrm;
@@ -1803,7 +1804,7 @@ k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
     __shared__ float s_d[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     __shared__ uint32_t s_i[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
-    const uint32_t n = min(ctr[CTR_PEND], B.cap_paths);
+    const uint32_t n = min(ctr[ctr_pend(fp.bounce)], B.cap_paths);
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
         V3 c = caustic_at<KNN>(fp.pm, xyz(B.p_x[r]), s_d + (KNN ? threadIdx.x : 0),
@@ -2149,30 +2150,24 @@ hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
         }                                                                     \
     } while (0)
 
+// The persistent trace grids take their stack overflow columns from ovf.
+static bool ovf_fits(const StackOvf &ovf, uint32_t blocks)
+{
+    return ovf.buf != nullptr && blocks <= ovf.blocks;
+}
+
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
-                        const SceneDev &S, const float4 *q_o,
-                        const float4 *q_d, float4 *hit, uint32_t *ctr,
+                        const SceneDev &S, const StackOvf &ovf,
+                        const TraceWork &w, uint32_t *ctr,
                         unsigned long long *stats)
 {
+    if (!ovf_fits(ovf, blocks))
+        return hipErrorInvalidValue;
     const BatchScene *nb = nullptr;
     const uint2 *ni = nullptr;
 #define MRT_GO(C, Q)                                                          \
     hipLaunchKernelGGL((k_trace<C, false, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
-                       0, st, S, q_o, q_d, hit, ctr, stats, nb, ni)
-    MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
-#undef MRT_GO
-    return hipGetLastError();
-}
-
-hipError_t launch_shadow(hipStream_t st, uint32_t blocks, bool count,
-                         const SceneDev &S, const float4 *s_o,
-                         const float4 *s_d, uint8_t *vis, uint32_t *ctr,
-                         unsigned long long *stats, uint32_t cap)
-{
-    const BatchScene *nb = nullptr;
-#define MRT_GO(C, Q)                                                          \
-    hipLaunchKernelGGL((k_shadow<C, false, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
-                       0, st, S, s_o, s_d, vis, ctr, stats, cap, nb)
+                       0, st, S, w, ctr, stats, nb, ni, ovf.buf)
     MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
 #undef MRT_GO
     return hipGetLastError();
@@ -2205,28 +2200,16 @@ hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
 }
 
 hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
-                         const SceneDev &S, const ShadeBufs &B,
-                         const float4 *q_o, const float4 *q_d, float4 *hit,
+                         const SceneDev &S, const StackOvf &ovf,
+                         const ShadeBufs &B, const TraceWork &w,
                          uint32_t *ctr, unsigned long long *stats)
 {
+    if (!ovf_fits(ovf, blocks))
+        return hipErrorInvalidValue;
 #define MRT_GO(C, Q)                                                          \
     hipLaunchKernelGGL((k_trace<C, true, Q>), dim3(blocks), dim3(TRACE_BLOCK),  \
-                       0, st, S, q_o, q_d, hit, ctr, stats, B.bs,              \
-                       (const uint2 *)B.slot_info)
-    MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
-#undef MRT_GO
-    return hipGetLastError();
-}
-
-hipError_t launch_bshadow(hipStream_t st, uint32_t blocks, bool count,
-                          const SceneDev &S, const ShadeBufs &B,
-                          const float4 *s_o, const float4 *s_d, uint8_t *vis,
-                          uint32_t *ctr, unsigned long long *stats,
-                          uint32_t cap)
-{
-#define MRT_GO(C, Q)                                                          \
-    hipLaunchKernelGGL((k_shadow<C, true, Q>), dim3(blocks), dim3(TRACE_BLOCK), \
-                       0, st, S, s_o, s_d, vis, ctr, stats, cap, B.bs)
+                       0, st, S, w, ctr, stats, B.bs,                          \
+                       (const uint2 *)B.slot_info, ovf.buf)
     MRT_DISPATCH2(count, (S.quad | S.alpha) != 0);
 #undef MRT_GO
     return hipGetLastError();
@@ -2267,9 +2250,11 @@ hipError_t launch_bnext(hipStream_t st, BatchCtl *ctl, uint32_t P)
 }
 
 hipError_t launch_advance(hipStream_t st, uint32_t *ctr,
-                          unsigned long long *stats, uint32_t cap)
+                          unsigned long long *stats, uint32_t cap,
+                          uint32_t bounce)
 {
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctr, stats, cap);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, ctr, stats, cap,
+                       bounce);
     return hipGetLastError();
 }
 
@@ -2303,15 +2288,12 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
     return hipGetLastError();
 }
 
-int trace_blocks_per_cu(bool shadow)
+int trace_blocks_per_cu()
 {
     int nb = 0;
-    hipError_t e = shadow
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, k_shadow<false, false, false>, TRACE_BLOCK, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, k_trace<false, false, false>, TRACE_BLOCK, 0);
-    if (e != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_trace<false, false, false>, TRACE_BLOCK, 0) != hipSuccess
+        || nb < 1)
         nb = 4;
     return nb;
 }

@@ -536,15 +536,16 @@ def main():
         r.render(spp * s, spp, args.depth)
     tm = r.stats()
     r.set_option(M.OPT_STATS, 0)
-    launches = prof_steps * args.depth
-    trace_ms = tm["ms_trace"] / launches
+    # k_trace launches: the closest-hit rays of a bounce together with the
+    # shadow rays of the bounce before (depth + 1 per render)
+    trace_ms = tm["ms_trace"] / max(tm["launches_trace"], 1)
     rays_c = cnt["rays_closest"]
     bytes_trace = (rays_c * (Q_RAY + Q_HIT) + cnt["nodes_closest"] * B_NODE
                    + cnt["prims_closest"] * B_PRIM)
     bytes_shadow = (cnt["rays_shadow"] * (Q_SRAY + Q_VIS)
                     + cnt["nodes_shadow"] * B_NODE
                     + cnt["prims_shadow"] * B_PRIM)
-    per_launch = bytes_trace / args.depth
+    per_launch = (bytes_trace + bytes_shadow) / max(cnt["launches_trace"], 1)
     achieved = per_launch / (trace_ms * 1e-3) / 1e9
     rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
         * value * 1e6
@@ -569,7 +570,9 @@ def main():
         "kernel_ms_per_step": {k: round(tm[f"ms_{k}"] / prof_steps, 3)
                                for k in ("raygen", "trace", "shade", "shadow",
                                          "accum", "total")},
-        "roofline": {"bound": "hbm", "kernel": "k_trace (closest hit)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_trace (closest-hit rays of a bounce + "
+                               "shadow rays of the bounce before)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -234,14 +234,17 @@ typedef struct mrt_stats {
     uint64_t prims_closest;    /* primitives tested by closest-hit rays       */
     uint64_t nodes_shadow;
     uint64_t prims_shadow;
-    uint64_t launches_trace;   /* closest-hit kernel launches                 */
-    uint64_t launches_shadow;
+    uint64_t launches_trace;   /* trace launches: the closest-hit rays of a
+                                  bounce with the shadow rays of the one
+                                  before                                      */
+    uint64_t launches_shadow;  /* shadow-only launches (0: shadow rays ride
+                                  in the trace launches)                      */
     uint64_t launches_shade;
     uint64_t nonfinite;        /* NaN/Inf samples zeroed                      */
     double   ms_raygen;        /* summed HIP-event times per kernel class     */
-    double   ms_trace;
-    double   ms_shade;
-    double   ms_shadow;
+    double   ms_trace;         /* trace launches (closest hit + shadow rays) */
+    double   ms_shade;         /* shade, light sums, queue advance           */
+    double   ms_shadow;        /* shadow-only launches                       */
     double   ms_accum;
     double   ms_total;         /* mrt_render wall time on the stream          */
     double   ms_caustic;       /* photon-map gather                           */
