@@ -20,6 +20,11 @@
 
 using namespace mrt;
 
+// Render lanes: a render's paths are split over up to this many streams.
+constexpr int MAX_LANES = 4;
+// Lanes get at least this many paths each (fewer lanes for small renders).
+constexpr uint32_t LANE_MIN_PATHS = 1u << 16;
+
 struct mrt_ctx {
     int          device = 0;
     hipStream_t  stream = nullptr;
@@ -27,8 +32,15 @@ struct mrt_ctx {
     char         err[512] = {0};
     int          num_cu = 0;
     uint32_t     trace_blocks = 0, grid_blocks = 0;
-    uint32_t    *d_ovf = nullptr;    // trace-stack overflow columns
-    uint32_t     ovf_blocks = 0;     // trace grid size d_ovf covers
+    uint32_t    *d_ovf = nullptr;    // trace-stack overflow columns, one
+                                     // set per render lane
+    uint32_t     ovf_blocks = 0;     // trace grid size each set covers
+    uint32_t     ovf_sets = 0;
+    // render lanes 1.. run on their own streams, forked from and joined to
+    // the context's stream (lane 0)
+    uint32_t     lanes = 2;          // MRT_OPT_LANES
+    hipStream_t  lane_stream[MAX_LANES] = {};
+    hipEvent_t   ev_fork = nullptr, ev_join[MAX_LANES] = {};
 
     // scene
     float4      *d_nodes = nullptr, *d_prims = nullptr, *d_planes = nullptr;
@@ -148,22 +160,34 @@ static hipError_t dalloc(T *&p, size_t count)
     return hipMalloc((void **)&p, count * sizeof(T));
 }
 
-// Overflow columns for persistent trace grids of up to `blocks` blocks.
-static hipError_t ensure_ovf(mrt_ctx *c, uint32_t blocks)
+// Overflow columns for persistent trace grids of up to `blocks` blocks, one
+// set for each of `sets` render lanes (their grids may run together).
+static hipError_t ensure_ovf(mrt_ctx *c, uint32_t blocks, uint32_t sets = 1)
 {
-    if (blocks <= c->ovf_blocks && c->d_ovf)
+    if (blocks <= c->ovf_blocks && sets <= c->ovf_sets && c->d_ovf)
         return hipSuccess;
+    blocks = std::max(blocks, c->ovf_blocks);
+    sets = std::max(sets, c->ovf_sets);
     hipError_t e = hipStreamSynchronize(c->stream);
+    for (int l = 1; l < MAX_LANES && e == hipSuccess; l++)
+        if (c->lane_stream[l])
+            e = hipStreamSynchronize(c->lane_stream[l]);
     if (e != hipSuccess)
         return e;
-    c->ovf_blocks = 0;
-    e = dalloc(c->d_ovf, (size_t)TRACE_OVF * blocks * TRACE_BLOCK);
-    if (e == hipSuccess)
+    c->ovf_blocks = c->ovf_sets = 0;
+    e = dalloc(c->d_ovf, (size_t)sets * TRACE_OVF * blocks * TRACE_BLOCK);
+    if (e == hipSuccess) {
         c->ovf_blocks = blocks;
+        c->ovf_sets = sets;
+    }
     return e;
 }
 
-static StackOvf stack_ovf(const mrt_ctx *c) { return StackOvf{c->d_ovf, c->ovf_blocks}; }
+static StackOvf stack_ovf(const mrt_ctx *c, uint32_t lane = 0)
+{
+    return StackOvf{c->d_ovf + (size_t)lane * TRACE_OVF * c->ovf_blocks * TRACE_BLOCK,
+                    c->ovf_blocks};
+}
 
 extern "C" const char *mrt_version(void) { return "mrt_hip 0.1 gfx950"; }
 
@@ -200,20 +224,30 @@ extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
     do {
         if (hipSetDevice(device) != hipSuccess
             || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)
+                   != hipSuccess
+            || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)
                    != hipSuccess) {
             rc = MRT_E_HIP;
             break;
         }
+        for (int l = 1; l < MAX_LANES && rc == MRT_OK; l++)
+            if (hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking)
+                    != hipSuccess
+                || hipEventCreateWithFlags(&c->ev_join[l], hipEventDisableTiming)
+                       != hipSuccess)
+                rc = MRT_E_HIP;
+        if (rc != MRT_OK)
+            break;
         size_t npx = (size_t)width * height;
         if (dalloc(c->d_accum, npx * 3) != hipSuccess
-            || dalloc(c->d_ctr, CTR_COUNT) != hipSuccess
+            || dalloc(c->d_ctr, MAX_LANES * CTR_COUNT) != hipSuccess
             || dalloc(c->d_stats, STAT_COUNT) != hipSuccess) {
             rc = MRT_E_NOMEM;
             break;
         }
         if (hipMemsetAsync(c->d_accum, 0, npx * 3 * sizeof(float), c->stream)
                 != hipSuccess
-            || hipMemsetAsync(c->d_ctr, 0, CTR_COUNT * sizeof(uint32_t),
+            || hipMemsetAsync(c->d_ctr, 0, MAX_LANES * CTR_COUNT * sizeof(uint32_t),
                               c->stream) != hipSuccess
             || hipMemsetAsync(c->d_stats, 0, STAT_COUNT * 8, c->stream)
                    != hipSuccess
@@ -301,6 +335,16 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     dfree(c->d_ctr);
     dfree(c->d_stats);
     dfree(c->d_ovf);
+    for (int l = 1; l < MAX_LANES; l++) {
+        if (c->lane_stream[l]) {
+            (void)hipStreamSynchronize(c->lane_stream[l]);
+            (void)hipStreamDestroy(c->lane_stream[l]);
+        }
+        if (c->ev_join[l])
+            (void)hipEventDestroy(c->ev_join[l]);
+    }
+    if (c->ev_fork)
+        (void)hipEventDestroy(c->ev_fork);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1110,10 +1154,14 @@ struct EvPair {
     int         cls;
     hipEvent_t  a, b;
 };
+// Per-launch HIP events, recorded on the stream the launches go to (st;
+// the context's stream unless a render lane sets its own).
 struct Timer {
     mrt_ctx              *c;
     bool                  on;
     std::vector<EvPair>   ev;
+    hipStream_t           st = nullptr;
+    hipStream_t stream() const { return st ? st : c->stream; }
     void begin(int cls)
     {
         if (!on)
@@ -1121,13 +1169,13 @@ struct Timer {
         EvPair p{cls, nullptr, nullptr};
         (void)hipEventCreate(&p.a);
         (void)hipEventCreate(&p.b);
-        (void)hipEventRecord(p.a, c->stream);
+        (void)hipEventRecord(p.a, stream());
         ev.push_back(p);
     }
     void end()
     {
         if (on && !ev.empty())
-            (void)hipEventRecord(ev.back().b, c->stream);
+            (void)hipEventRecord(ev.back().b, stream());
     }
     void collect()
     {
@@ -1162,6 +1210,94 @@ struct Timer {
         }                                                                    \
     } while (0)
 
+// One render lane: P paths of samples fp.sample0.. on stream st, using the
+// queue entries from path `off` (shadow entries from off * rpp) and the
+// counter block ctr.  Bounce b: its closest-hit rays and the shadow rays of
+// bounce b - 1 in one launch, then the light sums of bounce b - 1, then
+// the shade of bounce b; a last launch takes the shadow rays of the last
+// bounce.
+static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
+                       const SceneDev &S, bool count, uint32_t max_depth,
+                       uint32_t P, uint32_t off, uint32_t rpp, uint32_t *ctr,
+                       const StackOvf &ovf)
+{
+    const size_t so = (size_t)off * rpp;
+    const uint32_t cap_shadow = P * rpp;
+    tm.st = st;
+    LAUNCH(c, tm, K_RAYGEN,
+           launch_raygen(st, fp, c->q_o[0] + off, c->q_d[0] + off,
+                         c->q_t[0] + off, c->acc + off, P, ctr));
+    const bool lit = !c->lights.empty() || c->has_env || c->has_photons;
+    const uint32_t gb = std::min((P + SHADE_BLOCK - 1) / SHADE_BLOCK,
+                                 c->grid_blocks);
+    ShadeBufs Bp;
+    FrameParams fpp = fp;
+    memset(&Bp, 0, sizeof(Bp));
+    for (uint32_t b = 0; b <= max_depth; b++) {
+        const bool last = b == max_depth;
+        const int cur = b & 1, nxt = cur ^ 1;
+        TraceWork w;
+        w.q_o = c->q_o[cur] + off;
+        w.q_d = c->q_d[cur] + off;
+        w.hit = c->hit + off;
+        w.s_o = c->s_o + so;
+        w.s_d = c->s_d + so;
+        w.vis = c->vis + so;
+        w.cap_shadow = cap_shadow;
+        w.closest = !last;
+        w.shadow_ctr = (b > 0 && lit) ? ctr_shadow(b - 1) : -1;
+        if (w.closest || w.shadow_ctr >= 0)
+            LAUNCH(c, tm, K_TRACE,
+                   launch_trace(st, c->trace_blocks, count, S, ovf, w, ctr,
+                                c->d_stats));
+        if (b > 0 && lit)
+            LAUNCH(c, tm, K_RESOLVE, launch_resolve(st, gb, fpp, Bp, ctr));
+        if (last) {
+            LAUNCH(c, tm, K_ADVANCE,
+                   launch_advance(st, ctr, c->d_stats, P, b));
+            break;
+        }
+        fp.bounce = b;
+        fp.last_bounce = (b + 1 == max_depth) ? 1u : 0u;
+        ShadeBufs B;
+        memset(&B, 0, sizeof(B));
+        B.q_o = c->q_o[cur] + off;
+        B.q_d = c->q_d[cur] + off;
+        B.q_t = c->q_t[cur] + off;
+        B.hit = c->hit + off;
+        B.qn_o = c->q_o[nxt] + off;
+        B.qn_d = c->q_d[nxt] + off;
+        B.qn_t = c->q_t[nxt] + off;
+        B.acc = c->acc + off;
+        B.planes = c->d_planes;
+        B.prims = c->d_prims;
+        B.mats = c->d_mats;
+        B.lights = c->d_lights;
+        B.nrm = c->d_nrm;
+        B.tx = tex_dev(c);
+        B.s_o = c->s_o + so;
+        B.s_d = c->s_d + so;
+        B.s_c = c->s_c + so;
+        B.vis = c->vis + so;
+        B.p_a = c->p_a + off;
+        B.p_b = c->p_b + off;
+        B.p_c = c->p_c + off;
+        B.p_x = c->p_x + off;
+        B.p_k = c->p_k + off;
+        B.cap_paths = P;
+        B.cap_shadow = cap_shadow;
+        LAUNCH(c, tm, K_SHADE, launch_shade(st, gb, fp, B, ctr));
+        if (c->has_photons)
+            LAUNCH(c, tm, K_CAUSTIC,
+                   launch_caustic(st, c->grid_blocks, fp, B, ctr));
+        LAUNCH(c, tm, K_ADVANCE, launch_advance(st, ctr, c->d_stats, P, b));
+        Bp = B;
+        fpp = fp;
+    }
+    tm.st = nullptr;
+    return MRT_OK;
+}
+
 extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
                           uint32_t max_depth)
 {
@@ -1181,7 +1317,8 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
     if (nb > n_samples)
         nb = n_samples;
     uint64_t P64 = (uint64_t)nb * WH;
-    uint64_t S64 = P64 * shadow_rays_per_path(c);
+    const uint32_t rpp = shadow_rays_per_path(c);
+    uint64_t S64 = P64 * rpp;
     if (S64 > 0xFFFFFFF0ull)
         return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
     int rc = ensure_wavefront(c, (uint32_t)P64, (uint32_t)S64);
@@ -1199,84 +1336,36 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
         (void)hipEventRecord(t0, c->stream);
     }
     for (uint32_t s = 0; s < n_samples; s += nb) {
-        uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
-        uint32_t P = n * WH;
+        const uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
         fp.sample0 = first_sample + s;
-        LAUNCH(c, tm, K_RAYGEN,
-               launch_raygen(c->stream, fp, c->q_o[0], c->q_d[0], c->q_t[0],
-                             c->acc, P, c->d_ctr));
-        // bounce b: its closest-hit rays and the shadow rays of bounce b - 1
-        // in one launch, then the light sums of bounce b - 1, then the
-        // shade of bounce b; a last launch takes the shadow rays of the
-        // last bounce.
-        const bool lit = !c->lights.empty() || c->has_env || c->has_photons;
-        const uint32_t gb = std::min((P + SHADE_BLOCK - 1) / SHADE_BLOCK,
-                                     c->grid_blocks);
-        ShadeBufs Bp;
-        FrameParams fpp = fp;
-        memset(&Bp, 0, sizeof(Bp));
-        for (uint32_t b = 0; b <= max_depth; b++) {
-            const bool last = b == max_depth;
-            const int cur = b & 1, nxt = cur ^ 1;
-            TraceWork w;
-            w.q_o = c->q_o[cur];
-            w.q_d = c->q_d[cur];
-            w.hit = c->hit;
-            w.s_o = c->s_o;
-            w.s_d = c->s_d;
-            w.vis = c->vis;
-            w.cap_shadow = c->cap_shadow;
-            w.closest = !last;
-            w.shadow_ctr = (b > 0 && lit) ? ctr_shadow(b - 1) : -1;
-            if (w.closest || w.shadow_ctr >= 0)
-                LAUNCH(c, tm, K_TRACE,
-                       launch_trace(c->stream, c->trace_blocks, count, S,
-                                    stack_ovf(c), w, c->d_ctr, c->d_stats));
-            if (b > 0 && lit)
-                LAUNCH(c, tm, K_RESOLVE,
-                       launch_resolve(c->stream, gb, fpp, Bp, c->d_ctr));
-            if (last) {
-                LAUNCH(c, tm, K_ADVANCE, launch_advance(c->stream, c->d_ctr,
-                                                      c->d_stats, c->cap_paths, b));
-                break;
+        // Lanes: contiguous ranges of the batch's paths on their own
+        // streams; while the launches of one wait for their longest ray,
+        // the others' kernels fill the GPU.  Each path keeps its slot (and
+        // with slot0 its pixel and sample), so the accumulation below sums
+        // the same values in the same order.
+        const uint32_t P = n * WH;
+        const uint32_t L = std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS));
+        if (ensure_ovf(c, c->trace_blocks, L) != hipSuccess)
+            return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
+        if (L > 1) {
+            HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+            for (uint32_t l = 1; l < L; l++)
+                HIPCHK(c, hipStreamWaitEvent(c->lane_stream[l], c->ev_fork, 0));
+        }
+        for (uint32_t l = 0; l < L; l++) {
+            const uint32_t o0 = (uint32_t)((uint64_t)P * l / L);
+            const uint32_t o1 = (uint32_t)((uint64_t)P * (l + 1) / L);
+            FrameParams fl = fp;
+            fl.slot0 = o0;
+            hipStream_t st = l ? c->lane_stream[l] : c->stream;
+            rc = render_lane(c, st, tm, fl, S, count, max_depth, o1 - o0, o0, rpp,
+                             c->d_ctr + (size_t)l * CTR_COUNT, stack_ovf(c, l));
+            if (rc != MRT_OK)
+                return rc;
+            if (l) {
+                HIPCHK(c, hipEventRecord(c->ev_join[l], st));
+                HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join[l], 0));
             }
-            fp.bounce = b;
-            fp.last_bounce = (b + 1 == max_depth) ? 1u : 0u;
-            ShadeBufs B;
-            memset(&B, 0, sizeof(B));
-            B.q_o = c->q_o[cur];
-            B.q_d = c->q_d[cur];
-            B.q_t = c->q_t[cur];
-            B.hit = c->hit;
-            B.qn_o = c->q_o[nxt];
-            B.qn_d = c->q_d[nxt];
-            B.qn_t = c->q_t[nxt];
-            B.acc = c->acc;
-            B.planes = c->d_planes;
-            B.prims = c->d_prims;
-            B.mats = c->d_mats;
-            B.lights = c->d_lights;
-            B.nrm = c->d_nrm;
-            B.tx = tex_dev(c);
-            B.s_o = c->s_o;
-            B.s_d = c->s_d;
-            B.s_c = c->s_c;
-            B.vis = c->vis;
-            B.p_a = c->p_a;
-            B.p_b = c->p_b;
-            B.p_c = c->p_c;
-            B.p_x = c->p_x;
-            B.p_k = c->p_k;
-            B.cap_paths = c->cap_paths;
-            B.cap_shadow = c->cap_shadow;
-            LAUNCH(c, tm, K_SHADE, launch_shade(c->stream, gb, fp, B, c->d_ctr));
-            if (c->has_photons)
-                LAUNCH(c, tm, K_CAUSTIC,
-                       launch_caustic(c->stream, c->grid_blocks, fp, B, c->d_ctr));
-            LAUNCH(c, tm, K_ADVANCE, launch_advance(c->stream, c->d_ctr, c->d_stats,
-                                                  c->cap_paths, b));
-            Bp = B;
-            fpp = fp;
         }
         LAUNCH(c, tm, K_ACCUM,
                launch_accumulate(c->stream, c->acc, c->d_accum, WH, n,
@@ -1446,6 +1535,11 @@ extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
         if (value != MRT_BUILDER_HOST && value != MRT_BUILDER_DEVICE)
             return fail(c, MRT_E_INVALID, "unknown BVH builder");
         c->builder = (int)value;
+        return MRT_OK;
+    case MRT_OPT_LANES:
+        if (!(value >= 1.0) || value > (double)MAX_LANES || value != floor(value))
+            return fail(c, MRT_E_INVALID, "render lanes must be 1..%d", MAX_LANES);
+        c->lanes = (uint32_t)value;
         return MRT_OK;
     default:
         return fail(c, MRT_E_INVALID, "unknown option %d", option);

@@ -108,6 +108,8 @@ struct FrameParams {
     uint32_t lens;
     uint32_t W, H, WH;
     uint32_t sample0;         // sample index of batch slot 0
+    uint32_t slot0;           // path index (sample-major over the frame) of
+                              // slot 0 of a render lane
     uint32_t bounce;
     uint32_t last_bounce;
     uint32_t n_lights;

@@ -199,7 +199,8 @@ k_raygen(FrameParams fp, float4 *__restrict__ q_o, float4 *__restrict__ q_d,
         reset_queues(ctr, P);
     if (i >= P)
         return;
-    uint32_t pixel = i % fp.WH, sample = fp.sample0 + i / fp.WH;
+    const uint32_t g = i + fp.slot0;
+    uint32_t pixel = g % fp.WH, sample = fp.sample0 + g / fp.WH;
     V3 o, d;
     primary_ray(fp, pixel, sample, o, d);
     q_o[i] = make_float4(o.x, o.y, o.z, 1.0f);
@@ -617,7 +618,9 @@ __device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
     return pc * (float)nw * (float)nh / (19.7392088021787172f * cose);
 }
 
-template <bool BATCH>
+// FULL = false compiles the HDRI and texture code out (scenes without
+// them, and every batch): fewer registers, more resident waves.
+template <bool BATCH, bool FULL>
 __global__ void __launch_bounds__(SHADE_BLOCK)
 k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
@@ -647,8 +650,8 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 Fp = &B.bs[sid].fp;
             }
             const FrameParams &F = *Fp;
-            pixel = local % F.WH;
-            sample = F.sample0 + local / F.WH;
+            pixel = (local + F.slot0) % F.WH;
+            sample = F.sample0 + (local + F.slot0) / F.WH;
             d = xyz(d4);
             thr = xyz(t4);
             uint32_t code = __float_as_uint(h4.y);
@@ -658,7 +661,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 float ao_last = o4.w;
                 V3 bg = v3(F.bg_clamped[0], F.bg_clamped[1], F.bg_clamped[2]);
                 float wmis = 1.0f;
-                if (F.has_env) {
+                if (FULL && F.has_env) {
                     bg = env_texel(fp, d);
                     if (F.estimator == MRT_EST_REFERENCE) {
                         bg = v3(fminf(fmaxf(bg.x, 0.0f), 16.0f),
@@ -684,7 +687,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 m.a = B.mats[3 * mat];
                 m.b = B.mats[3 * mat + 1];
                 m.c = B.mats[3 * mat + 2];
-                if (F.textured)
+                if (FULL && F.textured)
                     apply_textures(B.tx, B.planes + 2 * (size_t)F.plane0, B.prims,
                                    h4, p, mat, nrm, m);
                 ao = m.b.w;
@@ -716,7 +719,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                         bool soft = ltype == MRT_LIGHT_POINT && l1.w != 0.0f && lss > 1;
                         n_rays += soft ? (uint32_t)lss : 1u;
                     }
-                    if (F.has_env) {
+                    if (FULL && F.has_env) {
                         U4 e = rng4(F.seed, pixel, sample, fp.bounce, 3u);
                         if (F.estimator == MRT_EST_REFERENCE) {
                             // importance_sampling (material_default.c:41-69)
@@ -947,8 +950,10 @@ __global__ void k_advance(uint32_t *__restrict__ ctr,
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const uint32_t prev = bounce + 1;      // parity of bounce - 1
-        stats[STAT_RAYS_CLOSEST] += ctr[CTR_CUR];
-        stats[STAT_RAYS_SHADOW] += ctr[ctr_shadow(prev)];
+        // atomics: the advance kernels of two render lanes may run together
+        atomicAdd(&stats[STAT_RAYS_CLOSEST], (unsigned long long)ctr[CTR_CUR]);
+        atomicAdd(&stats[STAT_RAYS_SHADOW],
+                  (unsigned long long)ctr[ctr_shadow(prev)]);
         ctr[CTR_CUR] = min(ctr[CTR_NEXT], cap);
         ctr[CTR_NEXT] = 0;
         ctr[ctr_shadow(prev)] = 0;
@@ -2176,8 +2181,12 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
-                       B, ctr);
+    if (fp.has_env || fp.textured)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(SHADE_BLOCK),
+                           0, st, fp, B, ctr);
+    else
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(SHADE_BLOCK),
+                           0, st, fp, B, ctr);
     return hipGetLastError();
 }
 
@@ -2218,7 +2227,8 @@ hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
 hipError_t launch_bshade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                          const ShadeBufs &B, uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
+    // batches take no HDRI and no textured material
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
                        B, ctr);
     return hipGetLastError();
 }

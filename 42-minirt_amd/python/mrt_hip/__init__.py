@@ -59,6 +59,7 @@ OPT_BLOCK = 5
 OPT_PHOTON_K = 6
 OPT_PHOTON_NORM = 7
 OPT_BUILDER = 8
+OPT_LANES = 10
 BUILDER_HOST = 0
 BUILDER_DEVICE = 1
 STATS_COUNT = 1

@@ -440,6 +440,8 @@ def main():
     ap.add_argument("--subdiv", type=int, default=8)
     ap.add_argument("--floor", type=int, default=256)
     ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--lanes", type=int, default=None,
+                    help="MRT_OPT_LANES (default: the library's)")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -505,6 +507,8 @@ def main():
     r.set_seed(0x5EED)
     spp = args.spp_per_step
     r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
+    if args.lanes is not None:
+        r.set_option(M.OPT_LANES, args.lanes)
 
     # 1) counting pass: nodes / primitives per ray (same sample indices)
     r.set_option(M.OPT_STATS, M.STATS_COUNT)

@@ -207,6 +207,12 @@ typedef struct mrt_scene {
 #define MRT_OPT_PHOTON_NORM 7  /* caustic norm factor (--caustic=..;..;NORM) */
 
 #define MRT_OPT_BUILDER     8  /* MRT_BUILDER_*                              */
+#define MRT_OPT_LANES      10  /* 1..4 render lanes: a render's paths split
+                                  into contiguous ranges traced on their own
+                                  HIP streams, so that one range's launches
+                                  fill the GPU while another's wait for their
+                                  longest ray (same image).  Default 2; a lane
+                                  gets at least 65,536 paths                */
 
 /* BVH builders (MRT_OPT_BUILDER). */
 #define MRT_BUILDER_HOST    0  /* binned SAH on the host (default)            */
