@@ -551,6 +551,7 @@ def main():
                     + cnt["prims_shadow"] * B_PRIM)
     per_launch = (bytes_trace + bytes_shadow) / max(cnt["launches_trace"], 1)
     achieved = per_launch / (trace_ms * 1e-3) / 1e9
+    step_achieved = (bytes_trace + bytes_shadow) / (dt / args.steps) / 1e9
     rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
         * value * 1e6
     traffic, traffic_src = measured_traffic()
@@ -582,7 +583,12 @@ def main():
                      "traffic": traffic,
                      "traffic_source": traffic_src,
                      "bytes_per_launch": round(per_launch),
-                     "avg_launch_ms": round(trace_ms, 4)},
+                     "avg_launch_ms": round(trace_ms, 4),
+                     # the render lanes' trace launches overlap, so the
+                     # per-launch rate above understates the GPU's: the
+                     # trace bytes of a whole step over the step's time
+                     "step_achieved": round(step_achieved, 1),
+                     "step_frac": round(step_achieved / HBM_PEAK_GBS, 4)},
         "bvh_build": {"host_sah_upload_s": round(t_up_host, 3),
                       "device_ploc_upload_s": round(t_up_dev, 4),
                       "device_nodes4": dev_nodes},
