@@ -1778,7 +1778,7 @@ struct mrt_batch {
     mrt_batch_tuning        tune{};
     hipGraph_t              graph = nullptr;
     hipGraphExec_t          gexec = nullptr;
-    uint32_t                g_key[5] = {0, 0, 0, 0, 0};
+    uint32_t                g_key[6] = {0, 0, 0, 0, 0, 0};
 };
 
 template <typename T>
@@ -2122,7 +2122,8 @@ static int batch_buffers(mrt_batch *b, uint32_t P)
     int rc = ensure_wavefront(c, P, (uint32_t)S64);
     if (rc != MRT_OK)
         return rc;
-    if (ensure_ovf(c, (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu)
+    if (ensure_ovf(c, (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu,
+                   std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS)))
         != hipSuccess)
         return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
     if (P > b->cap_slot) {
@@ -2185,50 +2186,97 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
         if (e != hipSuccess)                                                 \
             return e;                                                        \
     } while (0)
-    BSTEP(K_RAYGEN, launch_braygen(c->stream, B, b->d_ctl, c->q_o[0], c->q_d[0],
-                                   c->q_t[0], P, c->d_ctr));
-    // the order of the plain render: the closest-hit rays of bounce k with
-    // the shadow rays of bounce k - 1, the light sums of k - 1, the shade of k
-    ShadeBufs Bp = B;
-    FrameParams fpp = fp;
-    for (uint32_t k = 0; k <= depth; k++) {
-        const bool last = k == depth;
-        const int cur = k & 1, nxt = cur ^ 1;
-        TraceWork w;
-        w.q_o = c->q_o[cur];
-        w.q_d = c->q_d[cur];
-        w.hit = c->hit;
-        w.s_o = c->s_o;
-        w.s_d = c->s_d;
-        w.vis = c->vis;
-        w.cap_shadow = c->cap_shadow;
-        w.closest = !last;
-        w.shadow_ctr = (k > 0 && b->has_lights) ? ctr_shadow(k - 1) : -1;
-        if (w.closest || w.shadow_ctr >= 0)
-            BSTEP(K_TRACE, launch_btrace(c->stream, tb, count, S, stack_ovf(c), B,
-                                         w, c->d_ctr, c->d_stats));
-        if (k > 0 && b->has_lights)
-            BSTEP(K_RESOLVE, launch_bresolve(c->stream, gb, fpp, Bp, c->d_ctr));
-        if (last) {
-            BSTEP(K_ADVANCE, launch_advance(c->stream, c->d_ctr, c->d_stats,
-                                            c->cap_paths, k));
-            break;
-        }
-        fp.bounce = k;
-        fp.last_bounce = (k + 1 == depth) ? 1u : 0u;
-        B.q_o = c->q_o[cur];
-        B.q_d = c->q_d[cur];
-        B.q_t = c->q_t[cur];
-        B.hit = c->hit;
-        B.qn_o = c->q_o[nxt];
-        B.qn_d = c->q_d[nxt];
-        B.qn_t = c->q_t[nxt];
-        BSTEP(K_SHADE, launch_bshade(c->stream, gb, fp, B, c->d_ctr));
-        BSTEP(K_ADVANCE, launch_advance(c->stream, c->d_ctr, c->d_stats,
-                                        c->cap_paths, k));
-        Bp = B;
-        fpp = fp;
+    // render lanes, as in mrt_render: contiguous parts of the pass on their
+    // own streams, joined before the accumulation (inside a captured graph
+    // the fork and join events carry the dependencies)
+    const uint32_t L = std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS));
+    const uint32_t R = b->max_rays;
+    if (L > 1) {
+        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess)
+            return e;
+        for (uint32_t l = 1; l < L; l++)
+            if ((e = hipStreamWaitEvent(c->lane_stream[l], c->ev_fork, 0))
+                != hipSuccess)
+                return e;
     }
+    for (uint32_t l = 0; l < L; l++) {
+        const uint32_t o0 = (uint32_t)((uint64_t)P * l / L);
+        const uint32_t Pl = (uint32_t)((uint64_t)P * (l + 1) / L) - o0;
+        const size_t so = (size_t)o0 * R;
+        hipStream_t st = l ? c->lane_stream[l] : c->stream;
+        uint32_t *ctr = c->d_ctr + (size_t)l * CTR_COUNT;
+        const StackOvf ovf = stack_ovf(c, l);
+        if (tm)
+            tm->st = st;
+        ShadeBufs Bl = B;
+        Bl.acc = B.acc + o0;
+        Bl.s_o = B.s_o + so;
+        Bl.s_d = B.s_d + so;
+        Bl.s_c = B.s_c + so;
+        Bl.vis = B.vis + so;
+        Bl.p_a = B.p_a + o0;
+        Bl.p_b = B.p_b + o0;
+        Bl.p_c = B.p_c + o0;
+        Bl.p_x = B.p_x + o0;
+        Bl.p_k = B.p_k + o0;
+        Bl.slot_info = B.slot_info + o0;
+        Bl.cap_paths = Pl;
+        Bl.cap_shadow = Pl * R;
+        const uint32_t gbl = std::min((Pl + SHADE_BLOCK - 1) / SHADE_BLOCK, gb);
+        BSTEP(K_RAYGEN, launch_braygen(st, Bl, b->d_ctl, c->q_o[0] + o0,
+                                       c->q_d[0] + o0, c->q_t[0] + o0, o0, Pl, P,
+                                       ctr));
+        // the order of the plain render: the closest-hit rays of bounce k with
+        // the shadow rays of bounce k - 1, the light sums of k - 1, the shade of k
+        FrameParams fl = fp;
+        ShadeBufs Bp = Bl;
+        FrameParams fpp = fl;
+        for (uint32_t k = 0; k <= depth; k++) {
+            const bool last = k == depth;
+            const int cur = k & 1, nxt = cur ^ 1;
+            TraceWork w;
+            w.q_o = c->q_o[cur] + o0;
+            w.q_d = c->q_d[cur] + o0;
+            w.hit = c->hit + o0;
+            w.s_o = Bl.s_o;
+            w.s_d = Bl.s_d;
+            w.vis = Bl.vis;
+            w.cap_shadow = Bl.cap_shadow;
+            w.closest = !last;
+            w.shadow_ctr = (k > 0 && b->has_lights) ? ctr_shadow(k - 1) : -1;
+            if (w.closest || w.shadow_ctr >= 0)
+                BSTEP(K_TRACE, launch_btrace(st, tb, count, S, ovf, Bl, w, ctr,
+                                             c->d_stats));
+            if (k > 0 && b->has_lights)
+                BSTEP(K_RESOLVE, launch_bresolve(st, gbl, fpp, Bp, ctr));
+            if (last) {
+                BSTEP(K_ADVANCE, launch_advance(st, ctr, c->d_stats, Pl, k));
+                break;
+            }
+            fl.bounce = k;
+            fl.last_bounce = (k + 1 == depth) ? 1u : 0u;
+            ShadeBufs Bk = Bl;
+            Bk.q_o = c->q_o[cur] + o0;
+            Bk.q_d = c->q_d[cur] + o0;
+            Bk.q_t = c->q_t[cur] + o0;
+            Bk.hit = c->hit + o0;
+            Bk.qn_o = c->q_o[nxt] + o0;
+            Bk.qn_d = c->q_d[nxt] + o0;
+            Bk.qn_t = c->q_t[nxt] + o0;
+            BSTEP(K_SHADE, launch_bshade(st, gbl, fl, Bk, ctr));
+            BSTEP(K_ADVANCE, launch_advance(st, ctr, c->d_stats, Pl, k));
+            Bp = Bk;
+            fpp = fl;
+        }
+        if (l) {
+            if ((e = hipEventRecord(c->ev_join[l], st)) != hipSuccess)
+                return e;
+            if ((e = hipStreamWaitEvent(c->stream, c->ev_join[l], 0)) != hipSuccess)
+                return e;
+        }
+    }
+    if (tm)
+        tm->st = nullptr;
     BSTEP(K_ACCUM, launch_baccumulate(c->stream, B, b->d_ctl, b->d_accum, P,
                                       c->d_ctr, c->d_stats));
     BSTEP(K_ACCUM, launch_bnext(c->stream, b->d_ctl, P));
@@ -2259,8 +2307,9 @@ static int batch_run(mrt_batch *b, uint32_t depth, uint64_t total)
         (void)hipEventRecord(t0, c->stream);
     }
     if (b->tune.use_graph && !count && !timing) {
-        const uint32_t key[5] = {depth, P, b->tune.trace_blocks_per_cu,
-                                 b->tune.shade_blocks_per_cu, c->cap_shadow};
+        const uint32_t key[6] = {depth, P, b->tune.trace_blocks_per_cu,
+                                 b->tune.shade_blocks_per_cu, c->cap_shadow,
+                                 c->lanes};
         if (!b->gexec || memcmp(key, b->g_key, sizeof(key)) != 0) {
             if (b->gexec)
                 (void)hipGraphExecDestroy(b->gexec);
@@ -2457,6 +2506,7 @@ extern "C" int mrt_batch_set_option(mrt_batch *b, int option, double value)
         return MRT_E_INVALID;
     switch (option) {
     case MRT_OPT_STATS:
+    case MRT_OPT_LANES:
     case MRT_OPT_BUILDER:
         if (option == MRT_OPT_BUILDER && b->committed)
             return fail(b->c, MRT_E_STATE, "batch already committed");

@@ -250,9 +250,13 @@ hipError_t launch_adjoint(hipStream_t st, uint32_t blocks, const FrameParams &fp
                           const AdjBufs &A, uint32_t P, uint32_t max_depth);
 
 // batched scenes (the same stages over the scene table of a batch)
+// Camera rays of a render lane's part [o0, o0 + Pl) of a pass of P paths
+// (B, q_* and ctr already that lane's); lane 0 (o0 = 0) also records the
+// pass's path count for k_baccumulate.
 hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
                           const BatchCtl *ctl, float4 *q_o, float4 *q_d,
-                          float4 *q_t, uint32_t P, uint32_t *ctr);
+                          float4 *q_t, uint32_t o0, uint32_t Pl, uint32_t P,
+                          uint32_t *ctr);
 hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
                          const SceneDev &S, const StackOvf &ovf,
                          const ShadeBufs &B, const TraceWork &w,

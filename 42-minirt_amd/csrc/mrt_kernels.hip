@@ -1382,20 +1382,22 @@ __global__ void __launch_bounds__(256)
 k_braygen(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
           float4 *__restrict__ q_o, float4 *__restrict__ q_d,
           float4 *__restrict__ q_t, float4 *__restrict__ acc,
-          uint2 *__restrict__ slot_info, uint32_t P,
+          uint2 *__restrict__ slot_info, uint32_t o0, uint32_t Pl, uint32_t P,
           uint32_t *__restrict__ ctr)
 {
     const unsigned long long base = ctl->base, total = ctl->total;
     const uint32_t n = base >= total ? 0u
                        : (uint32_t)min((unsigned long long)P, total - base);
+    const uint32_t nl = n > o0 ? min(Pl, n - o0) : 0u;   // this lane's paths
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        reset_queues(ctr, n);
-        ctr[CTR_PASS] = n;
+        reset_queues(ctr, nl);
+        if (o0 == 0)
+            ctr[CTR_PASS] = n;
     }
-    if (i >= n)
+    if (i >= nl)
         return;
-    const unsigned long long g = base + i;
+    const unsigned long long g = base + o0 + i;
     const uint32_t sid = scene_at(bs, ctl->n_scenes, g);
     const FrameParams &F = bs[sid].fp;
     const uint32_t local = (uint32_t)(g - bs[sid].path0);
@@ -2201,10 +2203,11 @@ hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
 
 hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
                           const BatchCtl *ctl, float4 *q_o, float4 *q_d,
-                          float4 *q_t, uint32_t P, uint32_t *ctr)
+                          float4 *q_t, uint32_t o0, uint32_t Pl, uint32_t P,
+                          uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_braygen, dim3((P + 255) / 256), dim3(256), 0, st, B.bs,
-                       ctl, q_o, q_d, q_t, B.acc, B.slot_info, P, ctr);
+    hipLaunchKernelGGL(k_braygen, dim3((Pl + 255) / 256), dim3(256), 0, st, B.bs,
+                       ctl, q_o, q_d, q_t, B.acc, B.slot_info, o0, Pl, P, ctr);
     return hipGetLastError();
 }
 

@@ -174,3 +174,25 @@ def test_device_built_batch_matches_device_built_plain_renders(scenes):
             ref = r.readback()
         np.testing.assert_allclose(b.readback(k), ref, rtol=2e-5, atol=1e-6)
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_lanes_equal_one_lane(graph):
+    """A pass large enough for four render lanes: every lane count gives the
+    same bits, replayed graph or direct launches."""
+    big = [(M.HostScene.plain(7, 256, 256), 256, 256, 3, 0x77),
+           (M.HostScene.mesh(3, 192, 108, subdiv=5, floor_cells=32), 192, 108,
+            4, 0x78),
+           (soft_light_scene(), 128, 96, 3, 0x79)]
+    imgs = {}
+    for lanes in (1, 2, 4):
+        b = make_batch(big, path_budget=1 << 20, use_graph=graph)
+        b.set_option(M.OPT_LANES, lanes)
+        b.render(DEPTH)
+        b.render(DEPTH)
+        imgs[lanes] = [b.readback(k) for k in range(len(big))]
+        b.close()
+    for lanes in (2, 4):
+        for k in range(len(big)):
+            assert np.array_equal(imgs[lanes][k], imgs[1][k]), (lanes, k)
