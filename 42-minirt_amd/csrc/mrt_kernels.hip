@@ -618,9 +618,11 @@ __device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
     return pc * (float)nw * (float)nh / (19.7392088021787172f * cose);
 }
 
-// FULL = false compiles the HDRI and texture code out (scenes without
-// them, and every batch): fewer registers, more resident waves.
-template <bool BATCH, bool FULL>
+// ENV: 0 compiles the HDRI code out, 1 keeps its MIS estimator, 2 its
+// reference estimator (the launcher picks by the scene's); TEX = false
+// compiles the texture code out.  Scenes without them, and every batch, get
+// fewer registers and more resident waves.
+template <bool BATCH, int ENV, bool TEX>
 __global__ void __launch_bounds__(SHADE_BLOCK)
 k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
@@ -661,9 +663,9 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 float ao_last = o4.w;
                 V3 bg = v3(F.bg_clamped[0], F.bg_clamped[1], F.bg_clamped[2]);
                 float wmis = 1.0f;
-                if (FULL && F.has_env) {
+                if (ENV && F.has_env) {
                     bg = env_texel(fp, d);
-                    if (F.estimator == MRT_EST_REFERENCE) {
+                    if (ENV == 2) {
                         bg = v3(fminf(fmaxf(bg.x, 0.0f), 16.0f),
                                 fminf(fmaxf(bg.y, 0.0f), 16.0f),
                                 fminf(fmaxf(bg.z, 0.0f), 16.0f));
@@ -687,7 +689,7 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 m.a = B.mats[3 * mat];
                 m.b = B.mats[3 * mat + 1];
                 m.c = B.mats[3 * mat + 2];
-                if (FULL && F.textured)
+                if (TEX && F.textured)
                     apply_textures(B.tx, B.planes + 2 * (size_t)F.plane0, B.prims,
                                    h4, p, mat, nrm, m);
                 ao = m.b.w;
@@ -719,9 +721,9 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                         bool soft = ltype == MRT_LIGHT_POINT && l1.w != 0.0f && lss > 1;
                         n_rays += soft ? (uint32_t)lss : 1u;
                     }
-                    if (FULL && F.has_env) {
+                    if (ENV && F.has_env) {
                         U4 e = rng4(F.seed, pixel, sample, fp.bounce, 3u);
-                        if (F.estimator == MRT_EST_REFERENCE) {
+                        if (ENV == 2) {
                             // importance_sampling (material_default.c:41-69)
                             const uint32_t ew = F.env_w, eh = F.env_h;
                             double vy = F.env_marg_inv[(int)((double)e.x * (eh - 1))];
@@ -2183,12 +2185,18 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr)
 {
-    if (fp.has_env || fp.textured)
-        hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(SHADE_BLOCK),
-                           0, st, fp, B, ctr);
-    else
-        hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(SHADE_BLOCK),
-                           0, st, fp, B, ctr);
+#define MRT_SHADE(E, T)                                                       \
+    hipLaunchKernelGGL((k_shade<false, E, T>), dim3(blocks), dim3(SHADE_BLOCK), 0, \
+                       st, fp, B, ctr)
+    const int env = !fp.has_env ? 0 : fp.estimator == MRT_EST_REFERENCE ? 2 : 1;
+    if (fp.textured) {
+        if (env == 2) MRT_SHADE(2, true); else if (env == 1) MRT_SHADE(1, true);
+        else MRT_SHADE(0, true);
+    } else {
+        if (env == 2) MRT_SHADE(2, false); else if (env == 1) MRT_SHADE(1, false);
+        else MRT_SHADE(0, false);
+    }
+#undef MRT_SHADE
     return hipGetLastError();
 }
 
@@ -2231,7 +2239,7 @@ hipError_t launch_bshade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                          const ShadeBufs &B, uint32_t *ctr)
 {
     // batches take no HDRI and no textured material
-    hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
+    hipLaunchKernelGGL((k_shade<true, 0, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, fp,
                        B, ctr);
     return hipGetLastError();
 }
