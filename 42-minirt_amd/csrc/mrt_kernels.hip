@@ -1760,23 +1760,23 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
         for (int iy = y0; iy <= y1; iy++)
             for (int ix = x0; ix <= x1; ix++) {
                 uint32_t b = hash_cell(ix, iy, iz, pm.table);
-                for (uint32_t j = pm.cstart[b]; j < pm.cend[b]; j++) {
-                    float4 P = pm.pos[j];
+                // one photon of the bucket (P its position, W its power when
+                // gathering by radius), in bucket order
+                auto take = [&](float4 P, float4 W, uint32_t j) {
                     if ((int)floorf(P.x / pm.cell) != ix
                         || (int)floorf(P.y / pm.cell) != iy
                         || (int)floorf(P.z / pm.cell) != iz)
-                        continue;
+                        return;
                     float ddx = P.x - x.x, ddy = P.y - x.y, ddz = P.z - x.z;
                     float d2 = (ddx * ddx + ddy * ddy) + ddz * ddz;
                     if (!(d2 <= cut))
-                        continue;
+                        return;
                     if (!knn) {
-                        float4 w = pm.pow[j];
-                        sum = add(sum, xyz(w));
-                        continue;
+                        sum = add(sum, xyz(W));
+                        return;
                     }
                     if (cnt == K && !(d2 < kd[(K - 1) * stride]))
-                        continue;
+                        return;
                     uint32_t pos = cnt < K ? cnt++ : K - 1;
                     while (pos > 0 && d2 < kd[(pos - 1) * stride]) {
                         kd[pos * stride] = kd[(pos - 1) * stride];
@@ -1785,7 +1785,24 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                     }
                     kd[pos * stride] = d2;
                     ki[pos * stride] = j;
+                };
+                const uint32_t je = pm.cend[b];
+                uint32_t j = pm.cstart[b];
+                // four records in flight at a time instead of one
+                for (; j + 4 <= je; j += 4) {
+                    float4 P[4], W[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        P[u] = pm.pos[j + u];
+                        W[u] = knn ? make_float4(0.f, 0.f, 0.f, 0.f) : pm.pow[j + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        take(P[u], W[u], j + u);
                 }
+                for (; j < je; j++)
+                    take(pm.pos[j],
+                         knn ? make_float4(0.f, 0.f, 0.f, 0.f) : pm.pow[j], j);
             }
     float area;
     if (knn) {
