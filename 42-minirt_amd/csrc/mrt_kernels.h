@@ -10,7 +10,7 @@ namespace mrt {
 #define MRT_TRACE_BLOCK 128
 #endif
 #ifndef MRT_TRACE_SS
-#define MRT_TRACE_SS 24
+#define MRT_TRACE_SS 16
 #endif
 constexpr int TRACE_BLOCK = MRT_TRACE_BLOCK;  // threads per trace block
 static_assert(TRACE_BLOCK % 64 == 0, "trace blocks are whole waves");
