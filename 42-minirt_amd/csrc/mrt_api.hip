@@ -672,9 +672,6 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     // the tables are built on the device from the uploaded texels (the
     // host builders in hdri_tables.c give the same bits)
     size_t n = (size_t)w * h;
-    std::vector<float4> tex(n);
-    for (size_t i = 0; i < n; i++)
-        tex[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
     std::vector<double> band(h - 1);
     const double pi = 3.14159265358979323846;
     for (uint32_t y = 0; y + 1 < h; y++) {
@@ -684,6 +681,7 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     }
     const double sc = exposure == 0.0f ? 1.0 : (double)powf(2.0f, exposure);
     double *d_band = nullptr, *d_scratch = nullptr;
+    float *d_rgb = nullptr;
     int rc = MRT_OK;
     if (dalloc(c->d_env_tex, n) != hipSuccess
         || dalloc(c->d_env_pdf, n) != hipSuccess
@@ -693,10 +691,15 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
         || dalloc(c->d_mis_cond, (size_t)(h - 1) * w) != hipSuccess
         || dalloc(c->d_mis_pcell, (size_t)(h - 1) * (w - 1)) != hipSuccess
         || dalloc(d_band, h - 1) != hipSuccess
-        || dalloc(d_scratch, 3 * (size_t)h + 2 + n) != hipSuccess)
+        || dalloc(d_scratch, 3 * (size_t)h + 2 + n) != hipSuccess
+        || dalloc(d_rgb, 3 * n) != hipSuccess)
         rc = fail(c, MRT_E_NOMEM, "device allocation for the HDRI failed");
+    // the texels as given (rgb), widened to float4 on the device
     if (rc == MRT_OK)
-        rc = upload(c, c->d_env_tex, tex.data(), n * 16);
+        rc = upload(c, d_rgb, rgb, n * 12);
+    if (rc == MRT_OK && launch_rgb_to_float4(c->stream, d_rgb, c->d_env_tex, n)
+                            != hipSuccess)
+        rc = fail(c, MRT_E_HIP, "HDRI texel copy failed");
     if (rc == MRT_OK)
         rc = upload(c, d_band, band.data(), (h - 1) * sizeof(double));
     if (rc == MRT_OK) {
@@ -709,6 +712,7 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
     }
     dfree(d_band);
     dfree(d_scratch);
+    dfree(d_rgb);
     if (rc != MRT_OK) {
         free_env(c);
         return rc;

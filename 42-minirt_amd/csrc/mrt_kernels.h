@@ -184,6 +184,9 @@ struct ShadeBufs {
 hipError_t launch_rebase(hipStream_t st, float4 *nodes, uint32_t n_nodes,
                          float4 *prims, float *nrm, uint32_t n_prims,
                          uint32_t node0, uint32_t prim0, uint32_t mat0);
+// tex[i] = (rgb[3i], rgb[3i + 1], rgb[3i + 2], 0) for i < n.
+hipError_t launch_rgb_to_float4(hipStream_t st, const float *rgb, float4 *tex,
+                                size_t n);
 // Builds the HDRI's reference and MIS tables from its texels (tex, w * h)
 // on the device, bit for bit as mrt_hdri_build_tables / _mis do on the
 // host.  sc = 2^exposure (or 1); band[y] = sin(e_hi) - sin(e_lo) of the h-1

@@ -2163,6 +2163,22 @@ k_env_point(const float4 *__restrict__ tex, uint32_t w, uint32_t h, double sc,
     }
 }
 
+__global__ void __launch_bounds__(256)
+k_rgb_to_float4(const float *__restrict__ rgb, float4 *__restrict__ tex, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        tex[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+}
+
+hipError_t launch_rgb_to_float4(hipStream_t st, const float *rgb, float4 *tex,
+                                size_t n)
+{
+    hipLaunchKernelGGL(k_rgb_to_float4, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, st, rgb, tex, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_env_tables(hipStream_t st, const float4 *tex, uint32_t w,
                              uint32_t h, double sc, const double *band,
                              double *scratch, float *pdf, float *marg_inv,
