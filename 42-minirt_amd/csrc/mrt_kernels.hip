@@ -1218,7 +1218,12 @@ __device__ __forceinline__ void lds_add3(float *g, V3 v)
         atomicAdd(g + 2, v.z);
 }
 
+// The replay waits on dependent fetches path after path; 3 waves per SIMD
+// (168 VGPRs, spilling into scratch) hide more of that than the compiler's
+// 2 (245 VGPRs): 63.6 -> 56.2 ms on the inverse-rendering workload, 4
+// waves no better.
 __global__ void __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(3, 3)))
 k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
           uint32_t max_depth)
 {
