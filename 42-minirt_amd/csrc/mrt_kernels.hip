@@ -650,8 +650,12 @@ __device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
 // reference estimator (the launcher picks by the scene's); TEX = false
 // compiles the texture code out.  Scenes without them, and every batch, get
 // fewer registers and more resident waves.
+// At least 4 waves per SIMD: the HDRI variants are held to 128 VGPRs
+// (spilling a little) instead of 160-186 and 2-3 waves, 2 % more
+// Msamples/s on the caustics workload; the others need fewer already.
 template <bool BATCH, int ENV, bool TEX>
 __global__ void __launch_bounds__(SHADE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(4)))
 k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
     const uint32_t n = ctr[CTR_CUR];
