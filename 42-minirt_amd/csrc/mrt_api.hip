@@ -1723,7 +1723,10 @@ extern "C" int mrt_adjoint(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples
             uint64_t P = (uint64_t)n * WH;
             fp.sample0 = first_sample + s;
             uint64_t blocks = (P + TRACE_BLOCK - 1) / TRACE_BLOCK;
-            uint64_t cap = (uint64_t)c->num_cu * 16;
+            // many short blocks: the block scheduler then balances paths of
+            // unequal length (56 ms with 16 blocks per CU, 48 ms from 128 on,
+            // inverse-rendering workload)
+            uint64_t cap = (uint64_t)c->num_cu * 128;
             e = launch_adjoint(c->stream, (uint32_t)(blocks < cap ? blocks : cap), fp,
                                S, B, A, (uint32_t)P, max_depth);
         }
