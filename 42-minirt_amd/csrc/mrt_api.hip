@@ -256,7 +256,9 @@ extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
             break;
         }
         c->trace_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu());
-        c->grid_blocks = (uint32_t)c->num_cu * 8u;
+        // grid of the grid-stride kernels (shade, light sums, gather): 32
+        // blocks per CU, 1.6 % faster renders than 8 (no change at 128)
+        c->grid_blocks = (uint32_t)c->num_cu * 32u;
         if (ensure_ovf(c, c->trace_blocks)
                 != hipSuccess) {
             rc = MRT_E_NOMEM;
