@@ -1834,7 +1834,7 @@ extern "C" int mrt_batch_create(int device, mrt_batch **out)
     b->c = c;
     b->tune.path_budget = 1u << 22;
     b->tune.trace_blocks_per_cu = c->trace_blocks / (uint32_t)c->num_cu;
-    b->tune.shade_blocks_per_cu = 8;
+    b->tune.shade_blocks_per_cu = 32;
     b->tune.use_graph = 1;
     *out = b;
     return MRT_OK;
@@ -2463,7 +2463,7 @@ extern "C" int mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
     std::vector<mrt_batch_tuning> cand;
     for (uint32_t pb : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24})
         for (uint32_t tbc : {occ, std::max(1u, occ / 2)})
-            for (uint32_t sbc : {4u, 8u}) {
+            for (uint32_t sbc : {8u, 32u}) {
                 if (pb > 2 * trial && pb != (1u << 20))
                     continue;
                 mrt_batch_tuning t{pb, tbc, sbc, 1, 0.0};
