@@ -466,8 +466,9 @@ def main():
                          "calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(rendering shards by sample index; see DESIGN.md)",
+        print("bench.py measures one GPU: there is no multi-GPU path (a "
+              "render would shard by sample index, each GPU summing its "
+              "samples, then one reduce of the accumulation buffers)",
               file=sys.stderr)
         sys.exit(2)
 
