@@ -1526,7 +1526,7 @@ extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
         c->path_budget = (uint32_t)value;
         return MRT_OK;
     case MRT_OPT_BLOCK:
-        return fail(c, MRT_E_UNSUPPORTED, "block size is fixed in this build");
+        return fail(c, MRT_E_UNSUPPORTED, "the trace block size is fixed when the library is compiled");
     case MRT_OPT_PHOTON_K:
         if (!(value >= 0.0) || value > 32.0)
             return fail(c, MRT_E_LIMIT, "photon k must be in 0..32");
