@@ -210,6 +210,18 @@ void orc_sample_ggx_f64(const double n[3], double rough, float u1, float u2,
 void orc_sample_cos_f64(const double n[3], float u1, float u2, double out[3]);
 void orc_unit_sphere_f64(const float u[4], double out[3]);
 
+/* The transmission branch of a smooth dielectric (refracted_ray,
+ * src/material/refraction.c:16-76, with its TIR fallback to
+ * metallic_color, src/material/metallic.c:25-48) for one direction d at a
+ * surface whose normal n faces it; power starts at 1.  Returns 1 when the
+ * ray was totally reflected. */
+int  orc_refract_f32(const double d[3], const double n[3], int front,
+                     float ior, float ior_all, const float color[3],
+                     double out[3], double power[3]);
+int  orc_refract_f64(const double d[3], const double n[3], int front,
+                     float ior, float ior_all, const float color[3],
+                     double out[3], double power[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -348,3 +348,22 @@ def make_hdri(rgb, exposure=0.0):
                  mc.ctypes.data, mp.ctypes.data)
     hd._keep = (rgb, pdf, marg, cond, mm, mc, mp)
     return hd
+
+
+def refract(d, n, front, ior, ior_all=1.0, color=(1.0, 1.0, 1.0), f64=True):
+    """Transmission branch of a smooth dielectric for one direction d at a
+    surface whose normal n faces it: (new direction, power, totally
+    reflected)."""
+    L = lib()
+    fn = L.orc_refract_f64 if f64 else L.orc_refract_f32
+    dp = C.POINTER(C.c_double)
+    fn.argtypes = [dp, dp, C.c_int, C.c_float, C.c_float,
+                   C.POINTER(C.c_float), dp, dp]
+    fn.restype = C.c_int
+    dd = (C.c_double * 3)(*map(float, d))
+    nn = (C.c_double * 3)(*map(float, n))
+    col = (C.c_float * 3)(*map(float, color))
+    out = (C.c_double * 3)()
+    pw = (C.c_double * 3)()
+    tir = fn(dd, nn, int(front), float(ior), float(ior_all), col, out, pw)
+    return np.array(out[:]), np.array(pw[:]), bool(tir)
