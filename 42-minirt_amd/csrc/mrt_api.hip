@@ -84,6 +84,7 @@ struct mrt_ctx {
 
     // accumulation
     float       *d_accum = nullptr;
+    float       *h_stage = nullptr;   // pinned staging of mrt_readback_add
     uint32_t    *d_ctr = nullptr;
     unsigned long long *d_stats = nullptr;
 
@@ -334,6 +335,8 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     free_env(c);
     free_photons(c);
     dfree(c->d_accum);
+    if (c->h_stage)
+        (void)hipHostFree(c->h_stage);
     dfree(c->d_ctr);
     dfree(c->d_stats);
     dfree(c->d_ovf);
@@ -1413,13 +1416,36 @@ extern "C" int mrt_readback_add(mrt_ctx *c, double *float_render)
 {
     if (!c || !float_render)
         return MRT_E_INVALID;
-    size_t n = (size_t)c->W * c->H * 3;
-    std::vector<float> tmp(n);
-    int rc = mrt_readback_f32(c, tmp.data());
-    if (rc != MRT_OK)
-        return rc;
-    for (size_t i = 0; i < n; i++)
-        float_render[i] += (double)tmp[i];
+    const size_t n = (size_t)c->W * c->H * 3;
+    if (!c->h_stage
+        && hipHostMalloc((void **)&c->h_stage, n * sizeof(float),
+                         hipHostMallocDefault) != hipSuccess) {
+        c->h_stage = nullptr;
+        return fail(c, MRT_E_NOMEM, "pinned read-back buffer (%zu floats)", n);
+    }
+    // get_result adds one dispatch's samples per call: the device sums are
+    // handed over and zeroed in stream order, so the next render starts
+    // from zero and every sample reaches float_render exactly once.
+    HIPCHK(c, hipMemcpyAsync(c->h_stage, c->d_accum, n * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_accum, 0, n * sizeof(float), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const float *src = c->h_stage;
+    auto add = [src, float_render](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++)
+            float_render[i] += (double)src[i];
+    };
+    // a 1080p frame is 6.2 M doubles: split the host-side add over threads
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 8u));
+    if (n < (1u << 20))
+        nt = 1;
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < nt; k++)
+        pool.emplace_back(add, n * k / nt, n * (k + 1) / nt);
+    add(0, n / nt);
+    for (auto &t : pool)
+        t.join();
     return MRT_OK;
 }
 

@@ -2,7 +2,8 @@
  * One render_frame tick (src/rendering/loop.c:36-63 -> render_gpu,
  * src/ray_tracing/render_gpu.c:35-61 -> draw_pixels, render.c:28-45) on the
  * device library: sample 0 clears the sums, every tick adds
- * samples_per_tick samples of paths into the host double buffer.
+ * samples_per_tick samples of paths into the host double buffer
+ * (mrt_readback_add hands the device sums over and zeroes them).
  */
 #include "mrt_host_internal.h"
 
@@ -23,9 +24,7 @@ int mrt_frame_tick(mrt_frame *f)
     uint32_t n = f->samples_per_tick ? f->samples_per_tick : 1;
     if (n > f->spp - f->sample)
         n = f->spp - f->sample;
-    int rc = mrt_clear(f->ctx);
-    if (rc == MRT_OK)
-        rc = mrt_render(f->ctx, f->sample, n, f->max_bounces);
+    int rc = mrt_render(f->ctx, f->sample, n, f->max_bounces);
     if (rc == MRT_OK)
         rc = mrt_readback_add(f->ctx, f->float_render);
     if (rc != MRT_OK)

@@ -372,6 +372,18 @@ mrt_load_rts = _sig("mrt_load_rts", C.c_int, C.c_char_p,
                     C.POINTER(C.c_double), C.c_uint32, C.c_uint32, _up)
 
 
+class FrameState(C.Structure):
+    """mrt_frame: one render_frame loop's state (include/mrt_host.h)."""
+    _fields_ = [("ctx", C.c_void_p), ("w", C.c_uint32), ("h", C.c_uint32),
+                ("sample", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32),
+                ("samples_per_tick", C.c_uint32),
+                ("float_render", C.POINTER(C.c_double))]
+
+
+mrt_frame_tick = _sig("mrt_frame_tick", C.c_int, C.POINTER(FrameState))
+
+
 def _f32(a):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a, a.ctypes.data_as(_fp)
@@ -879,6 +891,12 @@ class Renderer:
                              gt.ctypes.data_as(_fp) if gt is not None else None))
         return gm, gt
 
+    def set_camera(self, pos, direction, fov=60.0, defocus=0.0, focus=1.0):
+        """mrt_set_camera: a new viewpoint for the uploaded scene."""
+        cam = Camera((C.c_float * 3)(*pos), (C.c_float * 3)(*direction), fov,
+                     defocus, focus)
+        self._ck(mrt_set_camera(self._ctx, C.byref(cam)))
+
     def render(self, first_sample, n_samples, max_depth):
         self._ck(mrt_render(self._ctx, first_sample, n_samples, max_depth))
 
@@ -894,9 +912,17 @@ class Renderer:
         return out
 
     def readback_add(self, float_render):
+        """get_result: adds the samples rendered since the last call into
+        the double buffer and zeroes the device sums."""
         assert float_render.dtype == np.float64 and float_render.flags.c_contiguous
         self._ck(mrt_readback_add(
             self._ctx, float_render.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def frame(self, float_render, spp, max_bounces, samples_per_tick=1,
+              sample=0):
+        """A render_frame loop over this context (mrt_frame_tick)."""
+        return Frame(self, float_render, spp, max_bounces, samples_per_tick,
+                     sample)
 
     def aov(self, kind, sample=0):
         n = self.width * self.height
@@ -928,6 +954,30 @@ class Renderer:
 
     def reset_stats(self):
         self._ck(mrt_reset_stats(self._ctx))
+
+
+class Frame:
+    """render_frame ticks (src/rendering/loop.c:36-63) into a host-owned
+    double buffer through mrt_frame_tick."""
+
+    def __init__(self, renderer, float_render, spp, max_bounces,
+                 samples_per_tick=1, sample=0):
+        assert float_render.dtype == np.float64
+        assert float_render.flags.c_contiguous
+        self._r = renderer
+        self.buf = float_render
+        self.state = FrameState(renderer._ctx, renderer.width, renderer.height,
+                                sample, spp, max_bounces, samples_per_tick,
+                                float_render.ctypes.data_as(
+                                    C.POINTER(C.c_double)))
+
+    @property
+    def sample(self):
+        return self.state.sample
+
+    def tick(self):
+        """Samples added by this tick (0 once spp is reached)."""
+        return self._r._ck(mrt_frame_tick(C.byref(self.state)))
 
 
 class Batch:

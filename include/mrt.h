@@ -327,12 +327,16 @@ int  mrt_caustic_at(mrt_ctx *ctx, uint32_t n, const float *pts, float *rgb);
 int  mrt_render(mrt_ctx *ctx, uint32_t first_sample, uint32_t n_samples,
                 uint32_t max_depth);
 
-/* Replaces get_result (src/gpu/use_shader.c:53-79): waits for the stream and
+/* Replaces get_result (src/gpu/use_shader.c:53-79): waits for the stream,
  * adds the device sums into the host-owned W*H*3 double buffer, as
- * add_fcolor_float3 does into screen.float_render. */
+ * add_fcolor_float3 does into screen.float_render, and zeroes the device
+ * sums.  Like get_result, which adds the one sample each dispatch wrote,
+ * every sample rendered since the previous call reaches float_render
+ * exactly once: the compute_frame_gpu sequence mrt_render(sample, 1) ->
+ * mrt_readback_add -> sample++ leaves k samples after k ticks. */
 int  mrt_readback_add(mrt_ctx *ctx, double *float_render);
 
-/* Copies the device sums (W*H*3 floats) to out. */
+/* Copies the device sums (W*H*3 floats) to out and leaves them in place. */
 int  mrt_readback_f32(mrt_ctx *ctx, float *out);
 
 /* Zeroes the device accumulation buffer. */
