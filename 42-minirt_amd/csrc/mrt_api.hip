@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -25,8 +26,43 @@ constexpr int MAX_LANES = 4;
 // Lanes get at least this many paths each (fewer lanes for small renders).
 constexpr uint32_t LANE_MIN_PATHS = 1u << 16;
 
+// Staging of mrt_trace_rays_noalloc: origins (3 floats), directions (3),
+// primitive ids (1 word) and t (1 float) of up to MRT_TRACE_NOALLOC_MAX rays,
+// 32 bytes a ray, 8 MiB in all.  Each GPU holds its own copy from the
+// library's load until the process ends.  Host code never passes `scratch`
+// itself (on the host it names a stand-in, not device memory): it uses the
+// address hipGetSymbolAddress returns on each device, kept in
+// g_scratch_dev[device] by mrt_init.
+constexpr size_t NOALLOC_WORDS = (size_t)MRT_TRACE_NOALLOC_MAX * 8;
+static __device__ __align__(256) float scratch[NOALLOC_WORDS];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_dev[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current).
+static hipError_t scratch_on_device(int device, float **out)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        if (!p)
+            return hipErrorInvalidSymbol;
+        g_scratch_dev[device] = static_cast<float *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return hipSuccess;
+}
+
 struct mrt_ctx {
     int          device = 0;
+    float       *scratch_dev = nullptr;  // this device's `scratch`
     hipStream_t  stream = nullptr;
     uint32_t     W = 0, H = 0;
     char         err[512] = {0};
@@ -253,6 +289,10 @@ extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
             || hipMemsetAsync(c->d_stats, 0, STAT_COUNT * 8, c->stream)
                    != hipSuccess
             || hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = MRT_E_HIP;
+            break;
+        }
+        if (scratch_on_device(device, &c->scratch_dev) != hipSuccess) {
             rc = MRT_E_HIP;
             break;
         }
@@ -1525,6 +1565,44 @@ extern "C" int mrt_trace_rays(mrt_ctx *c, uint32_t n, const float *orig,
     (void)hipFree(d_p);
     if (e != hipSuccess)
         return fail(c, MRT_E_HIP, "trace_rays failed: %s", hipGetErrorString(e));
+    return MRT_OK;
+}
+
+extern "C" int mrt_trace_rays_noalloc(mrt_ctx *c, uint32_t n,
+                                      const float *orig, const float *dir,
+                                      uint32_t *prim_id, float *t)
+{
+    if (!c || (n && (!orig || !dir || !prim_id || !t)))
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE,
+                    "mrt_trace_rays_noalloc before mrt_upload_scene");
+    if (n > MRT_TRACE_NOALLOC_MAX)
+        return fail(c, MRT_E_LIMIT,
+                    "mrt_trace_rays_noalloc takes at most %u rays (got %u)",
+                    MRT_TRACE_NOALLOC_MAX, n);
+    if (n == 0)
+        return MRT_OK;
+    if (!c->scratch_dev)
+        return fail(c, MRT_E_STATE, "no staging address for this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    // fixed partition of the staging array: every region is written (host
+    // copies, then the kernel for each of the n rays) before it is read
+    const size_t M = MRT_TRACE_NOALLOC_MAX;
+    float *d_o = c->scratch_dev;
+    float *d_d = c->scratch_dev + 3 * M;
+    uint32_t *d_p = reinterpret_cast<uint32_t *>(c->scratch_dev + 6 * M);
+    float *d_t = c->scratch_dev + 7 * M;
+    const size_t v = (size_t)n * 3 * sizeof(float);
+    HIPCHK(c, hipMemcpyAsync(d_o, orig, v, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_d, dir, v, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_trace_rays(c->stream, scene_dev(c), c->d_prims, n, d_o,
+                                d_d, d_p, d_t));
+    HIPCHK(c, hipMemcpyAsync(prim_id, d_p, (size_t)n * 4,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MRT_OK;
 }
 

@@ -253,6 +253,9 @@ mrt_render_aov = _sig("mrt_render_aov", C.c_int, _ctx_p, C.c_int, C.c_uint32,
                       _vp)
 mrt_trace_rays = _sig("mrt_trace_rays", C.c_int, _ctx_p, C.c_uint32, _fp, _fp,
                       _up, _fp)
+mrt_trace_rays_noalloc = _sig("mrt_trace_rays_noalloc", C.c_int, _ctx_p,
+                              C.c_uint32, _fp, _fp, _up, _fp)
+TRACE_NOALLOC_MAX = 262144
 mrt_set_option = _sig("mrt_set_option", C.c_int, _ctx_p, C.c_int, C.c_double)
 mrt_get_stats = _sig("mrt_get_stats", C.c_int, _ctx_p, C.POINTER(Stats))
 mrt_reset_stats = _sig("mrt_reset_stats", C.c_int, _ctx_p)
@@ -945,6 +948,19 @@ class Renderer:
         self._ck(mrt_trace_rays(self._ctx, n, po, pd,
                                 prim.ctypes.data_as(_up),
                                 t.ctypes.data_as(_fp)))
+        return prim, t
+
+    def trace_rays_noalloc(self, orig, dirs):
+        """mrt_trace_rays through the library's fixed staging array (no
+        device allocation; at most TRACE_NOALLOC_MAX rays)."""
+        o, po = _f32(orig)
+        d, pd = _f32(dirs)
+        n = o.size // 3
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        self._ck(mrt_trace_rays_noalloc(self._ctx, n, po, pd,
+                                        prim.ctypes.data_as(_up),
+                                        t.ctypes.data_as(_fp)))
         return prim, t
 
     def stats(self):

@@ -354,6 +354,19 @@ int  mrt_render_aov(mrt_ctx *ctx, int aov, uint32_t sample, void *out);
 int  mrt_trace_rays(mrt_ctx *ctx, uint32_t n, const float *orig,
                     const float *dir, uint32_t *prim_id, float *t);
 
+/* No counterpart in the reference.  mrt_trace_rays without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight interactive loops such as an object picker called per
+ * click, harnesses that forbid hipMalloc): the rays and results are staged
+ * in a fixed 8 MiB device array that the library holds on each GPU until
+ * the process ends.  At most MRT_TRACE_NOALLOC_MAX rays per call (larger
+ * n returns MRT_E_LIMIT).  All contexts on one GPU share that array: calls
+ * that overlap on one GPU (from several host threads) race, so the caller
+ * serialises them.  Same results as mrt_trace_rays. */
+#define MRT_TRACE_NOALLOC_MAX 262144u
+int  mrt_trace_rays_noalloc(mrt_ctx *ctx, uint32_t n, const float *orig,
+                            const float *dir, uint32_t *prim_id, float *t);
+
 int  mrt_set_option(mrt_ctx *ctx, int option, double value);
 int  mrt_get_stats(mrt_ctx *ctx, mrt_stats *out);
 int  mrt_reset_stats(mrt_ctx *ctx);
