@@ -1606,12 +1606,27 @@ extern "C" int mrt_trace_rays_noalloc(mrt_ctx *c, uint32_t n,
     return MRT_OK;
 }
 
+extern "C" int mrt_set_seed(mrt_ctx *c, uint64_t seed)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    c->seed = seed;
+    return MRT_OK;
+}
+
 extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
 {
     if (!c)
         return MRT_E_INVALID;
     switch (option) {
     case MRT_OPT_SEED:
+        // a double holds integers exactly only up to 2^53: larger keys go
+        // through mrt_set_seed, so that no key is silently rounded
+        if (!(value >= 0.0) || value > 9007199254740992.0
+            || value != floor(value))
+            return fail(c, MRT_E_INVALID,
+                        "seed must be an integer in [0, 2^53] here; use "
+                        "mrt_set_seed for a full 64-bit key");
         c->seed = (uint64_t)value;
         return MRT_OK;
     case MRT_OPT_ESTIMATOR:

@@ -110,7 +110,7 @@ int main(int argc, char **argv)
         mrt_hs_free(hs);
         return 1;
     }
-    mrt_set_option(ctx, MRT_OPT_SEED, (double)seed);
+    mrt_set_seed(ctx, (uint64_t)seed);
     if ((rc = mrt_upload_scene(ctx, mrt_hs_view(hs))) != MRT_OK) {
         fprintf(stderr, "Error\n%s\n", mrt_last_error(ctx));
         mrt_destroy(ctx);

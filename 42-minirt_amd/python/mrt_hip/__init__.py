@@ -257,6 +257,7 @@ mrt_trace_rays_noalloc = _sig("mrt_trace_rays_noalloc", C.c_int, _ctx_p,
                               C.c_uint32, _fp, _fp, _up, _fp)
 TRACE_NOALLOC_MAX = 262144
 mrt_set_option = _sig("mrt_set_option", C.c_int, _ctx_p, C.c_int, C.c_double)
+mrt_set_seed = _sig("mrt_set_seed", C.c_int, _ctx_p, C.c_uint64)
 mrt_get_stats = _sig("mrt_get_stats", C.c_int, _ctx_p, C.POINTER(Stats))
 mrt_reset_stats = _sig("mrt_reset_stats", C.c_int, _ctx_p)
 mrt_synchronize = _sig("mrt_synchronize", C.c_int, _ctx_p)
@@ -790,7 +791,8 @@ class Renderer:
         self._ck(mrt_set_option(self._ctx, opt, float(value)))
 
     def set_seed(self, seed):
-        self.set_option(OPT_SEED, seed)
+        """The full 64-bit RNG key (mrt_set_seed)."""
+        self._ck(mrt_set_seed(self._ctx, int(seed) & (2 ** 64 - 1)))
 
     def upload_hdri(self, rgb, exposure=0.0):
         a = np.ascontiguousarray(rgb, dtype=np.float32)

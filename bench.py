@@ -104,45 +104,17 @@ def measured_traffic():
         return None, None
 
 
-RESOLUTIONS = [(256, 256), (512, 512), (960, 540), (1920, 1080)]
-SPPS = [16, 64, 256]
-
-
-def batched_scenes(seed, n, tri_lo, tri_hi, spp_div):
-    """Scene list of the batched workload: displaced icospheres over a
-    heightfield floor whose triangle count is log-uniform in [tri_lo,
-    tri_hi] (subdivision level from the count, floor cells for the rest),
-    resolution and spp drawn from the fixed sets, all seeded."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(n):
-        target = float(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
-        sub = 3
-        while 20 * 4 ** (sub + 1) <= target:
-            sub += 1
-        rest = max(target - 20 * 4 ** sub, 2.0)
-        cells = max(1, int(round((rest / 2) ** 0.5)))
-        w, h = RESOLUTIONS[int(rng.integers(len(RESOLUTIONS)))]
-        spp = max(1, SPPS[int(rng.integers(len(SPPS)))] // spp_div)
-        out.append({"seed": 1000 + k, "subdiv": sub, "floor": cells,
-                    "w": w, "h": h, "spp": spp,
-                    "rng_seed": int(rng.integers(1 << 62))})
-    return out
-
-
 def batched_leg(M, args):
     """Batched vs sequential plain rendering of the same scenes and samples
     (the images are equal up to fp32 summation order, so the error is equal
     too)."""
     from concurrent.futures import ThreadPoolExecutor
+    from mrt_hip.workloads import batched_scene, batched_scenes
     desc = batched_scenes(args.batch_seed, args.batch_scenes, 1e4, 2e6,
                           args.batch_spp_div)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(16) as ex:
-        scenes = list(ex.map(lambda d: M.HostScene.mesh(
-            d["seed"], d["w"], d["h"], subdiv=d["subdiv"],
-            floor_cells=d["floor"], amp=0.15), desc))
+        scenes = list(ex.map(batched_scene, desc))
     ntri = [s.view().contents.n_triangles for s in scenes]
     log(f"batched: {len(scenes)} scenes, {sum(ntri)} triangles generated in "
         f"{time.perf_counter() - t0:.1f} s")

@@ -196,7 +196,8 @@ typedef struct mrt_scene {
 #define MRT_EST_MIS        1   /* proper solid-angle pdf + balance heuristic  */
 
 /* Options for mrt_set_option. */
-#define MRT_OPT_SEED        1  /* 64-bit RNG key (value cast to uint64)      */
+#define MRT_OPT_SEED        1  /* RNG key, an integer in [0, 2^53] (a larger
+                                  key needs mrt_set_seed)                   */
 #define MRT_OPT_ESTIMATOR   2  /* MRT_EST_*                                  */
 #define MRT_OPT_STATS       3  /* MRT_STATS_* flags                          */
 #define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront batch        */
@@ -368,6 +369,11 @@ int  mrt_trace_rays_noalloc(mrt_ctx *ctx, uint32_t n, const float *orig,
                             const float *dir, uint32_t *prim_id, float *t);
 
 int  mrt_set_option(mrt_ctx *ctx, int option, double value);
+/* The 64-bit RNG key of the context's camera paths and photons; the
+ * reference seeds its generator from RANDOM_SEED (src/math/random.c:17-30).
+ * Samples are keyed by (seed, pixel, sample, bounce), so equal seeds give
+ * equal images on any device. */
+int  mrt_set_seed(mrt_ctx *ctx, uint64_t seed);
 int  mrt_get_stats(mrt_ctx *ctx, mrt_stats *out);
 int  mrt_reset_stats(mrt_ctx *ctx);
 int  mrt_synchronize(mrt_ctx *ctx);

@@ -196,3 +196,72 @@ def test_batch_lanes_equal_one_lane(graph):
     for lanes in (2, 4):
         for k in range(len(big)):
             assert np.array_equal(imgs[lanes][k], imgs[1][k]), (lanes, k)
+
+
+@pytest.mark.gpu
+def test_batch_matches_the_oracle():
+    """Uneven scenes in one batch, each compared with the oracle's
+    restatement of path_trace (path_trace.c:50-76) on the same RNG keys:
+    the plain three-sphere scene, a small mesh scene, and the smallest
+    displaced icosphere of the 64-scene batched workload's generator (its
+    geometry as generated, rendered at 64x48)."""
+    import oracle_ffi as O
+    from mrt_hip.workloads import batched_scene, batched_scenes, triangles
+    desc = min(batched_scenes(4, 64), key=triangles)
+    cases = [(M.HostScene.plain(7, 64, 36), 64, 36, 5, 0x1111),
+             (M.HostScene.mesh(3, 48, 27, subdiv=4, floor_cells=16), 48, 27,
+              4, 0x3333),
+             (batched_scene(desc, 64, 48), 64, 48, 4, desc["rng_seed"])]
+    assert cases[2][0].view().contents.n_triangles == triangles(desc)
+    b = make_batch(cases, path_budget=4096)
+    b.render(DEPTH)
+    for k, (hs, w, h, spp, seed) in enumerate(cases):
+        img = b.readback(k).astype(np.float64)
+        view = hs.view()
+        bvh = O.make_bvh(M.BuiltBvh(view))
+        ref, _ = O.render(view, bvh, w, h, 0, spp, DEPTH, seed)
+        diff = np.abs(img - ref)
+        close = diff <= 1e-4 + 1e-4 * np.abs(ref)
+        print(f"scene {k}: close {close.mean():.6f}, "
+              f"mean abs diff {diff.mean():.3e}")
+        assert close.mean() >= 0.995, k
+        assert diff.mean() <= 1e-3 * max(ref.mean(), 1e-6), k
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batched_workload_properties():
+    """The full 64-scene batched workload (24.96 M triangles, 4.74 G
+    samples): every path is traced once, no sample is non-finite, every
+    scene's image is finite and lit, and two scenes (the smallest and the
+    largest by samples) equal their plain renders."""
+    from concurrent.futures import ThreadPoolExecutor
+    from mrt_hip.workloads import batched_scene, batched_scenes
+    desc = batched_scenes(4, 64)
+    with ThreadPoolExecutor(8) as ex:
+        scenes = list(ex.map(batched_scene, desc))
+    b = M.Batch()
+    for hs, d in zip(scenes, desc):
+        b.add(hs, d["w"], d["h"], d["spp"], d["rng_seed"])
+    b.commit()
+    b.render(DEPTH)
+    st = b.stats()
+    assert st["paths"] == sum(d["w"] * d["h"] * d["spp"] for d in desc)
+    assert st["nonfinite"] == 0
+    imgs = {}
+    for k, d in enumerate(desc):
+        img = b.readback(k)
+        assert np.isfinite(img).all(), k
+        assert img.mean() > 0, k
+        imgs[k] = img
+    b.close()
+    cost = [d["w"] * d["h"] * d["spp"] for d in desc]
+    for k in (int(np.argmin(cost)), int(np.argmax(cost))):
+        d = desc[k]
+        with M.Renderer(d["w"], d["h"]) as r:
+            r.upload(scenes[k])
+            r.set_seed(d["rng_seed"])
+            r.render(0, d["spp"], DEPTH)
+            ref = r.readback()
+        np.testing.assert_allclose(imgs[k], ref, rtol=1e-4, atol=1e-5,
+                                   err_msg=f"scene {k}")
