@@ -139,7 +139,11 @@ struct mrt_ctx {
     uint64_t     seed = 0x6D696E6952542121ull;
     int          estimator = MRT_EST_REFERENCE;
     int          stats_on = 0;
-    uint32_t     path_budget = 1u << 22;
+    // paths in flight per wavefront batch: 32 M (16 spp at 1080p, about
+    // 8 GB of queues) so that each bounce launch is long against the tail
+    // of its slowest rays (4 M: 856, 8 M: 1055, 16 M: 1142, 32 M: 1185
+    // Msamples/s on the 1080p mesh workload)
+    uint32_t     path_budget = 1u << 25;
 
     // statistics
     uint64_t     paths = 0;
@@ -1347,6 +1351,32 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
     return MRT_OK;
 }
 
+extern "C" int mrt_reserve(mrt_ctx *c, uint32_t n_samples)
+{
+    if (!c)
+        return MRT_E_INVALID;
+    if (!c->has_scene)
+        return fail(c, MRT_E_STATE, "mrt_reserve before mrt_upload_scene");
+    if (n_samples == 0)
+        return MRT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t WH = c->W * c->H;
+    uint32_t nb = std::max(1u, c->path_budget / WH);
+    nb = std::min(nb, n_samples);
+    const uint64_t P64 = (uint64_t)nb * WH;
+    const uint64_t S64 = P64 * shadow_rays_per_path(c);
+    if (S64 > 0xFFFFFFF0ull)
+        return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
+    int rc = ensure_wavefront(c, (uint32_t)P64, (uint32_t)S64);
+    if (rc != MRT_OK)
+        return rc;
+    const uint32_t L = std::max(1u, std::min(c->lanes,
+                                             (uint32_t)P64 / LANE_MIN_PATHS));
+    if (ensure_ovf(c, c->trace_blocks, L) != hipSuccess)
+        return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
+    return MRT_OK;
+}
+
 extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
                           uint32_t max_depth)
 {
@@ -1951,7 +1981,7 @@ extern "C" int mrt_batch_create(int device, mrt_batch **out)
         return MRT_E_NOMEM;
     }
     b->c = c;
-    b->tune.path_budget = 1u << 22;
+    b->tune.path_budget = 1u << 25;
     b->tune.trace_blocks_per_cu = c->trace_blocks / (uint32_t)c->num_cu;
     b->tune.shade_blocks_per_cu = 32;
     b->tune.use_graph = 1;
@@ -2578,12 +2608,12 @@ extern "C" int mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
     const int saved_stats = c->stats_on;
     c->stats_on = 0;
     const uint32_t occ = std::max(1u, c->trace_blocks / (uint32_t)c->num_cu);
-    const uint64_t trial = std::min<uint64_t>(b->total, 1ull << 24);
+    const uint64_t trial = std::min<uint64_t>(b->total, 1ull << 26);
     std::vector<mrt_batch_tuning> cand;
-    for (uint32_t pb : {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24})
+    for (uint32_t pb : {1u << 22, 1u << 23, 1u << 24, 1u << 25})
         for (uint32_t tbc : {occ, std::max(1u, occ / 2)})
             for (uint32_t sbc : {8u, 32u}) {
-                if (pb > 2 * trial && pb != (1u << 20))
+                if (pb > 2 * trial && pb != (1u << 22))
                     continue;
                 mrt_batch_tuning t{pb, tbc, sbc, 1, 0.0};
                 cand.push_back(t);

@@ -258,6 +258,7 @@ mrt_trace_rays_noalloc = _sig("mrt_trace_rays_noalloc", C.c_int, _ctx_p,
 TRACE_NOALLOC_MAX = 262144
 mrt_set_option = _sig("mrt_set_option", C.c_int, _ctx_p, C.c_int, C.c_double)
 mrt_set_seed = _sig("mrt_set_seed", C.c_int, _ctx_p, C.c_uint64)
+mrt_reserve = _sig("mrt_reserve", C.c_int, _ctx_p, C.c_uint32)
 mrt_get_stats = _sig("mrt_get_stats", C.c_int, _ctx_p, C.POINTER(Stats))
 mrt_reset_stats = _sig("mrt_reset_stats", C.c_int, _ctx_p)
 mrt_synchronize = _sig("mrt_synchronize", C.c_int, _ctx_p)
@@ -901,6 +902,10 @@ class Renderer:
         cam = Camera((C.c_float * 3)(*pos), (C.c_float * 3)(*direction), fov,
                      defocus, focus)
         self._ck(mrt_set_camera(self._ctx, C.byref(cam)))
+
+    def reserve(self, n_samples):
+        """Allocate the queues of renders of n_samples per call now."""
+        self._ck(mrt_reserve(self._ctx, n_samples))
 
     def render(self, first_sample, n_samples, max_depth):
         self._ck(mrt_render(self._ctx, first_sample, n_samples, max_depth))

@@ -24,7 +24,9 @@ METRIC = ("Msamples/s @1080p (batched vs plain GPU vs host CPU) at ≤1% "
           "image error; adjoint/fwd time")
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of the encodings the kernels read and write
-B_NODE = 128         # one 4-wide node: four child boxes + child links
+B_NODE = 112         # one 4-wide node as traversal reads it: four child
+                     # boxes (96 B) + four child links (16 B); the node's
+                     # last 16 B are padding that is never loaded
 B_PRIM = 48          # one primitive record
 Q_RAY = 32           # ray read by the trace kernel (origin, direction)
 Q_HIT = 16           # hit record written
@@ -36,45 +38,118 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def usable_cores():
+    """Cores this process may run on: its CPU affinity, capped by the
+    cgroup CPU quota when one is set (a GPU box grants a share of a larger
+    machine); also the machine's nproc and CPU model."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        q, per = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    use = min(aff, quota) if quota else aff
+    return use, {"nproc": os.cpu_count(), "affinity": aff,
+                 "cgroup_quota_cores": quota, "model": model}
+
+
 def cpu_baseline(hs, width, height, max_depth, seed, budget_s):
     """The oracle (fp64, the reference's recursive traversal) on a band of
-    rows of the same frame, all host cores, then one thread."""
+    rows of the same frame: on every core this process may use (median of
+    three runs), then on one thread."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import mrt_hip as M
     import oracle_ffi as O
     view = hs.view()
     bvh = O.make_bvh(M.BuiltBvh(view))
-    cores = min(16, os.cpu_count() or 1)
+    cores, host = usable_cores()
+
+    def run(threads, rows, spp):
+        r0 = height // 2 - rows // 2
+        t0 = time.perf_counter()
+        _, cnt = O.render(view, bvh, width, height, 0, spp, max_depth, seed,
+                          f64=True, traversal=O.TRAV_REFERENCE,
+                          threads=threads, rows=(r0, r0 + rows))
+        return time.perf_counter() - t0, cnt["rays_closest"] + cnt["rays_shadow"]
+
     res = {}
-    for threads, share in ((cores, 0.75), (1, 0.25)):
-        # grow the band of rows, then the samples per pixel, until the run
-        # takes a good share of the budget
-        rows, spp, t = 4, 1, 0.0
+    for threads, share, reps in ((cores, 0.75, 3), (1, 0.25, 1)):
+        # grow the band of rows, then the samples per pixel, until one run
+        # takes its share of the budget
+        rows, spp = 4, 1
+        per_run = budget_s * share / reps
         while True:
-            r0 = height // 2 - rows // 2
-            t0 = time.perf_counter()
-            O.render(view, bvh, width, height, 0, spp, max_depth, seed,
-                     f64=True, traversal=O.TRAV_REFERENCE, threads=threads,
-                     rows=(r0, r0 + rows))
-            t = time.perf_counter() - t0
-            if t >= budget_s * share * 0.4 or (rows >= height and spp >= 64):
+            t, _ = run(threads, rows, spp)
+            if t >= per_run * 0.4 or (rows >= height and spp >= 64):
                 break
-            grow = max(2.0, min(8.0, budget_s * share * 0.5 / max(t, 1e-3)))
+            grow = max(2.0, min(8.0, per_run * 0.5 / max(t, 1e-3)))
             if rows < height:
                 rows = min(height, int(rows * grow))
             else:
                 spp = min(64, int(spp * grow))
-        res[threads] = (rows * width * spp / t / 1e6, rows, spp, t)
-    v, rows, spp, t = res[cores]
-    v1, rows1, spp1, t1 = res[1]
+        runs = sorted(run(threads, rows, spp) for _ in range(reps))
+        t, rays = runs[len(runs) // 2]
+        res[threads] = (rows * width * spp / t / 1e6, rays / t, rows, spp, t,
+                        [round(x[0], 2) for x in runs])
+    v, rps, rows, spp, t, times = res[cores]
+    v1, rps1, rows1, spp1, t1, _ = res[1]
     return {"value": round(v, 4), "unit": "Msamples/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "rays_per_s": round(rps, 1),
             "sample": f"rows {rows} of the {width}x{height} frame at {spp} "
                       f"spp, depth {max_depth}, oracle fp64 with the "
-                      f"reference's recursive BVH traversal, {t:.1f} s",
+                      f"reference's recursive BVH traversal, median of 3 "
+                      f"runs {times} s",
+            "host": host,
+            "per_core_msamples_per_s": round(v / cores, 4),
             "single_thread": {"value": round(v1, 4), "cores": 1,
+                              "rays_per_s": round(rps1, 1),
                               "sample": f"{rows1} rows at {spp1} spp, "
                                         f"{t1:.1f} s"}}
+
+
+def image_error(r, W, H, depth, seed):
+    """Mean relative per-pixel error, |I - R| / max(R, 1e-3) over the pixels
+    and channels of a band of rows, of the GPU image at 64, 256 and 1024
+    spp against the committed 16,384-spp fp64 oracle render of the same
+    band (tests/golden/make_mesh1080_band.py), and the spp at which the
+    1/sqrt(spp) fit reaches 1 %."""
+    import numpy as np
+    f = ROOT / "tests" / "golden" / "mesh1080_band_16k.npz"
+    if not f.exists():
+        return None
+    ref = np.load(f)
+    r0, r1 = (int(x) for x in ref["rows"])
+    R = ref["mean"].astype(np.float64)
+    if int(ref["seed"]) != seed or int(ref["depth"]) != depth \
+            or R.shape != (r1 - r0, W, 3):
+        return None
+    out = {}
+    for spp in (64, 256, 1024):
+        r.clear()
+        r.render(0, spp, depth)
+        img = r.readback()[r0:r1].astype(np.float64) / spp
+        out[spp] = float(np.mean(np.abs(img - R) / np.maximum(R, 1e-3)))
+    r.clear()
+    e256 = out[256]
+    return {"reference": f"oracle fp64, reference traversal, 16384 spp, "
+                         f"rows {r0}-{r1 - 1} of {W}x{H}",
+            "epsilon": 1e-3,
+            "mean_rel_error": {str(k): round(v, 5) for k, v in out.items()},
+            "at_256_spp_within_1pct": e256 <= 0.01,
+            "spp_for_1pct": int(np.ceil(256 * (e256 / 0.01) ** 2))}
 
 
 def copy_probe():
@@ -148,42 +223,26 @@ def batched_leg(M, args):
     t_batch = time.perf_counter() - t0
     log(f"batched: {samples / t_batch / 1e6:.1f} Msamples/s")
     b.close()
-    # 64 sequential plain launches: one context per scene, uploaded
-    # beforehand, each rendering all of its samples in one call
-    t0 = time.perf_counter()
-
-    def mk(args_):
-        hs, d = args_
-        r = M.Renderer(d["w"], d["h"])
-        r.upload(hs)
-        r.set_seed(d["rng_seed"])
-        return r
-    with ThreadPoolExecutor(16) as ex:
-        ctxs = list(ex.map(mk, zip(scenes, desc)))
-    log(f"plain: {len(ctxs)} contexts uploaded in "
-        f"{time.perf_counter() - t0:.1f} s")
-    ctxs[0].render(0, 1, args.depth)           # warm-up
-    ctxs[0].synchronize()
-    ctxs[0].clear()
-    t0 = time.perf_counter()
-    for r, d in zip(ctxs, desc):
-        r.render(0, d["spp"], args.depth)
-        r.synchronize()
-    t_plain = time.perf_counter() - t0
-    log(f"plain sequential: {samples / t_plain / 1e6:.1f} Msamples/s")
-    # the drop-in's own contract: one sample per render_frame tick
-    # (compute_frame_gpu), every scene in turn, the stream drained per tick
-    for r in ctxs:
-        r.clear()
-    t0 = time.perf_counter()
-    for r, d in zip(ctxs, desc):
-        for smp in range(d["spp"]):
-            r.render(smp, 1, args.depth)
+    # 64 sequential plain launches: per scene a context with the library's
+    # defaults, uploaded and its queues reserved outside the timed region,
+    # then one mrt_render of all its samples (timed, with its stream
+    # drained); contexts are released one after the other, as 64 of them
+    # at the default path budget would not fit in HBM together
+    t_plain = 0.0
+    for k, (hs, d) in enumerate(zip(scenes, desc)):
+        with M.Renderer(d["w"], d["h"]) as r:
+            r.upload(hs)
+            r.set_seed(d["rng_seed"])
+            r.reserve(d["spp"])
+            if k == 0:
+                r.render(0, 1, args.depth)     # warm-up (code objects)
+                r.clear()
             r.synchronize()
-    t_tick = time.perf_counter() - t0
-    log(f"plain one sample per tick: {samples / t_tick / 1e6:.1f} Msamples/s")
-    for r in ctxs:
-        r.close()
+            t0 = time.perf_counter()
+            r.render(0, d["spp"], args.depth)
+            r.synchronize()
+            t_plain += time.perf_counter() - t0
+    log(f"plain sequential: {samples / t_plain / 1e6:.1f} Msamples/s")
     return {
         "workload": f"{len(desc)} scenes, triangles log-uniform in [1e4, 2e6] "
                     f"({sum(ntri)} total), resolutions "
@@ -192,11 +251,8 @@ def batched_leg(M, args):
         "samples": samples, "triangles": sum(ntri),
         "msamples_per_s": round(samples / t_batch / 1e6, 2),
         "plain_sequential_msamples_per_s": round(samples / t_plain / 1e6, 2),
-        "plain_per_tick_msamples_per_s": round(samples / t_tick / 1e6, 2),
         "speedup_vs_plain": round(t_plain / t_batch, 3),
-        "speedup_vs_plain_per_tick": round(t_tick / t_batch, 3),
         "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
-        "plain_per_tick_s": round(t_tick, 3),
         "commit_s": round(t_commit, 2),
         "commit_device_builder_s": round(t_commit_dev, 2),
         "tuning": tuning,
@@ -404,9 +460,9 @@ def caustics_leg(M, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--spp-per-step", type=int, default=2)
+    ap.add_argument("--spp-per-step", type=int, default=16)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--subdiv", type=int, default=8)
@@ -417,6 +473,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ticks", action="store_true",
+                    help="skip the drop-in per-tick measurement")
+    ap.add_argument("--no-image-error", action="store_true",
+                    help="skip the image-error renders")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp-div", type=int, default=1,
                     help="divide the batched workload's spp set by this")
@@ -505,7 +565,10 @@ def main():
     samples = args.steps * spp * W * H
     value = samples / dt / 1e6
 
-    # 3) per-kernel HIP-event times on the library's stream
+    # 3) per-kernel HIP-event times, on one render lane so that no other
+    # lane's kernels overlap a timed launch (the roofline's per-launch
+    # duration), then the lane setting of the timed steps again
+    r.set_option(M.OPT_LANES, 1)
     r.set_option(M.OPT_STATS, M.STATS_TIME)
     r.reset_stats()
     prof_steps = 2
@@ -513,6 +576,7 @@ def main():
         r.render(spp * s, spp, args.depth)
     tm = r.stats()
     r.set_option(M.OPT_STATS, 0)
+    r.set_option(M.OPT_LANES, args.lanes if args.lanes is not None else 2)
     # k_trace launches: the closest-hit rays of a bounce together with the
     # shadow rays of the bounce before (depth + 1 per render)
     trace_ms = tm["ms_trace"] / max(tm["launches_trace"], 1)
@@ -522,12 +586,44 @@ def main():
     bytes_shadow = (cnt["rays_shadow"] * (Q_SRAY + Q_VIS)
                     + cnt["nodes_shadow"] * B_NODE
                     + cnt["prims_shadow"] * B_PRIM)
-    per_launch = (bytes_trace + bytes_shadow) / max(cnt["launches_trace"], 1)
+    # bytes of one render over the launches of one render in the timing
+    # pass (one lane: depth + 1 launches)
+    launches_per_render = max(tm["launches_trace"] / prof_steps, 1.0)
+    per_launch = (bytes_trace + bytes_shadow) / launches_per_render
     achieved = per_launch / (trace_ms * 1e-3) / 1e9
     step_achieved = (bytes_trace + bytes_shadow) / (dt / args.steps) / 1e9
     rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
         * value * 1e6
     traffic, traffic_src = measured_traffic()
+
+    # 4) the drop-in's own contract (compute_frame_gpu): one sample per
+    # render_frame tick, each followed by mrt_readback_add into the host's
+    # double buffer (PCIe transfer and host-side add included)
+    import numpy as np
+    ticks = None
+    if not args.no_ticks:
+        float_render = np.zeros((H, W, 3), np.float64)
+        r.render(0, 1, args.depth)
+        r.readback_add(float_render)           # warm-up (pinned buffer)
+        n_ticks = 16
+        t0 = time.perf_counter()
+        for s in range(n_ticks):
+            r.render(1 + s, 1, args.depth)
+            r.readback_add(float_render)
+        t_ticks = time.perf_counter() - t0
+        per_tick = n_ticks * W * H / t_ticks / 1e6
+        log(f"drop-in ticks: {per_tick:.1f} Msamples/s "
+            f"({t_ticks / n_ticks * 1e3:.2f} ms per tick)")
+        ticks = {"msamples_per_s": round(per_tick, 2),
+                 "ms_per_tick": round(t_ticks / n_ticks * 1e3, 3),
+                 "sequence": "mrt_render(sample, 1) + mrt_readback_add into "
+                             "a host double buffer, per tick (PCIe "
+                             "included)"}
+    err = None
+    if not args.no_image_error:
+        err = image_error(r, W, H, args.depth, 0x5EED)
+        if err:
+            log(f"image error: {err['mean_rel_error']}")
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -548,7 +644,14 @@ def main():
         "kernel_ms_per_step": {k: round(tm[f"ms_{k}"] / prof_steps, 3)
                                for k in ("raygen", "trace", "shade", "shadow",
                                          "accum", "total")},
+        "drop_in_per_tick": ticks,
+        "image_error": err,
         "roofline": {"bound": "hbm",
+                     # the peak this frac is taken against is HBM's; what
+                     # holds the kernel below it is the latency of dependent
+                     # node fetches and divergence (occupancy, VALU use and
+                     # DRAM bytes well under the algorithmic ones)
+                     "limiter": "latency",
                      "kernel": "k_trace (closest-hit rays of a bounce + "
                                "shadow rays of the bounce before)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -557,9 +660,9 @@ def main():
                      "traffic_source": traffic_src,
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4),
-                     # the render lanes' trace launches overlap, so the
-                     # per-launch rate above understates the GPU's: the
-                     # trace bytes of a whole step over the step's time
+                     "timed_on": "one render lane (no overlapping launch)",
+                     # the trace bytes of a whole timed step (default
+                     # lanes) over the step's time
                      "step_achieved": round(step_achieved, 1),
                      "step_frac": round(step_achieved / HBM_PEAK_GBS, 4)},
         "bvh_build": {"host_sah_upload_s": round(t_up_host, 3),

@@ -200,7 +200,8 @@ typedef struct mrt_scene {
                                   key needs mrt_set_seed)                   */
 #define MRT_OPT_ESTIMATOR   2  /* MRT_EST_*                                  */
 #define MRT_OPT_STATS       3  /* MRT_STATS_* flags                          */
-#define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront batch        */
+#define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront batch
+                                  (default 2^25: 16 spp at 1080p, ~8 GB)    */
 #define MRT_OPT_BLOCK       5  /* threads per block for the trace kernels    */
 #define MRT_OPT_PHOTON_K    6  /* caustic gather: k nearest photons (default
                                   15, get_caustic.c); 0 = every photon within
@@ -327,6 +328,13 @@ int  mrt_caustic_at(mrt_ctx *ctx, uint32_t n, const float *pts, float *rgb);
  * accumulation buffer.  Asynchronous on the context's stream. */
 int  mrt_render(mrt_ctx *ctx, uint32_t first_sample, uint32_t n_samples,
                 uint32_t max_depth);
+
+/* No counterpart in the reference: allocates the wavefront queues that
+ * renders of n_samples samples per call need (at most the path budget), so
+ * that the first such mrt_render does not allocate (an interactive host's
+ * first frame, or a timed render).  mrt_render allocates them itself when
+ * this was not called. */
+int  mrt_reserve(mrt_ctx *ctx, uint32_t n_samples);
 
 /* Replaces get_result (src/gpu/use_shader.c:53-79): waits for the stream,
  * adds the device sums into the host-owned W*H*3 double buffer, as

@@ -134,7 +134,7 @@ def test_autotune_then_render(scenes, plain_images):
     b = make_batch(scenes)
     t = b.autotune(DEPTH)
     print("chosen", t)
-    assert t["path_budget"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
+    assert t["path_budget"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25)
     assert t["ms_per_mpath"] > 0
     # the trial renders are cleared
     assert not b.readback(0).any()
