@@ -144,6 +144,7 @@ struct mrt_ctx {
     // of its slowest rays (4 M: 856, 8 M: 1055, 16 M: 1142, 32 M: 1185
     // Msamples/s on the 1080p mesh workload)
     uint32_t     path_budget = 1u << 25;
+    bool         trace_top = true;   // MRT_OPT_TRACE_TOP
 
     // statistics
     uint64_t     paths = 0;
@@ -216,7 +217,7 @@ static hipError_t ensure_ovf(mrt_ctx *c, uint32_t blocks, uint32_t sets = 1)
     if (e != hipSuccess)
         return e;
     c->ovf_blocks = c->ovf_sets = 0;
-    e = dalloc(c->d_ovf, (size_t)sets * TRACE_OVF * blocks * TRACE_BLOCK);
+    e = dalloc(c->d_ovf, (size_t)sets * TRACE_OVF_BUF * blocks * TRACE_BLOCK);
     if (e == hipSuccess) {
         c->ovf_blocks = blocks;
         c->ovf_sets = sets;
@@ -226,7 +227,7 @@ static hipError_t ensure_ovf(mrt_ctx *c, uint32_t blocks, uint32_t sets = 1)
 
 static StackOvf stack_ovf(const mrt_ctx *c, uint32_t lane = 0)
 {
-    return StackOvf{c->d_ovf + (size_t)lane * TRACE_OVF * c->ovf_blocks * TRACE_BLOCK,
+    return StackOvf{c->d_ovf + (size_t)lane * TRACE_OVF_BUF * c->ovf_blocks * TRACE_BLOCK,
                     c->ovf_blocks};
 }
 
@@ -1302,7 +1303,7 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         if (w.closest || w.shadow_ctr >= 0)
             LAUNCH(c, tm, K_TRACE,
                    launch_trace(st, c->trace_blocks, count, S, ovf, w, ctr,
-                                c->d_stats));
+                                c->d_stats, c->trace_top));
         if (b > 0 && lit)
             LAUNCH(c, tm, K_RESOLVE, launch_resolve(st, gb, fpp, Bp, ctr));
         if (last) {
@@ -1673,6 +1674,11 @@ extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
         if (!(value >= 1.0) || value > 1.0e9)
             return fail(c, MRT_E_INVALID, "path budget out of range");
         c->path_budget = (uint32_t)value;
+        return MRT_OK;
+    case MRT_OPT_TRACE_TOP:
+        if (value != 0.0 && value != 1.0)
+            return fail(c, MRT_E_INVALID, "trace top cache is 0 or 1");
+        c->trace_top = value != 0.0;
         return MRT_OK;
     case MRT_OPT_BLOCK:
         return fail(c, MRT_E_UNSUPPORTED, "the trace block size is fixed when the library is compiled");

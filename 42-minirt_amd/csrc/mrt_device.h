@@ -633,14 +633,19 @@ __device__ __forceinline__ void cx(float &ka, int32_t &ca, float &kb, int32_t &c
 // below tlimit.
 // EXT = false drops the quadric and alpha tests from the leaf loop (scenes
 // without them keep the traversal's register count and code size).
-template <bool ANY, bool COUNT, int SS, bool EXT = true>
+// TOP: nodes 0 .. ntop-1 (the top levels, laid out breadth-first at the
+// front of the node array) are read from `top`, the block's copy in LDS
+// (7 float4 per node), instead of from global memory.
+template <bool ANY, bool COUNT, int SS, bool EXT = true, bool TOP = false>
 __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
                                          int stride, uint32_t *ovf,
                                          uint32_t ovf_stride,
                                          float &t_out, uint32_t &hit_slot,
                                          float &bu_out, float &bv_out,
-                                         uint32_t &nodes, uint32_t &prims)
+                                         uint32_t &nodes, uint32_t &prims,
+                                         const float4 *top = nullptr,
+                                         uint32_t ntop = 0)
 {
     if (S.n_nodes == 0)
         return false;
@@ -671,9 +676,16 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
     // Inner node: push the hit children but the nearest, return the nearest
     // (or the next stack entry when no child is hit).
     auto node = [&](int32_t nc) -> int32_t {
-        const float4 *n = S.nodes + (size_t)nc * 8;
-        float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-        float4 cw = n[6];
+        float4 lx, hx, ly, hy, lz, hz, cw;
+        if (TOP && (uint32_t)nc < ntop) {
+            const float4 *n = top + (size_t)nc * 7;
+            lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
+            cw = n[6];
+        } else {
+            const float4 *n = S.nodes + (size_t)nc * 8;
+            lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
+            cw = n[6];
+        }
         if (COUNT)
             nodes++;
         float t0, t1, t2, t3;
@@ -780,20 +792,23 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
 
 // Closest hit over BVH + planes (hit_register_all, hit_register.c:109-123).
 // code: leaf-order slot, HIT_PLANE | k, or HIT_NONE.
-template <bool COUNT, int SS, bool EXT = true>
+template <bool COUNT, int SS, bool EXT = true, bool TOP = false>
 __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
                                             uint32_t *lds_code, int stride,
                                             uint32_t *ovf, uint32_t ovf_stride,
                                             float &t, uint32_t &code,
                                             float &bu, float &bv,
-                                            uint32_t &nodes, uint32_t &prims)
+                                            uint32_t &nodes, uint32_t &prims,
+                                            const float4 *top = nullptr,
+                                            uint32_t ntop = 0)
 {
     float tb = IT_MAX_F;
     uint32_t slot = HIT_NONE;
     float u = 0.0f, v = 0.0f;
-    bool hit = traverse<false, COUNT, SS, EXT>(S, o, d, IT_MAX_F, lds_code,
-                                               stride, ovf, ovf_stride, tb, slot,
-                                               u, v, nodes, prims);
+    bool hit = traverse<false, COUNT, SS, EXT, TOP>(S, o, d, IT_MAX_F, lds_code,
+                                                    stride, ovf, ovf_stride, tb,
+                                                    slot, u, v, nodes, prims,
+                                                    top, ntop);
     for (uint32_t k = 0; k < S.n_planes; k++) {
         float tp;
         if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
@@ -812,19 +827,22 @@ __device__ __forceinline__ bool closest_hit(const SceneDev &S, V3 o, V3 d,
 
 // Occlusion below tlimit: BVH primitives below min(tlimit, 1000), planes
 // below tlimit (check_plight_hit, src/ray_tracing/light/p_light.c:48-64).
-template <bool COUNT, int SS, bool EXT = true>
+template <bool COUNT, int SS, bool EXT = true, bool TOP = false>
 __device__ __forceinline__ bool occluded(const SceneDev &S, V3 o, V3 d,
                                          float tlimit, uint32_t *lds_code,
                                          int stride, uint32_t *ovf,
                                          uint32_t ovf_stride,
-                                         uint32_t &nodes, uint32_t &prims)
+                                         uint32_t &nodes, uint32_t &prims,
+                                         const float4 *top = nullptr,
+                                         uint32_t ntop = 0)
 {
     float tl = fminf(tlimit, IT_MAX_F);
     float t;
     uint32_t slot;
     float u, v;
-    if (traverse<true, COUNT, SS, EXT>(S, o, d, tl, lds_code, stride, ovf,
-                                       ovf_stride, t, slot, u, v, nodes, prims))
+    if (traverse<true, COUNT, SS, EXT, TOP>(S, o, d, tl, lds_code, stride, ovf,
+                                            ovf_stride, t, slot, u, v, nodes,
+                                            prims, top, ntop))
         return true;
     for (uint32_t k = 0; k < S.n_planes; k++) {
         float tp;

@@ -16,6 +16,19 @@ constexpr int TRACE_BLOCK = MRT_TRACE_BLOCK;  // threads per trace block
 static_assert(TRACE_BLOCK % 64 == 0, "trace blocks are whole waves");
 constexpr int TRACE_SS = MRT_TRACE_SS;        // stack entries kept in LDS
 constexpr int TRACE_OVF = STACK4_MAX - TRACE_SS;  // the rest, per lane
+// The trace kernel that keeps the tree's top nodes in LDS: TOP_NODES nodes
+// (the first levels, breadth-first at the front of the node array, 112 B
+// each) and a TRACE_SS_TOP-entry stack per lane, 9,520 B per 128-thread
+// block, so that 16 blocks (8 waves per SIMD) still fit in a CU's 160 KB.
+#ifndef MRT_TOP_NODES
+#define MRT_TOP_NODES 21
+#endif
+constexpr int TOP_NODES = MRT_TOP_NODES;
+constexpr int TRACE_SS_TOP = 14;
+static_assert(TOP_NODES <= (int)MRT_BVH4_TOP, "cached nodes are laid out first");
+// per-lane columns of the overflow buffer: enough for either stack split
+constexpr int TRACE_OVF_BUF = STACK4_MAX - (TRACE_SS_TOP < TRACE_SS ? TRACE_SS_TOP
+                                                                    : TRACE_SS);
 
 // Device buffer for the traversal stack entries beyond the LDS part, one
 // column per lane of a persistent trace grid of up to `blocks` blocks
@@ -218,7 +231,7 @@ struct TraceWork {
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
                         const SceneDev &S, const StackOvf &ovf,
                         const TraceWork &w, uint32_t *ctr,
-                        unsigned long long *stats);
+                        unsigned long long *stats, bool top);
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr);
 hipError_t launch_resolve(hipStream_t st, uint32_t blocks,

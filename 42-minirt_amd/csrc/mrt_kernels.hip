@@ -237,12 +237,13 @@ __device__ __forceinline__ uint32_t tag_light(uint32_t tag) { return (tag >> 12)
 // whatever the number of rays).  Chunks [0, ncc) hold closest-hit rays,
 // [ncc, ncc + ncs) shadow rays; a chunk is one kind, so the branch is
 // wave-uniform.
-template <bool COUNT, bool BATCH, bool EXT>
+template <bool COUNT, bool BATCH, bool EXT, int SS = TRACE_SS, bool TOP = false>
 __device__ __forceinline__ void
 trace_work(const SceneDev &S, const TraceWork &w, uint32_t *__restrict__ ctr,
            unsigned long long *__restrict__ stats,
            const BatchScene *__restrict__ bs, const uint2 *__restrict__ slot_info,
-           uint32_t *__restrict__ ovf_buf, uint32_t *s_code)
+           uint32_t *__restrict__ ovf_buf, uint32_t *s_code,
+           const float4 *top = nullptr, uint32_t ntop = 0)
 {
     const uint32_t nc = w.closest ? ctr[CTR_CUR] : 0u;
     const uint32_t ns = w.shadow_ctr >= 0 ? min(ctr[w.shadow_ctr], w.cap_shadow) : 0u;
@@ -271,10 +272,10 @@ trace_work(const SceneDev &S, const TraceWork &w, uint32_t *__restrict__ ctr,
                                                       code, bu, bv, nodes_c,
                                                       prims_c);
                 } else {
-                    closest_hit<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), lc,
-                                                      TRACE_BLOCK, ovf, os, t,
-                                                      code, bu, bv, nodes_c,
-                                                      prims_c);
+                    closest_hit<COUNT, SS, EXT, TOP>(S, xyz(o4), xyz(d4), lc,
+                                                     TRACE_BLOCK, ovf, os, t,
+                                                     code, bu, bv, nodes_c,
+                                                     prims_c, top, ntop);
                 }
                 w.hit[i] = make_float4(t, __uint_as_float(code), bu, bv);
             }
@@ -290,9 +291,10 @@ trace_work(const SceneDev &S, const TraceWork &w, uint32_t *__restrict__ ctr,
                                                          lc, TRACE_BLOCK, ovf, os,
                                                          nodes_s, prims_s);
                 } else {
-                    occ = occluded<COUNT, TRACE_SS, EXT>(S, xyz(o4), xyz(d4), o4.w,
-                                                         lc, TRACE_BLOCK, ovf, os,
-                                                         nodes_s, prims_s);
+                    occ = occluded<COUNT, SS, EXT, TOP>(S, xyz(o4), xyz(d4), o4.w,
+                                                        lc, TRACE_BLOCK, ovf, os,
+                                                        nodes_s, prims_s, top,
+                                                        ntop);
                 }
                 w.vis[i] = occ ? 0 : 1;
             }
@@ -320,6 +322,24 @@ k_trace(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
 {
     __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
     trace_work<COUNT, BATCH, false>(S, w, ctr, stats, bs, slot_info, ovf_buf, s_code);
+}
+
+// k_trace with the tree's top nodes staged in LDS once per block (the
+// persistent grid's blocks live for the whole launch), for single scenes.
+template <bool COUNT>
+__global__ void __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(8, 8)))
+k_trace_top(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
+            unsigned long long *__restrict__ stats, uint32_t *__restrict__ ovf_buf)
+{
+    __shared__ uint32_t s_code[TRACE_SS_TOP * TRACE_BLOCK];
+    __shared__ float4 s_top[TOP_NODES * 7];
+    const uint32_t ntop = min((uint32_t)TOP_NODES, S.n_nodes);
+    for (uint32_t k = threadIdx.x; k < ntop * 7; k += TRACE_BLOCK)
+        s_top[k] = S.nodes[(size_t)(k / 7) * 8 + k % 7];
+    __syncthreads();
+    trace_work<COUNT, false, false, TRACE_SS_TOP, true>(
+        S, w, ctr, stats, nullptr, nullptr, ovf_buf, s_code, s_top, ntop);
 }
 
 // With quadrics or alpha-tested textures: the longer leaf code keeps the
@@ -2241,12 +2261,23 @@ static bool ovf_fits(const StackOvf &ovf, uint32_t blocks)
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
                         const SceneDev &S, const StackOvf &ovf,
                         const TraceWork &w, uint32_t *ctr,
-                        unsigned long long *stats)
+                        unsigned long long *stats, bool top)
 {
     if (!ovf_fits(ovf, blocks))
         return hipErrorInvalidValue;
     const BatchScene *nb = nullptr;
     const uint2 *ni = nullptr;
+    // the tree's top levels from LDS: needs the root at node 0 and the
+    // plain (triangle / sphere) leaf code
+    if (top && S.root == 0 && !(S.quad | S.alpha)) {
+        if (count)
+            hipLaunchKernelGGL(k_trace_top<true>, dim3(blocks), dim3(TRACE_BLOCK),
+                               0, st, S, w, ctr, stats, ovf.buf);
+        else
+            hipLaunchKernelGGL(k_trace_top<false>, dim3(blocks), dim3(TRACE_BLOCK),
+                               0, st, S, w, ctr, stats, ovf.buf);
+        return hipGetLastError();
+    }
 #define MRT_GO(C, Q)                                                          \
     do {                                                                      \
         if (Q)                                                                \

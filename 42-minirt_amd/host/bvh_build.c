@@ -658,6 +658,57 @@ static int32_t collapse4(col4 *C, int32_t code, uint32_t level)
     return (int32_t)k;
 }
 
+/* Renumbers the nodes so that the first n_top of a breadth-first walk from
+ * the root come first, in that order, and the others follow in their
+ * pre-order: the top levels then form one contiguous block at the front,
+ * which a kernel can copy into LDS with one linear loop. */
+static int top_first(mrt_bvh4_node **nodes, uint32_t n, uint32_t n_top)
+{
+    if (n <= 1)
+        return MRT_OK;
+    if (n_top > n)
+        n_top = n;
+    uint32_t *order = malloc((size_t)n_top * sizeof(uint32_t));
+    uint32_t *nid = malloc((size_t)n * sizeof(uint32_t));
+    mrt_bvh4_node *out = malloc((size_t)n * sizeof(mrt_bvh4_node));
+    if (!order || !nid || !out) {
+        free(order);
+        free(nid);
+        free(out);
+        return MRT_E_NOMEM;
+    }
+    for (uint32_t k = 0; k < n; k++)
+        nid[k] = UINT32_MAX;
+    const mrt_bvh4_node *in = *nodes;
+    uint32_t head = 0, tail = 0;
+    order[tail++] = 0;
+    nid[0] = 0;
+    while (head < tail && tail < n_top) {
+        const mrt_bvh4_node *nd = &in[order[head++]];
+        for (int i = 0; i < 4 && tail < n_top; i++)
+            if (nd->child[i] >= 0) {
+                nid[nd->child[i]] = tail;
+                order[tail++] = (uint32_t)nd->child[i];
+            }
+    }
+    uint32_t next = tail;
+    for (uint32_t k = 0; k < n; k++)
+        if (nid[k] == UINT32_MAX)
+            nid[k] = next++;
+    for (uint32_t k = 0; k < n; k++) {
+        mrt_bvh4_node nd = in[k];
+        for (int i = 0; i < 4; i++)
+            if (nd.child[i] >= 0)
+                nd.child[i] = (int32_t)nid[nd.child[i]];
+        out[nid[k]] = nd;
+    }
+    free(order);
+    free(nid);
+    free(*nodes);
+    *nodes = out;
+    return MRT_OK;
+}
+
 int mrt_bvh4_build(const mrt_bvh *b, mrt_bvh4 *out)
 {
     if (!b || !out)
@@ -669,6 +720,8 @@ int mrt_bvh4_build(const mrt_bvh *b, mrt_bvh4 *out)
     memset(&C, 0, sizeof(C));
     C.b = b;
     (void)collapse4(&C, 0, 1);
+    if (!C.err)
+        C.err = top_first(&C.nodes, C.n, MRT_BVH4_TOP);
     if (C.err) {
         free(C.nodes);
         return C.err;

@@ -471,6 +471,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=None,
                     help="MRT_OPT_LANES (default: the library's)")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--top-cache", type=int, default=None,
+                    help="MRT_OPT_TRACE_TOP (default: the library's)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ticks", action="store_true",
@@ -542,6 +544,8 @@ def main():
     r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
     if args.lanes is not None:
         r.set_option(M.OPT_LANES, args.lanes)
+    if args.top_cache is not None:
+        r.set_option(M.OPT_TRACE_TOP, args.top_cache)
 
     # 1) counting pass: nodes / primitives per ray (same sample indices)
     r.set_option(M.OPT_STATS, M.STATS_COUNT)

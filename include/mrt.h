@@ -216,6 +216,12 @@ typedef struct mrt_scene {
                                   longest ray (same image).  Default 2; a lane
                                   gets at least 65,536 paths                */
 
+#define MRT_OPT_TRACE_TOP  11  /* 1 (default): the trace kernel of scenes of
+                                  triangles and spheres keeps the tree's top
+                                  levels (the first nodes, laid out
+                                  breadth-first) in LDS; 0: every node from
+                                  global memory (same results)              */
+
 /* BVH builders (MRT_OPT_BUILDER). */
 #define MRT_BUILDER_HOST    0  /* binned SAH on the host (default)            */
 #define MRT_BUILDER_DEVICE  1  /* PLOC clustering on the device; falls back to

@@ -79,10 +79,15 @@ typedef struct mrt_bvh4_node {
 } mrt_bvh4_node;                      /* 128 bytes */
 
 #define MRT_BVH4_MAX_DEPTH 40u        /* stack: 3 entries per level + 1   */
+#define MRT_BVH4_TOP       64u        /* nodes laid out breadth-first at the
+                                         front of the array (the top levels
+                                         the trace kernels may keep in LDS) */
 
 typedef struct mrt_bvh4 {
     uint32_t       n_nodes;
-    mrt_bvh4_node *nodes;             /* node 0 is the root, pre-order      */
+    mrt_bvh4_node *nodes;             /* node 0 is the root; the first
+                                         MRT_BVH4_TOP nodes in breadth-first
+                                         order, the rest in pre-order       */
     uint32_t       depth;             /* deepest node level (root = 1)      */
 } mrt_bvh4;
 
