@@ -120,6 +120,8 @@ struct mrt_ctx {
 
     // accumulation
     float       *d_accum = nullptr;
+    float4      *d_tape = nullptr;    // the adjoint's tape
+    size_t       tape_cap = 0;        // float4 entries
     float       *h_stage = nullptr;   // pinned staging of mrt_readback_add
     uint32_t    *d_ctr = nullptr;
     unsigned long long *d_stats = nullptr;
@@ -380,6 +382,7 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     free_env(c);
     free_photons(c);
     dfree(c->d_accum);
+    dfree(c->d_tape);
     if (c->h_stage)
         (void)hipHostFree(c->h_stage);
     dfree(c->d_ctr);
@@ -1273,7 +1276,7 @@ struct Timer {
 static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
                        const SceneDev &S, bool count, uint32_t max_depth,
                        uint32_t P, uint32_t off, uint32_t rpp, uint32_t *ctr,
-                       const StackOvf &ovf)
+                       const StackOvf &ovf, float4 *tape = nullptr)
 {
     const size_t so = (size_t)off * rpp;
     const uint32_t cap_shadow = P * rpp;
@@ -1340,6 +1343,8 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         B.p_k = c->p_k + off;
         B.cap_paths = P;
         B.cap_shadow = cap_shadow;
+        B.tape = tape;
+        B.tape_P = P;
         LAUNCH(c, tm, K_SHADE, launch_shade(st, gb, fp, B, ctr));
         if (c->has_photons)
             LAUNCH(c, tm, K_CAUSTIC,
@@ -1857,36 +1862,70 @@ extern "C" int mrt_adjoint(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples
             || (g_hdri && (e = launch_fill_u32(c->stream, (uint32_t *)d_gt, 0u, ntex))
                               != hipSuccess))
             break;
+        // Record: the forward wavefront (the same kernels, queues and RNG
+        // keys as mrt_render, one lane) replays the paths and writes each
+        // vertex's record to the tape; backward: one thread per path walks
+        // its records in reverse.  At most 16 M paths per pass (a tape of
+        // (2 + 4 depth) x 16 B per path: 9 GB at depth 8).
         FrameParams fp = frame_params(c);
+        fp.slot0 = 0;
         SceneDev S = scene_dev(c);
-        ShadeBufs B;
-        memset(&B, 0, sizeof(B));
-        B.planes = c->d_planes;
-        B.prims = c->d_prims;
-        B.mats = c->d_mats;
-        B.lights = c->d_lights;
-        B.nrm = c->d_nrm;
-        B.tx = tex_dev(c);
         AdjBufs A;
         A.dimg = d_img;
         A.g_mat = d_gm;
         A.g_tex = d_gt;
         A.n_mat = c->n_mats;
         A.inv_spp = 1.0f / (float)n_samples;
-        uint32_t nb = c->path_budget / WH;
+        uint32_t nb = std::min(c->path_budget, 1u << 24) / WH;
         nb = nb < 1 ? 1 : (nb > n_samples ? n_samples : nb);
-        for (uint32_t s = 0; s < n_samples && e == hipSuccess; s += nb) {
-            uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
-            uint64_t P = (uint64_t)n * WH;
-            fp.sample0 = first_sample + s;
-            uint64_t blocks = (P + TRACE_BLOCK - 1) / TRACE_BLOCK;
-            // many short blocks: the block scheduler then balances paths of
-            // unequal length (56 ms with 16 blocks per CU, 48 ms from 128 on,
-            // inverse-rendering workload)
-            uint64_t cap = (uint64_t)c->num_cu * 128;
-            e = launch_adjoint(c->stream, (uint32_t)(blocks < cap ? blocks : cap), fp,
-                               S, B, A, (uint32_t)P, max_depth);
+        const uint32_t P = nb * WH;
+        const uint32_t rpp = shadow_rays_per_path(c);
+        const uint64_t S64 = (uint64_t)P * rpp;
+        if (S64 > 0xFFFFFFF0ull) {
+            rc = fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
+            break;
         }
+        if ((rc = ensure_wavefront(c, P, (uint32_t)S64)) != MRT_OK)
+            break;
+        if (ensure_ovf(c, c->trace_blocks, 1) != hipSuccess) {
+            rc = fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
+            break;
+        }
+        const size_t tape_n = tape_row(max_depth, 0) * (size_t)P;
+        if (tape_n > c->tape_cap) {
+            dfree(c->d_tape);
+            c->tape_cap = 0;
+            if (dalloc(c->d_tape, tape_n) != hipSuccess) {
+                rc = fail(c, MRT_E_NOMEM, "adjoint tape (%zu bytes) does not fit",
+                          tape_n * 16);
+                break;
+            }
+            c->tape_cap = tape_n;
+        }
+        ShadeBufs Bb;
+        memset(&Bb, 0, sizeof(Bb));
+        Bb.mats = c->d_mats;
+        Bb.tape = c->d_tape;
+        Timer tm{c, false, {}};
+        for (uint32_t s = 0; s < n_samples && e == hipSuccess && rc == MRT_OK;
+             s += nb) {
+            const uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
+            const uint32_t Pn = n * WH;
+            fp.sample0 = first_sample + s;
+            Bb.tape_P = Pn;
+            if ((e = launch_tape_init(c->stream, c->d_tape, Pn, max_depth))
+                != hipSuccess)
+                break;
+            rc = render_lane(c, c->stream, tm, fp, S, false, max_depth, Pn, 0,
+                             rpp, c->d_ctr, stack_ovf(c, 0), c->d_tape);
+            if (rc != MRT_OK)
+                break;
+            const uint32_t blocks = std::min((Pn + 255) / 256,
+                                             (uint32_t)c->num_cu * 16u);
+            e = launch_adjoint_back(c->stream, blocks, fp, Bb, A, Pn);
+        }
+        if (rc != MRT_OK)
+            break;
         if (e != hipSuccess)
             break;
         if ((e = hipMemcpyAsync(g_mat, d_gm, ng * 4, hipMemcpyDeviceToHost,

@@ -174,6 +174,17 @@ struct BatchCtl {
 
 constexpr uint32_t TAG_ENV = 0x40000000u;   // shadow-ray tag of the HDRI ray
 
+// The adjoint's tape: float4 rows of tape_P entries (one per path slot).
+// Rows 0 and 1 end each path: (escape radiance before thr, its texel
+// weight) and (texel index, vertex count); vertex b has rows
+// tape_row(b, 0..3): (thr in, material), (scatter factor, GGX score),
+// (albedo derivative, environment-sample weight), (next-event radiance,
+// environment-sample texel).
+__host__ __device__ constexpr size_t tape_row(uint32_t bounce, uint32_t k)
+{
+    return 2 + 4 * (size_t)bounce + k;
+}
+
 struct ShadeBufs {
     const float4 *q_o, *q_d, *q_t, *hit;
     float4       *qn_o, *qn_d, *qn_t;
@@ -194,6 +205,9 @@ struct ShadeBufs {
     const BatchScene *bs;
     uint2            *slot_info;
     TexDev            tx;      // textures (when FrameParams::textured)
+    // the adjoint's recording replay only (else null)
+    float4           *tape;
+    uint32_t          tape_P;
 };
 
 // Moves one scene's nodes / records, already copied into a batch's shared
@@ -266,9 +280,13 @@ struct AdjBufs {
     uint32_t     n_mat;
     float        inv_spp;
 };
-hipError_t launch_adjoint(hipStream_t st, uint32_t blocks, const FrameParams &fp,
-                          const SceneDev &S, const ShadeBufs &B,
-                          const AdjBufs &A, uint32_t P, uint32_t max_depth);
+// Rows 0-1 of the tape for P paths: nothing escaped, max_depth vertices.
+hipError_t launch_tape_init(hipStream_t st, float4 *tape, uint32_t P,
+                            uint32_t max_depth);
+// The backward pass over the tape of a recorded replay of P paths.
+hipError_t launch_adjoint_back(hipStream_t st, uint32_t blocks,
+                               const FrameParams &fp, const ShadeBufs &B,
+                               const AdjBufs &A, uint32_t P);
 
 // batched scenes (the same stages over the scene table of a batch)
 // Camera rays of a render lane's part [o0, o0 + Pl) of a pass of P paths

@@ -500,6 +500,113 @@ __device__ __forceinline__ bool scatter(const RngKey &rk, float ior_all, V3 nrm,
     return false;
 }
 
+// d/d(roughness) of log p(m) for the GGX micro-normal that ggx_sample draws
+// from u2: p(m) = D(m) cos(theta_m), alpha = roughness^2,
+// d log D / d alpha = 2/alpha - 4 alpha / (alpha^2 + tan^2 theta).
+__device__ __forceinline__ float ggx_score(float rough, float u2)
+{
+    float a = rough * rough, a2 = a * a;
+    float c2 = (1.0f - u2) / (1.0f + (a2 - 1.0f) * u2);
+    float t2 = (1.0f - c2) / fmaxf(c2, 1e-30f);
+    return (2.0f / a - 4.0f * a / (a2 + t2)) * (2.0f * rough);
+}
+
+__device__ __forceinline__ void metal_adj(const RngKey &rk, V3 n, const Mat &m,
+                                          V3 &d, V3 &f, V3 &df, float &score)
+{
+    float cth = -dot(unit(d), n);
+    V3 mn = n;
+    if (m.b.x > 0.0f) {
+        U4 g = rk(2u);
+        mn = ggx_sample(n, m.b.x, g.x, g.y);
+        score = ggx_score(m.b.x, g.x);
+    }
+    d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
+    float p5 = schlick_pow5(cth);
+    f = v3(m.a.x + (1.0f - m.a.x) * p5, m.a.y + (1.0f - m.a.y) * p5,
+           m.a.z + (1.0f - m.a.z) * p5);
+    df = v3(1.0f - p5, 1.0f - p5, 1.0f - p5);
+}
+
+__device__ __forceinline__ void refract_adj(const RngKey &rk, float ior_all, V3 n,
+                                            bool front, const Mat &m, V3 &d,
+                                            V3 &f, V3 &df, float &score)
+{
+    float ior = m.b.y;
+    float eta = front ? ior_all / ior : ior / ior_all;
+    d = unit(d);
+    float cth = -dot(d, n);
+    if (cth > 1.0f)
+        cth = 1.0f;
+    float sth = sqrtf(1.0f - cth * cth);
+    if (eta * sth > 1.0f) {
+        metal_adj(rk, n, m, d, f, df, score);
+        return;
+    }
+    float c2 = dot(neg(d), n);
+    if (c2 > 1.0f)
+        c2 = 1.0f;
+    V3 perp = mul(add(d, mul(n, c2)), eta);
+    V3 par = mul(n, -sqrtf(fabsf(1.0f - dot(perp, perp))));
+    V3 out = unit(add(par, perp));
+    if (m.b.x > 0.0f) {
+        U4 g = rk(2u);
+        out = ggx_sample(out, m.b.x, g.x, g.y);
+        score = ggx_score(m.b.x, g.x);
+    }
+    d = out;
+    f = v3(m.a.x, m.a.y, m.a.z);
+    df = v3(1.0f, 1.0f, 1.0f);
+}
+
+// scatter() with the factor, its albedo derivative and the GGX score.
+__device__ __forceinline__ bool scatter_adj(const RngKey &rk, float ior_all, V3 nrm,
+                                            bool front, const Mat &m, V3 &d,
+                                            V3 &f, V3 &df, float &score)
+{
+    f = v3(1.0f, 1.0f, 1.0f);
+    df = v3(0.0f, 0.0f, 0.0f);
+    score = 0.0f;
+    bool is_metal;
+    if (m.a.w == 1.0f) {
+        is_metal = true;
+    } else if (m.a.w == 0.0f) {
+        is_metal = false;
+    } else {
+        U4 u = rk(1u);
+        is_metal = m.a.w > u.x;
+    }
+    if (is_metal) {
+        metal_adj(rk, nrm, m, d, f, df, score);
+        return false;
+    }
+    if (!(m.b.y > 0.0f))
+        return true;
+    U4 u = rk(1u);
+    float cth = -dot(unit(d), nrm);
+    float q = (m.b.y - 1.0f) / (m.b.y + 1.0f);
+    float f0 = powf(q, 2.0f);
+    float refl = f0 + (1.0f - f0) * schlick_pow5(cth);
+    if (refl >= u.y && m.b.y != 1.0f) {
+        V3 mn = nrm;
+        if (m.b.x > 0.0f) {
+            U4 g = rk(2u);
+            mn = ggx_sample(nrm, m.b.x, g.x, g.y);
+            score = ggx_score(m.b.x, g.x);
+        }
+        d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
+        return false;
+    }
+    if (m.b.z == 1.0f) {
+        refract_adj(rk, ior_all, nrm, front, m, d, f, df, score);
+        return false;
+    }
+    if (m.b.z == 0.0f || m.b.z < u.z)
+        return true;
+    refract_adj(rk, ior_all, nrm, front, m, d, f, df, score);
+    return false;
+}
+
 // Hit point, normal facing the ray, side and material of a hit record
 // h4 = (t, code, u, v): hit_plane (hit_plane.c:55-63), hit_sphere with
 // set_normal_face (hit_sphere.c:37-46, normal.c:16-29), hit_triangle's
@@ -635,6 +742,16 @@ __device__ __forceinline__ V3 env_texel(const FrameParams &fp, V3 dir)
     return v3(t.x * fp.env_scale, t.y * fp.env_scale, t.z * fp.env_scale);
 }
 
+// Texel index of env_texel.
+__device__ __forceinline__ uint32_t env_index(const FrameParams &fp, V3 dir)
+{
+    V3 d = unit(dir);
+    double v = 0.5 + asin((double)d.y) / PI_D;
+    double u = 0.5 - atan2((double)d.z, (double)d.x) / (2.0 * PI_D);
+    int x = (int)(u * (fp.env_w - 1)), y = (int)(v * (fp.env_h - 1));
+    return (uint32_t)y * fp.env_w + (uint32_t)x;
+}
+
 // Largest index i in [0, n-1] with cdf[i] <= u (cdf[0] = 0).
 __device__ __forceinline__ uint32_t cdf_search(const float *cdf, uint32_t n, float u)
 {
@@ -673,7 +790,10 @@ __device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
 // At least 4 waves per SIMD: the HDRI variants are held to 128 VGPRs
 // (spilling a little) instead of 160-186 and 2-3 waves, 2 % more
 // Msamples/s on the caustics workload; the others need fewer already.
-template <bool BATCH, int ENV, bool TEX>
+// REC: the adjoint's recording replay (single scene, no textures, the MIS
+// or no environment, exact light sums): the same paths and image, and per
+// vertex a tape record for the backward pass (see k_adjoint_back).
+template <bool BATCH, int ENV, bool TEX, bool REC = false>
 __global__ void __launch_bounds__(SHADE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(4)))
 k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
@@ -732,6 +852,21 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                 a.y = a.y + bg.y * thr.y * wmis;
                 a.z = a.z + bg.z * thr.z * wmis;
                 B.acc[slot] = a;
+                if (REC) {
+                    // the radiance the path escapes with (before thr), its
+                    // texel and weight, and the number of vertices
+                    uint32_t et = HIT_NONE;
+                    float eg = 0.0f;
+                    if (ENV && F.has_env) {
+                        et = env_index(fp, d);
+                        eg = fp.env_scale * ao_last * wmis;
+                    }
+                    float4 *te = B.tape;
+                    te[slot] = make_float4(bg.x * wmis, bg.y * wmis, bg.z * wmis, eg);
+                    te[B.tape_P + slot] = make_float4(__uint_as_float(et),
+                                                      __uint_as_float(fp.bounce),
+                                                      0.0f, 0.0f);
+                }
             } else {
                 uint32_t mat;
                 bool front;
@@ -754,7 +889,25 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     B.acc[slot] = a;
                 }
                 const RngKey rk{F.seed, pixel, sample, fp.bounce, 0u};
-                bool diffuse = scatter(rk, F.ior_all, nrm, front, m, d, thr);
+                const V3 thr_in = thr;
+                V3 f_adj = v3(1.f, 1.f, 1.f), df_adj = v3(0.f, 0.f, 0.f);
+                float sc_adj = 0.0f, ng_adj = 0.0f;
+                uint32_t nt_adj = HIT_NONE;
+                bool diffuse;
+                if (REC) {
+                    // scatter() with the factor it multiplies into thr, its
+                    // albedo derivative and the GGX score
+                    diffuse = scatter_adj(rk, F.ior_all, nrm, front, m, d, f_adj,
+                                          df_adj, sc_adj);
+                    if (diffuse) {
+                        f_adj = v3(m.a.x, m.a.y, m.a.z);
+                        df_adj = v3(1.f, 1.f, 1.f);
+                    } else {
+                        thr = had(thr, f_adj);
+                    }
+                } else {
+                    diffuse = scatter(rk, F.ior_all, nrm, front, m, d, thr);
+                }
                 if (diffuse) {
                     // default_mat (src/material/material_default.c:71-92)
                     kind = SC_DIFFUSE;
@@ -817,6 +970,11 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                                 env_rad = had(mul(env_texel(fp, env_d), ao), thr);
                                 env_rad = mul(env_rad, (cn / (3.14159265358979324f * pl)) * wl);
                                 env_ray = true;
+                                if (REC) {
+                                    nt_adj = env_index(fp, env_d);
+                                    ng_adj = fp.env_scale * ao
+                                             * ((cn / (3.14159265358979324f * pl)) * wl);
+                                }
                             }
                         }
                         n_rays += env_ray ? 1u : 0u;
@@ -827,6 +985,21 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                     want_pend = (n_rays || F.has_photons) ? 1u : 0u;
                 }
                 want_next = fp.last_bounce ? 0u : 1u;
+                if (REC) {
+                    // this vertex: throughput in, factor and score, albedo
+                    // derivative and the environment light sample's weight;
+                    // the next-event sum comes from k_resolve when the
+                    // vertex has a pending record (else it is zero)
+                    float4 *tv = B.tape + tape_row(fp.bounce, 0) * B.tape_P + slot;
+                    tv[0] = make_float4(thr_in.x, thr_in.y, thr_in.z,
+                                        __uint_as_float(mat));
+                    tv[B.tape_P] = make_float4(f_adj.x, f_adj.y, f_adj.z, sc_adj);
+                    tv[2 * (size_t)B.tape_P] = make_float4(df_adj.x, df_adj.y,
+                                                           df_adj.z, ng_adj);
+                    tv[3 * (size_t)B.tape_P] = make_float4(0.f, 0.f, 0.f,
+                                                           __uint_as_float(HIT_NONE));
+                    env_omc = __uint_as_float(nt_adj);   // to k_resolve
+                }
             }
         }
         // allocations: every lane of the wave takes part
@@ -918,7 +1091,7 @@ __device__ __forceinline__ float lq(bool exact, float acc, float add)
     return exact ? s : floorf(s);
 }
 
-template <bool BATCH>
+template <bool BATCH, bool REC = false>
 __global__ void __launch_bounds__(256)
 k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
 {
@@ -981,12 +1154,13 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         s.x = s.x + a.x * dl.x;
         s.y = s.y + a.y * dl.y;
         s.z = s.z + a.z * dl.z;
-        if (b.w && B.vis[end]) {
+        const bool env_vis = b.w && B.vis[end];
+        if (env_vis) {
             float4 c = B.p_c[r];
             s.x = s.x + c.x;
             s.y = s.y + c.y;
             s.z = s.z + c.z;
-            if (F.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
+            if (!REC && F.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
                 float4 t = B.qn_t[b.x];
                 t.x = t.x * c.w;
                 t.y = t.y * c.w;
@@ -995,6 +1169,22 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             }
         }
         B.acc[slot] = s;
+        if (REC) {
+            // the vertex's next-event radiance (before thr and albedo): the
+            // exact light sum and the visible environment sample
+            float4 *tv = B.tape + tape_row(fp.bounce, 2) * B.tape_P + slot;
+            uint32_t nt = HIT_NONE;
+            V3 N = dl;
+            if (env_vis) {
+                nt = __float_as_uint(B.p_c[r].w);
+                if (nt != HIT_NONE) {
+                    const float ng = tv[0].w;
+                    const float4 tx = fp.env_tex[nt];
+                    N = add(N, v3(tx.x * ng, tx.y * ng, tx.z * ng));
+                }
+            }
+            tv[B.tape_P] = make_float4(N.x, N.y, N.z, __uint_as_float(nt));
+        }
     }
 }
 
@@ -1057,181 +1247,6 @@ k_accumulate(const float4 *__restrict__ acc, float *__restrict__ accum,
 // (the sampled micro-normal m moves with r; everything else is fixed).
 // The HDRI sampling tables stay those of the uploaded map.
 
-// Texel index of env_texel.
-__device__ __forceinline__ uint32_t env_index(const FrameParams &fp, V3 dir)
-{
-    V3 d = unit(dir);
-    double v = 0.5 + asin((double)d.y) / PI_D;
-    double u = 0.5 - atan2((double)d.z, (double)d.x) / (2.0 * PI_D);
-    int x = (int)(u * (fp.env_w - 1)), y = (int)(v * (fp.env_h - 1));
-    return (uint32_t)y * fp.env_w + (uint32_t)x;
-}
-
-// d/d(roughness) of log p(m) for the GGX micro-normal that ggx_sample draws
-// from u2: p(m) = D(m) cos(theta_m), alpha = roughness^2,
-// d log D / d alpha = 2/alpha - 4 alpha / (alpha^2 + tan^2 theta).
-__device__ __forceinline__ float ggx_score(float rough, float u2)
-{
-    float a = rough * rough, a2 = a * a;
-    float c2 = (1.0f - u2) / (1.0f + (a2 - 1.0f) * u2);
-    float t2 = (1.0f - c2) / fmaxf(c2, 1e-30f);
-    return (2.0f / a - 4.0f * a / (a2 + t2)) * (2.0f * rough);
-}
-
-__device__ __forceinline__ void metal_adj(const RngKey &rk, V3 n, const Mat &m,
-                                          V3 &d, V3 &f, V3 &df, float &score)
-{
-    float cth = -dot(unit(d), n);
-    V3 mn = n;
-    if (m.b.x > 0.0f) {
-        U4 g = rk(2u);
-        mn = ggx_sample(n, m.b.x, g.x, g.y);
-        score = ggx_score(m.b.x, g.x);
-    }
-    d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
-    float p5 = schlick_pow5(cth);
-    f = v3(m.a.x + (1.0f - m.a.x) * p5, m.a.y + (1.0f - m.a.y) * p5,
-           m.a.z + (1.0f - m.a.z) * p5);
-    df = v3(1.0f - p5, 1.0f - p5, 1.0f - p5);
-}
-
-__device__ __forceinline__ void refract_adj(const RngKey &rk, float ior_all, V3 n,
-                                            bool front, const Mat &m, V3 &d,
-                                            V3 &f, V3 &df, float &score)
-{
-    float ior = m.b.y;
-    float eta = front ? ior_all / ior : ior / ior_all;
-    d = unit(d);
-    float cth = -dot(d, n);
-    if (cth > 1.0f)
-        cth = 1.0f;
-    float sth = sqrtf(1.0f - cth * cth);
-    if (eta * sth > 1.0f) {
-        metal_adj(rk, n, m, d, f, df, score);
-        return;
-    }
-    float c2 = dot(neg(d), n);
-    if (c2 > 1.0f)
-        c2 = 1.0f;
-    V3 perp = mul(add(d, mul(n, c2)), eta);
-    V3 par = mul(n, -sqrtf(fabsf(1.0f - dot(perp, perp))));
-    V3 out = unit(add(par, perp));
-    if (m.b.x > 0.0f) {
-        U4 g = rk(2u);
-        out = ggx_sample(out, m.b.x, g.x, g.y);
-        score = ggx_score(m.b.x, g.x);
-    }
-    d = out;
-    f = v3(m.a.x, m.a.y, m.a.z);
-    df = v3(1.0f, 1.0f, 1.0f);
-}
-
-// scatter() with the factor, its albedo derivative and the GGX score.
-__device__ __forceinline__ bool scatter_adj(const RngKey &rk, float ior_all, V3 nrm,
-                                            bool front, const Mat &m, V3 &d,
-                                            V3 &f, V3 &df, float &score)
-{
-    f = v3(1.0f, 1.0f, 1.0f);
-    df = v3(0.0f, 0.0f, 0.0f);
-    score = 0.0f;
-    bool is_metal;
-    if (m.a.w == 1.0f) {
-        is_metal = true;
-    } else if (m.a.w == 0.0f) {
-        is_metal = false;
-    } else {
-        U4 u = rk(1u);
-        is_metal = m.a.w > u.x;
-    }
-    if (is_metal) {
-        metal_adj(rk, nrm, m, d, f, df, score);
-        return false;
-    }
-    if (!(m.b.y > 0.0f))
-        return true;
-    U4 u = rk(1u);
-    float cth = -dot(unit(d), nrm);
-    float q = (m.b.y - 1.0f) / (m.b.y + 1.0f);
-    float f0 = powf(q, 2.0f);
-    float refl = f0 + (1.0f - f0) * schlick_pow5(cth);
-    if (refl >= u.y && m.b.y != 1.0f) {
-        V3 mn = nrm;
-        if (m.b.x > 0.0f) {
-            U4 g = rk(2u);
-            mn = ggx_sample(nrm, m.b.x, g.x, g.y);
-            score = ggx_score(m.b.x, g.x);
-        }
-        d = unit(sub(d, mul(mn, 2.0f * dot(d, mn))));
-        return false;
-    }
-    if (m.b.z == 1.0f) {
-        refract_adj(rk, ior_all, nrm, front, m, d, f, df, score);
-        return false;
-    }
-    if (m.b.z == 0.0f || m.b.z < u.z)
-        return true;
-    refract_adj(rk, ior_all, nrm, front, m, d, f, df, score);
-    return false;
-}
-
-// Unquantised light sum / 255 at a diffuse hit (the shade + resolve pair with
-// MRT_SCENE_LIGHT_EXACT), shadow rays traced inline.
-__device__ __forceinline__ V3 lights_exact(const FrameParams &fp, const SceneDev &S,
-                                           const float4 *L, V3 p, V3 nrm,
-                                           uint32_t pixel, uint32_t sample,
-                                           uint32_t bounce, uint32_t *lc, uint32_t *ovf,
-                                           uint32_t &nodes, uint32_t &np)
-{
-    float lr = 0.0f, lg = 0.0f, lb = 0.0f;
-    V3 view = unit(sub(v3(fp.cam_pos[0], fp.cam_pos[1], fp.cam_pos[2]), p));
-    for (uint32_t k = 0; k < fp.n_lights; k++) {
-        float4 l0 = L[3 * k], l1 = L[3 * k + 1], l2 = L[3 * k + 2];
-        uint32_t ltype = __float_as_uint(l2.x);
-        int32_t lss = __float_as_int(l2.y);
-        bool point = ltype == MRT_LIGHT_POINT;
-        V3 lpos = xyz(l0);
-        V3 ld = point ? unit(sub(lpos, p)) : neg(lpos);
-        float ndl = dot(nrm, ld);
-        if (ndl <= 0.0f)
-            continue;
-        V3 nl = neg(ld);
-        V3 refl = sub(nl, mul(nrm, 2.0f * dot(nl, nrm)));
-        float rdv = fmaxf(dot(refl, view), 0.0f);
-        float sp64 = pow64(rdv);
-        bool soft = point && l1.w != 0.0f && lss > 1;
-        uint32_t cnt = soft ? (uint32_t)lss : 1u, nvis = 0;
-        for (uint32_t s = 0; s < cnt; s++) {
-            V3 sd;
-            float tmax;
-            if (!point) {
-                sd = unit(ld);
-                tmax = 2.0f * IT_MAX_F;
-            } else {
-                V3 tgt = lpos;
-                if (soft) {
-                    U4 u = rng4(fp.seed, pixel, sample, bounce, 16u + k * 4096u + s);
-                    tgt = add(lpos, mul(unit_sphere_pt(u), l1.w));
-                }
-                V3 dl = sub(tgt, p);
-                sd = unit(dl);
-                tmax = sqrtf(dot(dl, dl));
-            }
-            if (!occluded<false, TRACE_SS>(S, p, sd, tmax, lc, TRACE_BLOCK, ovf, 1u,
-                                           nodes, np))
-                nvis++;
-        }
-        if (!nvis)
-            continue;
-        float br = l0.w;
-        float factor = point && soft ? (float)nvis / (float)cnt : 1.0f;
-        float diff = br * ndl * factor, spec = br * sp64 * factor;
-        lr += l1.x * diff + l1.x * spec;
-        lg += l1.y * diff + l1.y * spec;
-        lb += l1.z * diff + l1.z * spec;
-    }
-    return v3(lr / 255.0f, lg / 255.0f, lb / 255.0f);
-}
-
 __device__ __forceinline__ void lds_add3(float *g, V3 v)
 {
     if (v.x != 0.0f)
@@ -1242,130 +1257,45 @@ __device__ __forceinline__ void lds_add3(float *g, V3 v)
         atomicAdd(g + 2, v.z);
 }
 
-// The replay waits on dependent fetches path after path; 3 waves per SIMD
-// (168 VGPRs, spilling into scratch) hide more of that than the compiler's
-// 2 (245 VGPRs): 63.6 -> 56.2 ms on the inverse-rendering workload, 4
-// waves no better.
-__global__ void __launch_bounds__(TRACE_BLOCK)
-__attribute__((amdgpu_waves_per_eu(3, 3)))
-k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
-          uint32_t max_depth)
+__global__ void __launch_bounds__(256)
+k_tape_init(float4 *__restrict__ tape, uint32_t P, uint32_t max_depth)
 {
-    __shared__ uint32_t s_code[TRACE_SS * TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P)
+        return;
+    tape[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    tape[(size_t)P + i] = make_float4(__uint_as_float(HIT_NONE),
+                                      __uint_as_float(max_depth), 0.0f, 0.0f);
+}
+
+// The backward pass of the path-replay adjoint, one path per thread over
+// the tape its recorded replay (k_shade / k_resolve with REC) wrote: from
+// the escape radiance X after the last vertex, vertex v = n-1 .. 0 gets
+//   dC/dalbedo_v = dl thr_v df_v (N_v + X),   dC/dE_v = dl thr_v,
+//   dC/drough_v  = score_v dl . (thr_v f_v X)  (score-function term),
+//   dC/dtexel    = dl thr_v f_v w_v           (environment sample),
+// then X = E_v + f_v (N_v + X).  Material sums go through LDS, texel
+// gradients to HBM with float atomics.
+__global__ void __launch_bounds__(256)
+k_adjoint_back(FrameParams fp, ShadeBufs B, AdjBufs A, uint32_t P)
+{
     __shared__ float s_g[ADJ_MAXMAT * ADJ_NG];
     for (uint32_t k = threadIdx.x; k < ADJ_MAXMAT * ADJ_NG; k += blockDim.x)
         s_g[k] = 0.0f;
     __syncthreads();
-    uint32_t *lc = s_code + threadIdx.x;
-    uint32_t ovf_l[TRACE_OVF];
-    uint32_t nodes = 0, np = 0;
+    const size_t TP = B.tape_P;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P;
          i += gridDim.x * blockDim.x) {
-        const uint32_t pixel = i % fp.WH, sample = fp.sample0 + i / fp.WH;
+        const uint32_t pixel = i % fp.WH;
         const V3 dl = v3(A.dimg[3 * (size_t)pixel] * A.inv_spp,
                          A.dimg[3 * (size_t)pixel + 1] * A.inv_spp,
                          A.dimg[3 * (size_t)pixel + 2] * A.inv_spp);
         if (dl.x == 0.0f && dl.y == 0.0f && dl.z == 0.0f)
             continue;
-        V3 o, d;
-        primary_ray(fp, pixel, sample, o, d);
-        uint32_t vm[ADJ_MAXD];
-        V3 vthr[ADJ_MAXD], vf[ADJ_MAXD], vdf[ADJ_MAXD], vN[ADJ_MAXD];
-        uint32_t vnt[ADJ_MAXD];
-        float vng[ADJ_MAXD], vsc[ADJ_MAXD];
-        uint32_t nv = 0;
-        V3 thr = v3(1.f, 1.f, 1.f), X = v3(0.f, 0.f, 0.f);
-        float ao_last = 1.0f, pb = 0.0f, eg = 0.0f;
-        uint32_t et = HIT_NONE;
-        for (uint32_t b = 0; b < max_depth; b++) {
-            float t, bu, bv;
-            uint32_t code;
-            if (!closest_hit<false, TRACE_SS>(S, o, d, lc, TRACE_BLOCK, ovf_l, 1u, t, code,
-                                              bu, bv, nodes, np)) {
-                // add_skybox; the MIS weight after a diffuse bounce
-                if (fp.has_env) {
-                    float w = 1.0f;
-                    if (pb > 0.0f) {
-                        float pl = env_pdf(fp, d);
-                        w = pb / (pb + pl);
-                    }
-                    et = env_index(fp, d);
-                    eg = fp.env_scale * ao_last * w;
-                    float4 tx = fp.env_tex[et];
-                    X = v3(tx.x * eg, tx.y * eg, tx.z * eg);
-                } else {
-                    X = mul(v3(fp.bg_clamped[0], fp.bg_clamped[1], fp.bg_clamped[2]),
-                            ao_last);
-                }
-                break;
-            }
-            V3 p, nrm;
-            bool front;
-            uint32_t mat;
-            surface(B.planes, B.prims, B.nrm, fp.one_sided != 0, o, d,
-                    make_float4(t, __uint_as_float(code), bu, bv), p, nrm, front,
-                    mat);
-            Mat m;
-            m.a = B.mats[3 * mat];
-            m.b = B.mats[3 * mat + 1];
-            m.c = B.mats[3 * mat + 2];
-            lds_add3(s_g + mat * ADJ_NG + 4, had(dl, thr));   // dC/dE = thr
-            const RngKey rk{fp.seed, pixel, sample, b, 0u};
-            V3 f, df, N = v3(0.f, 0.f, 0.f);
-            float sc, ng = 0.0f;
-            uint32_t nt = HIT_NONE;
-            if (scatter_adj(rk, fp.ior_all, nrm, front, m, d, f, df, sc)) {
-                f = v3(m.a.x, m.a.y, m.a.z);
-                df = v3(1.f, 1.f, 1.f);
-                if (fp.n_lights)
-                    N = lights_exact(fp, S, B.lights, p, nrm, pixel, sample, b, lc,
-                                     ovf_l, nodes, np);
-                if (fp.has_env) {
-                    U4 e = rng4(fp.seed, pixel, sample, b, 3u);
-                    const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
-                    uint32_t y = cdf_search(fp.mis_marg, nh, e.x);
-                    uint32_t x = cdf_search(fp.mis_cond + (size_t)y * fp.env_w, nw, e.y);
-                    float pc = fp.mis_pcell[(size_t)y * nw + x];
-                    float vv = ((float)y + e.z) / (float)nh;
-                    float uu = ((float)x + e.w) / (float)nw;
-                    float el = (vv - 0.5f) * 3.14159265358979324f;
-                    float ph = (0.5f - uu) * 6.28318530717958648f;
-                    float ce = cosf(el);
-                    V3 ed = v3(ce * cosf(ph), sinf(el), ce * sinf(ph));
-                    float cn = dot(nrm, ed);
-                    if (pc > 0.0f && ce > 0.0f && cn > 0.0f
-                        && !occluded<false, TRACE_SS>(S, p, ed, 2.0f * IT_MAX_F, lc,
-                                                      TRACE_BLOCK, ovf_l, 1u, nodes,
-                                                      np)) {
-                        float pl = pc * (float)nw * (float)nh
-                                   / (19.7392088021787172f * ce);
-                        float wl = pl / (pl + cn / 3.14159265358979324f);
-                        nt = env_index(fp, ed);
-                        ng = fp.env_scale * m.b.w
-                             * ((cn / (3.14159265358979324f * pl)) * wl);
-                        float4 tx = fp.env_tex[nt];
-                        N = add(N, v3(tx.x * ng, tx.y * ng, tx.z * ng));
-                    }
-                }
-                U4 g = rng4(fp.seed, pixel, sample, b, 2u);
-                d = cos_hemi(nrm, g.z, g.w);
-                pb = dot(nrm, d) / 3.14159265358979324f;
-            } else {
-                pb = 0.0f;
-            }
-            vm[nv] = mat;
-            vthr[nv] = thr;
-            vf[nv] = f;
-            vdf[nv] = df;
-            vN[nv] = N;
-            vnt[nv] = nt;
-            vng[nv] = ng;
-            vsc[nv] = sc;
-            nv++;
-            thr = had(thr, f);
-            o = p;
-            ao_last = m.b.w;
-        }
+        const float4 e0 = B.tape[i], e1 = B.tape[TP + i];
+        V3 X = xyz(e0);
+        const float eg = e0.w;
+        const uint32_t et = __float_as_uint(e1.x), nv = __float_as_uint(e1.y);
         if (nv == 0) {
             if (A.g_tex && et != HIT_NONE) {
                 atomicAdd(&A.g_tex[3 * (size_t)et], dl.x * eg);
@@ -1375,21 +1305,26 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
             continue;
         }
         for (int v = (int)nv - 1; v >= 0; v--) {
-            const uint32_t mt = vm[v];
-            const V3 Av = add(vN[v], X);
-            const V3 tf = had(vthr[v], vf[v]);
-            lds_add3(s_g + mt * ADJ_NG, had(had(dl, vthr[v]), had(vdf[v], Av)));
-            if (vsc[v] != 0.0f) {
-                float r = vsc[v] * dot(dl, had(tf, X));
+            const float4 *tv = B.tape + tape_row((uint32_t)v, 0) * TP + i;
+            const float4 t0 = tv[0], t1 = tv[TP], t2 = tv[2 * TP], t3 = tv[3 * TP];
+            const uint32_t mt = __float_as_uint(t0.w);
+            const V3 thr = xyz(t0), f = xyz(t1), df = xyz(t2), N = xyz(t3);
+            const float sc = t1.w, ng = t2.w;
+            const uint32_t nt = __float_as_uint(t3.w);
+            const V3 Av = add(N, X);
+            const V3 tf = had(thr, f);
+            lds_add3(s_g + mt * ADJ_NG + 4, had(dl, thr));   // dC/dE = thr
+            lds_add3(s_g + mt * ADJ_NG, had(had(dl, thr), had(df, Av)));
+            if (sc != 0.0f) {
+                float r = sc * dot(dl, had(tf, X));
                 if (r != 0.0f)
                     atomicAdd(s_g + mt * ADJ_NG + 3, r);
             }
             if (A.g_tex) {
-                if (vnt[v] != HIT_NONE) {
-                    float w = vng[v];
-                    atomicAdd(&A.g_tex[3 * (size_t)vnt[v]], dl.x * tf.x * w);
-                    atomicAdd(&A.g_tex[3 * (size_t)vnt[v] + 1], dl.y * tf.y * w);
-                    atomicAdd(&A.g_tex[3 * (size_t)vnt[v] + 2], dl.z * tf.z * w);
+                if (nt != HIT_NONE) {
+                    atomicAdd(&A.g_tex[3 * (size_t)nt], dl.x * tf.x * ng);
+                    atomicAdd(&A.g_tex[3 * (size_t)nt + 1], dl.y * tf.y * ng);
+                    atomicAdd(&A.g_tex[3 * (size_t)nt + 2], dl.z * tf.z * ng);
                 }
                 if (v == (int)nv - 1 && et != HIT_NONE) {
                     atomicAdd(&A.g_tex[3 * (size_t)et], dl.x * tf.x * eg);
@@ -1398,9 +1333,9 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
                 }
             }
             const float4 mc = B.mats[3 * mt + 2];
-            V3 Ev = mc.x > 0.0f ? v3(mc.y * mc.x, mc.z * mc.x, mc.w * mc.x)
-                                : v3(0.f, 0.f, 0.f);
-            X = add(Ev, had(vf[v], Av));
+            const V3 Ev = mc.x > 0.0f ? v3(mc.y * mc.x, mc.z * mc.x, mc.w * mc.x)
+                                      : v3(0.f, 0.f, 0.f);
+            X = add(Ev, had(f, Av));
         }
     }
     __syncthreads();
@@ -1409,12 +1344,19 @@ k_adjoint(FrameParams fp, SceneDev S, ShadeBufs B, AdjBufs A, uint32_t P,
             atomicAdd(&A.g_mat[k], s_g[k]);
 }
 
-hipError_t launch_adjoint(hipStream_t st, uint32_t blocks, const FrameParams &fp,
-                          const SceneDev &S, const ShadeBufs &B,
-                          const AdjBufs &A, uint32_t P, uint32_t max_depth)
+hipError_t launch_tape_init(hipStream_t st, float4 *tape, uint32_t P,
+                            uint32_t max_depth)
 {
-    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(TRACE_BLOCK), 0, st, fp, S, B,
-                       A, P, max_depth);
+    hipLaunchKernelGGL(k_tape_init, dim3((P + 255) / 256), dim3(256), 0, st, tape,
+                       P, max_depth);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_back(hipStream_t st, uint32_t blocks,
+                               const FrameParams &fp, const ShadeBufs &B,
+                               const AdjBufs &A, uint32_t P)
+{
+    hipLaunchKernelGGL(k_adjoint_back, dim3(blocks), dim3(256), 0, st, fp, B, A, P);
     return hipGetLastError();
 }
 
@@ -2301,6 +2243,18 @@ hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
     hipLaunchKernelGGL((k_shade<false, E, T>), dim3(blocks), dim3(SHADE_BLOCK), 0, \
                        st, fp, B, ctr)
     const int env = !fp.has_env ? 0 : fp.estimator == MRT_EST_REFERENCE ? 2 : 1;
+    if (B.tape) {
+        // the adjoint's recording replay: no textures, MIS or no environment
+        if (fp.textured || env == 2)
+            return hipErrorInvalidValue;
+        if (env == 1)
+            hipLaunchKernelGGL((k_shade<false, 1, false, true>), dim3(blocks),
+                               dim3(SHADE_BLOCK), 0, st, fp, B, ctr);
+        else
+            hipLaunchKernelGGL((k_shade<false, 0, false, true>), dim3(blocks),
+                               dim3(SHADE_BLOCK), 0, st, fp, B, ctr);
+        return hipGetLastError();
+    }
     if (fp.textured) {
         if (env == 2) MRT_SHADE(2, true); else if (env == 1) MRT_SHADE(1, true);
         else MRT_SHADE(0, true);
@@ -2316,8 +2270,12 @@ hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
                           uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_resolve<false>, dim3(blocks), dim3(256), 0, st, fp, B,
-                       ctr);
+    if (B.tape)
+        hipLaunchKernelGGL((k_resolve<false, true>), dim3(blocks), dim3(256), 0, st,
+                           fp, B, ctr);
+    else
+        hipLaunchKernelGGL((k_resolve<false, false>), dim3(blocks), dim3(256), 0,
+                           st, fp, B, ctr);
     return hipGetLastError();
 }
 
@@ -2367,7 +2325,7 @@ hipError_t launch_bresolve(hipStream_t st, uint32_t blocks,
                            const FrameParams &fp, const ShadeBufs &B,
                            uint32_t *ctr)
 {
-    hipLaunchKernelGGL(k_resolve<true>, dim3(blocks), dim3(256), 0, st, fp, B,
+    hipLaunchKernelGGL((k_resolve<true, false>), dim3(blocks), dim3(256), 0, st, fp, B,
                        ctr);
     return hipGetLastError();
 }
