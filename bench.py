@@ -339,27 +339,45 @@ def adjoint_leg(M, args):
                                                                  emis.shape),
                                   0.0, None), dtype=torch.float32,
                           requires_grad=True)
-    # texels as log-scale factors over the perturbed map
-    env_start = torch.tensor(env * np.exp(rng.normal(0, 0.2, env.shape)),
-                             dtype=torch.float32)
-    p_log = torch.zeros(env.shape, dtype=torch.float32, requires_grad=True)
+    # the HDRI: a smooth perturbation (a 16 x 8 grid of log-scale factors
+    # per channel, sigma 0.3, bilinearly stretched over the 256 x 128
+    # texels) and the same parametrisation to optimise; the adjoint's
+    # per-texel gradients reach the grid's cells through autograd
+    import torch.nn.functional as tnf
+    GX, GY = 16, 8
+
+    def field(coarse):
+        return tnf.interpolate(coarse.permute(2, 0, 1)[None], size=env.shape[:2],
+                               mode="bilinear", align_corners=False)[0] \
+            .permute(1, 2, 0)
+    noise = torch.tensor(rng.normal(0, 0.3, (GY, GX, 3)), dtype=torch.float32)
+    env_start = torch.tensor(env, dtype=torch.float32) * torch.exp(field(noise))
+    p_log = torch.zeros((GY, GX, 3), dtype=torch.float32, requires_grad=True)
     opt = torch.optim.Adam([
         {"params": [p_alb, p_emis], "lr": args.adj_lr},
         {"params": [p_rough], "lr": args.adj_lr * 0.5},
-        {"params": [p_log], "lr": args.adj_lr_env}])
+        # eps on the field's scale: cells that only weakly constrain the
+        # image (below the horizon, seen by a few escaping rays) get small,
+        # noisy gradients that Adam's normalisation would otherwise turn
+        # into full-size steps
+        {"params": [p_log], "lr": args.adj_lr_env, "eps": args.adj_env_eps}])
     alb_start = p_alb.detach().numpy().copy()
     rough_start = p_rough.detach().numpy().copy()
     losses = []
     t0 = time.perf_counter()
     for step in range(args.adam_steps):
         opt.zero_grad()
-        p_env = env_start * torch.exp(p_log)
+        p_env = env_start * torch.exp(field(p_log))
         f0 = 0 if args.adj_crn else spp * (2 * step + 1)
         out = diff.render(r, p_alb, p_rough, p_emis, p_env, spp, depth,
                           first_sample=f0,
                           grad_first_sample=None if args.adj_crn else f0 + spp)
         loss = ((out - target) ** 2).sum()
         loss.backward()
+        if step == 0:
+            g = p_log.grad.abs()
+            log(f"adam: field gradient |g| upper half mean {g[GY // 2:].mean():.3e}, "
+                f"lower half mean {g[:GY // 2].mean():.3e}")
         opt.step()
         with torch.no_grad():
             p_alb.clamp_(0.0, 1.0)
@@ -371,6 +389,13 @@ def adjoint_leg(M, args):
             log(f"adam step {step}: loss {losses[-1]:.4e}, albedo rmse {ae:.4f}")
     t_opt = time.perf_counter() - t0
     r.close()
+    env_end = (env_start * torch.exp(field(p_log))).detach().numpy()
+
+    def log_rmse(e, upper=False):
+        # upper: the rows above the horizon (row 0 is the nadir)
+        sl = slice(env.shape[0] // 2, None) if upper else slice(None)
+        return float(np.sqrt(np.mean(np.log((e[sl] + 1e-6)
+                                            / (env[sl] + 1e-6)) ** 2)))
     return {
         "workload": f"inverse rendering 512x512, {spp} spp, depth {depth}, "
                     f"{ntri} triangles, 16 materials (albedo, roughness, "
@@ -389,11 +414,12 @@ def adjoint_leg(M, args):
             (rough_start - rough) ** 2))), 5),
         "roughness_rmse_end": round(float(np.sqrt(np.mean(
             (p_rough.detach().numpy() - rough) ** 2))), 5),
-        "hdri_log_rmse_start": round(float(np.sqrt(np.mean(np.log(
-            (env_start.numpy() + 1e-6) / (env + 1e-6)) ** 2))), 5),
-        "hdri_log_rmse_end": round(float(np.sqrt(np.mean(np.log(
-            ((env_start * torch.exp(p_log)).detach().numpy() + 1e-6)
-            / (env + 1e-6)) ** 2))), 5),
+        "hdri_log_rmse_start": round(log_rmse(env_start.numpy()), 5),
+        "hdri_log_rmse_end": round(log_rmse(env_end), 5),
+        "hdri_log_rmse_upper_start": round(log_rmse(env_start.numpy(), True), 5),
+        "hdri_log_rmse_upper_end": round(log_rmse(env_end, True), 5),
+        "hdri_parametrisation": f"{GX}x{GY} grid of log-scale factors per "
+                                f"channel, bilinear over the texels",
         "samples": "common random numbers (the target's)" if args.adj_crn
                    else "fresh samples every step, gradient from a disjoint "
                         "sample range",
@@ -491,7 +517,8 @@ def main():
     ap.add_argument("--adam-steps", type=int, default=200)
     ap.add_argument("--adj-seed", type=int, default=5)
     ap.add_argument("--adj-lr", type=float, default=0.01)
-    ap.add_argument("--adj-lr-env", type=float, default=0.005)
+    ap.add_argument("--adj-lr-env", type=float, default=0.05)
+    ap.add_argument("--adj-env-eps", type=float, default=1.0)
     ap.add_argument("--adj-crn", action="store_true",
                     help="optimise on the target's own samples")
     ap.add_argument("--only-batched", action="store_true")
