@@ -243,7 +243,50 @@ def batched_leg(M, args):
             r.synchronize()
             t_plain += time.perf_counter() - t0
     log(f"plain sequential: {samples / t_plain / 1e6:.1f} Msamples/s")
+    # the same 64 geometries as small renders (256^2, 16 spp, 1 M paths
+    # each), where a plain render is short and its launches' tails weigh:
+    # both sides on device-built trees (fast uploads, the same trees)
+    small = {}
+    sw, sh, sspp = 256, 256, 16
+    bs_ = M.Batch()
+    bs_.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
+    for hs, d in zip(scenes, desc):
+        bs_.add(hs, sw, sh, sspp, d["rng_seed"])
+    bs_.commit()
+    small["tuning"] = bs_.autotune(args.depth)
+    bs_.render(args.depth)
+    bs_.clear()
+    t0 = time.perf_counter()
+    bs_.render(args.depth)
+    bs_.readback(0)
+    t_sb = time.perf_counter() - t0
+    bs_.close()
+    t_sp = 0.0
+    for k, (hs, d) in enumerate(zip(scenes, desc)):
+        with M.Renderer(sw, sh) as r:
+            r.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
+            r.upload(hs)
+            r.set_seed(d["rng_seed"])
+            r.reserve(sspp)
+            if k == 0:
+                r.render(0, 1, args.depth)
+                r.clear()
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.render(0, sspp, args.depth)
+            r.synchronize()
+            t_sp += time.perf_counter() - t0
+    ssamples = len(desc) * sw * sh * sspp
+    small.update({
+        "workload": f"the same {len(desc)} scenes at {sw}x{sh}, {sspp} spp",
+        "samples": ssamples,
+        "msamples_per_s": round(ssamples / t_sb / 1e6, 2),
+        "plain_sequential_msamples_per_s": round(ssamples / t_sp / 1e6, 2),
+        "speedup_vs_plain": round(t_sp / t_sb, 3)})
+    log(f"small batch: {small['msamples_per_s']} vs plain "
+        f"{small['plain_sequential_msamples_per_s']} Msamples/s")
     return {
+        "small_scenes": small,
         "workload": f"{len(desc)} scenes, triangles log-uniform in [1e4, 2e6] "
                     f"({sum(ntri)} total), resolutions "
                     f"{{256^2, 512^2, 960x540, 1920x1080}}, spp "

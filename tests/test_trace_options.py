@@ -1,0 +1,109 @@
+"""Trace-kernel choices that must not change the image: the LDS copy of
+the BVH's top levels (MRT_OPT_TRACE_TOP), the device builder's and the host
+builder's breadth-first top layout, queue reservation (mrt_reserve), and
+the converged 1080p image against the committed 16k-spp oracle band."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mrt_hip as M
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+STAT_KEYS = ("rays_closest", "rays_shadow", "nodes_closest", "prims_closest",
+             "nodes_shadow", "prims_shadow")
+
+
+def test_host_tree_top_is_breadth_first():
+    """The first nodes of the host-built 4-wide tree are a breadth-first
+    walk from the root (what the trace kernel copies into LDS), and the
+    renumbering keeps the tree: every node is reached exactly once."""
+    hs = M.HostScene.mesh(3, 32, 32, subdiv=5, floor_cells=32)
+    b = M.BuiltBvh(hs.view())
+    ch = b.node4_children().reshape(-1, 4)
+    n = len(ch)
+    assert n > 64
+    order, head = [0], 0
+    while head < len(order) and len(order) < 64:
+        for c in ch[order[head]]:
+            if c >= 0 and len(order) < 64:
+                order.append(int(c))
+        head += 1
+    assert order == list(range(len(order)))
+    seen = np.zeros(n, int)
+    stack = [0]
+    while stack:
+        k = stack.pop()
+        seen[k] += 1
+        stack.extend(int(c) for c in ch[k] if c >= 0)
+    assert (seen == 1).all()
+
+
+def test_trace_top_option_is_validated():
+    try:
+        r = M.Renderer(8, 8)
+    except M.MrtError as e:
+        assert e.code == M.MRT_E_NODEVICE
+        return
+    with r:
+        with pytest.raises(M.MrtError):
+            r.set_option(M.OPT_TRACE_TOP, 2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("builder", [M.BUILDER_HOST, M.BUILDER_DEVICE])
+def test_top_cache_gives_the_same_bits(builder):
+    hs = M.HostScene.mesh(3, 256, 144, subdiv=6, floor_cells=48)
+    out = {}
+    for top in (0, 1):
+        with M.Renderer(256, 144) as r:
+            r.set_option(M.OPT_BUILDER, builder)
+            r.upload(hs)
+            r.set_seed(77)
+            r.set_option(M.OPT_TRACE_TOP, top)
+            r.set_option(M.OPT_STATS, M.STATS_COUNT)
+            r.render(0, 4, 8)
+            st = r.stats()
+            out[top] = (r.readback(), {k: st[k] for k in STAT_KEYS})
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
+@pytest.mark.gpu
+def test_reserve_then_render_equals_render():
+    hs = M.HostScene.plain(7, 96, 54)
+    imgs = []
+    for reserve in (False, True):
+        with M.Renderer(96, 54) as r:
+            r.upload(hs)
+            r.set_seed(5)
+            if reserve:
+                r.reserve(6)
+            r.render(0, 6, 8)
+            imgs.append(r.readback())
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+
+
+@pytest.mark.gpu
+def test_1080p_band_converges_to_the_16k_oracle():
+    """The benchmarked workload's image error against the committed
+    16,384-spp fp64 oracle band (tests/golden/make_mesh1080_band.py): the
+    mean relative error falls as 1/sqrt(spp) and is below 1.5 % at 1024
+    spp (its value there is about 1.1 %)."""
+    ref = np.load(GOLDEN / "mesh1080_band_16k.npz")
+    r0, r1 = (int(x) for x in ref["rows"])
+    R = ref["mean"].astype(np.float64)
+    hs = M.HostScene.mesh(2, 1920, 1080)
+    errs = {}
+    with M.Renderer(1920, 1080) as r:
+        r.upload(hs)
+        r.set_seed(int(ref["seed"]))
+        for spp in (64, 1024):
+            r.clear()
+            r.render(0, spp, int(ref["depth"]))
+            img = r.readback()[r0:r1].astype(np.float64) / spp
+            errs[spp] = float(np.mean(np.abs(img - R) / np.maximum(R, 1e-3)))
+    print("errors", errs)
+    assert errs[1024] < 0.015
+    # 16x the samples: about a quarter of the error
+    assert 3.0 < errs[64] / errs[1024] < 5.0
