@@ -1348,7 +1348,8 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         LAUNCH(c, tm, K_SHADE, launch_shade(st, gb, fp, B, ctr));
         if (c->has_photons)
             LAUNCH(c, tm, K_CAUSTIC,
-                   launch_caustic(st, c->grid_blocks, fp, B, ctr));
+                   launch_caustic(st, c->grid_blocks, fp, B, ctr, count,
+                                  c->d_stats));
         LAUNCH(c, tm, K_ADVANCE, launch_advance(st, ctr, c->d_stats, P, b));
         Bp = B;
         fpp = fp;
@@ -1739,6 +1740,9 @@ extern "C" int mrt_get_stats(mrt_ctx *c, mrt_stats *out)
     out->ms_accum = c->ms[K_ACCUM];
     out->ms_total = c->ms[5];
     out->ms_caustic = c->ms[6];
+    out->gather_queries = st[STAT_GATHER_QUERIES];
+    out->gather_photons = st[STAT_GATHER_PHOTONS];
+    out->gather_cells = st[STAT_GATHER_CELLS];
     return MRT_OK;
 }
 

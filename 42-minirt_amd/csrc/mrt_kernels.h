@@ -94,6 +94,9 @@ enum : int {
     STAT_NODES_SHADOW = 4,
     STAT_PRIMS_SHADOW = 5,
     STAT_NONFINITE = 6,
+    STAT_GATHER_QUERIES = 7,   // caustic gathers (pending records)
+    STAT_GATHER_PHOTONS = 8,   // photon records they examined
+    STAT_GATHER_CELLS = 9,     // grid cells (bucket bounds) they read
     STAT_COUNT = 16
 };
 
@@ -327,7 +330,7 @@ hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
 hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
-                          uint32_t *ctr);
+                          uint32_t *ctr, bool count, unsigned long long *stats);
 hipError_t launch_caustic_at(hipStream_t st, const PhotonMapDev &pm,
                              uint32_t n, const float *pts, float *out);
 

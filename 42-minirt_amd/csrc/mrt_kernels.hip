@@ -1744,7 +1744,9 @@ k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
 constexpr int KNN_MAX = 32;
 template <bool KNN>
 __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
-                                         float *kd, uint32_t *ki, int stride)
+                                         float *kd, uint32_t *ki, int stride,
+                                         uint32_t *examined = nullptr,
+                                         uint32_t *cells = nullptr)
 {
     // the cells meeting the cube [x - s, x + s]^3, s = the search distance
     // (the cell edge, widened by 1e-4 against rounding): two or three per
@@ -1758,9 +1760,24 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
     const float cut = knn ? pm.radius : pm.radius * pm.radius;
     V3 sum = v3(0.f, 0.f, 0.f);
     uint32_t cnt = 0;
+    // the search reaches at most distance rmax from x (k-NN: squared
+    // distances up to the radius; by radius: the radius): a cell whose box
+    // lies farther holds no photon the tests below would take, so it is
+    // skipped (about a quarter of the 27 cells; the sums are unchanged)
+    const float rmax2 = cut * 1.001f;
     for (int iz = z0; iz <= z1; iz++)
         for (int iy = y0; iy <= y1; iy++)
             for (int ix = x0; ix <= x1; ix++) {
+                {
+                    const float bx = fmaxf(fmaxf((float)ix * pm.cell - x.x,
+                                                 x.x - (float)(ix + 1) * pm.cell), 0.0f);
+                    const float by = fmaxf(fmaxf((float)iy * pm.cell - x.y,
+                                                 x.y - (float)(iy + 1) * pm.cell), 0.0f);
+                    const float bz = fmaxf(fmaxf((float)iz * pm.cell - x.z,
+                                                 x.z - (float)(iz + 1) * pm.cell), 0.0f);
+                    if ((bx * bx + by * by) + bz * bz > rmax2)
+                        continue;
+                }
                 uint32_t b = hash_cell(ix, iy, iz, pm.table);
                 // one photon of the bucket (P its position, W its power when
                 // gathering by radius), in bucket order
@@ -1790,6 +1807,10 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                 };
                 const uint32_t je = pm.cend[b];
                 uint32_t j = pm.cstart[b];
+                if (examined) {
+                    *examined += je - j;
+                    *cells += 1;
+                }
                 // four records in flight at a time instead of one
                 for (; j + 4 <= je; j += 4) {
                     float4 P[4], W[4];
@@ -1826,18 +1847,28 @@ constexpr int CAUSTIC_BLOCK = 128;
 
 // The k-NN lists live in LDS; the fixed-radius gather needs none, so its
 // instantiation keeps the CU's LDS free for more resident waves.
-template <bool KNN>
+// COUNT: also count the gathers and the photon records they examine.
+template <bool KNN, bool COUNT>
 __global__ void __launch_bounds__(CAUSTIC_BLOCK)
-k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
+k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr,
+          unsigned long long *__restrict__ stats)
 {
     __shared__ float s_d[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     __shared__ uint32_t s_i[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     const uint32_t n = min(ctr[ctr_pend(fp.bounce)], B.cap_paths);
+    uint32_t queries = 0, examined = 0, cells = 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
         V3 c = caustic_at<KNN>(fp.pm, xyz(B.p_x[r]), s_d + (KNN ? threadIdx.x : 0),
-                               s_i + (KNN ? threadIdx.x : 0), CAUSTIC_BLOCK);
+                               s_i + (KNN ? threadIdx.x : 0), CAUSTIC_BLOCK,
+                               COUNT ? &examined : nullptr, &cells);
         B.p_k[r] = make_float4(c.x, c.y, c.z, 0.0f);
+        queries++;
+    }
+    if (COUNT) {
+        wave_add_u64(&stats[STAT_GATHER_QUERIES], queries);
+        wave_add_u64(&stats[STAT_GATHER_PHOTONS], examined);
+        wave_add_u64(&stats[STAT_GATHER_CELLS], cells);
     }
 }
 
@@ -1909,14 +1940,17 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n)
 
 hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,
-                          uint32_t *ctr)
+                          uint32_t *ctr, bool count, unsigned long long *stats)
 {
-    if (fp.pm.k)
-        hipLaunchKernelGGL(k_caustic<true>, dim3(blocks), dim3(CAUSTIC_BLOCK), 0,
-                           st, fp, B, ctr);
-    else
-        hipLaunchKernelGGL(k_caustic<false>, dim3(blocks), dim3(CAUSTIC_BLOCK), 0,
-                           st, fp, B, ctr);
+#define MRT_GATHER(K, C)                                                      \
+    hipLaunchKernelGGL((k_caustic<K, C>), dim3(blocks), dim3(CAUSTIC_BLOCK), 0, \
+                       st, fp, B, ctr, stats)
+    if (fp.pm.k) {
+        if (count) MRT_GATHER(true, true); else MRT_GATHER(true, false);
+    } else {
+        if (count) MRT_GATHER(false, true); else MRT_GATHER(false, false);
+    }
+#undef MRT_GATHER
     return hipGetLastError();
 }
 

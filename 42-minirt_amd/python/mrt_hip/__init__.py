@@ -169,7 +169,9 @@ class Stats(C.Structure):
         "launches_shadow", "launches_shade", "nonfinite")] + \
         [(n, C.c_double) for n in (
             "ms_raygen", "ms_trace", "ms_shade", "ms_shadow", "ms_accum",
-            "ms_total", "ms_caustic")]
+            "ms_total", "ms_caustic")] + \
+        [(n, C.c_uint64) for n in ("gather_queries", "gather_photons",
+                                   "gather_cells")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

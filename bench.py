@@ -489,6 +489,8 @@ def caustics_leg(M, args):
     r.set_option(M.OPT_ESTIMATOR, M.EST_MIS)
     r.set_seed(0xCA05)
     r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
+    if args.lanes is not None:
+        r.set_option(M.OPT_LANES, args.lanes)
     log(f"caustics: {ntri} triangles, setup {time.perf_counter() - t0:.1f} s "
         f"(HDRI tables {t_tab:.2f} s)")
     r.build_photons(1000, 0.02, k=0, norm=1.0)      # warm-up
@@ -506,12 +508,26 @@ def caustics_leg(M, args):
     r.synchronize()
     dt = time.perf_counter() - t1
     value = steps * spp * W * H / dt / 1e6
+    r.set_option(M.OPT_STATS, M.STATS_COUNT)
+    r.reset_stats()
+    r.render(0, spp, args.depth)
+    cnt = r.stats()
+    # per-kernel times on one lane (no overlapping launches)
+    r.set_option(M.OPT_LANES, 1)
     r.set_option(M.OPT_STATS, M.STATS_TIME)
     r.reset_stats()
     r.render(0, spp, args.depth)
     tm = r.stats()
     r.close()
     log(f"caustics: {value:.1f} Msamples/s")
+    # the gather's algorithmic bytes: per query the hit point read and the
+    # term written (16 + 16 B), per grid cell searched its bucket's start
+    # and end (8 B), per photon record examined its position and power
+    # (16 + 16 B)
+    q, ph, gc = (cnt["gather_queries"], cnt["gather_photons"],
+                 cnt["gather_cells"])
+    g_bytes = q * (16 + 16) + gc * 8 + ph * 32
+    g_ms = tm["ms_caustic"]
     return {
         "workload": f"mesh scene + glass sphere + 512^2 water heightfield "
                     f"({ntri} triangles), {W}x{H}, 2048x1024 HDRI (0.5 deg "
@@ -520,6 +536,13 @@ def caustics_leg(M, args):
         "msamples_per_s": round(value, 2), "steps": steps,
         "photon_build_ms": round(t_ph * 1e3, 1), "photons_stored": n_ph,
         "hdri_tables_ms": round(t_tab * 1e3, 1),
+        "gather": {"queries": q, "photons_examined": ph, "cells": gc,
+                   "photons_per_query": round(ph / max(q, 1), 2),
+                   "algorithmic_bytes": g_bytes, "ms": round(g_ms, 3),
+                   "achieved_gbs": round(g_bytes / (g_ms * 1e-3) / 1e9, 1)
+                   if g_ms > 0 else None,
+                   "frac": round(g_bytes / (g_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 4) if g_ms > 0 else None},
         "kernel_ms_per_step": {k: round(tm[f"ms_{k}"], 3)
                                for k in ("raygen", "trace", "shade", "caustic",
                                          "shadow", "accum", "total")},

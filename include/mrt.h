@@ -262,6 +262,9 @@ typedef struct mrt_stats {
     double   ms_accum;
     double   ms_total;         /* mrt_render wall time on the stream          */
     double   ms_caustic;       /* photon-map gather                           */
+    uint64_t gather_queries;   /* caustic gathers (MRT_STATS_COUNT)           */
+    uint64_t gather_photons;   /* photon records those gathers examined       */
+    uint64_t gather_cells;     /* grid cells whose bucket bounds they read    */
 } mrt_stats;
 
 typedef struct mrt_ctx mrt_ctx;
