@@ -111,8 +111,8 @@ struct mrt_ctx {
     bool         has_env = false;
 
     // photon map
-    float4      *d_ph_pos = nullptr, *d_ph_pow = nullptr;
-    uint32_t    *d_cstart = nullptr, *d_cend = nullptr;
+    float4      *d_ph_rec = nullptr;   // 2 per photon: position, power
+    uint2       *d_range = nullptr;    // per bucket: first, one past last
     uint32_t     ph_n = 0, ph_table = 0, ph_k = 15;
     float        ph_cell = 0.0f, ph_radius = 0.0f, ph_norm = 1.0f;
     bool         has_photons = false;
@@ -362,10 +362,8 @@ static void free_env(mrt_ctx *c);
 
 static void free_photons(mrt_ctx *c)
 {
-    dfree(c->d_ph_pos);
-    dfree(c->d_ph_pow);
-    dfree(c->d_cstart);
-    dfree(c->d_cend);
+    dfree(c->d_ph_rec);
+    dfree(c->d_range);
     c->ph_n = 0;
     c->has_photons = false;
 }
@@ -924,10 +922,8 @@ extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t n_per, float radius)
         const float cell = c->ph_k ? sqrtf(radius) : radius;
         if (dalloc(k_in, n) != hipSuccess || dalloc(k_out, n) != hipSuccess
             || dalloc(v_in, n) != hipSuccess || dalloc(v_out, n) != hipSuccess
-            || dalloc(c->d_ph_pos, n) != hipSuccess
-            || dalloc(c->d_ph_pow, n) != hipSuccess
-            || dalloc(c->d_cstart, table) != hipSuccess
-            || dalloc(c->d_cend, table) != hipSuccess) {
+            || dalloc(c->d_ph_rec, 2 * (size_t)n) != hipSuccess
+            || dalloc(c->d_range, table) != hipSuccess) {
             rc = fail(c, MRT_E_NOMEM, "photon map (%u) does not fit", n);
             break;
         }
@@ -944,11 +940,11 @@ extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t n_per, float radius)
             || (e = rocprim::radix_sort_pairs(tmp, tmp_bytes, k_in, k_out, v_in,
                                               v_out, n, 0, end_bit, c->stream))
                    != hipSuccess
-            || (e = launch_fill_u32(c->stream, c->d_cstart, 0u, table)) != hipSuccess
-            || (e = launch_fill_u32(c->stream, c->d_cend, 0u, table)) != hipSuccess
+            || (e = launch_fill_u32(c->stream, (uint32_t *)c->d_range, 0u,
+                                    2 * (size_t)table)) != hipSuccess
             || (e = launch_photon_reorder(c->stream, raw_pos, raw_pow, k_out,
-                                          v_out, n, c->d_ph_pos, c->d_ph_pow,
-                                          c->d_cstart, c->d_cend)) != hipSuccess
+                                          v_out, n, c->d_ph_rec,
+                                          c->d_range)) != hipSuccess
             || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
             break;
         c->ph_n = n;
@@ -985,26 +981,25 @@ extern "C" int mrt_get_photons(mrt_ctx *c, uint32_t max, float *pos,
     uint32_t n = c->ph_n < max ? c->ph_n : max;
     if (n == 0)
         return MRT_OK;
-    std::vector<float4> p(n), w(n);
-    HIPCHK(c, hipMemcpyAsync(p.data(), c->d_ph_pos, (size_t)n * 16,
-                             hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(w.data(), c->d_ph_pow, (size_t)n * 16,
+    std::vector<float4> rec(2 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(rec.data(), c->d_ph_rec, (size_t)n * 32,
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (uint32_t i = 0; i < n; i++) {
+        const float4 &P = rec[2 * (size_t)i], &W = rec[2 * (size_t)i + 1];
         if (pos) {
-            pos[3 * i] = p[i].x;
-            pos[3 * i + 1] = p[i].y;
-            pos[3 * i + 2] = p[i].z;
+            pos[3 * i] = P.x;
+            pos[3 * i + 1] = P.y;
+            pos[3 * i + 2] = P.z;
         }
         if (power) {
-            power[3 * i] = w[i].x;
-            power[3 * i + 1] = w[i].y;
-            power[3 * i + 2] = w[i].z;
+            power[3 * i] = W.x;
+            power[3 * i + 1] = W.y;
+            power[3 * i + 2] = W.z;
         }
         if (ids) {
             uint32_t g;
-            memcpy(&g, &p[i].w, 4);
+            memcpy(&g, &P.w, 4);
             ids[i] = g;
         }
     }
@@ -1015,10 +1010,8 @@ static PhotonMapDev photon_map_dev(const mrt_ctx *c)
 {
     PhotonMapDev pm;
     memset(&pm, 0, sizeof(pm));
-    pm.pos = c->d_ph_pos;
-    pm.pow = c->d_ph_pow;
-    pm.cstart = c->d_cstart;
-    pm.cend = c->d_cend;
+    pm.rec = c->d_ph_rec;
+    pm.range = c->d_range;
     pm.n = c->ph_n;
     pm.table = c->ph_table;
     pm.cell = c->ph_cell;

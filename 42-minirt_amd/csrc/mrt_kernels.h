@@ -102,11 +102,12 @@ enum : int {
 
 // Photon map in HBM, sorted by hash bucket (hash_cell of
 // src/gpu/convert_caustic_utils.c:15-24 modulo `table`), photon id second.
+// The map's records are interleaved, photon j at rec[2 j] = (position,
+// photon id) and rec[2 j + 1] = (power, 0): a bucket's few photons then
+// share cache lines instead of touching one line in each of two arrays.
 struct PhotonMapDev {
-    const float4   *pos;      // (position, photon id)
-    const float4   *pow;      // (power, 0)
-    const uint32_t *cstart;   // first photon of each bucket
-    const uint32_t *cend;     // one past the last
+    const float4   *rec;      // 2 per photon, in bucket order
+    const uint2    *range;    // per bucket: first photon, one past the last
     uint32_t n, table;
     float    cell;            // grid cell edge >= the search distance
     uint32_t k;               // k nearest (0: every photon within radius)
@@ -325,8 +326,7 @@ hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
                                  const float4 *pow,
                                  const unsigned long long *keys,
                                  const uint32_t *vals, uint32_t n,
-                                 float4 *pos_out, float4 *pow_out,
-                                 uint32_t *cstart, uint32_t *cend);
+                                 float4 *rec, uint2 *range);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,

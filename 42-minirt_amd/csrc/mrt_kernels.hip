@@ -1710,20 +1710,20 @@ __global__ void __launch_bounds__(256)
 k_photon_reorder(const float4 *__restrict__ pos, const float4 *__restrict__ pow,
                  const unsigned long long *__restrict__ keys,
                  const uint32_t *__restrict__ vals, uint32_t n,
-                 float4 *__restrict__ pos_out, float4 *__restrict__ pow_out,
-                 uint32_t *__restrict__ cstart, uint32_t *__restrict__ cend)
+                 float4 *__restrict__ rec,
+                 uint2 *__restrict__ range)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     uint32_t s = vals[i];
-    pos_out[i] = pos[s];
-    pow_out[i] = pow[s];
+    rec[2 * (size_t)i] = pos[s];
+    rec[2 * (size_t)i + 1] = pow[s];
     uint32_t b = (uint32_t)(keys[i] >> 32);
     if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != b)
-        cstart[b] = i;
+        range[b].x = i;
     if (i + 1 == n || (uint32_t)(keys[i + 1] >> 32) != b)
-        cend[b] = i + 1;
+        range[b].y = i + 1;
 }
 
 __global__ void __launch_bounds__(256)
@@ -1805,8 +1805,9 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                     kd[pos * stride] = d2;
                     ki[pos * stride] = j;
                 };
-                const uint32_t je = pm.cend[b];
-                uint32_t j = pm.cstart[b];
+                const uint2 br = pm.range[b];
+                const uint32_t je = br.y;
+                uint32_t j = br.x;
                 if (examined) {
                     *examined += je - j;
                     *cells += 1;
@@ -1816,16 +1817,18 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                     float4 P[4], W[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        P[u] = pm.pos[j + u];
-                        W[u] = knn ? make_float4(0.f, 0.f, 0.f, 0.f) : pm.pow[j + u];
+                        P[u] = pm.rec[2 * (size_t)(j + u)];
+                        W[u] = knn ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                   : pm.rec[2 * (size_t)(j + u) + 1];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; u++)
                         take(P[u], W[u], j + u);
                 }
                 for (; j < je; j++)
-                    take(pm.pos[j],
-                         knn ? make_float4(0.f, 0.f, 0.f, 0.f) : pm.pow[j], j);
+                    take(pm.rec[2 * (size_t)j],
+                         knn ? make_float4(0.f, 0.f, 0.f, 0.f)
+                             : pm.rec[2 * (size_t)j + 1], j);
             }
     float area;
     if (knn) {
@@ -1835,7 +1838,7 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
         if (area < 1e-9f)
             return v3(0.f, 0.f, 0.f);
         for (uint32_t j = 0; j < K; j++)
-            sum = add(sum, xyz(pm.pow[ki[j * stride]]));
+            sum = add(sum, xyz(pm.rec[2 * (size_t)ki[j * stride] + 1]));
     } else {
         area = 3.14159265358979324f * cut;
     }
@@ -1917,12 +1920,10 @@ hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
                                  const float4 *pow,
                                  const unsigned long long *keys,
                                  const uint32_t *vals, uint32_t n,
-                                 float4 *pos_out, float4 *pow_out,
-                                 uint32_t *cstart, uint32_t *cend)
+                                 float4 *rec, uint2 *range)
 {
     hipLaunchKernelGGL(k_photon_reorder, dim3((n + 255) / 256), dim3(256), 0,
-                       st, pos, pow, keys, vals, n, pos_out, pow_out, cstart,
-                       cend);
+                       st, pos, pow, keys, vals, n, rec, range);
     return hipGetLastError();
 }
 
