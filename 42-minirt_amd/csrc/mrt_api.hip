@@ -113,6 +113,7 @@ struct mrt_ctx {
     // photon map
     float4      *d_ph_rec = nullptr;   // 2 per photon: position, power
     uint2       *d_range = nullptr;    // per bucket: first, one past last
+    uint32_t    *d_occ = nullptr;      // per bucket: one bit, nonempty
     uint32_t     ph_n = 0, ph_table = 0, ph_k = 15;
     float        ph_cell = 0.0f, ph_radius = 0.0f, ph_norm = 1.0f;
     bool         has_photons = false;
@@ -364,6 +365,7 @@ static void free_photons(mrt_ctx *c)
 {
     dfree(c->d_ph_rec);
     dfree(c->d_range);
+    dfree(c->d_occ);
     c->ph_n = 0;
     c->has_photons = false;
 }
@@ -923,7 +925,8 @@ extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t n_per, float radius)
         if (dalloc(k_in, n) != hipSuccess || dalloc(k_out, n) != hipSuccess
             || dalloc(v_in, n) != hipSuccess || dalloc(v_out, n) != hipSuccess
             || dalloc(c->d_ph_rec, 2 * (size_t)n) != hipSuccess
-            || dalloc(c->d_range, table) != hipSuccess) {
+            || dalloc(c->d_range, table) != hipSuccess
+            || dalloc(c->d_occ, (table + 31) / 32) != hipSuccess) {
             rc = fail(c, MRT_E_NOMEM, "photon map (%u) does not fit", n);
             break;
         }
@@ -942,9 +945,11 @@ extern "C" int mrt_build_photons(mrt_ctx *c, uint32_t n_per, float radius)
                    != hipSuccess
             || (e = launch_fill_u32(c->stream, (uint32_t *)c->d_range, 0u,
                                     2 * (size_t)table)) != hipSuccess
+            || (e = launch_fill_u32(c->stream, c->d_occ, 0u, (table + 31) / 32))
+                   != hipSuccess
             || (e = launch_photon_reorder(c->stream, raw_pos, raw_pow, k_out,
                                           v_out, n, c->d_ph_rec,
-                                          c->d_range)) != hipSuccess
+                                          c->d_range, c->d_occ)) != hipSuccess
             || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
             break;
         c->ph_n = n;
@@ -1012,6 +1017,7 @@ static PhotonMapDev photon_map_dev(const mrt_ctx *c)
     memset(&pm, 0, sizeof(pm));
     pm.rec = c->d_ph_rec;
     pm.range = c->d_range;
+    pm.occ = c->d_occ;
     pm.n = c->ph_n;
     pm.table = c->ph_table;
     pm.cell = c->ph_cell;
@@ -1736,6 +1742,7 @@ extern "C" int mrt_get_stats(mrt_ctx *c, mrt_stats *out)
     out->gather_queries = st[STAT_GATHER_QUERIES];
     out->gather_photons = st[STAT_GATHER_PHOTONS];
     out->gather_cells = st[STAT_GATHER_CELLS];
+    out->gather_lookups = st[STAT_GATHER_LOOKUPS];
     return MRT_OK;
 }
 

@@ -96,7 +96,8 @@ enum : int {
     STAT_NONFINITE = 6,
     STAT_GATHER_QUERIES = 7,   // caustic gathers (pending records)
     STAT_GATHER_PHOTONS = 8,   // photon records they examined
-    STAT_GATHER_CELLS = 9,     // grid cells (bucket bounds) they read
+    STAT_GATHER_CELLS = 9,     // nonempty buckets whose bounds they read
+    STAT_GATHER_LOOKUPS = 10,  // grid cells they looked up (occupancy bits)
     STAT_COUNT = 16
 };
 
@@ -108,6 +109,7 @@ enum : int {
 struct PhotonMapDev {
     const float4   *rec;      // 2 per photon, in bucket order
     const uint2    *range;    // per bucket: first photon, one past the last
+    const uint32_t *occ;      // one bit per bucket: holds photons
     uint32_t n, table;
     float    cell;            // grid cell edge >= the search distance
     uint32_t k;               // k nearest (0: every photon within radius)
@@ -326,7 +328,7 @@ hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
                                  const float4 *pow,
                                  const unsigned long long *keys,
                                  const uint32_t *vals, uint32_t n,
-                                 float4 *rec, uint2 *range);
+                                 float4 *rec, uint2 *range, uint32_t *occ);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 hipError_t launch_caustic(hipStream_t st, uint32_t blocks,
                           const FrameParams &fp, const ShadeBufs &B,

@@ -1711,7 +1711,7 @@ k_photon_reorder(const float4 *__restrict__ pos, const float4 *__restrict__ pow,
                  const unsigned long long *__restrict__ keys,
                  const uint32_t *__restrict__ vals, uint32_t n,
                  float4 *__restrict__ rec,
-                 uint2 *__restrict__ range)
+                 uint2 *__restrict__ range, uint32_t *__restrict__ occ)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1720,8 +1720,10 @@ k_photon_reorder(const float4 *__restrict__ pos, const float4 *__restrict__ pow,
     rec[2 * (size_t)i] = pos[s];
     rec[2 * (size_t)i + 1] = pow[s];
     uint32_t b = (uint32_t)(keys[i] >> 32);
-    if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != b)
+    if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != b) {
         range[b].x = i;
+        atomicOr(&occ[b >> 5], 1u << (b & 31u));
+    }
     if (i + 1 == n || (uint32_t)(keys[i + 1] >> 32) != b)
         range[b].y = i + 1;
 }
@@ -1746,7 +1748,8 @@ template <bool KNN>
 __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                                          float *kd, uint32_t *ki, int stride,
                                          uint32_t *examined = nullptr,
-                                         uint32_t *cells = nullptr)
+                                         uint32_t *cells = nullptr,
+                                         uint32_t *lookups = nullptr)
 {
     // the cells meeting the cube [x - s, x + s]^3, s = the search distance
     // (the cell edge, widened by 1e-4 against rounding): two or three per
@@ -1805,6 +1808,12 @@ __device__ __forceinline__ V3 caustic_at(const PhotonMapDev &pm, V3 x,
                     kd[pos * stride] = d2;
                     ki[pos * stride] = j;
                 };
+                // the occupancy bits (2 bits per photon, L2-resident) turn
+                // most empty cells away before their bucket bounds are read
+                if (examined)
+                    *lookups += 1;
+                if (!((pm.occ[b >> 5] >> (b & 31u)) & 1u))
+                    continue;
                 const uint2 br = pm.range[b];
                 const uint32_t je = br.y;
                 uint32_t j = br.x;
@@ -1859,12 +1868,12 @@ k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr,
     __shared__ float s_d[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     __shared__ uint32_t s_i[KNN ? KNN_MAX * CAUSTIC_BLOCK : 1];
     const uint32_t n = min(ctr[ctr_pend(fp.bounce)], B.cap_paths);
-    uint32_t queries = 0, examined = 0, cells = 0;
+    uint32_t queries = 0, examined = 0, cells = 0, lookups = 0;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n;
          r += gridDim.x * blockDim.x) {
         V3 c = caustic_at<KNN>(fp.pm, xyz(B.p_x[r]), s_d + (KNN ? threadIdx.x : 0),
                                s_i + (KNN ? threadIdx.x : 0), CAUSTIC_BLOCK,
-                               COUNT ? &examined : nullptr, &cells);
+                               COUNT ? &examined : nullptr, &cells, &lookups);
         B.p_k[r] = make_float4(c.x, c.y, c.z, 0.0f);
         queries++;
     }
@@ -1872,6 +1881,7 @@ k_caustic(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr,
         wave_add_u64(&stats[STAT_GATHER_QUERIES], queries);
         wave_add_u64(&stats[STAT_GATHER_PHOTONS], examined);
         wave_add_u64(&stats[STAT_GATHER_CELLS], cells);
+        wave_add_u64(&stats[STAT_GATHER_LOOKUPS], lookups);
     }
 }
 
@@ -1920,10 +1930,10 @@ hipError_t launch_photon_reorder(hipStream_t st, const float4 *pos,
                                  const float4 *pow,
                                  const unsigned long long *keys,
                                  const uint32_t *vals, uint32_t n,
-                                 float4 *rec, uint2 *range)
+                                 float4 *rec, uint2 *range, uint32_t *occ)
 {
     hipLaunchKernelGGL(k_photon_reorder, dim3((n + 255) / 256), dim3(256), 0,
-                       st, pos, pow, keys, vals, n, rec, range);
+                       st, pos, pow, keys, vals, n, rec, range, occ);
     return hipGetLastError();
 }
 

@@ -171,7 +171,7 @@ class Stats(C.Structure):
             "ms_raygen", "ms_trace", "ms_shade", "ms_shadow", "ms_accum",
             "ms_total", "ms_caustic")] + \
         [(n, C.c_uint64) for n in ("gather_queries", "gather_photons",
-                                   "gather_cells")]
+                                   "gather_cells", "gather_lookups")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

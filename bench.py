@@ -521,12 +521,12 @@ def caustics_leg(M, args):
     r.close()
     log(f"caustics: {value:.1f} Msamples/s")
     # the gather's algorithmic bytes: per query the hit point read and the
-    # term written (16 + 16 B), per grid cell searched its bucket's start
-    # and end (8 B), per photon record examined its position and power
-    # (16 + 16 B)
-    q, ph, gc = (cnt["gather_queries"], cnt["gather_photons"],
-                 cnt["gather_cells"])
-    g_bytes = q * (16 + 16) + gc * 8 + ph * 32
+    # term written (16 + 16 B), per grid cell looked up its occupancy word
+    # (4 B), per nonempty bucket its bounds (8 B), per photon record
+    # examined its position and power (16 + 16 B)
+    q, ph, gc, gl = (cnt["gather_queries"], cnt["gather_photons"],
+                     cnt["gather_cells"], cnt["gather_lookups"])
+    g_bytes = q * (16 + 16) + gl * 4 + gc * 8 + ph * 32
     g_ms = tm["ms_caustic"]
     return {
         "workload": f"mesh scene + glass sphere + 512^2 water heightfield "
@@ -536,7 +536,8 @@ def caustics_leg(M, args):
         "msamples_per_s": round(value, 2), "steps": steps,
         "photon_build_ms": round(t_ph * 1e3, 1), "photons_stored": n_ph,
         "hdri_tables_ms": round(t_tab * 1e3, 1),
-        "gather": {"queries": q, "photons_examined": ph, "cells": gc,
+        "gather": {"queries": q, "photons_examined": ph, "cells_looked_up": gl,
+                   "buckets_read": gc,
                    "photons_per_query": round(ph / max(q, 1), 2),
                    "algorithmic_bytes": g_bytes, "ms": round(g_ms, 3),
                    "achieved_gbs": round(g_bytes / (g_ms * 1e-3) / 1e9, 1)

@@ -264,7 +264,8 @@ typedef struct mrt_stats {
     double   ms_caustic;       /* photon-map gather                           */
     uint64_t gather_queries;   /* caustic gathers (MRT_STATS_COUNT)           */
     uint64_t gather_photons;   /* photon records those gathers examined       */
-    uint64_t gather_cells;     /* grid cells whose bucket bounds they read    */
+    uint64_t gather_cells;     /* nonempty buckets whose bounds they read     */
+    uint64_t gather_lookups;   /* grid cells they looked up                   */
 } mrt_stats;
 
 typedef struct mrt_ctx mrt_ctx;
