@@ -1924,8 +1924,11 @@ extern "C" int mrt_adjoint(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples
                              rpp, c->d_ctr, stack_ovf(c, 0), c->d_tape);
             if (rc != MRT_OK)
                 break;
+            // one resident wave of blocks (4 per CU with the texel hash's
+            // LDS), so each block's table gathers many paths before its
+            // flush
             const uint32_t blocks = std::min((Pn + 255) / 256,
-                                             (uint32_t)c->num_cu * 16u);
+                                             (uint32_t)c->num_cu * 4u);
             e = launch_adjoint_back(c->stream, blocks, fp, Bb, A, Pn);
         }
         if (rc != MRT_OK)

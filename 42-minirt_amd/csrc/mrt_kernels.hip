@@ -1276,12 +1276,47 @@ k_tape_init(float4 *__restrict__ tape, uint32_t P, uint32_t max_depth)
 //   dC/dtexel    = dl thr_v f_v w_v           (environment sample),
 // then X = E_v + f_v (N_v + X).  Material sums go through LDS, texel
 // gradients to HBM with float atomics.
+// Texel gradients of a block are summed in an LDS hash table first (open
+// addressing on the texel index, a few probes, then the HBM atomic as the
+// fallback) and added to HBM once per entry at the block's end: the
+// environment samples concentrate on the few texels of the sun, whose
+// HBM atomics from every path would otherwise queue on the same addresses.
+constexpr uint32_t TEX_HASH = 2048;       // entries per block (24 KB of sums)
+constexpr uint32_t TEX_PROBES = 8;
+
+__device__ __forceinline__ void texel_add(float *__restrict__ g_tex,
+                                          uint32_t *s_key, float *s_val,
+                                          uint32_t t, V3 v)
+{
+    uint32_t h = (t * 2654435761u) >> (32 - 11);   // log2(TEX_HASH) = 11
+    for (uint32_t p = 0; p < TEX_PROBES; p++, h = (h + 1) & (TEX_HASH - 1)) {
+        uint32_t k = s_key[h];
+        if (k == HIT_NONE)
+            k = atomicCAS(&s_key[h], HIT_NONE, t);
+        if (k == HIT_NONE || k == t) {
+            atomicAdd(&s_val[3 * h], v.x);
+            atomicAdd(&s_val[3 * h + 1], v.y);
+            atomicAdd(&s_val[3 * h + 2], v.z);
+            return;
+        }
+    }
+    atomicAdd(&g_tex[3 * (size_t)t], v.x);
+    atomicAdd(&g_tex[3 * (size_t)t + 1], v.y);
+    atomicAdd(&g_tex[3 * (size_t)t + 2], v.z);
+}
+
 __global__ void __launch_bounds__(256)
 k_adjoint_back(FrameParams fp, ShadeBufs B, AdjBufs A, uint32_t P)
 {
     __shared__ float s_g[ADJ_MAXMAT * ADJ_NG];
+    __shared__ uint32_t s_key[TEX_HASH];
+    __shared__ float s_val[3 * TEX_HASH];
     for (uint32_t k = threadIdx.x; k < ADJ_MAXMAT * ADJ_NG; k += blockDim.x)
         s_g[k] = 0.0f;
+    for (uint32_t k = threadIdx.x; k < TEX_HASH; k += blockDim.x) {
+        s_key[k] = HIT_NONE;
+        s_val[3 * k] = s_val[3 * k + 1] = s_val[3 * k + 2] = 0.0f;
+    }
     __syncthreads();
     const size_t TP = B.tape_P;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P;
@@ -1297,11 +1332,8 @@ k_adjoint_back(FrameParams fp, ShadeBufs B, AdjBufs A, uint32_t P)
         const float eg = e0.w;
         const uint32_t et = __float_as_uint(e1.x), nv = __float_as_uint(e1.y);
         if (nv == 0) {
-            if (A.g_tex && et != HIT_NONE) {
-                atomicAdd(&A.g_tex[3 * (size_t)et], dl.x * eg);
-                atomicAdd(&A.g_tex[3 * (size_t)et + 1], dl.y * eg);
-                atomicAdd(&A.g_tex[3 * (size_t)et + 2], dl.z * eg);
-            }
+            if (A.g_tex && et != HIT_NONE)
+                texel_add(A.g_tex, s_key, s_val, et, mul(dl, eg));
             continue;
         }
         for (int v = (int)nv - 1; v >= 0; v--) {
@@ -1321,16 +1353,12 @@ k_adjoint_back(FrameParams fp, ShadeBufs B, AdjBufs A, uint32_t P)
                     atomicAdd(s_g + mt * ADJ_NG + 3, r);
             }
             if (A.g_tex) {
-                if (nt != HIT_NONE) {
-                    atomicAdd(&A.g_tex[3 * (size_t)nt], dl.x * tf.x * ng);
-                    atomicAdd(&A.g_tex[3 * (size_t)nt + 1], dl.y * tf.y * ng);
-                    atomicAdd(&A.g_tex[3 * (size_t)nt + 2], dl.z * tf.z * ng);
-                }
-                if (v == (int)nv - 1 && et != HIT_NONE) {
-                    atomicAdd(&A.g_tex[3 * (size_t)et], dl.x * tf.x * eg);
-                    atomicAdd(&A.g_tex[3 * (size_t)et + 1], dl.y * tf.y * eg);
-                    atomicAdd(&A.g_tex[3 * (size_t)et + 2], dl.z * tf.z * eg);
-                }
+                if (nt != HIT_NONE)
+                    texel_add(A.g_tex, s_key, s_val, nt,
+                              v3(dl.x * tf.x * ng, dl.y * tf.y * ng, dl.z * tf.z * ng));
+                if (v == (int)nv - 1 && et != HIT_NONE)
+                    texel_add(A.g_tex, s_key, s_val, et,
+                              v3(dl.x * tf.x * eg, dl.y * tf.y * eg, dl.z * tf.z * eg));
             }
             const float4 mc = B.mats[3 * mt + 2];
             const V3 Ev = mc.x > 0.0f ? v3(mc.y * mc.x, mc.z * mc.x, mc.w * mc.x)
@@ -1342,6 +1370,15 @@ k_adjoint_back(FrameParams fp, ShadeBufs B, AdjBufs A, uint32_t P)
     for (uint32_t k = threadIdx.x; k < A.n_mat * ADJ_NG; k += blockDim.x)
         if (s_g[k] != 0.0f)
             atomicAdd(&A.g_mat[k], s_g[k]);
+    if (A.g_tex)
+        for (uint32_t k = threadIdx.x; k < TEX_HASH; k += blockDim.x) {
+            const uint32_t t = s_key[k];
+            if (t == HIT_NONE)
+                continue;
+            atomicAdd(&A.g_tex[3 * (size_t)t], s_val[3 * k]);
+            atomicAdd(&A.g_tex[3 * (size_t)t + 1], s_val[3 * k + 1]);
+            atomicAdd(&A.g_tex[3 * (size_t)t + 2], s_val[3 * k + 2]);
+        }
 }
 
 hipError_t launch_tape_init(hipStream_t st, float4 *tape, uint32_t P,
