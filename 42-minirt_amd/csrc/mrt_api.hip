@@ -148,6 +148,8 @@ struct mrt_ctx {
     // Msamples/s on the 1080p mesh workload)
     uint32_t     path_budget = 1u << 25;
     bool         trace_top = true;   // MRT_OPT_TRACE_TOP
+    bool         trace_refill = true;   // MRT_OPT_TRACE_REFILL
+    uint32_t     refill_blocks = 0;     // its resident grid
 
     // statistics
     uint64_t     paths = 0;
@@ -305,6 +307,8 @@ extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
             break;
         }
         c->trace_blocks = (uint32_t)(c->num_cu * trace_blocks_per_cu());
+        c->refill_blocks = std::min(c->trace_blocks,
+                                    (uint32_t)(c->num_cu * refill_blocks_per_cu()));
         // grid of the grid-stride kernels (shade, light sums, gather): 32
         // blocks per CU, 1.6 % faster renders than 8 (no change at 128)
         c->grid_blocks = (uint32_t)c->num_cu * 32u;
@@ -1305,7 +1309,8 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         if (w.closest || w.shadow_ctr >= 0)
             LAUNCH(c, tm, K_TRACE,
                    launch_trace(st, c->trace_blocks, count, S, ovf, w, ctr,
-                                c->d_stats, c->trace_top));
+                                c->d_stats, c->trace_top, c->trace_refill,
+                                c->refill_blocks));
         if (b > 0 && lit)
             LAUNCH(c, tm, K_RESOLVE, launch_resolve(st, gb, fpp, Bp, ctr));
         if (last) {
@@ -1679,6 +1684,11 @@ extern "C" int mrt_set_option(mrt_ctx *c, int option, double value)
         if (!(value >= 1.0) || value > 1.0e9)
             return fail(c, MRT_E_INVALID, "path budget out of range");
         c->path_budget = (uint32_t)value;
+        return MRT_OK;
+    case MRT_OPT_TRACE_REFILL:
+        if (value != 0.0 && value != 1.0)
+            return fail(c, MRT_E_INVALID, "trace refill is 0 or 1");
+        c->trace_refill = value != 0.0;
         return MRT_OK;
     case MRT_OPT_TRACE_TOP:
         if (value != 0.0 && value != 1.0)

@@ -251,7 +251,8 @@ struct TraceWork {
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
                         const SceneDev &S, const StackOvf &ovf,
                         const TraceWork &w, uint32_t *ctr,
-                        unsigned long long *stats, bool top);
+                        unsigned long long *stats, bool top, bool refill,
+                        uint32_t refill_blocks);
 hipError_t launch_shade(hipStream_t st, uint32_t blocks, const FrameParams &fp,
                         const ShadeBufs &B, uint32_t *ctr);
 hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
@@ -274,6 +275,8 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                              const float *orig, const float *dir,
                              uint32_t *prim_id, float *t);
 int trace_blocks_per_cu();
+// resident blocks per CU of the per-lane refill trace kernel
+int refill_blocks_per_cu();
 
 // Path-replay adjoint: dL/dI per pixel in, gradients out.
 constexpr int ADJ_MAXD = 8;      // deepest path the adjoint replays

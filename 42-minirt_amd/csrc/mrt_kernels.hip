@@ -342,6 +342,266 @@ k_trace_top(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
         S, w, ctr, stats, nullptr, nullptr, ovf_buf, s_code, s_top, ntop);
 }
 
+// k_trace_top with per-lane refill (single scenes of triangles and
+// spheres).  A wave takes 64-item chunks as k_trace does (closest-hit rays
+// and then shadow rays numbered as one item space), but a lane whose ray is
+// done takes the next item of the wave's chunk instead of idling until the
+// wave's slowest ray ends: secondary rays of one wave differ several times
+// over in traversal length.  Refills happen when REFILL_MIN lanes are idle
+// (or all of them), so their loads and set-up run for many lanes at once.
+// The traversal is traverse()'s, one node or one leaf per iteration, with
+// the ray's kind per lane: closest-hit rays sort the hit children, shadow
+// rays keep slot order and stop at the first hit, as there.
+constexpr uint32_t REFILL_MIN = 16;
+
+// 6 waves per SIMD (80 VGPRs, 12 B of scratch): 1,263 against 1,169
+// Msamples/s for the 8-wave k_trace_top on the 1080p mesh workload (one
+// lane); at 8 waves it spills 68 B per lane and halves the rate, at 5 it
+// spills nothing but hides less latency (1,199).  REFILL_MIN 16 against
+// 4 / 8 / 32: 1,224 / 1,250 / 1,224.
+template <bool COUNT>
+__global__ void __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(6, 6)))
+k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
+               unsigned long long *__restrict__ stats, uint32_t *__restrict__ ovf_buf)
+{
+    constexpr int SS = TRACE_SS_TOP;
+    __shared__ uint32_t s_code[SS * TRACE_BLOCK];
+    __shared__ float4 s_top[TOP_NODES * 7];
+    const uint32_t ntop = min((uint32_t)TOP_NODES, S.n_nodes);
+    for (uint32_t k = threadIdx.x; k < ntop * 7; k += TRACE_BLOCK)
+        s_top[k] = S.nodes[(size_t)(k / 7) * 8 + k % 7];
+    __syncthreads();
+    const uint32_t nc = w.closest ? ctr[CTR_CUR] : 0u;
+    const uint32_t ns = w.shadow_ctr >= 0 ? min(ctr[w.shadow_ctr], w.cap_shadow) : 0u;
+    const uint32_t nall = nc + ns;
+    const uint32_t nch = nall / CHUNK + (nall % CHUNK != 0);
+    const uint32_t W = gridDim.x * (TRACE_BLOCK / 64);
+    uint32_t *lds = s_code + threadIdx.x;
+    uint32_t *ovf = ovf_buf + blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    const uint32_t os = gridDim.x * TRACE_BLOCK;
+    const int32_t DONE = INT32_MIN;
+    const float INF = __int_as_float(0x7f800000);
+    const uint32_t lane = lane_id();
+    // wave-uniform: the chunk being handed out, [pos, end)
+    uint32_t chunk = blockIdx.x * (TRACE_BLOCK / 64) + (threadIdx.x >> 6);
+    uint32_t set = blockIdx.x % NSET, tries = 0;
+    uint32_t pos = 0, end = 0;
+    bool more = chunk < nch;
+    if (more) {
+        pos = chunk * CHUNK;
+        end = min(pos + CHUNK, nall);
+    }
+    // per lane: the item and its ray's traversal state
+    uint32_t item = HIT_NONE;
+    bool any = false, hit = false;
+    V3 o = v3(0.f, 0.f, 0.f), d = o, inv = o;
+    float tclose = 0.0f, bu = 0.0f, bv = 0.0f;
+    uint32_t hslot = 0, nodes_c = 0, prims_c = 0, nodes_s = 0, prims_s = 0;
+    int sp = 0;
+    int32_t code = DONE;
+    auto pop = [&]() -> int32_t {
+        if (sp == 0)
+            return DONE;
+        sp--;
+        return (int32_t)(sp < SS ? lds[sp * TRACE_BLOCK]
+                                 : ovf[(uint32_t)(sp - SS) * os]);
+    };
+    auto push = [&](int32_t cc) {
+        if (sp < SS)
+            lds[sp * TRACE_BLOCK] = (uint32_t)cc;
+        else if (sp < STACK4_MAX)
+            ovf[(uint32_t)(sp - SS) * os] = (uint32_t)cc;
+        sp = sp < STACK4_MAX ? sp + 1 : sp;
+    };
+    for (;;) {
+        uint64_t idle = __ballot(item == HIT_NONE);
+        if (idle && (more || pos < end)
+            && (__popcll(idle) >= REFILL_MIN || idle == __ballot(true))) {
+            // hand the idle lanes the next items of the wave's chunk,
+            // claiming further chunks as it runs out
+            while (idle) {
+                if (pos >= end) {
+                    if (!more || !next_chunk(&ctr[CTR_SETS], nch, W, set, tries, chunk)) {
+                        more = false;
+                        break;
+                    }
+                    pos = chunk * CHUNK;
+                    end = min(pos + CHUNK, nall);
+                }
+                // keep the wave's cursor in scalar registers
+                pos = __builtin_amdgcn_readfirstlane(pos);
+                end = __builtin_amdgcn_readfirstlane(end);
+                const uint32_t avail = end - pos;
+                const uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+                if (((idle >> lane) & 1ull) && rank < avail) {
+                    item = pos + rank;
+                    float4 o4, d4;
+                    if (item < nc) {
+                        o4 = w.q_o[item];
+                        d4 = w.q_d[item];
+                        any = false;
+                        tclose = IT_MAX_F;
+                    } else {
+                        o4 = w.s_o[item - nc];
+                        d4 = w.s_d[item - nc];
+                        any = true;
+                        tclose = fminf(o4.w, IT_MAX_F);
+                    }
+                    o = xyz(o4);
+                    d = xyz(d4);
+                    inv = v3(inv_dir(d.x), inv_dir(d.y), inv_dir(d.z));
+                    hit = false;
+                    hslot = HIT_NONE;
+                    bu = bv = 0.0f;
+                    sp = 0;
+                    code = S.n_nodes ? (int32_t)S.root : DONE;
+                }
+                pos += min((uint32_t)__popcll(idle), avail);
+                idle = __ballot(item == HIT_NONE);
+            }
+        }
+        if (__ballot(item != HIT_NONE) == 0) {
+            if (!more && pos >= end)
+                break;
+            continue;
+        }
+        if (item == HIT_NONE)
+            continue;
+        // one traversal step (none for a ray of a scene without a BVH,
+        // which starts at DONE: only its planes are tested below)
+        bool found = false;
+        if (code == DONE) {
+        } else if (code >= 0) {
+            float4 lx, hx, ly, hy, lz, hz, cw;
+            if ((uint32_t)code < ntop) {
+                const float4 *n = s_top + (size_t)code * 7;
+                lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
+                cw = n[6];
+            } else {
+                const float4 *n = S.nodes + (size_t)code * 8;
+                lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
+                cw = n[6];
+            }
+            if (COUNT) {
+                if (any)
+                    nodes_s++;
+                else
+                    nodes_c++;
+            }
+            float t0, t1, t2, t3;
+            float k0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, tclose, t0) ? t0 : INF;
+            float k1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, tclose, t1) ? t1 : INF;
+            float k2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, tclose, t2) ? t2 : INF;
+            float k3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, tclose, t3) ? t3 : INF;
+            int32_t c0 = __float_as_int(cw.x), c1 = __float_as_int(cw.y),
+                    c2 = __float_as_int(cw.z), c3 = __float_as_int(cw.w);
+            if (any) {
+                if (k0 == INF) { k0 = k1; c0 = c1; k1 = k2; c1 = c2; k2 = k3; c2 = c3; k3 = INF; }
+                if (k0 == INF) { k0 = k1; c0 = c1; k1 = k2; c1 = c2; k2 = INF; }
+                if (k0 == INF) { k0 = k1; c0 = c1; k1 = INF; }
+                if (k1 == INF) { k1 = k2; c1 = c2; k2 = k3; c2 = c3; k3 = INF; }
+                if (k1 == INF) { k1 = k2; c1 = c2; k2 = INF; }
+                if (k2 == INF) { k2 = k3; c2 = c3; k3 = INF; }
+            } else {
+                cx(k0, c0, k1, c1);
+                cx(k2, c2, k3, c3);
+                cx(k0, c0, k2, c2);
+                cx(k1, c1, k3, c3);
+                cx(k1, c1, k2, c2);
+            }
+            if (k0 == INF) {
+                code = pop();
+            } else {
+                if (k3 != INF)
+                    push(c3);
+                if (k2 != INF)
+                    push(c2);
+                if (k1 != INF)
+                    push(c1);
+                code = c0;
+            }
+        } else {
+            // a leaf's records, two at a time (as traverse())
+            auto test = [&](float4 A, float4 Bq, float4 Cq, uint32_t slot) -> bool {
+                if (COUNT) {
+                    if (any)
+                        prims_s++;
+                    else
+                        prims_c++;
+                }
+                float tt, uu = 0.0f, vv = 0.0f;
+                bool h;
+                if (__float_as_uint(Bq.w) == MRT_PRIMKIND_SPHERE)
+                    h = isect_sph(o, d, A, Bq, tt);
+                else
+                    h = isect_tri(o, d, A, Bq, Cq, S.one_sided != 0, tt, uu, vv);
+                if (h && tt < tclose) {
+                    tclose = tt;
+                    hit = true;
+                    hslot = slot;
+                    bu = uu;
+                    bv = vv;
+                    return any;
+                }
+                return false;
+            };
+            uint32_t v = ~(uint32_t)code, first = v >> 3, cnt = (v & 7u) + 1u;
+            for (uint32_t i = 0; i < cnt && !found; i += 2) {
+                const float4 *p = S.prims + (size_t)(first + i) * 3;
+                float4 A0 = p[0], B0 = p[1], C0 = p[2], A1, B1, C1;
+                const bool two = i + 1 < cnt;
+                if (two) {
+                    A1 = p[3];
+                    B1 = p[4];
+                    C1 = p[5];
+                }
+                found = test(A0, B0, C0, first + i);
+                if (!found && two)
+                    found = test(A1, B1, C1, first + i + 1);
+            }
+            if (!found)
+                code = pop();
+        }
+        if (found || code == DONE) {
+            if (!any) {
+                // closest_hit: the planes after the BVH
+                float tb = hit ? tclose : IT_MAX_F;
+                uint32_t slot = hit ? hslot : HIT_NONE;
+                bool hh = hit;
+                for (uint32_t k = 0; k < S.n_planes; k++) {
+                    float tp;
+                    if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
+                        && (!hh || tp < tb)) {
+                        hh = true;
+                        tb = tp;
+                        slot = HIT_PLANE | k;
+                    }
+                }
+                w.hit[item] = make_float4(tb, __uint_as_float(hh ? slot : HIT_NONE),
+                                          bu, bv);
+            } else {
+                bool occ = hit;
+                const float tlim = occ ? 0.0f : w.s_o[item - nc].w;
+                for (uint32_t k = 0; k < S.n_planes && !occ; k++) {
+                    float tp;
+                    if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
+                        && tp < tlim)
+                        occ = true;
+                }
+                w.vis[item - nc] = occ ? 0 : 1;
+            }
+            item = HIT_NONE;
+        }
+    }
+    if (COUNT) {
+        wave_add_u64(&stats[STAT_NODES_CLOSEST], nodes_c);
+        wave_add_u64(&stats[STAT_PRIMS_CLOSEST], prims_c);
+        wave_add_u64(&stats[STAT_NODES_SHADOW], nodes_s);
+        wave_add_u64(&stats[STAT_PRIMS_SHADOW], prims_s);
+    }
+}
+
 // With quadrics or alpha-tested textures: the longer leaf code keeps the
 // registers it needs (8 waves would spill it).
 template <bool COUNT, bool BATCH>
@@ -2285,14 +2545,30 @@ static bool ovf_fits(const StackOvf &ovf, uint32_t blocks)
 hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
                         const SceneDev &S, const StackOvf &ovf,
                         const TraceWork &w, uint32_t *ctr,
-                        unsigned long long *stats, bool top)
+                        unsigned long long *stats, bool top, bool refill,
+                        uint32_t refill_blocks)
 {
     if (!ovf_fits(ovf, blocks))
         return hipErrorInvalidValue;
     const BatchScene *nb = nullptr;
     const uint2 *ni = nullptr;
     // the tree's top levels from LDS: needs the root at node 0 and the
-    // plain (triangle / sphere) leaf code
+    // plain (triangle / sphere) leaf code; a scene without a BVH (planes
+    // only) takes k_trace_top
+    if (top && refill && S.n_nodes > 0 && S.root == 0 && !(S.quad | S.alpha)
+        && refill_blocks > 0) {
+        if (!ovf_fits(ovf, refill_blocks))
+            return hipErrorInvalidValue;
+        // the grid: the blocks that stay resident at the kernel's 6 waves
+        // per SIMD (refill_blocks_per_cu), not the 8-wave kernels' count
+        if (count)
+            hipLaunchKernelGGL((k_trace_refill<true>), dim3(refill_blocks),
+                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf);
+        else
+            hipLaunchKernelGGL((k_trace_refill<false>), dim3(refill_blocks),
+                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf);
+        return hipGetLastError();
+    }
     if (top && S.root == 0 && !(S.quad | S.alpha)) {
         if (count)
             hipLaunchKernelGGL(k_trace_top<true>, dim3(blocks), dim3(TRACE_BLOCK),
@@ -2466,6 +2742,16 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                        dim3(TRACE_BLOCK), 0, st, S, prims, n, orig, dir,
                        prim_id, t);
     return hipGetLastError();
+}
+
+int refill_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_trace_refill<false>, TRACE_BLOCK, 0) != hipSuccess
+        || nb < 1)
+        nb = 4;
+    return nb;
 }
 
 int trace_blocks_per_cu()

@@ -53,6 +53,7 @@ EST_MIS = 1
 
 OPT_SEED = 1
 OPT_TRACE_TOP = 11
+OPT_TRACE_REFILL = 12
 OPT_ESTIMATOR = 2
 OPT_STATS = 3
 OPT_PATH_BUDGET = 4

@@ -38,6 +38,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def trace_kernel_name(args):
+    """The trace kernel the library launches for a triangle / sphere scene
+    under the options bench.py set (both default to on)."""
+    top = args.top_cache is None or args.top_cache == 1
+    refill = args.refill is None or args.refill == 1
+    if top and refill:
+        return "k_trace_refill"
+    return "k_trace_top" if top else "k_trace"
+
+
 def usable_cores():
     """Cores this process may run on: its CPU affinity, capped by the
     cgroup CPU quota when one is set (a GPU box grants a share of a larger
@@ -566,6 +576,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--top-cache", type=int, default=None,
                     help="MRT_OPT_TRACE_TOP (default: the library's)")
+    ap.add_argument("--refill", type=int, default=None,
+                    help="MRT_OPT_TRACE_REFILL (default: the library's)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ticks", action="store_true",
@@ -640,6 +652,8 @@ def main():
         r.set_option(M.OPT_LANES, args.lanes)
     if args.top_cache is not None:
         r.set_option(M.OPT_TRACE_TOP, args.top_cache)
+    if args.refill is not None:
+        r.set_option(M.OPT_TRACE_REFILL, args.refill)
 
     # 1) counting pass: nodes / primitives per ray (same sample indices)
     r.set_option(M.OPT_STATS, M.STATS_COUNT)
@@ -750,7 +764,8 @@ def main():
                      # node fetches and divergence (occupancy, VALU use and
                      # DRAM bytes well under the algorithmic ones)
                      "limiter": "latency",
-                     "kernel": "k_trace (closest-hit rays of a bounce + "
+                     "kernel": trace_kernel_name(args)
+                               + " (closest-hit rays of a bounce + "
                                "shadow rays of the bounce before)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -222,6 +222,13 @@ typedef struct mrt_scene {
                                   breadth-first) in LDS; 0: every node from
                                   global memory (same results)              */
 
+#define MRT_OPT_TRACE_REFILL 12 /* 1 (default): with MRT_OPT_TRACE_TOP, a
+                                  lane of the trace kernel whose ray is done
+                                  takes the next ray of its wave's chunk
+                                  instead of waiting for the wave's slowest
+                                  ray; 0: a wave traces 64 rays to the end
+                                  (same results)                            */
+
 /* BVH builders (MRT_OPT_BUILDER). */
 #define MRT_BUILDER_HOST    0  /* binned SAH on the host (default)            */
 #define MRT_BUILDER_DEVICE  1  /* PLOC clustering on the device; falls back to

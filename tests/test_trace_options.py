@@ -48,25 +48,65 @@ def test_trace_top_option_is_validated():
     with r:
         with pytest.raises(M.MrtError):
             r.set_option(M.OPT_TRACE_TOP, 2)
+        with pytest.raises(M.MrtError):
+            r.set_option(M.OPT_TRACE_REFILL, 3)
+
+
+def planes_and_one_sphere(spheres):
+    """Planes only (no BVH: no node, no primitive), or planes and one
+    sphere (a tree of one leaf under the root)."""
+    hs = M.HostScene.empty()
+    a = hs.add_material(color=(0.8, 0.3, 0.3))
+    b = hs.add_material(color=(0.9, 0.9, 0.9), metallic=1.0, roughness=0.2)
+    hs.add_plane((0, 0, 0), (0, 1, 0), a)
+    hs.add_plane((0, 0, 4), (0, 0, -1), b)
+    if spheres:
+        hs.add_sphere((0.3, 0.8, 1.0), 0.8, b)
+    hs.add_light((0, 3, 0), 1.0, (255, 255, 255))
+    hs.set_ambient(0.1, (255, 255, 255))
+    hs.set_camera((0, 1, -3), (0, 0, 1), fov=70.0)
+    hs.set_resolution(256, 144)
+    return hs
+
+
+SCENES = {
+    "mesh": lambda: M.HostScene.mesh(3, 256, 144, subdiv=6, floor_cells=48),
+    "plain": lambda: M.HostScene.plain(7, 256, 144),
+    "planes": lambda: planes_and_one_sphere(False),
+    "one_leaf": lambda: planes_and_one_sphere(True),
+}
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("builder", [M.BUILDER_HOST, M.BUILDER_DEVICE])
-def test_top_cache_gives_the_same_bits(builder):
-    hs = M.HostScene.mesh(3, 256, 144, subdiv=6, floor_cells=48)
+@pytest.mark.parametrize("scene", list(SCENES))
+def test_trace_variants_give_the_same_bits(builder, scene):
+    """Every global-node / LDS-top / per-lane-refill trace kernel gives the
+    same image and the same node and primitive counts, down to scenes
+    without a tree or with a single leaf."""
+    hs = SCENES[scene]()
     out = {}
-    for top in (0, 1):
+    for top, refill in ((0, 0), (1, 0), (1, 1)):
         with M.Renderer(256, 144) as r:
             r.set_option(M.OPT_BUILDER, builder)
             r.upload(hs)
             r.set_seed(77)
             r.set_option(M.OPT_TRACE_TOP, top)
+            r.set_option(M.OPT_TRACE_REFILL, refill)
             r.set_option(M.OPT_STATS, M.STATS_COUNT)
             r.render(0, 4, 8)
             st = r.stats()
-            out[top] = (r.readback(), {k: st[k] for k in STAT_KEYS})
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
+            img_count = r.readback()
+            r.set_option(M.OPT_STATS, 0)
+            r.clear()
+            r.render(0, 4, 8)
+            out[(top, refill)] = (r.readback(), {k: st[k] for k in STAT_KEYS},
+                                  img_count)
+    ref = out[(0, 0)]
+    for key, val in out.items():
+        np.testing.assert_array_equal(val[0], ref[0], err_msg=str(key))
+        np.testing.assert_array_equal(val[2], ref[0], err_msg=str(key))
+        assert val[1] == ref[1], key
 
 
 @pytest.mark.gpu
