@@ -177,13 +177,15 @@ def copy_probe():
                       "copy_probe_bytes_written": 4 * n, "launches": 4}))
 
 
-def measured_traffic():
-    """HBM bytes per k_trace launch from the committed counter summary
-    (two rocprofv3 --pmc passes over this bench, scaled by the copy probe),
-    or None when it is absent."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of the trace kernel `kernel` from the committed
+    counter summary (two rocprofv3 --pmc passes over this bench, scaled by
+    the copy probe), or None when it is absent or covers another kernel."""
     f = ROOT / "profiles" / "k_trace_mesh1080p_hbm.json"
     try:
         d = json.loads(f.read_text())
+        if not d["kernel"].startswith(f"mrt::{kernel}<"):
+            return None, None
         return d["hbm_bytes_per_launch"], f.relative_to(ROOT).as_posix()
     except (OSError, KeyError, ValueError):
         return None, None
@@ -706,7 +708,7 @@ def main():
     step_achieved = (bytes_trace + bytes_shadow) / (dt / args.steps) / 1e9
     rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
         * value * 1e6
-    traffic, traffic_src = measured_traffic()
+    traffic, traffic_src = measured_traffic(trace_kernel_name(args))
 
     # 4) the drop-in's own contract (compute_frame_gpu): one sample per
     # render_frame tick, each followed by mrt_readback_add into the host's
@@ -759,10 +761,13 @@ def main():
         "drop_in_per_tick": ticks,
         "image_error": err,
         "roofline": {"bound": "hbm",
-                     # the peak this frac is taken against is HBM's; what
-                     # holds the kernel below it is the latency of dependent
-                     # node fetches and divergence (occupancy, VALU use and
-                     # DRAM bytes well under the algorithmic ones)
+                     # the peak this frac is taken against is HBM's.  The
+                     # algorithmic bytes count every node visit at 112 B as
+                     # if fetched from HBM, while LDS (the tree's top), L1
+                     # and L2 serve about 70 % of them, so frac can pass 1;
+                     # dram_frac is the measured DRAM-side traffic over the
+                     # same launch time.  What holds the kernel is divergent
+                     # traversal work (VALU busy, a third of lanes active)
                      "limiter": "latency",
                      "kernel": trace_kernel_name(args)
                                + " (closest-hit rays of a bounce + "
@@ -771,6 +776,10 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_source": traffic_src,
+                     "dram_achieved": (round(traffic / (trace_ms * 1e-3) / 1e9, 1)
+                                       if traffic else None),
+                     "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9
+                                         / HBM_PEAK_GBS, 4) if traffic else None),
                      "bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(trace_ms, 4),
                      "timed_on": "one render lane (no overlapping launch)",
