@@ -354,11 +354,12 @@ k_trace_top(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
 // rays keep slot order and stop at the first hit, as there.
 constexpr uint32_t REFILL_MIN = 16;
 
-// 6 waves per SIMD (80 VGPRs, 12 B of scratch): 1,263 against 1,169
-// Msamples/s for the 8-wave k_trace_top on the 1080p mesh workload (one
-// lane); at 8 waves it spills 68 B per lane and halves the rate, at 5 it
-// spills nothing but hides less latency (1,199).  REFILL_MIN 16 against
-// 4 / 8 / 32: 1,224 / 1,250 / 1,224.
+// 6 waves per SIMD (78 VGPRs, no scratch): 1,379 against 1,225 Msamples/s
+// for the 8-wave k_trace_top on the 1080p mesh workload (one lane); at 8
+// waves it spilled 68 B per lane and halved the rate, at 5 it hid less
+// latency.  REFILL_MIN 16 against 8 / 24: 1,367 / 1,364.  A leaf takes one
+// loop iteration per pair of records, the rest of it staying the lane's
+// code (1,251 -> 1,379 against an inner loop over the whole leaf).
 template <bool COUNT>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(6, 6)))
@@ -522,7 +523,9 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                 code = c0;
             }
         } else {
-            // a leaf's records, two at a time (as traverse())
+            // a leaf's next two records (one pair per iteration: a lane
+            // with more left keeps the rest of the leaf as its code, so no
+            // lane of the wave waits in an inner loop over a long leaf)
             auto test = [&](float4 A, float4 Bq, float4 Cq, uint32_t slot) -> bool {
                 if (COUNT) {
                     if (any)
@@ -546,22 +549,21 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                 }
                 return false;
             };
-            uint32_t v = ~(uint32_t)code, first = v >> 3, cnt = (v & 7u) + 1u;
-            for (uint32_t i = 0; i < cnt && !found; i += 2) {
-                const float4 *p = S.prims + (size_t)(first + i) * 3;
-                float4 A0 = p[0], B0 = p[1], C0 = p[2], A1, B1, C1;
-                const bool two = i + 1 < cnt;
-                if (two) {
-                    A1 = p[3];
-                    B1 = p[4];
-                    C1 = p[5];
-                }
-                found = test(A0, B0, C0, first + i);
-                if (!found && two)
-                    found = test(A1, B1, C1, first + i + 1);
+            const uint32_t v = ~(uint32_t)code, first = v >> 3, cnt = (v & 7u) + 1u;
+            const float4 *p = S.prims + (size_t)first * 3;
+            float4 A0 = p[0], B0 = p[1], C0 = p[2], A1, B1, C1;
+            const bool two = cnt > 1;
+            if (two) {
+                A1 = p[3];
+                B1 = p[4];
+                C1 = p[5];
             }
+            found = test(A0, B0, C0, first);
+            if (!found && two)
+                found = test(A1, B1, C1, first + 1);
+            // the records after this pair: the same leaf code two on
             if (!found)
-                code = pop();
+                code = cnt > 2 ? (int32_t)~(((first + 2u) << 3) | (cnt - 3u)) : pop();
         }
         if (found || code == DONE) {
             if (!any) {
