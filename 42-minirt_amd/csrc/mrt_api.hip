@@ -21,10 +21,18 @@
 
 using namespace mrt;
 
-// Render lanes: a render's paths are split over up to this many streams.
+// Render lanes: up to this many passes of a render in flight, each on its
+// own stream with its own queues.
 constexpr int MAX_LANES = 4;
-// Lanes get at least this many paths each (fewer lanes for small renders).
+// Batch lanes get at least this many paths each (fewer for small passes).
 constexpr uint32_t LANE_MIN_PATHS = 1u << 16;
+// A pass starts once the pass before it has traced this bounce: its first,
+// fullest launches then overlap the other pass's later ones, whose grids
+// wait for a few long rays with most of the GPU idle (bounce 0: 1,506
+// Msamples/s on the 1080p mesh workload at 256 spp in passes of 32, against
+// 1,498 / 1,492 / 1,471 for bounces 1 / 2 / 3 and 1,461 for a pass split in
+// two concurrent halves).
+constexpr uint32_t LANE_STAGGER_BOUNCE = 0;
 
 // Staging of mrt_trace_rays_noalloc: origins (3 floats), directions (3),
 // primitive ids (1 word) and t (1 float) of up to MRT_TRACE_NOALLOC_MAX rays,
@@ -77,6 +85,7 @@ struct mrt_ctx {
     uint32_t     lanes = 2;          // MRT_OPT_LANES
     hipStream_t  lane_stream[MAX_LANES] = {};
     hipEvent_t   ev_fork = nullptr, ev_join[MAX_LANES] = {};
+    hipEvent_t   ev_stag[MAX_LANES] = {}, ev_acc[MAX_LANES] = {};
 
     // scene
     float4      *d_nodes = nullptr, *d_prims = nullptr, *d_planes = nullptr;
@@ -142,11 +151,12 @@ struct mrt_ctx {
     uint64_t     seed = 0x6D696E6952542121ull;
     int          estimator = MRT_EST_REFERENCE;
     int          stats_on = 0;
-    // paths in flight per wavefront batch: 32 M (16 spp at 1080p, about
-    // 8 GB of queues) so that each bounce launch is long against the tail
-    // of its slowest rays (4 M: 856, 8 M: 1055, 16 M: 1142, 32 M: 1185
-    // Msamples/s on the 1080p mesh workload)
-    uint32_t     path_budget = 1u << 25;
+    // paths in flight per wavefront pass: 128 M (64 spp at 1080p, about
+    // 33 GB of queues per render lane) so that each bounce launch is long
+    // against the tail of its slowest rays (1080p mesh workload at 256 spp,
+    // two lanes: 32 M: 1,461, 64 M: 1,501, 128 M: 1,521, 256 M: 1,539
+    // Msamples/s; earlier, at 2 spp per step: 4 M: 856, 32 M: 1,185)
+    uint32_t     path_budget = 1u << 27;
     bool         trace_top = true;   // MRT_OPT_TRACE_TOP
     bool         trace_refill = true;   // MRT_OPT_TRACE_REFILL
     uint32_t     refill_blocks = 0;     // its resident grid
@@ -277,10 +287,14 @@ extern "C" int mrt_init(int device, uint32_t width, uint32_t height,
             rc = MRT_E_HIP;
             break;
         }
-        for (int l = 1; l < MAX_LANES && rc == MRT_OK; l++)
-            if (hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking)
-                    != hipSuccess
+        for (int l = 0; l < MAX_LANES && rc == MRT_OK; l++)
+            if ((l && hipStreamCreateWithFlags(&c->lane_stream[l],
+                                               hipStreamNonBlocking) != hipSuccess)
                 || hipEventCreateWithFlags(&c->ev_join[l], hipEventDisableTiming)
+                       != hipSuccess
+                || hipEventCreateWithFlags(&c->ev_stag[l], hipEventDisableTiming)
+                       != hipSuccess
+                || hipEventCreateWithFlags(&c->ev_acc[l], hipEventDisableTiming)
                        != hipSuccess)
                 rc = MRT_E_HIP;
         if (rc != MRT_OK)
@@ -381,6 +395,9 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
+    for (int l = 1; l < MAX_LANES; l++)
+        if (c->lane_stream[l])
+            (void)hipStreamSynchronize(c->lane_stream[l]);
     free_wavefront(c);
     free_scene(c);
     free_env(c);
@@ -392,13 +409,12 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     dfree(c->d_ctr);
     dfree(c->d_stats);
     dfree(c->d_ovf);
-    for (int l = 1; l < MAX_LANES; l++) {
-        if (c->lane_stream[l]) {
-            (void)hipStreamSynchronize(c->lane_stream[l]);
+    for (int l = 0; l < MAX_LANES; l++) {
+        if (c->lane_stream[l])
             (void)hipStreamDestroy(c->lane_stream[l]);
-        }
-        if (c->ev_join[l])
-            (void)hipEventDestroy(c->ev_join[l]);
+        for (hipEvent_t e : {c->ev_join[l], c->ev_stag[l], c->ev_acc[l]})
+            if (e)
+                (void)hipEventDestroy(e);
     }
     if (c->ev_fork)
         (void)hipEventDestroy(c->ev_fork);
@@ -1279,7 +1295,8 @@ struct Timer {
 static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
                        const SceneDev &S, bool count, uint32_t max_depth,
                        uint32_t P, uint32_t off, uint32_t rpp, uint32_t *ctr,
-                       const StackOvf &ovf, float4 *tape = nullptr)
+                       const StackOvf &ovf, float4 *tape = nullptr,
+                       hipEvent_t ev_after = nullptr, uint32_t ev_bounce = 0)
 {
     const size_t so = (size_t)off * rpp;
     const uint32_t cap_shadow = P * rpp;
@@ -1311,6 +1328,11 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
                    launch_trace(st, c->trace_blocks, count, S, ovf, w, ctr,
                                 c->d_stats, c->trace_top, c->trace_refill,
                                 c->refill_blocks));
+        // the next pass may start (ev_after) once this bounce is traced
+        if (ev_after && (b == ev_bounce || last)) {
+            HIPCHK(c, hipEventRecord(ev_after, st));
+            ev_after = nullptr;
+        }
         if (b > 0 && lit)
             LAUNCH(c, tm, K_RESOLVE, launch_resolve(st, gb, fpp, Bp, ctr));
         if (last) {
@@ -1362,6 +1384,20 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
     return MRT_OK;
 }
 
+// Lanes of a render of n_samples in passes of nb samples (P64 paths, S64
+// shadow-ray slots each): one per pass in flight, at most MRT_OPT_LANES,
+// each with a pass's worth of queues; 0 when one pass's shadow queue
+// alone is beyond 2^32 entries.
+static uint32_t render_lanes(const mrt_ctx *c, uint32_t n_samples, uint32_t nb,
+                             uint64_t P64, uint64_t S64)
+{
+    const uint32_t npass = (n_samples + nb - 1) / nb;
+    uint32_t L = std::max(1u, std::min(c->lanes, npass));
+    while (L > 1 && (S64 * L > 0xFFFFFFF0ull || P64 * L > 0xFFFFFFF0ull))
+        L--;
+    return S64 > 0xFFFFFFF0ull ? 0u : L;
+}
+
 extern "C" int mrt_reserve(mrt_ctx *c, uint32_t n_samples)
 {
     if (!c)
@@ -1376,13 +1412,12 @@ extern "C" int mrt_reserve(mrt_ctx *c, uint32_t n_samples)
     nb = std::min(nb, n_samples);
     const uint64_t P64 = (uint64_t)nb * WH;
     const uint64_t S64 = P64 * shadow_rays_per_path(c);
-    if (S64 > 0xFFFFFFF0ull)
+    const uint32_t L = render_lanes(c, n_samples, nb, P64, S64);
+    if (!L)
         return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
-    int rc = ensure_wavefront(c, (uint32_t)P64, (uint32_t)S64);
+    int rc = ensure_wavefront(c, (uint32_t)(P64 * L), (uint32_t)(S64 * L));
     if (rc != MRT_OK)
         return rc;
-    const uint32_t L = std::max(1u, std::min(c->lanes,
-                                             (uint32_t)P64 / LANE_MIN_PATHS));
     if (ensure_ovf(c, c->trace_blocks, L) != hipSuccess)
         return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
     return MRT_OK;
@@ -1406,14 +1441,24 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
         nb = 1;
     if (nb > n_samples)
         nb = n_samples;
-    uint64_t P64 = (uint64_t)nb * WH;
+    const uint64_t P64 = (uint64_t)nb * WH;
     const uint32_t rpp = shadow_rays_per_path(c);
-    uint64_t S64 = P64 * rpp;
-    if (S64 > 0xFFFFFFF0ull)
+    const uint64_t S64 = P64 * rpp;
+    // Lanes: the render's passes go round-robin to up to MRT_OPT_LANES
+    // streams, each with a pass's worth of queues, counters and stack
+    // columns.  A pass starts once the one before it has traced bounce
+    // LANE_STAGGER_BOUNCE, so the launches of one fill the GPU while the
+    // other's late bounces wait for their longest rays.  Each pass adds its
+    // sums after the pass before it (ev_acc), so the image is the same sum
+    // in the same order whatever the lanes.
+    const uint32_t L = render_lanes(c, n_samples, nb, P64, S64);
+    if (!L)
         return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
-    int rc = ensure_wavefront(c, (uint32_t)P64, (uint32_t)S64);
+    int rc = ensure_wavefront(c, (uint32_t)(P64 * L), (uint32_t)(S64 * L));
     if (rc != MRT_OK)
         return rc;
+    if (ensure_ovf(c, c->trace_blocks, L) != hipSuccess)
+        return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
     FrameParams fp = frame_params(c);
     SceneDev S = scene_dev(c);
     const bool count = (c->stats_on & MRT_STATS_COUNT) != 0;
@@ -1425,42 +1470,45 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
         (void)hipEventCreate(&t1);
         (void)hipEventRecord(t0, c->stream);
     }
-    for (uint32_t s = 0; s < n_samples; s += nb) {
-        const uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
-        fp.sample0 = first_sample + s;
-        // Lanes: contiguous ranges of the batch's paths on their own
-        // streams; while the launches of one wait for their longest ray,
-        // the others' kernels fill the GPU.  Each path keeps its slot (and
-        // with slot0 its pixel and sample), so the accumulation below sums
-        // the same values in the same order.
-        const uint32_t P = n * WH;
-        const uint32_t L = std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS));
-        if (ensure_ovf(c, c->trace_blocks, L) != hipSuccess)
-            return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
-        if (L > 1) {
-            HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-            for (uint32_t l = 1; l < L; l++)
-                HIPCHK(c, hipStreamWaitEvent(c->lane_stream[l], c->ev_fork, 0));
-        }
-        for (uint32_t l = 0; l < L; l++) {
-            const uint32_t o0 = (uint32_t)((uint64_t)P * l / L);
-            const uint32_t o1 = (uint32_t)((uint64_t)P * (l + 1) / L);
-            FrameParams fl = fp;
-            fl.slot0 = o0;
-            hipStream_t st = l ? c->lane_stream[l] : c->stream;
-            rc = render_lane(c, st, tm, fl, S, count, max_depth, o1 - o0, o0, rpp,
-                             c->d_ctr + (size_t)l * CTR_COUNT, stack_ovf(c, l));
-            if (rc != MRT_OK)
-                return rc;
-            if (l) {
-                HIPCHK(c, hipEventRecord(c->ev_join[l], st));
-                HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join[l], 0));
-            }
-        }
-        LAUNCH(c, tm, K_ACCUM,
-               launch_accumulate(c->stream, c->acc, c->d_accum, WH, n,
-                                 c->d_stats));
+    if (L > 1) {
+        // every lane after what the context's stream holds (uploads, clears)
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        for (uint32_t l = 1; l < L; l++)
+            HIPCHK(c, hipStreamWaitEvent(c->lane_stream[l], c->ev_fork, 0));
     }
+    uint32_t l = 0;
+    for (uint32_t s = 0, k = 0; s < n_samples; s += nb, k++) {
+        const uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
+        const uint32_t P = n * WH;
+        const uint32_t prev = l;
+        l = k % L;
+        hipStream_t st = l ? c->lane_stream[l] : c->stream;
+        FrameParams fl = fp;
+        fl.sample0 = first_sample + s;
+        fl.slot0 = 0;
+        if (k && L > 1)
+            HIPCHK(c, hipStreamWaitEvent(st, c->ev_stag[prev], 0));
+        const uint32_t off = (uint32_t)(P64 * l);
+        rc = render_lane(c, st, tm, fl, S, count, max_depth, P, off, rpp,
+                         c->d_ctr + (size_t)l * CTR_COUNT, stack_ovf(c, l),
+                         nullptr, L > 1 ? c->ev_stag[l] : nullptr,
+                         LANE_STAGGER_BOUNCE);
+        if (rc != MRT_OK)
+            return rc;
+        if (k && L > 1)
+            HIPCHK(c, hipStreamWaitEvent(st, c->ev_acc[prev], 0));
+        tm.st = st;
+        LAUNCH(c, tm, K_ACCUM,
+               launch_accumulate(st, c->acc + off, c->d_accum, WH, n,
+                                 c->d_stats));
+        tm.st = nullptr;
+        if (L > 1)
+            HIPCHK(c, hipEventRecord(c->ev_acc[l], st));
+    }
+    // the context's stream after the last pass's sums (which follow every
+    // pass before it)
+    if (l)
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_acc[l], 0));
     c->paths += (uint64_t)n_samples * WH;
     if (timing) {
         (void)hipEventRecord(t1, c->stream);

@@ -177,14 +177,16 @@ def copy_probe():
                       "copy_probe_bytes_written": 4 * n, "launches": 4}))
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, pass_paths):
     """HBM bytes per launch of the trace kernel `kernel` from the committed
     counter summary (two rocprofv3 --pmc passes over this bench, scaled by
-    the copy probe), or None when it is absent or covers another kernel."""
+    the copy probe), or None when it is absent or covers another kernel or
+    another pass size."""
     f = ROOT / "profiles" / "k_trace_mesh1080p_hbm.json"
     try:
         d = json.loads(f.read_text())
-        if not d["kernel"].startswith(f"mrt::{kernel}<"):
+        if (not d["kernel"].startswith(f"mrt::{kernel}<")
+                or d.get("paths_per_pass") != pass_paths):
             return None, None
         return d["hbm_bytes_per_launch"], f.relative_to(ROOT).as_posix()
     except (OSError, KeyError, ValueError):
@@ -500,7 +502,8 @@ def caustics_leg(M, args):
     t_tab = time.perf_counter() - t1
     r.set_option(M.OPT_ESTIMATOR, M.EST_MIS)
     r.set_seed(0xCA05)
-    r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
+    if args.path_budget is not None:
+        r.set_option(M.OPT_PATH_BUDGET, args.path_budget)
     if args.lanes is not None:
         r.set_option(M.OPT_LANES, args.lanes)
     log(f"caustics: {ntri} triangles, setup {time.perf_counter() - t0:.1f} s "
@@ -567,7 +570,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--spp-per-step", type=int, default=16)
+    ap.add_argument("--spp-per-step", type=int, default=256)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--subdiv", type=int, default=8)
@@ -580,6 +583,8 @@ def main():
                     help="MRT_OPT_TRACE_TOP (default: the library's)")
     ap.add_argument("--refill", type=int, default=None,
                     help="MRT_OPT_TRACE_REFILL (default: the library's)")
+    ap.add_argument("--path-budget", type=int, default=None,
+                    help="MRT_OPT_PATH_BUDGET (default: the library's)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ticks", action="store_true",
@@ -649,7 +654,8 @@ def main():
     log(f"upload (device BVH build): {t_up_dev:.3f} s, {dev_nodes} 4-wide nodes")
     r.set_seed(0x5EED)
     spp = args.spp_per_step
-    r.set_option(M.OPT_PATH_BUDGET, spp * W * H)
+    if args.path_budget is not None:
+        r.set_option(M.OPT_PATH_BUDGET, args.path_budget)
     if args.lanes is not None:
         r.set_option(M.OPT_LANES, args.lanes)
     if args.top_cache is not None:
@@ -708,7 +714,10 @@ def main():
     step_achieved = (bytes_trace + bytes_shadow) / (dt / args.steps) / 1e9
     rays_per_s = (cnt["rays_closest"] + cnt["rays_shadow"]) / (spp * W * H) \
         * value * 1e6
-    traffic, traffic_src = measured_traffic(trace_kernel_name(args))
+    # the pass size: depth + 1 trace launches per pass
+    passes = max(1, round(launches_per_render / (args.depth + 1)))
+    traffic, traffic_src = measured_traffic(trace_kernel_name(args),
+                                            spp * W * H // passes)
 
     # 4) the drop-in's own contract (compute_frame_gpu): one sample per
     # render_frame tick, each followed by mrt_readback_add into the host's
@@ -781,6 +790,7 @@ def main():
                      "dram_frac": (round(traffic / (trace_ms * 1e-3) / 1e9
                                          / HBM_PEAK_GBS, 4) if traffic else None),
                      "bytes_per_launch": round(per_launch),
+                     "paths_per_pass": spp * W * H // passes,
                      "avg_launch_ms": round(trace_ms, 4),
                      "timed_on": "one render lane (no overlapping launch)",
                      # the trace bytes of a whole timed step (default

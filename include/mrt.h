@@ -200,8 +200,9 @@ typedef struct mrt_scene {
                                   key needs mrt_set_seed)                   */
 #define MRT_OPT_ESTIMATOR   2  /* MRT_EST_*                                  */
 #define MRT_OPT_STATS       3  /* MRT_STATS_* flags                          */
-#define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront batch
-                                  (default 2^25: 16 spp at 1080p, ~8 GB)    */
+#define MRT_OPT_PATH_BUDGET 4  /* paths in flight per wavefront pass
+                                  (default 2^27: 64 spp at 1080p, ~33 GB of
+                                  queues per render lane)                   */
 #define MRT_OPT_BLOCK       5  /* threads per block for the trace kernels    */
 #define MRT_OPT_PHOTON_K    6  /* caustic gather: k nearest photons (default
                                   15, get_caustic.c); 0 = every photon within
@@ -209,12 +210,16 @@ typedef struct mrt_scene {
 #define MRT_OPT_PHOTON_NORM 7  /* caustic norm factor (--caustic=..;..;NORM) */
 
 #define MRT_OPT_BUILDER     8  /* MRT_BUILDER_*                              */
-#define MRT_OPT_LANES      10  /* 1..4 render lanes: a render's paths split
-                                  into contiguous ranges traced on their own
-                                  HIP streams, so that one range's launches
-                                  fill the GPU while another's wait for their
-                                  longest ray (same image).  Default 2; a lane
-                                  gets at least 65,536 paths                */
+#define MRT_OPT_LANES      10  /* 1..4 render lanes: the passes of a render
+                                  of more samples than the path budget holds
+                                  run on up to this many HIP streams, each
+                                  with a pass's queues, a pass starting once
+                                  the one before it has traced its first
+                                  bounce, so that one pass's launches fill
+                                  the GPU while the other's late bounces wait
+                                  for their longest rays (same image).
+                                  Default 2; the passes of a batch are split
+                                  into this many parts of >= 65,536 paths  */
 
 #define MRT_OPT_TRACE_TOP  11  /* 1 (default): the trace kernel of scenes of
                                   triangles and spheres keeps the tree's top

@@ -1,7 +1,8 @@
-"""Render lanes (MRT_OPT_LANES): a render's paths split into contiguous
-ranges traced on their own HIP streams.  Every path keeps its slot, pixel
-and sample, so images and traversal statistics must equal the one-lane
-render's bit for bit (needs a GPU)."""
+"""Render lanes (MRT_OPT_LANES): the passes of a render larger than the
+path budget run on their own HIP streams, overlapping.  Every path keeps
+its pixel and sample and each pass adds its sums after the one before it,
+so images and traversal statistics must equal the one-lane render's bit
+for bit (needs a GPU)."""
 import numpy as np
 import pytest
 
@@ -12,11 +13,12 @@ pytestmark = pytest.mark.gpu
 
 STAT_KEYS = ("rays_closest", "rays_shadow", "nodes_closest", "prims_closest",
              "nodes_shadow", "prims_shadow", "nonfinite")
-W, H, SPP = 256, 256, 5          # 327,680 paths: four lanes of >= 65,536
+W, H, SPP = 256, 256, 5
+ONE_SAMPLE = W * H               # a path budget of one sample: five passes
 
 
 def render(hs, lanes, env=None, est=M.EST_MIS, photons=False, spp=SPP,
-           count=True, budget=None):
+           count=True, budget=ONE_SAMPLE):
     with M.Renderer(W, H) as r:
         r.upload(hs)
         r.set_seed(0x1A7E5)
@@ -67,16 +69,28 @@ def test_lanes_equal_one_lane(scenes, name):
         img, st = render(hs, lanes, env, est, photons)
         assert np.array_equal(img, ref), (name, lanes)
         assert st == ref_st, (name, lanes)
+    # one pass of every sample: the lanes have nothing to overlap
+    one, one_st = render(hs, 1, env, est, photons, budget=None)
+    img, st = render(hs, 4, env, est, photons, budget=None)
+    assert np.array_equal(img, one) and st == one_st
 
 
 def test_lanes_with_several_batches(scenes):
-    """A path budget below the render's paths: batches of 2 samples, each
-    split over the lanes, the last batch smaller."""
+    """Passes of 2 samples, the last one smaller, on three lanes, then a
+    second render call into the same sums: equal to one lane."""
     hs = scenes["mesh"][0]
-    budget = 2 * W * H
-    ref, ref_st = render(hs, 1, budget=budget, count=False)
-    img, _ = render(hs, 3, budget=budget, count=False)
-    assert np.array_equal(img, ref)
+    out = []
+    for lanes in (1, 3):
+        with M.Renderer(W, H) as r:
+            r.upload(hs)
+            r.set_seed(0x1A7E5)
+            r.set_option(M.OPT_PATH_BUDGET, 2 * W * H)
+            r.set_option(M.OPT_LANES, lanes)
+            r.render(0, SPP, 8)
+            r.render(SPP, 3, 8)
+            out.append(r.readback())
+    assert out[0].any()
+    assert np.array_equal(out[1], out[0])
 
 
 def test_lanes_option_is_validated():
