@@ -1316,6 +1316,7 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         TraceWork w;
         w.q_o = c->q_o[cur] + off;
         w.q_d = c->q_d[cur] + off;
+        w.q_t = c->q_t[cur] + off;
         w.hit = c->hit + off;
         w.s_o = c->s_o + so;
         w.s_d = c->s_d + so;
@@ -2043,10 +2044,16 @@ struct mrt_batch {
     uint32_t                max_rays = 1;
     bool                    has_lights = false;
     bool                    has_quad = false;
+    // the batched refill trace kernel applies (triangle / sphere scenes of
+    // one sidedness, at most BATCH_LDS_SCENES of them) on its resident grid
+    bool                    refill_ok = false;
+    bool                    one_sided = false;
+    uint32_t                refill_blocks = 0;
+    uint32_t                n_planes_all = 0;
     mrt_batch_tuning        tune{};
     hipGraph_t              graph = nullptr;
     hipGraphExec_t          gexec = nullptr;
-    uint32_t                g_key[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t                g_key[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 template <typename T>
@@ -2091,10 +2098,14 @@ extern "C" int mrt_batch_create(int device, mrt_batch **out)
         return MRT_E_NOMEM;
     }
     b->c = c;
-    b->tune.path_budget = 1u << 25;
+    // the autotuner's choice on the 64-scene batched workload
+    b->tune.path_budget = 1u << 27;
     b->tune.trace_blocks_per_cu = c->trace_blocks / (uint32_t)c->num_cu;
     b->tune.shade_blocks_per_cu = 32;
-    b->tune.use_graph = 1;
+    b->tune.use_graph = 0;
+    b->tune.trace_refill = 1;
+    b->refill_blocks = std::min(c->trace_blocks,
+                                (uint32_t)(c->num_cu * refill_blocks_per_cu(true)));
     *out = b;
     return MRT_OK;
 }
@@ -2266,6 +2277,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     b->max_rays = 1;
     b->has_lights = false;
     b->has_quad = false;
+    b->n_planes_all = 0;
+    bool sides_agree = true;
     for (size_t i = 0; i < n && rc == MRT_OK; i++) {
         const BatchSrc &S = *b->src[i];
         const uint32_t nn4 = on_dev[i] ? dv[i].n_nodes : bv4[i].n_nodes;
@@ -2295,6 +2308,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         T.fp.estimator = MRT_EST_REFERENCE;
         T.fp.light0 = light0;
         T.fp.plane0 = plane0;
+        T.light0 = light0;
+        T.flags = T.fp.flags;
         T.root = nn4 ? node0s[i] : 0u;
         T.n_nodes = nn4;
         T.n_planes = S.view.n_planes;
@@ -2311,6 +2326,10 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         }
         b->max_rays = std::max(b->max_rays, rays);
         b->has_lights = b->has_lights || !S.lights.empty();
+        b->n_planes_all += S.view.n_planes;
+        if (i == 0)
+            b->one_sided = T.fp.one_sided != 0;
+        sides_agree = sides_agree && (T.fp.one_sided != 0) == b->one_sided;
     }
     if (rc == MRT_OK && accum0 >= (1ull << 32))
         rc = fail(c, MRT_E_LIMIT, "batch images exceed 2^32 floats");
@@ -2318,6 +2337,8 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
         free_built();
         return rc;
     }
+    b->refill_ok = !b->has_quad && sides_agree && n <= BATCH_LDS_SCENES
+                   && b->refill_blocks > 0;
     HIPCHK(c, hipSetDevice(c->device));
     batch_free_device(b);
     BatchCtl ctl{0, path0, (uint32_t)n, 0};
@@ -2380,46 +2401,56 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
     return MRT_OK;
 }
 
-// Queues for P paths per pass.
-static int batch_buffers(mrt_batch *b, uint32_t P)
+// Queues for P paths per pass: one pass split over L lanes (a captured
+// pass), or L passes in flight on L lanes (pipelined).
+static int batch_buffers(mrt_batch *b, uint32_t P, uint32_t L, bool pipelined)
 {
     mrt_ctx *c = b->c;
-    uint64_t S64 = (uint64_t)P * b->max_rays;
-    if (S64 > 0xFFFFFFF0ull)
+    const uint64_t Q64 = pipelined ? (uint64_t)P * L : P;
+    const uint64_t S64 = Q64 * b->max_rays;
+    if (S64 > 0xFFFFFFF0ull || Q64 > 0xFFFFFFF0ull)
         return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
-    int rc = ensure_wavefront(c, P, (uint32_t)S64);
+    int rc = ensure_wavefront(c, (uint32_t)Q64, (uint32_t)S64);
     if (rc != MRT_OK)
         return rc;
-    if (ensure_ovf(c, (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu,
-                   std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS)))
-        != hipSuccess)
+    const uint32_t blocks = std::max((uint32_t)c->num_cu * b->tune.trace_blocks_per_cu,
+                                     b->refill_blocks);
+    if (ensure_ovf(c, blocks, L) != hipSuccess)
         return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
-    if (P > b->cap_slot) {
-        if (dalloc(b->d_slot, P) != hipSuccess) {
+    if (Q64 > b->cap_slot) {
+        if (dalloc(b->d_slot, Q64) != hipSuccess) {
             b->cap_slot = 0;
-            return fail(c, MRT_E_NOMEM, "slot table (%u) does not fit", P);
+            return fail(c, MRT_E_NOMEM, "slot table (%llu) does not fit",
+                        (unsigned long long)Q64);
         }
-        b->cap_slot = P;
+        b->cap_slot = (uint32_t)Q64;
     }
     return MRT_OK;
 }
 
-// One pass: the next P paths of the batch, all bounces, accumulation.
-static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
-                             bool count, Timer *tm)
+// The launch shapes and shared buffers of a batch's passes.
+struct BatchShape {
+    SceneDev  S;
+    ShadeBufs B;
+    uint32_t  tb, gb, rb;    // trace grid, shade grid, refill grid (0: k_trace)
+};
+
+static BatchShape batch_shape(const mrt_batch *b, uint32_t P)
 {
-    mrt_ctx *c = b->c;
-    SceneDev S;
-    memset(&S, 0, sizeof(S));
+    const mrt_ctx *c = b->c;
+    BatchShape sh;
+    memset(&sh, 0, sizeof(sh));
+    SceneDev &S = sh.S;
     S.nodes = b->d_nodes;
     S.prims = b->d_prims;
     S.aux = b->d_nrm;
     S.planes = b->d_planes;
     S.quad = b->has_quad ? 1u : 0u;
-    FrameParams fp;
-    memset(&fp, 0, sizeof(fp));
-    ShadeBufs B;
-    memset(&B, 0, sizeof(B));
+    // for the batched refill kernel: the sidedness every scene shares, and
+    // whether any scene has planes (their total)
+    S.one_sided = b->one_sided ? 1u : 0u;
+    S.n_planes = b->n_planes_all;
+    ShadeBufs &B = sh.B;
     B.acc = c->acc;
     B.planes = b->d_planes;
     B.prims = b->d_prims;
@@ -2439,10 +2470,13 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
     B.cap_shadow = c->cap_shadow;
     B.bs = b->d_bs;
     B.slot_info = b->d_slot;
-    const uint32_t tb = (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu;
-    uint32_t gb = (P + SHADE_BLOCK - 1) / SHADE_BLOCK;
-    gb = std::min(gb, (uint32_t)c->num_cu * b->tune.shade_blocks_per_cu);
-    hipError_t e;
+    sh.tb = (uint32_t)c->num_cu * b->tune.trace_blocks_per_cu;
+    sh.gb = std::min((P + SHADE_BLOCK - 1) / SHADE_BLOCK,
+                     (uint32_t)c->num_cu * b->tune.shade_blocks_per_cu);
+    sh.rb = (b->tune.trace_refill && b->refill_ok) ? b->refill_blocks : 0u;
+    return sh;
+}
+
 #define BSTEP(cls, expr)                                                     \
     do {                                                                     \
         c->launches[cls]++;                                                  \
@@ -2454,11 +2488,113 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
         if (e != hipSuccess)                                                 \
             return e;                                                        \
     } while (0)
-    // render lanes, as in mrt_render: contiguous parts of the pass on their
-    // own streams, joined before the accumulation (inside a captured graph
-    // the fork and join events carry the dependencies)
-    const uint32_t L = std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS));
+
+// One render lane's part of a pass: paths [o0, o0 + Pl) of the pass of P
+// paths starting at global path `base` (BASE_FROM_CTL: the device
+// cursor), on stream st with lane `lane`'s counters and stack columns and
+// the queue entries from qoff (shadow entries from qoff * max_rays).  When
+// ev_after is given it is recorded once bounce LANE_STAGGER_BOUNCE is traced.
+static hipError_t batch_lane(mrt_batch *b, const BatchShape &sh, uint32_t depth,
+                             hipStream_t st, uint32_t lane,
+                             unsigned long long base, uint32_t P, uint32_t o0,
+                             uint32_t Pl, uint32_t qoff, bool count, Timer *tm,
+                             hipEvent_t ev_after, uint32_t top_base = 0,
+                             uint32_t ntop = 0, uint32_t s_lo = 0,
+                             uint32_t s_hi = ~0u)
+{
+    mrt_ctx *c = b->c;
+    hipError_t e;
     const uint32_t R = b->max_rays;
+    const size_t so = (size_t)qoff * R;
+    uint32_t *ctr = c->d_ctr + (size_t)lane * CTR_COUNT;
+    const StackOvf ovf = stack_ovf(c, lane);
+    if (tm)
+        tm->st = st;
+    ShadeBufs Bl = sh.B;
+    Bl.acc = sh.B.acc + qoff;
+    Bl.s_o = sh.B.s_o + so;
+    Bl.s_d = sh.B.s_d + so;
+    Bl.s_c = sh.B.s_c + so;
+    Bl.vis = sh.B.vis + so;
+    Bl.p_a = sh.B.p_a + qoff;
+    Bl.p_b = sh.B.p_b + qoff;
+    Bl.p_c = sh.B.p_c + qoff;
+    Bl.p_x = sh.B.p_x + qoff;
+    Bl.p_k = sh.B.p_k + qoff;
+    Bl.slot_info = sh.B.slot_info + qoff;
+    Bl.cap_paths = Pl;
+    Bl.cap_shadow = Pl * R;
+    const uint32_t gbl = std::min((Pl + SHADE_BLOCK - 1) / SHADE_BLOCK, sh.gb);
+    BSTEP(K_RAYGEN, launch_braygen(st, Bl, b->d_ctl, base, s_lo, s_hi,
+                                   c->q_o[0] + qoff, c->q_d[0] + qoff,
+                                   c->q_t[0] + qoff, o0, Pl, P, ctr));
+    // the order of the plain render: the closest-hit rays of bounce k with
+    // the shadow rays of bounce k - 1, the light sums of k - 1, the shade of k
+    FrameParams fl;
+    memset(&fl, 0, sizeof(fl));
+    ShadeBufs Bp = Bl;
+    FrameParams fpp = fl;
+    const uint32_t n_scenes = (uint32_t)b->hbs.size();
+    for (uint32_t k = 0; k <= depth; k++) {
+        const bool last = k == depth;
+        const int cur = k & 1, nxt = cur ^ 1;
+        TraceWork w;
+        w.q_o = c->q_o[cur] + qoff;
+        w.q_d = c->q_d[cur] + qoff;
+        w.q_t = c->q_t[cur] + qoff;
+        w.hit = c->hit + qoff;
+        w.s_o = Bl.s_o;
+        w.s_d = Bl.s_d;
+        w.vis = Bl.vis;
+        w.cap_shadow = Bl.cap_shadow;
+        w.closest = !last;
+        w.shadow_ctr = (k > 0 && b->has_lights) ? ctr_shadow(k - 1) : -1;
+        if (w.closest || w.shadow_ctr >= 0)
+            BSTEP(K_TRACE, launch_btrace(st, sh.tb, count, sh.S, ovf, Bl, w, ctr,
+                                         c->d_stats, sh.rb, n_scenes, top_base,
+                                         ntop));
+        if (ev_after && (k == LANE_STAGGER_BOUNCE || last)) {
+            if ((e = hipEventRecord(ev_after, st)) != hipSuccess)
+                return e;
+            ev_after = nullptr;
+        }
+        if (k > 0 && b->has_lights)
+            BSTEP(K_RESOLVE, launch_bresolve(st, gbl, fpp, Bp, ctr));
+        if (last) {
+            BSTEP(K_ADVANCE, launch_advance(st, ctr, c->d_stats, Pl, k));
+            break;
+        }
+        fl.bounce = k;
+        fl.last_bounce = (k + 1 == depth) ? 1u : 0u;
+        ShadeBufs Bk = Bl;
+        Bk.q_o = c->q_o[cur] + qoff;
+        Bk.q_d = c->q_d[cur] + qoff;
+        Bk.q_t = c->q_t[cur] + qoff;
+        Bk.hit = c->hit + qoff;
+        Bk.qn_o = c->q_o[nxt] + qoff;
+        Bk.qn_d = c->q_d[nxt] + qoff;
+        Bk.qn_t = c->q_t[nxt] + qoff;
+        BSTEP(K_SHADE, launch_bshade(st, gbl, fl, Bk, ctr));
+        BSTEP(K_ADVANCE, launch_advance(st, ctr, c->d_stats, Pl, k));
+        Bp = Bk;
+        fpp = fl;
+    }
+    if (tm)
+        tm->st = nullptr;
+    return hipSuccess;
+}
+
+// One pass of P paths at the device cursor, split into contiguous parts on
+// the render lanes (joined before the accumulation; inside a captured graph
+// the fork and join events carry the dependencies), then the cursor moves
+// on.  This is the pass a graph captures and replays.
+static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
+                             bool count, Timer *tm)
+{
+    mrt_ctx *c = b->c;
+    const BatchShape sh = batch_shape(b, P);
+    hipError_t e;
+    const uint32_t L = std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS));
     if (L > 1) {
         if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess)
             return e;
@@ -2470,72 +2606,10 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
     for (uint32_t l = 0; l < L; l++) {
         const uint32_t o0 = (uint32_t)((uint64_t)P * l / L);
         const uint32_t Pl = (uint32_t)((uint64_t)P * (l + 1) / L) - o0;
-        const size_t so = (size_t)o0 * R;
         hipStream_t st = l ? c->lane_stream[l] : c->stream;
-        uint32_t *ctr = c->d_ctr + (size_t)l * CTR_COUNT;
-        const StackOvf ovf = stack_ovf(c, l);
-        if (tm)
-            tm->st = st;
-        ShadeBufs Bl = B;
-        Bl.acc = B.acc + o0;
-        Bl.s_o = B.s_o + so;
-        Bl.s_d = B.s_d + so;
-        Bl.s_c = B.s_c + so;
-        Bl.vis = B.vis + so;
-        Bl.p_a = B.p_a + o0;
-        Bl.p_b = B.p_b + o0;
-        Bl.p_c = B.p_c + o0;
-        Bl.p_x = B.p_x + o0;
-        Bl.p_k = B.p_k + o0;
-        Bl.slot_info = B.slot_info + o0;
-        Bl.cap_paths = Pl;
-        Bl.cap_shadow = Pl * R;
-        const uint32_t gbl = std::min((Pl + SHADE_BLOCK - 1) / SHADE_BLOCK, gb);
-        BSTEP(K_RAYGEN, launch_braygen(st, Bl, b->d_ctl, c->q_o[0] + o0,
-                                       c->q_d[0] + o0, c->q_t[0] + o0, o0, Pl, P,
-                                       ctr));
-        // the order of the plain render: the closest-hit rays of bounce k with
-        // the shadow rays of bounce k - 1, the light sums of k - 1, the shade of k
-        FrameParams fl = fp;
-        ShadeBufs Bp = Bl;
-        FrameParams fpp = fl;
-        for (uint32_t k = 0; k <= depth; k++) {
-            const bool last = k == depth;
-            const int cur = k & 1, nxt = cur ^ 1;
-            TraceWork w;
-            w.q_o = c->q_o[cur] + o0;
-            w.q_d = c->q_d[cur] + o0;
-            w.hit = c->hit + o0;
-            w.s_o = Bl.s_o;
-            w.s_d = Bl.s_d;
-            w.vis = Bl.vis;
-            w.cap_shadow = Bl.cap_shadow;
-            w.closest = !last;
-            w.shadow_ctr = (k > 0 && b->has_lights) ? ctr_shadow(k - 1) : -1;
-            if (w.closest || w.shadow_ctr >= 0)
-                BSTEP(K_TRACE, launch_btrace(st, tb, count, S, ovf, Bl, w, ctr,
-                                             c->d_stats));
-            if (k > 0 && b->has_lights)
-                BSTEP(K_RESOLVE, launch_bresolve(st, gbl, fpp, Bp, ctr));
-            if (last) {
-                BSTEP(K_ADVANCE, launch_advance(st, ctr, c->d_stats, Pl, k));
-                break;
-            }
-            fl.bounce = k;
-            fl.last_bounce = (k + 1 == depth) ? 1u : 0u;
-            ShadeBufs Bk = Bl;
-            Bk.q_o = c->q_o[cur] + o0;
-            Bk.q_d = c->q_d[cur] + o0;
-            Bk.q_t = c->q_t[cur] + o0;
-            Bk.hit = c->hit + o0;
-            Bk.qn_o = c->q_o[nxt] + o0;
-            Bk.qn_d = c->q_d[nxt] + o0;
-            Bk.qn_t = c->q_t[nxt] + o0;
-            BSTEP(K_SHADE, launch_bshade(st, gbl, fl, Bk, ctr));
-            BSTEP(K_ADVANCE, launch_advance(st, ctr, c->d_stats, Pl, k));
-            Bp = Bk;
-            fpp = fl;
-        }
+        if ((e = batch_lane(b, sh, depth, st, l, BASE_FROM_CTL, P, o0, Pl, o0,
+                            count, tm, nullptr)) != hipSuccess)
+            return e;
         if (l) {
             if ((e = hipEventRecord(c->ev_join[l], st)) != hipSuccess)
                 return e;
@@ -2543,14 +2617,103 @@ static hipError_t batch_pass(mrt_batch *b, uint32_t depth, uint32_t P,
                 return e;
         }
     }
-    if (tm)
-        tm->st = nullptr;
-    BSTEP(K_ACCUM, launch_baccumulate(c->stream, B, b->d_ctl, b->d_accum, P,
-                                      c->d_ctr, c->d_stats));
+    BSTEP(K_ACCUM, launch_baccumulate(c->stream, sh.B, b->d_ctl, BASE_FROM_CTL,
+                                      b->d_accum, P, c->d_ctr, c->d_stats));
     BSTEP(K_ACCUM, launch_bnext(c->stream, b->d_ctl, P));
-#undef BSTEP
     return hipSuccess;
 }
+
+// The scenes [s_lo, s_hi] that global paths [base, base + P) fall in, and
+// the one holding most of them: the batched refill kernel keeps the top of
+// its tree in LDS for that pass (every scene's tree starts at its root with
+// its top levels first).
+static void pass_top(const mrt_batch *b, uint64_t base, uint32_t P,
+                     uint32_t &top_base, uint32_t &ntop, uint32_t &s_lo,
+                     uint32_t &s_hi)
+{
+    const std::vector<BatchScene> &h = b->hbs;
+    size_t lo = 0, hi = h.size();
+    while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if (h[mid].path0 + h[mid].paths <= base)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    const uint64_t end = base + P;
+    uint64_t best = 0;
+    top_base = 0;
+    ntop = 0;
+    s_lo = (uint32_t)std::min(lo, h.size() - 1);
+    s_hi = s_lo;
+    for (size_t i = lo; i < h.size() && h[i].path0 < end; i++) {
+        s_hi = (uint32_t)i;
+        const uint64_t a = std::max<uint64_t>(base, h[i].path0);
+        const uint64_t z = std::min<uint64_t>(end, h[i].path0 + h[i].paths);
+        if (z > a && z - a > best && h[i].n_nodes) {
+            best = z - a;
+            top_base = h[i].root;
+            ntop = std::min<uint32_t>((uint32_t)TOP_NODES, h[i].n_nodes);
+        }
+    }
+}
+
+// Passes of P paths from global path 0 to `total`, launched directly and
+// pipelined over the render lanes as mrt_render's passes are: pass k runs
+// whole on lane k % L with that lane's queues, starts once pass k - 1 has
+// traced bounce LANE_STAGGER_BOUNCE, and adds its sums after pass k - 1's,
+// so the sums are those of the captured pass replayed pass after pass.
+static hipError_t batch_pipelined(mrt_batch *b, uint32_t depth, uint32_t P,
+                                  uint32_t L, uint64_t total, bool count, Timer *tm)
+{
+    mrt_ctx *c = b->c;
+    const BatchShape sh = batch_shape(b, P);
+    hipError_t e;
+    if (L > 1) {
+        if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess)
+            return e;
+        for (uint32_t l = 1; l < L; l++)
+            if ((e = hipStreamWaitEvent(c->lane_stream[l], c->ev_fork, 0))
+                != hipSuccess)
+                return e;
+    }
+    uint32_t l = 0;
+    uint64_t k = 0;
+    for (uint64_t base = 0; base < total; base += P, k++) {
+        const uint32_t prev = l;
+        l = (uint32_t)(k % L);
+        hipStream_t st = l ? c->lane_stream[l] : c->stream;
+        if (k && L > 1 && (e = hipStreamWaitEvent(st, c->ev_stag[prev], 0)) != hipSuccess)
+            return e;
+        const uint32_t qoff = P * l;
+        uint32_t top_base = 0, ntop = 0, s_lo = 0, s_hi = 0;
+        pass_top(b, base, P, top_base, ntop, s_lo, s_hi);
+        if ((e = batch_lane(b, sh, depth, st, l, base, P, 0, P, qoff, count, tm,
+                            L > 1 ? c->ev_stag[l] : nullptr, top_base, ntop,
+                            s_lo, s_hi))
+            != hipSuccess)
+            return e;
+        if (k && L > 1 && (e = hipStreamWaitEvent(st, c->ev_acc[prev], 0)) != hipSuccess)
+            return e;
+        ShadeBufs Bl = sh.B;
+        Bl.acc = sh.B.acc + qoff;
+        Bl.slot_info = sh.B.slot_info + qoff;
+        if (tm)
+            tm->st = st;
+        BSTEP(K_ACCUM, launch_baccumulate(st, Bl, b->d_ctl, base, b->d_accum, P,
+                                          c->d_ctr + (size_t)l * CTR_COUNT,
+                                          c->d_stats));
+        if (tm)
+            tm->st = nullptr;
+        if (L > 1 && (e = hipEventRecord(c->ev_acc[l], st)) != hipSuccess)
+            return e;
+    }
+    // the context's stream after the last pass's sums
+    if (l && (e = hipStreamWaitEvent(c->stream, c->ev_acc[l], 0)) != hipSuccess)
+        return e;
+    return hipSuccess;
+}
+#undef BSTEP
 
 // Renders global paths [0, total) of the batch in passes of P.
 static int batch_run(mrt_batch *b, uint32_t depth, uint64_t total)
@@ -2559,25 +2722,31 @@ static int batch_run(mrt_batch *b, uint32_t depth, uint64_t total)
     uint32_t P = (uint32_t)std::min<uint64_t>(b->tune.path_budget, total);
     if (P == 0)
         return MRT_OK;
-    int rc = batch_buffers(b, P);
+    const uint64_t passes = (total + P - 1) / P;
+    const bool count = (c->stats_on & MRT_STATS_COUNT) != 0;
+    const bool timing = (c->stats_on & MRT_STATS_TIME) != 0;
+    const bool graph = b->tune.use_graph && !count && !timing;
+    // lanes: a captured pass splits into up to MRT_OPT_LANES parts of at
+    // least LANE_MIN_PATHS; direct launches pipeline up to that many passes
+    const uint32_t L = graph
+        ? std::max(1u, std::min(c->lanes, P / LANE_MIN_PATHS))
+        : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c->lanes, passes));
+    int rc = batch_buffers(b, P, L, !graph);
     if (rc != MRT_OK)
         return rc;
     BatchCtl ctl{0, total, (uint32_t)b->hbs.size(), 0};
     HIPCHK(c, hipMemcpyAsync(b->d_ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice,
                              c->stream));
-    const uint64_t passes = (total + P - 1) / P;
-    const bool count = (c->stats_on & MRT_STATS_COUNT) != 0;
-    const bool timing = (c->stats_on & MRT_STATS_TIME) != 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (timing) {
         (void)hipEventCreate(&t0);
         (void)hipEventCreate(&t1);
         (void)hipEventRecord(t0, c->stream);
     }
-    if (b->tune.use_graph && !count && !timing) {
-        const uint32_t key[6] = {depth, P, b->tune.trace_blocks_per_cu,
+    if (graph) {
+        const uint32_t key[7] = {depth, P, b->tune.trace_blocks_per_cu,
                                  b->tune.shade_blocks_per_cu, c->cap_shadow,
-                                 c->lanes};
+                                 c->lanes, b->tune.trace_refill};
         if (!b->gexec || memcmp(key, b->g_key, sizeof(key)) != 0) {
             if (b->gexec)
                 (void)hipGraphExecDestroy(b->gexec);
@@ -2603,13 +2772,11 @@ static int batch_run(mrt_batch *b, uint32_t depth, uint64_t total)
             HIPCHK(c, hipGraphLaunch(b->gexec, c->stream));
     } else {
         Timer tm{c, timing, {}};
-        for (uint64_t k = 0; k < passes; k++) {
-            hipError_t e = batch_pass(b, depth, P, count, &tm);
-            if (e != hipSuccess) {
-                tm.collect();
-                return fail(c, MRT_E_HIP, "kernel launch failed: %s",
-                            hipGetErrorString(e));
-            }
+        hipError_t e = batch_pipelined(b, depth, P, L, total, count, &tm);
+        if (e != hipSuccess) {
+            tm.collect();
+            return fail(c, MRT_E_HIP, "kernel launch failed: %s",
+                        hipGetErrorString(e));
         }
         tm.collect();
     }
@@ -2672,11 +2839,13 @@ extern "C" int mrt_batch_clear(mrt_batch *b)
 
 static int check_tuning(mrt_batch *b, const mrt_batch_tuning *t)
 {
-    if (t->path_budget < 1024 || t->path_budget > (1u << 26))
-        return fail(b->c, MRT_E_LIMIT, "path budget must be in 2^10..2^26");
+    if (t->path_budget < 1024 || t->path_budget > (1u << 27))
+        return fail(b->c, MRT_E_LIMIT, "path budget must be in 2^10..2^27");
     if (t->trace_blocks_per_cu < 1 || t->trace_blocks_per_cu > 32
         || t->shade_blocks_per_cu < 1 || t->shade_blocks_per_cu > 32)
         return fail(b->c, MRT_E_LIMIT, "blocks per CU must be in 1..32");
+    if (t->use_graph > 1 || t->trace_refill > 1)
+        return fail(b->c, MRT_E_INVALID, "use_graph and trace_refill are 0 or 1");
     return MRT_OK;
 }
 
@@ -2718,32 +2887,24 @@ extern "C" int mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
     const int saved_stats = c->stats_on;
     c->stats_on = 0;
     const uint32_t occ = std::max(1u, c->trace_blocks / (uint32_t)c->num_cu);
-    const uint64_t trial = std::min<uint64_t>(b->total, 1ull << 26);
-    std::vector<mrt_batch_tuning> cand;
-    for (uint32_t pb : {1u << 22, 1u << 23, 1u << 24, 1u << 25})
-        for (uint32_t tbc : {occ, std::max(1u, occ / 2)})
-            for (uint32_t sbc : {8u, 32u}) {
-                if (pb > 2 * trial && pb != (1u << 22))
-                    continue;
-                mrt_batch_tuning t{pb, tbc, sbc, 1, 0.0};
-                cand.push_back(t);
-            }
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     mrt_batch_tuning best = b->tune;
     double best_ms = 1e300;
     int rc = MRT_OK;
-    for (mrt_batch_tuning &t : cand) {
+    // times the first `trial` paths under t (after a warm-up render that
+    // also captures the graph) and keeps the fastest
+    auto try_one = [&](mrt_batch_tuning t, uint64_t trial) {
         b->tune = t;
-        rc = batch_run(b, max_depth, trial);          // warm-up and capture
+        rc = batch_run(b, max_depth, trial);
         if (rc != MRT_OK)
-            break;
+            return;
         (void)hipEventRecord(e0, c->stream);
         rc = batch_run(b, max_depth, trial);
         (void)hipEventRecord(e1, c->stream);
         if (rc != MRT_OK)
-            break;
+            return;
         (void)hipEventSynchronize(e1);
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
@@ -2752,6 +2913,35 @@ extern "C" int mrt_batch_autotune(mrt_batch *b, uint32_t max_depth,
             best_ms = t.ms_per_mpath;
             best = t;
         }
+    };
+    // stage 1, the launch shape at 16 M-path passes over the first 64 M
+    // paths: trace kernel (the refill kernel on its resident grid, or
+    // k_trace on one of two grids), shade grid, and a captured pass
+    // replayed or passes pipelined on the lanes
+    const uint64_t trial1 = std::min<uint64_t>(b->total, 1ull << 26);
+    const uint32_t pb1 = (uint32_t)std::min<uint64_t>(1u << 24, trial1);
+    for (uint32_t tk = 0; tk < 3 && rc == MRT_OK; tk++)
+        for (uint32_t sbc : {8u, 32u})
+            for (uint32_t graph : {1u, 0u}) {
+                const uint32_t refill = tk == 0 ? 1u : 0u;
+                if ((refill && !b->refill_ok) || rc != MRT_OK)
+                    continue;
+                const uint32_t tbc = tk == 2 ? std::max(1u, occ / 2) : occ;
+                try_one(mrt_batch_tuning{std::max(pb1, 1024u), tbc, sbc, graph,
+                                         refill, 0.0},
+                        trial1);
+            }
+    // stage 2, the path budget of that shape over the first 256 M paths
+    // (longer passes: each bounce launch is long against its slowest rays)
+    const uint64_t trial2 = std::min<uint64_t>(b->total, 1ull << 28);
+    const mrt_batch_tuning shape = best;
+    best_ms = 1e300;
+    for (uint32_t pb : {1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26, 1u << 27}) {
+        if (rc != MRT_OK || (pb > trial2 && pb != (1u << 22)))
+            continue;
+        mrt_batch_tuning t = shape;
+        t.path_budget = pb;
+        try_one(t, trial2);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -2781,8 +2971,8 @@ extern "C" int mrt_batch_set_option(mrt_batch *b, int option, double value)
         return mrt_set_option(b->c, option, value);
     case MRT_OPT_PATH_BUDGET: {
         mrt_batch_tuning t = b->tune;
-        if (!(value >= 1024.0) || value > (double)(1u << 26))
-            return fail(b->c, MRT_E_LIMIT, "path budget must be in 2^10..2^26");
+        if (!(value >= 1024.0) || value > (double)(1u << 27))
+            return fail(b->c, MRT_E_LIMIT, "path budget must be in 2^10..2^27");
         t.path_budget = (uint32_t)value;
         return mrt_batch_set_tuning(b, &t);
     }

@@ -26,6 +26,10 @@ constexpr int TRACE_OVF = STACK4_MAX - TRACE_SS;  // the rest, per lane
 constexpr int TOP_NODES = MRT_TOP_NODES;
 constexpr int TRACE_SS_TOP = 14;
 static_assert(TOP_NODES <= (int)MRT_BVH4_TOP, "cached nodes are laid out first");
+// Scenes whose roots the batched refill trace kernel keeps in LDS (2 KB
+// beside its 7 KB stack and 2.3 KB tree top: 12 blocks, 6 waves per SIMD,
+// still fit a CU); larger batches take k_trace.
+constexpr uint32_t BATCH_LDS_SCENES = 512;
 // per-lane columns of the overflow buffer: enough for either stack split
 constexpr int TRACE_OVF_BUF = STACK4_MAX - (TRACE_SS_TOP < TRACE_SS ? TRACE_SS_TOP
                                                                     : TRACE_SS);
@@ -163,11 +167,13 @@ struct FrameParams {
 // shared arrays, and its share of the global path index space
 // (W*H*spp paths, sample-major).
 struct BatchScene {
-    FrameParams fp;
     uint32_t    root, n_nodes, n_planes;
     uint32_t    accum0;       // first float of its W*H*3 sums
     unsigned long long path0; // first global path index
     unsigned long long paths; // W*H*spp
+    uint32_t    light0, flags;   // fp.light0, fp.flags: in the first cache
+                                 // line, for the light sums of k_resolve
+    FrameParams fp;
 };
 
 // Device-side cursor of a batch: the first global path of the current
@@ -241,6 +247,7 @@ hipError_t launch_raygen(hipStream_t st, const FrameParams &fp, float4 *q_o,
 // closest) and the shadow rays counted at ctr[shadow_ctr] (when >= 0).
 struct TraceWork {
     const float4 *q_o, *q_d;      // closest-hit rays: origin | ao, dir | slot
+    const float4 *q_t;            // a batch's: .w holds the ray's scene
     float4       *hit;            // (t, code, u, v) per closest-hit ray
     const float4 *s_o, *s_d;      // shadow rays: origin | tmax, dir | tag
     uint8_t      *vis;            // 1: the shadow ray reaches its light
@@ -275,8 +282,9 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
                              const float *orig, const float *dir,
                              uint32_t *prim_id, float *t);
 int trace_blocks_per_cu();
-// resident blocks per CU of the per-lane refill trace kernel
-int refill_blocks_per_cu();
+// resident blocks per CU of the per-lane refill trace kernel (single
+// scenes, or batch = true)
+int refill_blocks_per_cu(bool batch = false);
 
 // Path-replay adjoint: dL/dI per pixel in, gradients out.
 constexpr int ADJ_MAXD = 8;      // deepest path the adjoint replays
@@ -298,17 +306,28 @@ hipError_t launch_adjoint_back(hipStream_t st, uint32_t blocks,
                                const AdjBufs &A, uint32_t P);
 
 // batched scenes (the same stages over the scene table of a batch)
+// A pass starts at global path `base`, or at the device cursor ctl->base
+// when base is BASE_FROM_CTL (a captured pass replayed as a graph).
+constexpr unsigned long long BASE_FROM_CTL = ~0ull;
 // Camera rays of a render lane's part [o0, o0 + Pl) of a pass of P paths
-// (B, q_* and ctr already that lane's); lane 0 (o0 = 0) also records the
-// pass's path count for k_baccumulate.
+// (B, q_* and ctr already that lane's), whose paths lie in scenes
+// [s_lo, s_hi]; lane 0 (o0 = 0) also records the pass's path count for
+// k_baccumulate.
 hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
-                          const BatchCtl *ctl, float4 *q_o, float4 *q_d,
+                          const BatchCtl *ctl, unsigned long long base,
+                          uint32_t s_lo, uint32_t s_hi, float4 *q_o, float4 *q_d,
                           float4 *q_t, uint32_t o0, uint32_t Pl, uint32_t P,
                           uint32_t *ctr);
+// refill_blocks > 0: the batched refill kernel on that grid (triangle /
+// sphere scenes of one sidedness, at most BATCH_LDS_SCENES of them), with
+// nodes [top_base, top_base + ntop) (ntop <= TOP_NODES, the top of one
+// scene's tree) staged in LDS; else k_trace on `blocks`.
 hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
                          const SceneDev &S, const StackOvf &ovf,
                          const ShadeBufs &B, const TraceWork &w,
-                         uint32_t *ctr, unsigned long long *stats);
+                         uint32_t *ctr, unsigned long long *stats,
+                         uint32_t refill_blocks, uint32_t n_scenes,
+                         uint32_t top_base, uint32_t ntop);
 hipError_t launch_bshade(hipStream_t st, uint32_t blocks,
                          const FrameParams &fp, const ShadeBufs &B,
                          uint32_t *ctr);
@@ -316,8 +335,9 @@ hipError_t launch_bresolve(hipStream_t st, uint32_t blocks,
                            const FrameParams &fp, const ShadeBufs &B,
                            uint32_t *ctr);
 hipError_t launch_baccumulate(hipStream_t st, const ShadeBufs &B,
-                              const BatchCtl *ctl, float *accum, uint32_t P,
-                              uint32_t *ctr, unsigned long long *stats);
+                              const BatchCtl *ctl, unsigned long long base,
+                              float *accum, uint32_t P, uint32_t *ctr,
+                              unsigned long long *stats);
 hipError_t launch_bnext(hipStream_t st, BatchCtl *ctl, uint32_t P);
 
 hipError_t launch_photon_emit(hipStream_t st, const FrameParams &fp,

@@ -6,7 +6,8 @@
 // with every queue in HBM as structure-of-arrays of 16-byte words:
 //   path queue   q_o = (origin, ao of the last material)
 //                q_d = (direction, path slot)
-//                q_t = (throughput, unused)
+//                q_t = (throughput, MIS pdf of the last BSDF sample; in a
+//                       batch, which has no HDRI, the path's scene)
 //   hit record   hit = (t, hit code, barycentric u, v)
 //   shadow queue s_o = (origin, t limit), s_d = (direction, light tag),
 //                s_c = (n.l, (r.v)^64), vis (1 = unoccluded)
@@ -360,18 +361,39 @@ constexpr uint32_t REFILL_MIN = 16;
 // latency.  REFILL_MIN 16 against 8 / 24: 1,367 / 1,364.  A leaf takes one
 // loop iteration per pair of records, the rest of it staying the lane's
 // code (1,251 -> 1,379 against an inner loop over the whole leaf).
-template <bool COUNT>
+//
+// BATCH: the rays of many scenes (triangles and spheres, one sidedness for
+// all) whose trees share the node array.  A ray starts at its scene's root,
+// read from a table of the scenes' roots in LDS: a closest-hit ray's scene
+// comes with the ray (q_t.w, written by the batch's raygen and shade), a
+// shadow ray's in its tag.  The ray state is the single scene's, so the
+// kernel keeps its registers; the LDS copy of a tree's top holds the top of
+// one scene's tree (the host names the scene with most of the pass's
+// paths), and a ray that ends reads its scene's planes only when the batch
+// has planes.
+template <bool COUNT, bool BATCH>
 __global__ void __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(6, 6)))
 k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
-               unsigned long long *__restrict__ stats, uint32_t *__restrict__ ovf_buf)
+               unsigned long long *__restrict__ stats, uint32_t *__restrict__ ovf_buf,
+               const BatchScene *__restrict__ bs, uint32_t n_scenes,
+               uint32_t top_base, uint32_t ntop)
 {
     constexpr int SS = TRACE_SS_TOP;
     __shared__ uint32_t s_code[SS * TRACE_BLOCK];
     __shared__ float4 s_top[TOP_NODES * 7];
-    const uint32_t ntop = min((uint32_t)TOP_NODES, S.n_nodes);
+    __shared__ uint32_t s_root[BATCH ? BATCH_LDS_SCENES : 1];
+    // the top ntop nodes of one tree, nodes [top_base, top_base + ntop):
+    // a single scene's (top_base 0), or those of the scene that holds most
+    // of a batch pass's paths (each scene's tree starts at its root, top
+    // levels first)
+    ntop = min(ntop, (uint32_t)TOP_NODES);
     for (uint32_t k = threadIdx.x; k < ntop * 7; k += TRACE_BLOCK)
-        s_top[k] = S.nodes[(size_t)(k / 7) * 8 + k % 7];
+        s_top[k] = S.nodes[(size_t)(top_base + k / 7) * 8 + k % 7];
+    if (BATCH) {
+        for (uint32_t k = threadIdx.x; k < n_scenes; k += TRACE_BLOCK)
+            s_root[k] = bs[k].n_nodes ? bs[k].root : (uint32_t)INT32_MIN;
+    }
     __syncthreads();
     const uint32_t nc = w.closest ? ctr[CTR_CUR] : 0u;
     const uint32_t ns = w.shadow_ctr >= 0 ? min(ctr[w.shadow_ctr], w.cap_shadow) : 0u;
@@ -438,14 +460,19 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                 if (((idle >> lane) & 1ull) && rank < avail) {
                     item = pos + rank;
                     float4 o4, d4;
+                    uint32_t sid = 0;
                     if (item < nc) {
                         o4 = w.q_o[item];
                         d4 = w.q_d[item];
+                        if (BATCH)
+                            sid = __float_as_uint(w.q_t[item].w);
                         any = false;
                         tclose = IT_MAX_F;
                     } else {
                         o4 = w.s_o[item - nc];
                         d4 = w.s_d[item - nc];
+                        if (BATCH)
+                            sid = tag_scene(__float_as_uint(d4.w));
                         any = true;
                         tclose = fminf(o4.w, IT_MAX_F);
                     }
@@ -456,7 +483,10 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                     hslot = HIT_NONE;
                     bu = bv = 0.0f;
                     sp = 0;
-                    code = S.n_nodes ? (int32_t)S.root : DONE;
+                    if (BATCH)
+                        code = (int32_t)s_root[sid];
+                    else
+                        code = S.n_nodes ? (int32_t)S.root : DONE;
                 }
                 pos += min((uint32_t)__popcll(idle), avail);
                 idle = __ballot(item == HIT_NONE);
@@ -479,8 +509,10 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                 if (code == DONE) {
                 } else if (code >= 0) {
                     float4 lx, hx, ly, hy, lz, hz, cw;
-                    if ((uint32_t)code < ntop) {
-                        const float4 *n = s_top + (size_t)code * 7;
+                    const uint32_t rel = BATCH ? (uint32_t)code - top_base
+                                               : (uint32_t)code;
+                    if (rel < ntop) {
+                        const float4 *n = s_top + (size_t)rel * 7;
                         lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
                         cw = n[6];
                     } else {
@@ -578,14 +610,26 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                         code = cnt > 2 ? (int32_t)~(((first + 2u) << 3) | (cnt - 3u)) : pop();
                 }
                 if (found || code == DONE) {
+                    // the ray's planes: the scene's, or in a batch with
+                    // planes those of the ray's scene (S.n_planes is then
+                    // the batch's total)
+                    const float4 *pl = S.planes;
+                    uint32_t npl = S.n_planes;
+                    if (BATCH && npl) {
+                        const BatchScene &b =
+                            bs[any ? tag_scene(__float_as_uint(w.s_d[item - nc].w))
+                                   : __float_as_uint(w.q_t[item].w)];
+                        pl = S.planes + 2 * (size_t)b.fp.plane0;
+                        npl = b.n_planes;
+                    }
                     if (!any) {
                         // closest_hit: the planes after the BVH
                         float tb = hit ? tclose : IT_MAX_F;
                         uint32_t slot = hit ? hslot : HIT_NONE;
                         bool hh = hit;
-                        for (uint32_t k = 0; k < S.n_planes; k++) {
+                        for (uint32_t k = 0; k < npl; k++) {
                             float tp;
-                            if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
+                            if (isect_pln(o, d, pl[2 * k], pl[2 * k + 1], tp)
                                 && (!hh || tp < tb)) {
                                 hh = true;
                                 tb = tp;
@@ -597,9 +641,9 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                     } else {
                         bool occ = hit;
                         const float tlim = occ ? 0.0f : w.s_o[item - nc].w;
-                        for (uint32_t k = 0; k < S.n_planes && !occ; k++) {
+                        for (uint32_t k = 0; k < npl && !occ; k++) {
                             float tp;
-                            if (isect_pln(o, d, S.planes[2 * k], S.planes[2 * k + 1], tp)
+                            if (isect_pln(o, d, pl[2 * k], pl[2 * k + 1], tp)
                                 && tp < tlim)
                                 occ = true;
                         }
@@ -1098,9 +1142,10 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
             slot = __float_as_uint(d4.w);
             uint32_t local = slot;
             if (BATCH) {
-                const uint2 si = B.slot_info[slot];
-                sid = si.x;
-                local = si.y;
+                // the scene comes with the path (q_t.w), so its frame loads
+                // beside the slot's local index instead of after it
+                sid = __float_as_uint(t4.w);
+                local = B.slot_info[slot].y;
                 Fp = &B.bs[sid].fp;
             }
             const FrameParams &F = *Fp;
@@ -1298,7 +1343,10 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         if (want_next) {
             B.qn_o[qn] = make_float4(p.x, p.y, p.z, ao);
             B.qn_d[qn] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
-            B.qn_t[qn] = make_float4(thr.x, thr.y, thr.z, pb_next);
+            // a batch has no HDRI: .w carries the path's scene instead of
+            // the BSDF pdf, for the batched refill trace kernel
+            B.qn_t[qn] = make_float4(thr.x, thr.y, thr.z,
+                                     BATCH ? __uint_as_float(sid) : pb_next);
         }
         if (kind == SC_DIFFUSE && n_rays) {
             V3 view = unit(sub(v3(F.cam_pos[0], F.cam_pos[1], F.cam_pos[2]), p));
@@ -1354,8 +1402,9 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         }
         if (want_pend) {
             B.p_a[pr] = make_float4(cp.x, cp.y, cp.z, __uint_as_float(slot));
+            // .w: bit 0 the environment ray, bits 1.. a batch path's scene
             B.p_b[pr] = make_uint4(want_next ? qn : HIT_NONE, j0, n_rays,
-                                   env_ray ? 1u : 0u);
+                                   (env_ray ? 1u : 0u) | (sid << 1));
             B.p_c[pr] = make_float4(env_rad.x, env_rad.y, env_rad.z, env_omc);
             if (F.has_photons)
                 B.p_x[pr] = make_float4(p.x, p.y, p.z, 0.0f);
@@ -1381,12 +1430,21 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         float4 a = B.p_a[r];
         uint4 b = B.p_b[r];
         uint32_t slot = __float_as_uint(a.w);
-        const FrameParams &F = BATCH ? B.bs[B.slot_info[slot].x].fp : fp;
-        const bool exact = (F.flags & MRT_SCENE_LIGHT_EXACT) != 0;
-        const float4 *L = B.lights + 3 * (size_t)F.light0;
+        // a batch path's scene is in p_b.w (bits 1..); its light range and
+        // flags sit in the first line of its record.  Batches have no HDRI
+        // and no photon map.
+        uint32_t flags = fp.flags, light0 = fp.light0;
+        if (BATCH) {
+            const BatchScene &bsc = B.bs[b.w >> 1];
+            flags = bsc.flags;
+            light0 = bsc.light0;
+        }
+        const bool env = !BATCH && (b.w & 1u);
+        const bool exact = (flags & MRT_SCENE_LIGHT_EXACT) != 0;
+        const float4 *L = B.lights + 3 * (size_t)light0;
         uint32_t j = b.y, end = b.y + b.z;
         float lr = 0.0f, lg = 0.0f, lb = 0.0f;
-        if (b.w)
+        if (env)
             end--;                      // the environment ray comes last
         while (j < end) {
             uint32_t tag = __float_as_uint(B.s_d[j].w);
@@ -1429,18 +1487,18 @@ k_resolve(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
         float4 s = B.acc[slot];
         // default_mat: total lighting = direct + caustics
         V3 dl = v3(lr / 255.0f, lg / 255.0f, lb / 255.0f);
-        if (F.has_photons)
+        if (!BATCH && fp.has_photons)
             dl = add(dl, xyz(B.p_k[r]));
         s.x = s.x + a.x * dl.x;
         s.y = s.y + a.y * dl.y;
         s.z = s.z + a.z * dl.z;
-        const bool env_vis = b.w && B.vis[end];
+        const bool env_vis = env && B.vis[end];
         if (env_vis) {
             float4 c = B.p_c[r];
             s.x = s.x + c.x;
             s.y = s.y + c.y;
             s.z = s.z + c.z;
-            if (!REC && F.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
+            if (!REC && fp.estimator == MRT_EST_REFERENCE && b.x != HIT_NONE) {
                 float4 t = B.qn_t[b.x];
                 t.x = t.x * c.w;
                 t.y = t.y * c.w;
@@ -1681,11 +1739,11 @@ hipError_t launch_adjoint_back(hipStream_t st, uint32_t blocks,
 // The global path index space is the scenes' W*H*spp ranges one after the
 // other (sample-major inside a scene); a pass takes P consecutive indices.
 
-// Scene holding global path g: the last scene with path0 <= g.
-__device__ __forceinline__ uint32_t scene_at(const BatchScene *bs, uint32_t ns,
-                                             unsigned long long g)
+// Scene holding global path g: the last scene with path0 <= g, searched
+// in scenes [lo, hi] (those of the pass).
+__device__ __forceinline__ uint32_t scene_at(const BatchScene *bs, uint32_t lo,
+                                             uint32_t hi, unsigned long long g)
 {
-    uint32_t lo = 0, hi = ns - 1;
     while (lo < hi) {
         uint32_t mid = (lo + hi + 1) >> 1;
         if (bs[mid].path0 <= g)
@@ -1698,12 +1756,15 @@ __device__ __forceinline__ uint32_t scene_at(const BatchScene *bs, uint32_t ns,
 
 __global__ void __launch_bounds__(256)
 k_braygen(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
+          unsigned long long base, uint32_t s_lo, uint32_t s_hi,
           float4 *__restrict__ q_o, float4 *__restrict__ q_d,
           float4 *__restrict__ q_t, float4 *__restrict__ acc,
           uint2 *__restrict__ slot_info, uint32_t o0, uint32_t Pl, uint32_t P,
           uint32_t *__restrict__ ctr)
 {
-    const unsigned long long base = ctl->base, total = ctl->total;
+    if (base == BASE_FROM_CTL)
+        base = ctl->base;
+    const unsigned long long total = ctl->total;
     const uint32_t n = base >= total ? 0u
                        : (uint32_t)min((unsigned long long)P, total - base);
     const uint32_t nl = n > o0 ? min(Pl, n - o0) : 0u;   // this lane's paths
@@ -1716,14 +1777,14 @@ k_braygen(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
     if (i >= nl)
         return;
     const unsigned long long g = base + o0 + i;
-    const uint32_t sid = scene_at(bs, ctl->n_scenes, g);
+    const uint32_t sid = scene_at(bs, s_lo, min(s_hi, ctl->n_scenes - 1), g);
     const FrameParams &F = bs[sid].fp;
     const uint32_t local = (uint32_t)(g - bs[sid].path0);
     V3 o, d;
     primary_ray(F, local % F.WH, F.sample0 + local / F.WH, o, d);
     q_o[i] = make_float4(o.x, o.y, o.z, 1.0f);
     q_d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(i));
-    q_t[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    q_t[i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(sid));
     acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     slot_info[i] = make_uint2(sid, local);
 }
@@ -1733,12 +1794,13 @@ k_braygen(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
 // i, i + W*H, ... inside the scene's part of the pass).
 __global__ void __launch_bounds__(256)
 k_baccumulate(const BatchScene *__restrict__ bs, const BatchCtl *__restrict__ ctl,
-              const float4 *__restrict__ acc, const uint2 *__restrict__ slot_info,
-              float *__restrict__ accum, const uint32_t *__restrict__ ctr,
-              unsigned long long *__restrict__ stats)
+              unsigned long long base, const float4 *__restrict__ acc,
+              const uint2 *__restrict__ slot_info, float *__restrict__ accum,
+              const uint32_t *__restrict__ ctr, unsigned long long *__restrict__ stats)
 {
     const uint32_t n = ctr[CTR_PASS];
-    const unsigned long long base = ctl->base;
+    if (base == BASE_FROM_CTL)
+        base = ctl->base;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t bad = 0;
     if (i < n) {
@@ -2581,12 +2643,15 @@ hipError_t launch_trace(hipStream_t st, uint32_t blocks, bool count,
             return hipErrorInvalidValue;
         // the grid: the blocks that stay resident at the kernel's 6 waves
         // per SIMD (refill_blocks_per_cu), not the 8-wave kernels' count
+        const uint32_t ntop = min((uint32_t)TOP_NODES, S.n_nodes);
         if (count)
-            hipLaunchKernelGGL((k_trace_refill<true>), dim3(refill_blocks),
-                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf);
+            hipLaunchKernelGGL((k_trace_refill<true, false>), dim3(refill_blocks),
+                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf,
+                               (const BatchScene *)nullptr, 0u, 0u, ntop);
         else
-            hipLaunchKernelGGL((k_trace_refill<false>), dim3(refill_blocks),
-                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf);
+            hipLaunchKernelGGL((k_trace_refill<false, false>), dim3(refill_blocks),
+                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf,
+                               (const BatchScene *)nullptr, 0u, 0u, ntop);
         return hipGetLastError();
     }
     if (top && S.root == 0 && !(S.quad | S.alpha)) {
@@ -2658,20 +2723,40 @@ hipError_t launch_resolve(hipStream_t st, uint32_t blocks,
 }
 
 hipError_t launch_braygen(hipStream_t st, const ShadeBufs &B,
-                          const BatchCtl *ctl, float4 *q_o, float4 *q_d,
+                          const BatchCtl *ctl, unsigned long long base,
+                          uint32_t s_lo, uint32_t s_hi, float4 *q_o, float4 *q_d,
                           float4 *q_t, uint32_t o0, uint32_t Pl, uint32_t P,
                           uint32_t *ctr)
 {
+    if (s_lo > s_hi)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_braygen, dim3((Pl + 255) / 256), dim3(256), 0, st, B.bs,
-                       ctl, q_o, q_d, q_t, B.acc, B.slot_info, o0, Pl, P, ctr);
+                       ctl, base, s_lo, s_hi, q_o, q_d, q_t, B.acc, B.slot_info,
+                       o0, Pl, P, ctr);
     return hipGetLastError();
 }
 
 hipError_t launch_btrace(hipStream_t st, uint32_t blocks, bool count,
                          const SceneDev &S, const StackOvf &ovf,
                          const ShadeBufs &B, const TraceWork &w,
-                         uint32_t *ctr, unsigned long long *stats)
+                         uint32_t *ctr, unsigned long long *stats,
+                         uint32_t refill_blocks, uint32_t n_scenes,
+                         uint32_t top_base, uint32_t ntop)
 {
+    if (refill_blocks > 0) {
+        if (!ovf_fits(ovf, refill_blocks) || n_scenes > BATCH_LDS_SCENES
+            || (S.quad | S.alpha) || w.q_t == nullptr || ntop > (uint32_t)TOP_NODES)
+            return hipErrorInvalidValue;
+        if (count)
+            hipLaunchKernelGGL((k_trace_refill<true, true>), dim3(refill_blocks),
+                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf,
+                               (const BatchScene *)B.bs, n_scenes, top_base, ntop);
+        else
+            hipLaunchKernelGGL((k_trace_refill<false, true>), dim3(refill_blocks),
+                               dim3(TRACE_BLOCK), 0, st, S, w, ctr, stats, ovf.buf,
+                               (const BatchScene *)B.bs, n_scenes, top_base, ntop);
+        return hipGetLastError();
+    }
     if (!ovf_fits(ovf, blocks))
         return hipErrorInvalidValue;
 #define MRT_GO(C, Q)                                                          \
@@ -2709,11 +2794,12 @@ hipError_t launch_bresolve(hipStream_t st, uint32_t blocks,
 }
 
 hipError_t launch_baccumulate(hipStream_t st, const ShadeBufs &B,
-                              const BatchCtl *ctl, float *accum, uint32_t P,
-                              uint32_t *ctr, unsigned long long *stats)
+                              const BatchCtl *ctl, unsigned long long base,
+                              float *accum, uint32_t P, uint32_t *ctr,
+                              unsigned long long *stats)
 {
     hipLaunchKernelGGL(k_baccumulate, dim3((P + 255) / 256), dim3(256), 0, st,
-                       B.bs, ctl, (const float4 *)B.acc,
+                       B.bs, ctl, base, (const float4 *)B.acc,
                        (const uint2 *)B.slot_info, accum, (const uint32_t *)ctr,
                        stats);
     return hipGetLastError();
@@ -2764,12 +2850,15 @@ hipError_t launch_trace_rays(hipStream_t st, const SceneDev &S,
     return hipGetLastError();
 }
 
-int refill_blocks_per_cu()
+int refill_blocks_per_cu(bool batch)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, k_trace_refill<false>, TRACE_BLOCK, 0) != hipSuccess
-        || nb < 1)
+    hipError_t e = batch
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, k_trace_refill<false, true>, TRACE_BLOCK, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, k_trace_refill<false, false>, TRACE_BLOCK, 0);
+    if (e != hipSuccess || nb < 1)
         nb = 4;
     return nb;
 }

@@ -181,7 +181,7 @@ class Stats(C.Structure):
 class BatchTuning(C.Structure):
     _fields_ = [("path_budget", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32),
-                ("ms_per_mpath", C.c_double)]
+                ("trace_refill", C.c_uint32), ("ms_per_mpath", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -221,6 +221,8 @@ def batched_leg(M, args):
     log(f"batched: commit with device BVH builds {t_commit_dev:.2f} s")
     t0 = time.perf_counter()
     b = M.Batch()
+    if args.lanes is not None:
+        b.set_option(M.OPT_LANES, args.lanes)
     for hs, d in zip(scenes, desc):
         b.add(hs, d["w"], d["h"], d["spp"], d["rng_seed"])
     b.commit()
@@ -245,6 +247,8 @@ def batched_leg(M, args):
     t_plain = 0.0
     for k, (hs, d) in enumerate(zip(scenes, desc)):
         with M.Renderer(d["w"], d["h"]) as r:
+            if args.lanes is not None:
+                r.set_option(M.OPT_LANES, args.lanes)
             r.upload(hs)
             r.set_seed(d["rng_seed"])
             r.reserve(d["spp"])

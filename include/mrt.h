@@ -458,7 +458,15 @@ typedef struct mrt_batch_tuning {
     uint32_t path_budget;        /* paths per pass                           */
     uint32_t trace_blocks_per_cu;/* persistent blocks of the trace kernels   */
     uint32_t shade_blocks_per_cu;/* blocks of the shading kernels            */
-    uint32_t use_graph;          /* replay a captured pass (1) or launch (0) */
+    uint32_t use_graph;          /* replay a captured pass (1), or launch the
+                                    passes directly, pipelined on the render
+                                    lanes (0)                                */
+    uint32_t trace_refill;       /* 1: the per-lane refill trace kernel on
+                                    its resident grid, where the batch allows
+                                    it (triangle / sphere scenes of one
+                                    sidedness, at most 512 scenes); 0: the
+                                    persistent trace kernel on
+                                    trace_blocks_per_cu blocks per CU        */
     double   ms_per_mpath;       /* measured by the autotuner, else 0        */
 } mrt_batch_tuning;
 

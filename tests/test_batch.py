@@ -134,13 +134,75 @@ def test_autotune_then_render(scenes, plain_images):
     b = make_batch(scenes)
     t = b.autotune(DEPTH)
     print("chosen", t)
-    assert t["path_budget"] in (1 << 22, 1 << 23, 1 << 24, 1 << 25)
+    assert t["path_budget"] in [1 << k for k in range(22, 28)]
     assert t["ms_per_mpath"] > 0
+    assert t["use_graph"] in (0, 1) and t["trace_refill"] in (0, 1)
     # the trial renders are cleared
     assert not b.readback(0).any()
     b.render(DEPTH)
     for k, ref in enumerate(plain_images):
         np.testing.assert_allclose(b.readback(k), ref, rtol=2e-5, atol=2e-6)
+    b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_refill_kernel_equals_k_trace(scenes, graph):
+    """The batched refill trace kernel (each ray starts at its scene's root,
+    read from LDS; planes read per scene at the ray's end) gives the bits and
+    the node / primitive counts of k_trace, on scenes with planes, soft
+    shadows, a planes-only scene and a mesh; one-sided triangles too."""
+    sub = scenes[:5]                 # without the quadric scene
+    one = M.HostScene.mesh(5, 40, 24, subdiv=3, floor_cells=8)
+    one.set_flags(M.SCENE_TRIANGLE_ONE_SIDED)
+    res = {}
+    for sided in ("two", "one"):
+        cases = sub if sided == "two" else [
+            (one, 40, 24, 3, 0x99),
+            (M.HostScene.mesh(6, 32, 20, subdiv=3, floor_cells=8), 32, 20, 2, 0x9A)]
+        if sided == "one":
+            cases[1][0].set_flags(M.SCENE_TRIANGLE_ONE_SIDED)
+        for refill in (0, 1):
+            b = make_batch(cases, path_budget=4096, use_graph=graph,
+                           trace_refill=refill)
+            b.render(DEPTH)
+            imgs = [b.readback(k) for k in range(len(cases))]
+            b.close()
+            c = make_batch(cases, path_budget=4096, use_graph=0,
+                           trace_refill=refill)
+            c.set_option(M.OPT_STATS, M.STATS_COUNT)
+            c.render(DEPTH)
+            st = c.stats()
+            c.close()
+            res[sided, refill] = (imgs, st)
+    for sided in ("two", "one"):
+        (a, sa), (b_, sb) = res[sided, 0], res[sided, 1]
+        for k, (x, y) in enumerate(zip(a, b_)):
+            assert np.array_equal(x, y), (sided, k)
+        for key in ("nodes_closest", "prims_closest", "nodes_shadow",
+                    "prims_shadow", "rays_closest", "rays_shadow"):
+            assert sa[key] == sb[key], (sided, key)
+
+
+@pytest.mark.gpu
+def test_batch_of_mixed_sidedness_falls_back_to_k_trace():
+    """Scenes of both sidedness in one batch: the refill kernel takes one
+    sidedness for all, so the batch traces with k_trace and still equals
+    the plain renders."""
+    one = M.HostScene.mesh(5, 40, 24, subdiv=3, floor_cells=8)
+    one.set_flags(M.SCENE_TRIANGLE_ONE_SIDED)
+    cases = [(one, 40, 24, 3, 0x99),
+             (M.HostScene.mesh(6, 32, 20, subdiv=3, floor_cells=8), 32, 20, 2,
+              0x9A)]
+    b = make_batch(cases, path_budget=4096, trace_refill=1)
+    b.render(DEPTH)
+    for k, (hs, w, h, spp, seed) in enumerate(cases):
+        with M.Renderer(w, h) as r:
+            r.upload(hs)
+            r.set_seed(seed)
+            r.render(0, spp, DEPTH)
+            ref = r.readback()
+        np.testing.assert_allclose(b.readback(k), ref, rtol=2e-5, atol=1e-6)
     b.close()
 
 
