@@ -312,12 +312,53 @@ static void subdivide(ico_ctx *c, const double a[3], const double b[3],
     subdivide(c, ab, be, ea, level - 1);
 }
 
-int mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
-                      uint32_t subdiv, float amp, uint64_t seed,
-                      int material_mix, uint32_t mat)
+/* Point (i, j) of the freq x freq triangular grid on icosahedron face
+ * (a, b, e): the direction of (freq - i - j) a + i b + e j, the three terms
+ * summed in the order of the vertices' indices, so that a point on an edge
+ * the face shares has the same bits in both faces (no cracks). */
+static void grid_point(const double V[12][3], const int f[3], uint32_t i,
+                       uint32_t j, uint32_t freq, double out[3])
 {
-    if (!hs || !centre || !(radius > 0.0f) || subdiv > 10)
-        return MRT_E_INVALID;
+    const double w[3] = {(double)(freq - i - j), (double)i, (double)j};
+    int o[3] = {0, 1, 2};
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2 - x; y++)
+            if (f[o[y]] > f[o[y + 1]]) {
+                int t = o[y];
+                o[y] = o[y + 1];
+                o[y + 1] = t;
+            }
+    for (int k = 0; k < 3; k++)
+        out[k] = (V[f[o[0]]][k] * w[o[0]] + V[f[o[1]]][k] * w[o[1]])
+                 + V[f[o[2]]][k] * w[o[2]];
+    normalize3(out);
+}
+
+/* The freq^2 triangles of face f's grid, wound as the face. */
+static void geodesic_face(ico_ctx *c, const double V[12][3], const int f[3],
+                          uint32_t freq)
+{
+    for (uint32_t j = 0; j < freq && !c->err; j++)
+        for (uint32_t i = 0; i + j < freq && !c->err; i++) {
+            double p00[3], p10[3], p01[3];
+            grid_point(V, f, i, j, freq, p00);
+            grid_point(V, f, i + 1, j, freq, p10);
+            grid_point(V, f, i, j + 1, freq, p01);
+            emit_face(c, p00, p10, p01);
+            if (i + j + 1 < freq) {
+                double p11[3];
+                grid_point(V, f, i + 1, j + 1, freq, p11);
+                emit_face(c, p10, p11, p01);
+            }
+        }
+}
+
+/* subdiv > 0: recursive 4-way splits (20 * 4^subdiv faces); freq > 0: each
+ * face split into a freq x freq triangular grid (20 * freq^2 faces). */
+static int icosphere(mrt_host_scene *hs, const float centre[3], float radius,
+                     uint32_t subdiv, uint32_t freq, float amp, uint64_t seed,
+                     int material_mix, uint32_t mat)
+{
     const double t = (1.0 + sqrt(5.0)) / 2.0;
     double V[12][3] = {{-1, t, 0}, {1, t, 0},  {-1, -t, 0}, {1, -t, 0},
                        {0, -1, t}, {0, 1, t},  {0, -1, -t}, {0, 1, -t},
@@ -347,9 +388,31 @@ int mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
     } else if (mat >= hs->n_mat) {
         return MRT_E_INVALID;
     }
-    for (int f = 0; f < 20; f++)
-        subdivide(&c, V[F[f][0]], V[F[f][1]], V[F[f][2]], subdiv);
+    for (int f = 0; f < 20; f++) {
+        if (freq)
+            geodesic_face(&c, (const double(*)[3])V, F[f], freq);
+        else
+            subdivide(&c, V[F[f][0]], V[F[f][1]], V[F[f][2]], subdiv);
+    }
     return c.err ? c.err : c.count;
+}
+
+int mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
+                      uint32_t subdiv, float amp, uint64_t seed,
+                      int material_mix, uint32_t mat)
+{
+    if (!hs || !centre || !(radius > 0.0f) || subdiv > 10)
+        return MRT_E_INVALID;
+    return icosphere(hs, centre, radius, subdiv, 0, amp, seed, material_mix, mat);
+}
+
+int mrt_gen_icosphere_freq(mrt_host_scene *hs, const float centre[3],
+                           float radius, uint32_t freq, float amp,
+                           uint64_t seed, int material_mix, uint32_t mat)
+{
+    if (!hs || !centre || !(radius > 0.0f) || freq < 1 || freq > 1024)
+        return MRT_E_INVALID;
+    return icosphere(hs, centre, radius, 0, freq, amp, seed, material_mix, mat);
 }
 
 /* ---- heightfield floor ----------------------------------------------------- */

@@ -352,6 +352,9 @@ mrt_gen_caustic_scene = _sig("mrt_gen_caustic_scene", C.c_int, C.c_uint64,
 mrt_gen_icosphere = _sig("mrt_gen_icosphere", C.c_int, _hs_p, _fp, C.c_float,
                          C.c_uint32, C.c_float, C.c_uint64, C.c_int,
                          C.c_uint32)
+mrt_gen_icosphere_freq = _sig("mrt_gen_icosphere_freq", C.c_int, _hs_p, _fp,
+                              C.c_float, C.c_uint32, C.c_float, C.c_uint64,
+                              C.c_int, C.c_uint32)
 mrt_gen_hdri_sky = _sig("mrt_gen_hdri_sky", C.c_int, C.c_uint64, C.c_uint32,
                         C.c_uint32, C.c_float, C.c_float, _fp)
 mrt_hdri_build_tables = _sig("mrt_hdri_build_tables", C.c_int, _fp,
@@ -613,8 +616,14 @@ class HostScene:
         return _check(mrt_hs_add_light(self._h, C.byref(L)))
 
     def add_icosphere(self, centre, radius, subdiv, amp, seed, mix=True,
-                      mat=0):
+                      mat=0, freq=None):
+        """A displaced icosphere: 20 * 4^subdiv triangles, or with freq
+        the geodesic sphere of 20 * freq^2."""
         c = np.asarray(centre, dtype=np.float32)
+        if freq is not None:
+            return _check(mrt_gen_icosphere_freq(self._h, c.ctypes.data_as(_fp),
+                                                 radius, freq, amp, seed,
+                                                 int(mix), mat))
         return _check(mrt_gen_icosphere(self._h, c.ctypes.data_as(_fp),
                                         radius, subdiv, amp, seed,
                                         int(mix), mat))

@@ -53,13 +53,15 @@ def render(renderer, albedo, roughness, emission, hdri=None, n_samples=64,
                                 grad_first_sample)
 
 
-def inverse_scene(seed, width, height, n_materials=16, subdiv=6,
-                  floor_cells=240, amp=0.15):
-    """The inverse-rendering scene: a displaced icosphere (subdiv levels)
-    over a heightfield floor, triangles drawn uniformly from n_materials
-    materials (lambert, GGX metals and emitters, seeded parameters), lit by
-    an HDRI (uploaded by the caller) with ambient ratio 1.  Returns the host
-    scene and the material parameters (albedo, roughness, emission)."""
+def inverse_scene(seed, width, height, n_materials=16, subdiv=None,
+                  floor_cells=8, amp=0.15, freq=100):
+    """The inverse-rendering scene: a displaced icosphere (the geodesic
+    sphere of 20 * freq^2 triangles, 200,000 by default, or subdiv levels of
+    4-way splits) over a flat floor of floor_cells^2 squares, triangles drawn
+    uniformly from n_materials materials (lambert, GGX metals and emitters,
+    seeded parameters), lit by an HDRI (uploaded by the caller) with ambient
+    ratio 1.  Returns the host scene and the material parameters (albedo,
+    roughness, emission)."""
     rng = np.random.default_rng(seed)
     hs = HostScene.empty()
     hs.set_resolution(width, height)
@@ -73,8 +75,12 @@ def inverse_scene(seed, width, height, n_materials=16, subdiv=6,
         hs.add_material(color=tuple(alb[k]), metallic=1.0 if metal else 0.0,
                         roughness=float(rough[k]), emission=1.0,
                         emission_color=tuple(emis[k]))
-    hs.add_icosphere((0.0, 1.2, 0.0), 1.0, subdiv, amp, seed, mix=2,
-                     mat=n_materials)
+    if subdiv is not None:
+        hs.add_icosphere((0.0, 1.2, 0.0), 1.0, subdiv, amp, seed, mix=2,
+                         mat=n_materials)
+    else:
+        hs.add_icosphere((0.0, 1.2, 0.0), 1.0, 0, amp, seed, mix=2,
+                         mat=n_materials, freq=freq)
     if floor_cells:
         # the floor: a flat grid of triangles drawn from the same palette
         n = floor_cells

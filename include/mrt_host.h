@@ -176,6 +176,14 @@ int  mrt_gen_icosphere(mrt_host_scene *hs, const float centre[3], float radius,
                                               0 .. mat-1 of hs */,
                        uint32_t mat);
 
+/* Adds a displaced geodesic icosphere of 20 * freq^2 triangles (each
+ * icosahedron face split into a freq x freq triangular grid projected onto
+ * the sphere; freq 100 gives 200,000 triangles), displaced and coloured as
+ * mrt_gen_icosphere's; returns the triangle count or < 0. */
+int  mrt_gen_icosphere_freq(mrt_host_scene *hs, const float centre[3],
+                            float radius, uint32_t freq, float amp,
+                            uint64_t seed, int material_mix, uint32_t mat);
+
 /* Equirectangular sky: analytic gradient plus a sun disc of the given
  * angular radius (degrees) and radiance at a seeded elevation in
  * [15, 60] degrees.  rgb receives w*h*3 floats, row 0 at the nadir. */

@@ -201,3 +201,55 @@ def test_hdri_sun_map_concentrates_samples():
     # nearly all inverse-marginal entries land in the sun's rows
     hit_rows = np.round(marg * 63).astype(int)
     assert np.isin(hit_rows, rows).mean() > 0.95
+
+
+def _triangles(hs):
+    v = hs.view().contents
+    return np.ctypeslib.as_array(v.tri_pos, (v.n_triangles * 9,)).reshape(-1, 3, 3).copy()
+
+
+@pytest.mark.parametrize("freq", [1, 2, 3, 7, 100])
+def test_geodesic_icosphere_count_and_closed(freq):
+    """20 freq^2 triangles; every edge is shared by exactly two triangles
+    with the same vertex bits (no cracks where icosahedron faces meet), and
+    each of the V - E + F = 2 vertices is used."""
+    hs = M.HostScene.empty()
+    hs.add_material(color=(0.5, 0.5, 0.5))
+    n = hs.add_icosphere((0.0, 1.0, 0.0), 1.0, 0, 0.15, 7, mix=False, mat=0,
+                         freq=freq)
+    assert n == 20 * freq * freq
+    tri = _triangles(hs)
+    assert tri.shape[0] == n
+    if freq > 7:
+        return
+    key = lambda p: tuple(np.asarray(p, np.float32).view(np.uint32))
+    edges = {}
+    verts = set()
+    for t in tri:
+        for a in range(3):
+            p, q = key(t[a]), key(t[(a + 1) % 3])
+            verts.add(p)
+            edges[frozenset((p, q))] = edges.get(frozenset((p, q)), 0) + 1
+    assert set(edges.values()) == {2}
+    assert len(verts) - len(edges) + n == 2
+
+
+def test_geodesic_icosphere_on_the_sphere_and_outward():
+    """Without displacement every vertex lies on the sphere, and every
+    triangle faces away from the centre; freq 1 is the icosahedron."""
+    hs = M.HostScene.empty()
+    hs.add_material(color=(0.5, 0.5, 0.5))
+    c = np.array([0.5, 1.0, -0.25])
+    hs.add_icosphere(tuple(c), 2.0, 0, 0.0, 3, mix=False, mat=0, freq=5)
+    tri = _triangles(hs).astype(np.float64)
+    r = np.linalg.norm(tri - c, axis=2)
+    np.testing.assert_allclose(r, 2.0, rtol=1e-6)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    assert (np.einsum("ij,ij->i", nrm, tri.mean(axis=1) - c) > 0).all()
+    ico = M.HostScene.empty()
+    ico.add_material(color=(0.5, 0.5, 0.5))
+    ico.add_icosphere((0.0, 0.0, 0.0), 1.0, 0, 0.0, 3, mix=False, mat=0, freq=1)
+    rec = M.HostScene.empty()
+    rec.add_material(color=(0.5, 0.5, 0.5))
+    rec.add_icosphere((0.0, 0.0, 0.0), 1.0, 0, 0.0, 3, mix=False, mat=0)
+    np.testing.assert_allclose(_triangles(ico), _triangles(rec), atol=1e-7)
