@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -67,6 +69,75 @@ static hipError_t scratch_on_device(int device, float **out)
     *out = g_scratch_dev[device];
     return hipSuccess;
 }
+
+// Host threads of a context that mrt_readback_add splits its add over (a
+// 1080p frame is 6.2 M doubles): started once, so a tick starts no thread.
+// run(f) calls f(0) on the caller and f(1 .. size-1) on the workers and
+// returns when every call has.
+class HostPool {
+public:
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : th_)
+            t.join();
+    }
+    unsigned size() const { return (unsigned)th_.size() + 1; }
+    void start(unsigned n)
+    {
+        for (unsigned k = 1; k < n; k++)
+            th_.emplace_back([this, k]() { work(k); });
+    }
+    void run(const std::function<void(unsigned)> &f)
+    {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            job_ = &f;
+            pending_ = (unsigned)th_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [this]() { return pending_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void work(unsigned k)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(unsigned)> *f;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&]() { return stop_ || gen_ != seen; });
+                if (stop_)
+                    return;
+                seen = gen_;
+                f = job_;
+            }
+            (*f)(k);
+            std::lock_guard<std::mutex> g(mu_);
+            if (--pending_ == 0)
+                done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(unsigned)> *job_ = nullptr;
+    uint64_t gen_ = 0;
+    unsigned pending_ = 0;
+    bool stop_ = false;
+};
+
+// mrt_readback_add copies the sums in chunks and adds each as it arrives
+constexpr int READBACK_CHUNKS = 8;
 
 struct mrt_ctx {
     int          device = 0;
@@ -133,6 +204,8 @@ struct mrt_ctx {
     float4      *d_tape = nullptr;    // the adjoint's tape
     size_t       tape_cap = 0;        // float4 entries
     float       *h_stage = nullptr;   // pinned staging of mrt_readback_add
+    hipEvent_t   ev_rb[READBACK_CHUNKS] = {};  // its chunks' arrival
+    HostPool    *pool = nullptr;      // its add's threads
     uint32_t    *d_ctr = nullptr;
     unsigned long long *d_stats = nullptr;
 
@@ -406,6 +479,10 @@ extern "C" void mrt_destroy(mrt_ctx *c)
     dfree(c->d_tape);
     if (c->h_stage)
         (void)hipHostFree(c->h_stage);
+    for (hipEvent_t e : c->ev_rb)
+        if (e)
+            (void)hipEventDestroy(e);
+    delete c->pool;
     dfree(c->d_ctr);
     dfree(c->d_stats);
     dfree(c->d_ovf);
@@ -1455,10 +1532,15 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
     const uint32_t L = render_lanes(c, n_samples, nb, P64, S64);
     if (!L)
         return fail(c, MRT_E_LIMIT, "shadow queue exceeds 2^32 entries");
+    // A render of one pass (a drop-in tick of 1 spp) splits that pass into
+    // contiguous parts traced at once on the lanes: each bounce launch is
+    // short, and the parts' launches overlap each other's slowest rays.
+    const uint32_t Ls = n_samples <= nb
+        ? std::max(1u, std::min(c->lanes, n_samples * WH / LANE_MIN_PATHS)) : 1u;
     int rc = ensure_wavefront(c, (uint32_t)(P64 * L), (uint32_t)(S64 * L));
     if (rc != MRT_OK)
         return rc;
-    if (ensure_ovf(c, c->trace_blocks, L) != hipSuccess)
+    if (ensure_ovf(c, c->trace_blocks, std::max(L, Ls)) != hipSuccess)
         return fail(c, MRT_E_NOMEM, "trace stack buffer does not fit");
     FrameParams fp = frame_params(c);
     SceneDev S = scene_dev(c);
@@ -1471,14 +1553,39 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
         (void)hipEventCreate(&t1);
         (void)hipEventRecord(t0, c->stream);
     }
-    if (L > 1) {
+    if (L > 1 || Ls > 1) {
         // every lane after what the context's stream holds (uploads, clears)
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-        for (uint32_t l = 1; l < L; l++)
+        for (uint32_t l = 1; l < std::max(L, Ls); l++)
             HIPCHK(c, hipStreamWaitEvent(c->lane_stream[l], c->ev_fork, 0));
     }
+    if (Ls > 1) {
+        // part l: slots [o0, o0 + Pl) of the pass, numbered as in the whole
+        // pass (fp.slot0), in the pass's queue entries from o0, with lane
+        // l's counters and stack columns; the pass's sums after every part
+        const uint32_t P = n_samples * WH;
+        FrameParams fl = fp;
+        fl.sample0 = first_sample;
+        for (uint32_t p = 0; p < Ls; p++) {
+            const uint32_t o0 = (uint32_t)((uint64_t)P * p / Ls);
+            const uint32_t Pl = (uint32_t)((uint64_t)P * (p + 1) / Ls) - o0;
+            hipStream_t st = p ? c->lane_stream[p] : c->stream;
+            fl.slot0 = o0;
+            rc = render_lane(c, st, tm, fl, S, count, max_depth, Pl, o0, rpp,
+                             c->d_ctr + (size_t)p * CTR_COUNT, stack_ovf(c, p));
+            if (rc != MRT_OK)
+                return rc;
+            if (p) {
+                HIPCHK(c, hipEventRecord(c->ev_join[p], st));
+                HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join[p], 0));
+            }
+        }
+        LAUNCH(c, tm, K_ACCUM,
+               launch_accumulate(c->stream, c->acc, c->d_accum, WH, n_samples,
+                                 c->d_stats));
+    }
     uint32_t l = 0;
-    for (uint32_t s = 0, k = 0; s < n_samples; s += nb, k++) {
+    for (uint32_t s = 0, k = 0; Ls == 1 && s < n_samples; s += nb, k++) {
         const uint32_t n = (n_samples - s) < nb ? (n_samples - s) : nb;
         const uint32_t P = n * WH;
         const uint32_t prev = l;
@@ -1547,35 +1654,58 @@ extern "C" int mrt_readback_add(mrt_ctx *c, double *float_render)
     if (!c || !float_render)
         return MRT_E_INVALID;
     const size_t n = (size_t)c->W * c->H * 3;
-    if (!c->h_stage
-        && hipHostMalloc((void **)&c->h_stage, n * sizeof(float),
-                         hipHostMallocDefault) != hipSuccess) {
-        c->h_stage = nullptr;
-        return fail(c, MRT_E_NOMEM, "pinned read-back buffer (%zu floats)", n);
+    if (!c->h_stage) {
+        if (hipHostMalloc((void **)&c->h_stage, n * sizeof(float),
+                          hipHostMallocDefault) != hipSuccess) {
+            c->h_stage = nullptr;
+            return fail(c, MRT_E_NOMEM, "pinned read-back buffer (%zu floats)", n);
+        }
+        for (hipEvent_t &e : c->ev_rb)
+            if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                e = nullptr;
+                return fail(c, MRT_E_HIP, "read-back events");
+            }
+        // a 1080p frame is 6.2 M doubles: the add runs on up to 8 threads
+        unsigned nt = std::thread::hardware_concurrency();
+        nt = std::max(1u, std::min(nt, 8u));
+        if (n < (1u << 20))
+            nt = 1;
+        c->pool = new (std::nothrow) HostPool();
+        if (!c->pool)
+            return fail(c, MRT_E_NOMEM, "read-back threads");
+        c->pool->start(nt);
     }
     // get_result adds one dispatch's samples per call: the device sums are
     // handed over and zeroed in stream order, so the next render starts
-    // from zero and every sample reaches float_render exactly once.
-    HIPCHK(c, hipMemcpyAsync(c->h_stage, c->d_accum, n * sizeof(float),
-                             hipMemcpyDeviceToHost, c->stream));
+    // from zero and every sample reaches float_render exactly once.  The
+    // copy goes in chunks, and each chunk is added while the next ones
+    // cross PCIe.
+    const int K = n >= (1u << 20) ? READBACK_CHUNKS : 1;
+    for (int k = 0; k < K; k++) {
+        const size_t a = n * k / K, b = n * (k + 1) / K;
+        HIPCHK(c, hipMemcpyAsync(c->h_stage + a, c->d_accum + a,
+                                 (b - a) * sizeof(float), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipEventRecord(c->ev_rb[k], c->stream));
+    }
     HIPCHK(c, hipMemsetAsync(c->d_accum, 0, n * sizeof(float), c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     const float *src = c->h_stage;
-    auto add = [src, float_render](size_t a, size_t b) {
-        for (size_t i = a; i < b; i++)
-            float_render[i] += (double)src[i];
-    };
-    // a 1080p frame is 6.2 M doubles: split the host-side add over threads
-    unsigned nt = std::thread::hardware_concurrency();
-    nt = std::max(1u, std::min(nt, 8u));
-    if (n < (1u << 20))
-        nt = 1;
-    std::vector<std::thread> pool;
-    for (unsigned k = 1; k < nt; k++)
-        pool.emplace_back(add, n * k / nt, n * (k + 1) / nt);
-    add(0, n / nt);
-    for (auto &t : pool)
-        t.join();
+    const unsigned T = c->pool->size();
+    std::atomic<int> bad{0};
+    c->pool->run([&](unsigned t) {
+        for (int k = 0; k < K; k++) {
+            if (hipEventSynchronize(c->ev_rb[k]) != hipSuccess) {
+                bad = 1;
+                return;
+            }
+            const size_t a = n * k / K, len = n * (k + 1) / K - a;
+            const size_t lo = a + len * t / T, hi = a + len * (t + 1) / T;
+            for (size_t i = lo; i < hi; i++)
+                float_render[i] += (double)src[i];
+        }
+    });
+    if (bad)
+        return fail(c, MRT_E_HIP, "read-back copy failed");
     return MRT_OK;
 }
 

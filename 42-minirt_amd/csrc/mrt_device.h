@@ -611,16 +611,18 @@ struct SceneDev {
 constexpr int STACK4_MAX = 3 * (int)MRT_BVH4_MAX_DEPTH + 1;
 
 // Compare-exchange of (entry distance, child code) pairs.
+// Compare-exchange of (key, code) pairs: the smaller key first, ties kept
+// in place.  Written as selects, so a divergent wave runs four v_cndmask
+// instead of a branch around the swap.
 __device__ __forceinline__ void cx(float &ka, int32_t &ca, float &kb, int32_t &cb)
 {
-    if (ka > kb) {
-        float t = ka;
-        ka = kb;
-        kb = t;
-        int32_t c = ca;
-        ca = cb;
-        cb = c;
-    }
+    const bool s = ka > kb;
+    const float lo = s ? kb : ka, hi = s ? ka : kb;
+    const int32_t cl = s ? cb : ca, ch = s ? ca : cb;
+    ka = lo;
+    kb = hi;
+    ca = cl;
+    cb = ch;
 }
 
 // Ordered traversal of the 4-wide BVH: one 128-byte node gives the four

@@ -537,24 +537,21 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                                       tclose, t3) ? t3 : INF;
                     int32_t c0 = __float_as_int(cw.x), c1 = __float_as_int(cw.y),
                             c2 = __float_as_int(cw.z), c3 = __float_as_int(cw.w);
+                    // shadow rays take the hit children in slot order (any
+                    // hit ends them): keyed by slot, the same network puts
+                    // them first in that order, so a wave that holds both
+                    // kinds of ray runs one branch-free sort
                     if (any) {
-                        if (k0 == INF) {
-                            k0 = k1; c0 = c1; k1 = k2; c1 = c2; k2 = k3; c2 = c3; k3 = INF;
-                        }
-                        if (k0 == INF) { k0 = k1; c0 = c1; k1 = k2; c1 = c2; k2 = INF; }
-                        if (k0 == INF) { k0 = k1; c0 = c1; k1 = INF; }
-                        if (k1 == INF) {
-                            k1 = k2; c1 = c2; k2 = k3; c2 = c3; k3 = INF;
-                        }
-                        if (k1 == INF) { k1 = k2; c1 = c2; k2 = INF; }
-                        if (k2 == INF) { k2 = k3; c2 = c3; k3 = INF; }
-                    } else {
-                        cx(k0, c0, k1, c1);
-                        cx(k2, c2, k3, c3);
-                        cx(k0, c0, k2, c2);
-                        cx(k1, c1, k3, c3);
-                        cx(k1, c1, k2, c2);
+                        k0 = k0 != INF ? 0.0f : INF;
+                        k1 = k1 != INF ? 1.0f : INF;
+                        k2 = k2 != INF ? 2.0f : INF;
+                        k3 = k3 != INF ? 3.0f : INF;
                     }
+                    cx(k0, c0, k1, c1);
+                    cx(k2, c2, k3, c3);
+                    cx(k0, c0, k2, c2);
+                    cx(k1, c1, k3, c3);
+                    cx(k1, c1, k2, c2);
                     if (k0 == INF) {
                         code = pop();
                     } else {

@@ -1,8 +1,9 @@
 """Render lanes (MRT_OPT_LANES): the passes of a render larger than the
-path budget run on their own HIP streams, overlapping.  Every path keeps
-its pixel and sample and each pass adds its sums after the one before it,
-so images and traversal statistics must equal the one-lane render's bit
-for bit (needs a GPU)."""
+path budget run on their own HIP streams, overlapping, and a render of one
+pass splits it into parts traced at once on the lanes.  Every path keeps
+its pixel and sample and the sums are added in the same order, so images
+and traversal statistics must equal the one-lane render's bit for bit
+(needs a GPU)."""
 import numpy as np
 import pytest
 
@@ -69,10 +70,13 @@ def test_lanes_equal_one_lane(scenes, name):
         img, st = render(hs, lanes, env, est, photons)
         assert np.array_equal(img, ref), (name, lanes)
         assert st == ref_st, (name, lanes)
-    # one pass of every sample: the lanes have nothing to overlap
+    # one pass of every sample (327,680 paths): the pass splits into
+    # contiguous parts traced at once on the lanes
     one, one_st = render(hs, 1, env, est, photons, budget=None)
-    img, st = render(hs, 4, env, est, photons, budget=None)
-    assert np.array_equal(img, one) and st == one_st
+    for lanes in (2, 3, 4):
+        img, st = render(hs, lanes, env, est, photons, budget=None)
+        assert np.array_equal(img, one), (name, lanes)
+        assert st == one_st, (name, lanes)
 
 
 def test_lanes_with_several_batches(scenes):
