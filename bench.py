@@ -426,6 +426,7 @@ def adjoint_leg(M, args):
         {"params": [p_log], "lr": args.adj_lr_env, "eps": args.adj_env_eps}])
     alb_start = p_alb.detach().numpy().copy()
     rough_start = p_rough.detach().numpy().copy()
+    metals = np.arange(len(rough)) % 4 == 1        # inverse_scene's GGX metals
     losses = []
     t0 = time.perf_counter()
     for step in range(args.adam_steps):
@@ -479,6 +480,12 @@ def adjoint_leg(M, args):
             (rough_start - rough) ** 2))), 5),
         "roughness_rmse_end": round(float(np.sqrt(np.mean(
             (p_rough.detach().numpy() - rough) ** 2))), 5),
+        # roughness changes the image only through the GGX metals (every
+        # fourth material); the lambert materials ignore it
+        "roughness_rmse_metals_start": round(float(np.sqrt(np.mean(
+            (rough_start[metals] - rough[metals]) ** 2))), 5),
+        "roughness_rmse_metals_end": round(float(np.sqrt(np.mean(
+            (p_rough.detach().numpy()[metals] - rough[metals]) ** 2))), 5),
         "hdri_log_rmse_start": round(log_rmse(env_start.numpy()), 5),
         "hdri_log_rmse_end": round(log_rmse(env_end), 5),
         "hdri_log_rmse_upper_start": round(log_rmse(env_start.numpy(), True), 5),
