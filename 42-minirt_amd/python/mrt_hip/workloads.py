@@ -1,5 +1,8 @@
 """Seeded synthetic workloads shared by the benchmark and the tests.
 
+The plain .rt scene: a ground plane and three spheres (lambert, metal,
+glass) on seeded jitter, one point light, as the text of a .rt file.
+
 The batched workload: many independent scenes, each a displaced icosphere
 over a heightfield floor (the procedural mesh scene at a given subdivision
 and floor size), with triangle counts log-uniform over a range, resolutions
@@ -45,3 +48,23 @@ def batched_scene(d, width=None, height=None):
 def triangles(d):
     """Triangle count of a description: 20 * 4^subdiv + 2 * floor^2."""
     return 20 * 4 ** d["subdiv"] + 2 * d["floor"] ** 2
+
+
+def config1_rt(seed=11, width=640, height=360):
+    """The plain 640x360 scene as .rt text: ground plane, lambert / metal
+    (fuzz 0.1) / glass (ior 1.5) unit spheres on seeded jitter within +-0.5
+    of x in {-2, 0, 2}, one point light (intensity 1), ambient 0.1, a
+    60 degree pinhole camera.  Returns the text and the spheres
+    (centre, radius) the generator placed."""
+    from . import HostScene
+    gen = HostScene.plain(seed, width, height)
+    sph = gen.arrays()["sph"]
+    lines = [f"W {width} {height}", "A 0.1 255,255,255", "C 0,1.5,-6 0,0,1 60",
+             "L 0,5,-2 1.0 255,255,255",
+             "mat lam 204,204,204 0 0 0 0 1 0 0,0,0",
+             "mat metal 204,204,204 1 0.1 0 0 1 0 0,0,0",
+             "mat glass 255,255,255 0 0 1.5 1 1 0 0,0,0",
+             "pl 0,0,0 0,1,0 lam"]
+    for (x, y, z, r), m in zip(sph, ("lam", "metal", "glass")):
+        lines.append(f"sp {x:.9g},{y:.9g},{z:.9g} {2 * r:.9g} {m}")
+    return "\n".join(lines) + "\n", sph

@@ -320,6 +320,68 @@ def batched_leg(M, args):
     }
 
 
+def config1_leg(M, args):
+    """The plain .rt scene (640x360, 64 spp, depth 8; survey config 1)
+    through the drop-in path: the .rt text read by the host front end,
+    uploaded and rendered on the GPU (median of 5 timed renders), then the
+    same 64 spp by the oracle on every usable core: fp64 with the
+    reference's recursive traversal (the timed CPU baseline, the whole
+    workload, not a sample) and fp32 in the kernels' operation order.  The
+    images are compared on the same RNG stream."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle_ffi as O
+    from mrt_hip.workloads import config1_rt
+    text, _ = config1_rt(11)
+    hs = M.HostScene.parse(text)
+    W, H = hs.resolution
+    spp, depth, seed = 64, 8, 0xC0F1
+    with M.Renderer(W, H) as r:
+        r.upload(hs)
+        r.set_seed(seed)
+        r.render(0, spp, depth)                # warm-up
+        ts = []
+        for _ in range(5):
+            r.clear()
+            r.synchronize()
+            t0 = time.perf_counter()
+            r.render(0, spp, depth)
+            r.synchronize()
+            ts.append(time.perf_counter() - t0)
+        img = r.readback().astype(np.float64)
+    t_gpu = sorted(ts)[len(ts) // 2]
+    view = hs.view()
+    bvh = O.make_bvh(M.BuiltBvh(view))
+    cores, host = usable_cores()
+    t0 = time.perf_counter()
+    ref64, cnt = O.render(view, bvh, W, H, 0, spp, depth, seed, f64=True,
+                          traversal=O.TRAV_REFERENCE, threads=cores)
+    t_cpu = time.perf_counter() - t0
+    ref32, _ = O.render(view, bvh, W, H, 0, spp, depth, seed, threads=cores)
+    samples = W * H * spp
+    d32 = np.abs(img - ref32)
+    rel64 = np.abs(img - ref64) / np.maximum(ref64, 1e-3 * spp)
+    out = {
+        "workload": f"plain .rt scene {W}x{H}, {spp} spp, depth {depth}: "
+                    "ground plane, lambert / metal (fuzz 0.1) / glass (ior "
+                    "1.5) spheres, one point light, ambient 0.1",
+        "samples": samples,
+        "gpu_msamples_per_s": round(samples / t_gpu / 1e6, 2),
+        "gpu_ms": round(t_gpu * 1e3, 3),
+        "cpu_msamples_per_s": round(samples / t_cpu / 1e6, 3),
+        "cpu_s": round(t_cpu, 3), "cpu_cores": cores, "cpu_kind": "port",
+        "cpu_rays_per_s": round((cnt["rays_closest"] + cnt["rays_shadow"])
+                                / t_cpu, 1),
+        "gpu_over_cpu": round(t_cpu / t_gpu, 1),
+        "parity_f32_channels_within_1e-4": round(float(
+            (d32 <= 1e-4 * spp + 1e-4 * np.abs(ref32)).mean()), 6),
+        "mean_rel_diff_vs_f64": round(float(rel64.mean()), 6),
+    }
+    log(f"config 1: GPU {out['gpu_msamples_per_s']} Msamples/s, CPU "
+        f"{out['cpu_msamples_per_s']} on {cores} cores")
+    return out
+
+
 def adjoint_leg(M, args):
     """Inverse rendering on the HIP path: forward (mean image over 64 spp) and
     path-replay adjoint timed on the same parameters, 32 gradients checked
@@ -613,6 +675,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-caustics", action="store_true")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the plain .rt scene (640x360, 64 spp)")
+    ap.add_argument("--only-config1", action="store_true")
     ap.add_argument("--only-caustics", action="store_true")
     ap.add_argument("--only-adjoint", action="store_true")
     ap.add_argument("--adam-steps", type=int, default=200)
@@ -647,6 +712,9 @@ def main():
         return
     if args.only_caustics:
         print(json.dumps({"caustics": caustics_leg(M, args)}), flush=True)
+        return
+    if args.only_config1:
+        print(json.dumps({"plain_rt": config1_leg(M, args)}), flush=True)
         return
     W, H = args.width, args.height
     t0 = time.perf_counter()
@@ -816,11 +884,14 @@ def main():
                       "device_ploc_upload_s": round(t_up_dev, 4),
                       "device_nodes4": dev_nodes},
     }
-    extra = not (args.no_batched and args.no_adjoint and args.no_caustics)
+    extra = not (args.no_batched and args.no_adjoint and args.no_caustics
+                 and args.no_config1)
     if extra:
         r.close()
     if not args.no_caustics:
         out["caustics"] = caustics_leg(M, args)
+    if not args.no_config1:
+        out["plain_rt"] = config1_leg(M, args)
     if not args.no_batched:
         out["batched"] = batched_leg(M, args)
     if not args.no_adjoint:

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import mrt_hip as M
+from mrt_hip.workloads import config1_rt
 import oracle_ffi as O
 
 pytestmark = pytest.mark.gpu
@@ -23,23 +24,6 @@ DEPTH = 8
 CAM_POS = (0.6, 1.9, -5.5)
 CAM_DIR = tuple(np.array([-0.1, -0.15, 1.0]) / np.linalg.norm([-0.1, -0.15, 1.0]))
 CAM_FOV = 55.0
-
-
-def config1_rt(seed=11):
-    """The plain 640x360 scene as .rt text: ground plane, lambert / metal
-    (fuzz 0.1) / glass (ior 1.5) unit spheres on seeded jitter, one point
-    light, ambient 0.1, a 60 degree pinhole camera."""
-    gen = M.HostScene.plain(seed, 640, 360)
-    sph = gen.arrays()["sph"]
-    lines = ["W 640 360", "A 0.1 255,255,255", "C 0,1.5,-6 0,0,1 60",
-             "L 0,5,-2 1.0 255,255,255",
-             "mat lam 204,204,204 0 0 0 0 1 0 0,0,0",
-             "mat metal 204,204,204 1 0.1 0 0 1 0 0,0,0",
-             "mat glass 255,255,255 0 0 1.5 1 1 0 0,0,0",
-             "pl 0,0,0 0,1,0 lam"]
-    for (x, y, z, r), m in zip(sph, ("lam", "metal", "glass")):
-        lines.append(f"sp {x:.9g},{y:.9g},{z:.9g} {2 * r:.9g} {m}")
-    return "\n".join(lines) + "\n", sph
 
 
 @pytest.fixture(scope="module")
