@@ -186,6 +186,8 @@ struct mrt_ctx {
     float       *d_env_pdf = nullptr, *d_env_marg_inv = nullptr;
     float       *d_env_cond_inv = nullptr, *d_mis_marg = nullptr;
     float       *d_mis_cond = nullptr, *d_mis_pcell = nullptr;
+    uint32_t    *d_mis_gm = nullptr, *d_mis_gc = nullptr;  // guide tables
+    uint32_t     guide_m = 0, guide_c = 0;
     uint32_t     env_w = 0, env_h = 0;
     float        env_exposure = 0.0f;
     bool         has_env = false;
@@ -802,6 +804,8 @@ static void free_env(mrt_ctx *c)
     dfree(c->d_mis_marg);
     dfree(c->d_mis_cond);
     dfree(c->d_mis_pcell);
+    dfree(c->d_mis_gm);
+    dfree(c->d_mis_gc);
     c->has_env = false;
 }
 
@@ -830,6 +834,12 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
         band[y] = sin(ehi) - sin(elo);
     }
     const double sc = exposure == 0.0f ? 1.0 : (double)powf(2.0f, exposure);
+    // guide tables: the powers of two at or above the CDFs' lengths
+    uint32_t Gm = 1, Gc = 1;
+    while (Gm < h - 1)
+        Gm <<= 1;
+    while (Gc < w - 1)
+        Gc <<= 1;
     double *d_band = nullptr, *d_scratch = nullptr;
     float *d_rgb = nullptr;
     int rc = MRT_OK;
@@ -840,6 +850,8 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
         || dalloc(c->d_mis_marg, h) != hipSuccess
         || dalloc(c->d_mis_cond, (size_t)(h - 1) * w) != hipSuccess
         || dalloc(c->d_mis_pcell, (size_t)(h - 1) * (w - 1)) != hipSuccess
+        || dalloc(c->d_mis_gm, (size_t)Gm + 1) != hipSuccess
+        || dalloc(c->d_mis_gc, (size_t)(h - 1) * (Gc + 1)) != hipSuccess
         || dalloc(d_band, h - 1) != hipSuccess
         || dalloc(d_scratch, 3 * (size_t)h + 2 + n) != hipSuccess
         || dalloc(d_rgb, 3 * n) != hipSuccess)
@@ -857,6 +869,9 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
                                          d_scratch, c->d_env_pdf, c->d_env_marg_inv,
                                          c->d_env_cond_inv, c->d_mis_marg,
                                          c->d_mis_cond, c->d_mis_pcell);
+        if (e == hipSuccess)
+            e = launch_env_guides(c->stream, c->d_mis_marg, c->d_mis_cond, w, h,
+                                  Gm, Gc, c->d_mis_gm, c->d_mis_gc);
         if (e != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
             rc = fail(c, MRT_E_HIP, "HDRI table build failed");
     }
@@ -867,6 +882,8 @@ extern "C" int mrt_upload_hdri(mrt_ctx *c, const float *rgb, uint32_t w,
         free_env(c);
         return rc;
     }
+    c->guide_m = Gm;
+    c->guide_c = Gc;
     c->env_w = w;
     c->env_h = h;
     c->env_exposure = exposure;
@@ -1224,6 +1241,10 @@ static FrameParams frame_params(const mrt_ctx *c)
         fp.mis_marg = c->d_mis_marg;
         fp.mis_cond = c->d_mis_cond;
         fp.mis_pcell = c->d_mis_pcell;
+        fp.mis_marg_guide = c->d_mis_gm;
+        fp.mis_cond_guide = c->d_mis_gc;
+        fp.guide_m = c->guide_m;
+        fp.guide_c = c->guide_c;
         fp.env_w = c->env_w;
         fp.env_h = c->env_h;
         float ex = c->env_exposure + c->ambient_ratio - 1.0f;

@@ -152,6 +152,11 @@ struct FrameParams {
     const float  *mis_marg;   // MIS tables (mrt_hdri_mis)
     const float  *mis_cond;
     const float  *mis_pcell;
+    // guide tables of the MIS CDFs (cdf_search_guided): guide_m + 1 entries
+    // for the marginal, guide_c + 1 per conditional row; powers of two
+    const uint32_t *mis_marg_guide;
+    const uint32_t *mis_cond_guide;
+    uint32_t guide_m, guide_c;
     uint32_t env_w, env_h;
     float    env_scale;       // 2^(exposure + ratio - 1)
     uint32_t has_env;
@@ -235,6 +240,13 @@ hipError_t launch_rgb_to_float4(hipStream_t st, const float *rgb, float4 *tex,
 // on the device, bit for bit as mrt_hdri_build_tables / _mis do on the
 // host.  sc = 2^exposure (or 1); band[y] = sin(e_hi) - sin(e_lo) of the h-1
 // MIS rows; scratch holds 3 h + 2 + w h doubles.
+// Guide tables of the MIS CDFs built by launch_env_tables: gm (Gm + 1
+// entries) for the marginal, gc ((h - 1) rows of Gc + 1) for the
+// conditional rows; Gm, Gc powers of two.
+hipError_t launch_env_guides(hipStream_t st, const float *mis_marg,
+                             const float *mis_cond, uint32_t w, uint32_t h,
+                             uint32_t Gm, uint32_t Gc, uint32_t *gm,
+                             uint32_t *gc);
 hipError_t launch_env_tables(hipStream_t st, const float4 *tex, uint32_t w,
                              uint32_t h, double sc, const double *band,
                              double *scratch, float *pdf, float *marg_inv,

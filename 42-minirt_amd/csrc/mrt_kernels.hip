@@ -1087,6 +1087,28 @@ __device__ __forceinline__ uint32_t cdf_search(const float *cdf, uint32_t n, flo
     return lo;
 }
 
+// cdf_search through a guide table g of G + 1 entries (G a power of two):
+// g[k] = cdf_search(cdf, n, k / G) for k < G and g[G] = n - 1.  The answer
+// is monotone in u, so for u in [k / G, (k + 1) / G) it lies in
+// [g[k], g[k + 1]]; k = floor(u G) and k / G are exact in fp32.  The search
+// over that range makes cdf_search's comparisons and returns its index,
+// after one guide load and a few entries instead of log2(n) dependent loads.
+__device__ __forceinline__ uint32_t cdf_search_guided(const float *cdf,
+                                                      const uint32_t *g,
+                                                      uint32_t G, float u)
+{
+    const uint32_t k = min((uint32_t)(u * (float)G), G - 1u);
+    uint32_t lo = g[k], hi = g[k + 1];
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Solid-angle density of the MIS environment sampler in direction d.
 __device__ __forceinline__ float env_pdf(const FrameParams &fp, V3 dir)
 {
@@ -1273,9 +1295,13 @@ k_shade(FrameParams fp, ShadeBufs B, uint32_t *__restrict__ ctr)
                         } else {
                             // light-sample half of the MIS pair
                             const uint32_t nw = F.env_w - 1, nh = F.env_h - 1;
-                            uint32_t y = cdf_search(F.mis_marg, nh, e.x);
-                            uint32_t x = cdf_search(F.mis_cond + (size_t)y * F.env_w,
-                                                    nw, e.y);
+                            uint32_t y = cdf_search_guided(F.mis_marg,
+                                                           F.mis_marg_guide,
+                                                           F.guide_m, e.x);
+                            uint32_t x = cdf_search_guided(
+                                F.mis_cond + (size_t)y * F.env_w,
+                                F.mis_cond_guide + (size_t)y * (F.guide_c + 1),
+                                F.guide_c, e.y);
                             float pc = F.mis_pcell[(size_t)y * nw + x];
                             float vv = ((float)y + e.z) / (float)nh;
                             float uu = ((float)x + e.w) / (float)nw;
@@ -1978,8 +2004,11 @@ k_photon_emit(FrameParams fp, SceneDev S, PhotonEmit pe, ShadeBufs B,
             } else {
                 const uint32_t nw = fp.env_w - 1, nh = fp.env_h - 1;
                 U4 e = rk(4u), u = rk(0u);
-                uint32_t y = cdf_search(fp.mis_marg, nh, e.x);
-                uint32_t x = cdf_search(fp.mis_cond + (size_t)y * fp.env_w, nw, e.y);
+                uint32_t y = cdf_search_guided(fp.mis_marg, fp.mis_marg_guide,
+                                               fp.guide_m, e.x);
+                uint32_t x = cdf_search_guided(
+                    fp.mis_cond + (size_t)y * fp.env_w,
+                    fp.mis_cond_guide + (size_t)y * (fp.guide_c + 1), fp.guide_c, e.y);
                 float pc = fp.mis_pcell[(size_t)y * nw + x];
                 float vv = ((float)y + e.z) / (float)nh;
                 float uu = ((float)x + e.w) / (float)nw;
@@ -2571,6 +2600,38 @@ hipError_t launch_rgb_to_float4(hipStream_t st, const float *rgb, float4 *tex,
 {
     hipLaunchKernelGGL(k_rgb_to_float4, dim3((unsigned)((n + 255) / 256)),
                        dim3(256), 0, st, rgb, tex, n);
+    return hipGetLastError();
+}
+
+// One thread per guide entry: the marginal's Gm + 1, then every
+// conditional row's Gc + 1.
+__global__ void __launch_bounds__(256)
+k_env_guides(const float *__restrict__ mis_marg, const float *__restrict__ mis_cond,
+             uint32_t w, uint32_t h, uint32_t Gm, uint32_t Gc,
+             uint32_t *__restrict__ gm, uint32_t *__restrict__ gc)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nh = h - 1, nw = w - 1;
+    const size_t nm = (size_t)Gm + 1, nc = (size_t)nh * (Gc + 1);
+    if (i < nm) {
+        const uint32_t k = (uint32_t)i;
+        gm[k] = k < Gm ? cdf_search(mis_marg, nh, (float)k / (float)Gm) : nh - 1;
+    } else if (i < nm + nc) {
+        const size_t j = i - nm;
+        const uint32_t y = (uint32_t)(j / (Gc + 1)), k = (uint32_t)(j % (Gc + 1));
+        gc[j] = k < Gc ? cdf_search(mis_cond + (size_t)y * w, nw,
+                                    (float)k / (float)Gc)
+                       : nw - 1;
+    }
+}
+
+hipError_t launch_env_guides(hipStream_t st, const float *mis_marg,
+                             const float *mis_cond, uint32_t w, uint32_t h,
+                             uint32_t Gm, uint32_t Gc, uint32_t *gm, uint32_t *gc)
+{
+    const size_t n = (size_t)Gm + 1 + (size_t)(h - 1) * (Gc + 1);
+    hipLaunchKernelGGL(k_env_guides, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                       0, st, mis_marg, mis_cond, w, h, Gm, Gc, gm, gc);
     return hipGetLastError();
 }
 

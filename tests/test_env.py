@@ -251,3 +251,46 @@ def test_gpu_hdri_tables_equal_host_tables_bitwise(case):
     np.testing.assert_array_equal(mm, om_)
     np.testing.assert_array_equal(mc, oc_)
     np.testing.assert_array_equal(mp, opc)
+
+
+def _cdf_search(cdf, n, u):
+    """The kernels' cdf_search: the last index in [0, n-1] whose cdf entry
+    is <= u (0 if none), by the same halving."""
+    lo, hi = 0, n - 1
+    while lo < hi:
+        mid = (lo + hi + 1) >> 1
+        if cdf[mid] <= u:
+            lo = mid
+        else:
+            hi = mid - 1
+    return lo
+
+
+def test_guided_cdf_search_equals_the_binary_search():
+    """cdf_search_guided's scheme (a guide g[k] = cdf_search(k / G), G a power
+    of two, then the search within [g[k], g[k + 1]]) returns cdf_search's
+    index for every u: random CDFs with flat runs (zero-pdf cells), u on the
+    guide boundaries k / G, just below them, and uniform draws on the
+    kernels' 2^-24 grid."""
+    rng = np.random.default_rng(7)
+    for n, G in ((1023, 1024), (2047, 2048), (5, 8), (1, 1), (300, 512)):
+        w = rng.exponential(size=n) * (rng.random(n) > 0.3)
+        w[0] = 0.0
+        cdf = np.concatenate([[0.0], np.cumsum(w[1:])]).astype(np.float32)
+        cdf = np.concatenate([cdf / max(cdf[-1], 1e-30), [1.0]]).astype(np.float32)
+        guide = [_cdf_search(cdf, n, np.float32(k) / np.float32(G))
+                 for k in range(G)] + [n - 1]
+        us = list(np.arange(G, dtype=np.float32) / np.float32(G))
+        us += [np.nextafter(u, np.float32(0)) for u in us[1:]]
+        us += list((rng.integers(0, 1 << 24, 4000) * 2.0 ** -24).astype(np.float32))
+        for u in us:
+            u = np.float32(u)
+            k = min(int(u * np.float32(G)), G - 1)
+            lo, hi = guide[k], guide[k + 1]
+            while lo < hi:
+                mid = (lo + hi + 1) >> 1
+                if cdf[mid] <= u:
+                    lo = mid
+                else:
+                    hi = mid - 1
+            assert lo == _cdf_search(cdf, n, u), (n, G, float(u))
