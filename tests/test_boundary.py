@@ -143,3 +143,29 @@ def test_set_camera_moves_the_view(scene):
         assert np.mean(ids_a == rid) >= 0.9999
     finally:
         scene.set_resolution(640, 360)
+
+
+def test_ticks_at_1080p_hand_over_every_sample_once():
+    """At 1920x1080 (6.2 M floats) the read-back copies in chunks and adds
+    each on the context's host threads as it arrives: each tick's sums land
+    in the host buffer exactly (the float sums widened to double and added),
+    and nothing is left on the device."""
+    w, h = 1920, 1080
+    hs = M.HostScene.plain(3, w, h)
+    with M.Renderer(w, h) as r:
+        r.upload(hs)
+        r.set_seed(SEED)
+        buf = np.zeros((h, w, 3), np.float64)
+        expect = np.zeros_like(buf)
+        for s in range(3):
+            # the tick's sums, read without handing them over, then the tick
+            with M.Renderer(w, h) as q:
+                q.upload(hs)
+                q.set_seed(SEED)
+                q.render(s, 1, DEPTH)
+                expect += q.readback().astype(np.float64)
+            r.render(s, 1, DEPTH)
+            r.readback_add(buf)
+            assert np.array_equal(buf, expect), s
+        assert not r.readback().any()
+        assert buf.any()
