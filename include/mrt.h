@@ -358,13 +358,16 @@ int  mrt_render(mrt_ctx *ctx, uint32_t first_sample, uint32_t n_samples,
  * this was not called. */
 int  mrt_reserve(mrt_ctx *ctx, uint32_t n_samples);
 
-/* Replaces get_result (src/gpu/use_shader.c:53-79): waits for the stream,
- * adds the device sums into the host-owned W*H*3 double buffer, as
- * add_fcolor_float3 does into screen.float_render, and zeroes the device
- * sums.  Like get_result, which adds the one sample each dispatch wrote,
- * every sample rendered since the previous call reaches float_render
- * exactly once: the compute_frame_gpu sequence mrt_render(sample, 1) ->
- * mrt_readback_add -> sample++ leaves k samples after k ticks. */
+/* Replaces get_result (src/gpu/use_shader.c:53-79): copies the device sums
+ * after every render before it, adds them into the host-owned W*H*3 double
+ * buffer, as add_fcolor_float3 does into screen.float_render, and zeroes
+ * the device sums in stream order (the next render starts from zero).  A
+ * frame of at least 2^20 floats is copied in chunks, each added on the
+ * context's host threads as it arrives.  Like get_result, which adds the
+ * one sample each dispatch wrote, every sample rendered since the previous
+ * call reaches float_render exactly once: the compute_frame_gpu sequence
+ * mrt_render(sample, 1) -> mrt_readback_add -> sample++ leaves k samples
+ * after k ticks.  Returns once float_render holds them. */
 int  mrt_readback_add(mrt_ctx *ctx, double *float_render);
 
 /* Copies the device sums (W*H*3 floats) to out and leaves them in place. */
