@@ -325,6 +325,28 @@ def batched_leg(M, args):
     }
 
 
+def ticks_leg(r, W, H, depth, n_ticks=16):
+    """The drop-in's per-tick sequence on renderer r (scene uploaded):
+    mrt_render(sample, 1) then mrt_readback_add into a host double buffer,
+    n_ticks times after one warm-up tick."""
+    import numpy as np
+    float_render = np.zeros((H, W, 3), np.float64)
+    r.render(0, 1, depth)
+    r.readback_add(float_render)               # warm-up (pinned buffer)
+    t0 = time.perf_counter()
+    for s in range(n_ticks):
+        r.render(1 + s, 1, depth)
+        r.readback_add(float_render)
+    t_ticks = time.perf_counter() - t0
+    per_tick = n_ticks * W * H / t_ticks / 1e6
+    log(f"drop-in ticks: {per_tick:.1f} Msamples/s "
+        f"({t_ticks / n_ticks * 1e3:.2f} ms per tick)")
+    return {"msamples_per_s": round(per_tick, 2),
+            "ms_per_tick": round(t_ticks / n_ticks * 1e3, 3),
+            "sequence": "mrt_render(sample, 1) + mrt_readback_add into a host "
+                        "double buffer, per tick (PCIe included)"}
+
+
 def config1_leg(M, args):
     """The plain .rt scene (640x360, 64 spp, depth 8; survey config 1)
     through the drop-in path: the .rt text read by the host front end,
@@ -683,6 +705,8 @@ def main():
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the plain .rt scene (640x360, 64 spp)")
     ap.add_argument("--only-config1", action="store_true")
+    ap.add_argument("--only-ticks", action="store_true",
+                    help="only the drop-in per-tick sequence on the mesh scene")
     ap.add_argument("--only-caustics", action="store_true")
     ap.add_argument("--only-adjoint", action="store_true")
     ap.add_argument("--adam-steps", type=int, default=200)
@@ -720,6 +744,17 @@ def main():
         return
     if args.only_config1:
         print(json.dumps({"plain_rt": config1_leg(M, args)}), flush=True)
+        return
+    if args.only_ticks:
+        hs = M.HostScene.mesh(args.seed, args.width, args.height,
+                              subdiv=args.subdiv, floor_cells=args.floor,
+                              amp=0.15)
+        with M.Renderer(args.width, args.height) as r:
+            if args.lanes is not None:
+                r.set_option(M.OPT_LANES, args.lanes)
+            r.upload(hs)
+            out = ticks_leg(r, args.width, args.height, args.depth)
+        print(json.dumps({"drop_in_per_tick": out}), flush=True)
         return
     W, H = args.width, args.height
     t0 = time.perf_counter()
@@ -810,26 +845,7 @@ def main():
     # 4) the drop-in's own contract (compute_frame_gpu): one sample per
     # render_frame tick, each followed by mrt_readback_add into the host's
     # double buffer (PCIe transfer and host-side add included)
-    import numpy as np
-    ticks = None
-    if not args.no_ticks:
-        float_render = np.zeros((H, W, 3), np.float64)
-        r.render(0, 1, args.depth)
-        r.readback_add(float_render)           # warm-up (pinned buffer)
-        n_ticks = 16
-        t0 = time.perf_counter()
-        for s in range(n_ticks):
-            r.render(1 + s, 1, args.depth)
-            r.readback_add(float_render)
-        t_ticks = time.perf_counter() - t0
-        per_tick = n_ticks * W * H / t_ticks / 1e6
-        log(f"drop-in ticks: {per_tick:.1f} Msamples/s "
-            f"({t_ticks / n_ticks * 1e3:.2f} ms per tick)")
-        ticks = {"msamples_per_s": round(per_tick, 2),
-                 "ms_per_tick": round(t_ticks / n_ticks * 1e3, 3),
-                 "sequence": "mrt_render(sample, 1) + mrt_readback_add into "
-                             "a host double buffer, per tick (PCIe "
-                             "included)"}
+    ticks = None if args.no_ticks else ticks_leg(r, W, H, args.depth)
     err = None
     if not args.no_image_error:
         err = image_error(r, W, H, args.depth, 0x5EED)
