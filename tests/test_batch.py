@@ -327,3 +327,26 @@ def test_batched_workload_properties():
             ref = r.readback()
         np.testing.assert_allclose(imgs[k], ref, rtol=1e-4, atol=1e-5,
                                    err_msg=f"scene {k}")
+
+
+@pytest.mark.gpu
+def test_batch_of_more_scenes_than_the_lds_root_table():
+    """600 small scenes (more than the refill kernel's 512-entry root
+    table): the batch traces with k_trace and every sampled scene equals
+    its plain render."""
+    cases = [(M.HostScene.plain(100 + k, 12, 8), 12, 8, 2, 0x9000 + k)
+             for k in range(600)]
+    b = make_batch(cases, path_budget=1 << 16)
+    b.render(DEPTH)
+    st = b.stats()
+    assert st["paths"] == 600 * 12 * 8 * 2
+    for k in (0, 1, 511, 512, 599):
+        hs, w, h, spp, seed = cases[k]
+        with M.Renderer(w, h) as r:
+            r.upload(hs)
+            r.set_seed(seed)
+            r.render(0, spp, DEPTH)
+            ref = r.readback()
+        np.testing.assert_allclose(b.readback(k), ref, rtol=2e-5, atol=2e-6,
+                                   err_msg=f"scene {k}")
+    b.close()
