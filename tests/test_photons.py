@@ -376,12 +376,17 @@ def test_gpu_photon_map_rebuild_and_disable():
 
 
 @pytest.mark.gpu
-def test_gpu_hdri_caustic_scene_matches_oracle():
+@pytest.mark.parametrize("W,H,sub,floor,water,n,radius", [
+    (64, 36, 3, 8, 24, 40000, 0.05),
+    (128, 72, 4, 16, 64, 200000, 0.03)])
+def test_gpu_hdri_caustic_scene_matches_oracle(W, H, sub, floor, water, n,
+                                               radius):
     """Water heightfield, glass sphere, HDRI with a sun, MIS estimator and
     environment photons gathered by fixed radius: the full image on the same
-    RNG stream against the oracle fed the GPU's photon map."""
-    W, H = 64, 36
-    hs = M.HostScene.caustic(4, W, H, subdiv=3, floor_cells=8, water_cells=24)
+    RNG stream against the oracle fed the GPU's photon map (the larger case:
+    a 64^2-cell water grid and 200 k photons)."""
+    hs = M.HostScene.caustic(4, W, H, subdiv=sub, floor_cells=floor,
+                             water_cells=water)
     view, bvh = _bvh(hs)
     sky = M.hdri_sky(5, 128, 64, sun_radius_deg=3.0, sun_radiance=300.0)
     r = M.Renderer(W, H)
@@ -389,14 +394,14 @@ def test_gpu_hdri_caustic_scene_matches_oracle():
     r.upload_hdri(sky)
     r.set_option(M.OPT_ESTIMATOR, M.EST_MIS)
     r.set_seed(SEED)
-    r.build_photons(40000, 0.05, k=0, norm=1.0)
+    r.build_photons(n, radius, k=0, norm=1.0)
     pos, pw, _ = r.photons()
     r.clear()
     r.render(0, 4, 8)
     img = r.readback().astype(np.float64)
     r.close()
     assert len(pos) > 100
-    pm = O.photon_map(pos, pw, k=0, radius=0.05, norm=1.0)
+    pm = O.photon_map(pos, pw, k=0, radius=radius, norm=1.0)
     hd = O.make_hdri(sky)
     ref, _ = O.render(view, bvh, W, H, 0, 4, 8, SEED, estimator=M.EST_MIS,
                       hdri=hd, photons=pm)
@@ -406,4 +411,7 @@ def test_gpu_hdri_caustic_scene_matches_oracle():
     diff = np.abs(img - ref)
     close = diff <= 1e-4 + 1e-4 * np.abs(ref)
     print(f"close {close.mean():.6f} mean diff {diff.mean():.3e}")
-    assert close.mean() >= 0.99
+    # a path whose refraction through the water differs by an ulp may leave
+    # on another branch: a few pixels, a small share of the image's energy
+    assert close.mean() >= 0.995
+    assert diff.mean() <= 1e-3 * ref.mean()
