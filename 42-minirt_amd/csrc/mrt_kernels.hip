@@ -343,16 +343,17 @@ k_trace_top(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
         S, w, ctr, stats, nullptr, nullptr, ovf_buf, s_code, s_top, ntop);
 }
 
-// k_trace_top with per-lane refill (single scenes of triangles and
-// spheres).  A wave takes 64-item chunks as k_trace does (closest-hit rays
+// k_trace_top with per-lane refill (scenes of triangles and spheres: one,
+// or with BATCH a batch's).  A wave takes 64-item chunks as k_trace does (closest-hit rays
 // and then shadow rays numbered as one item space), but a lane whose ray is
 // done takes the next item of the wave's chunk instead of idling until the
 // wave's slowest ray ends: secondary rays of one wave differ several times
 // over in traversal length.  Refills happen when REFILL_MIN lanes are idle
 // (or all of them), so their loads and set-up run for many lanes at once.
-// The traversal is traverse()'s, one node or one leaf per iteration, with
-// the ray's kind per lane: closest-hit rays sort the hit children, shadow
-// rays keep slot order and stop at the first hit, as there.
+// The traversal is traverse()'s, one node or one pair of leaf records per
+// iteration, with the ray's kind per lane: closest-hit rays sort the hit
+// children, shadow rays keep slot order and stop at the first hit, as
+// there.
 constexpr uint32_t REFILL_MIN = 16;
 
 // 6 waves per SIMD (78 VGPRs, no scratch): 1,379 against 1,225 Msamples/s
