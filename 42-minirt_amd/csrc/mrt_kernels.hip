@@ -424,12 +424,16 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
     uint32_t hslot = 0, nodes_c = 0, prims_c = 0, nodes_s = 0, prims_s = 0;
     int sp = 0;
     int32_t code = DONE;
+    // the top entry, DONE for an empty stack: an LDS read at a clamped slot
+    // (in bounds for every lane, so it needs no branch), replaced by the
+    // overflow column's entry when the top lies there
     auto pop = [&]() -> int32_t {
-        if (sp == 0)
-            return DONE;
-        sp--;
-        return (int32_t)(sp < SS ? lds[sp * TRACE_BLOCK]
-                                 : ovf[(uint32_t)(sp - SS) * os]);
+        const int t = sp - 1;
+        int32_t v = (int32_t)lds[(uint32_t)min(max(t, 0), SS - 1) * TRACE_BLOCK];
+        if (t >= SS)
+            v = (int32_t)ovf[(uint32_t)(t - SS) * os];
+        sp = max(t, 0);
+        return t < 0 ? DONE : v;
     };
     auto push = [&](int32_t cc) {
         if (sp < SS)
@@ -556,12 +560,28 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                     if (k0 == INF) {
                         code = pop();
                     } else {
-                        if (k3 != INF)
-                            push(c3);
-                        if (k2 != INF)
-                            push(c2);
-                        if (k1 != INF)
-                            push(c1);
+                        // the hit children after the nearest, farthest
+                        // first: hit keys are a prefix of the sorted four, so
+                        // with np of them after the nearest, slot sp + j takes
+                        // child np - j.  Three slots are written whatever np
+                        // is (those at sp + np and above are free), without a
+                        // branch per child, while they all lie in LDS
+                        const uint32_t np = (uint32_t)(k1 != INF) + (uint32_t)(k2 != INF)
+                                          + (uint32_t)(k3 != INF);
+                        if (sp + 3 <= SS) {
+                            lds[(uint32_t)sp * TRACE_BLOCK] =
+                                (uint32_t)(np == 3u ? c3 : (np == 2u ? c2 : c1));
+                            lds[(uint32_t)(sp + 1) * TRACE_BLOCK] = (uint32_t)(np == 3u ? c2 : c1);
+                            lds[(uint32_t)(sp + 2) * TRACE_BLOCK] = (uint32_t)c1;
+                            sp += (int)np;
+                        } else {
+                            if (k3 != INF)
+                                push(c3);
+                            if (k2 != INF)
+                                push(c2);
+                            if (k1 != INF)
+                                push(c1);
+                        }
                         code = c0;
                     }
                 } else {
