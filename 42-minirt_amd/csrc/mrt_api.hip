@@ -1675,26 +1675,35 @@ extern "C" int mrt_readback_add(mrt_ctx *c, double *float_render)
     if (!c || !float_render)
         return MRT_E_INVALID;
     const size_t n = (size_t)c->W * c->H * 3;
-    if (!c->h_stage) {
-        if (hipHostMalloc((void **)&c->h_stage, n * sizeof(float),
-                          hipHostMallocDefault) != hipSuccess) {
-            c->h_stage = nullptr;
-            return fail(c, MRT_E_NOMEM, "pinned read-back buffer (%zu floats)", n);
+    // set-up at the first call; each part is made once, so a call after a
+    // failed set-up picks up where it stopped
+    if (!c->h_stage && hipHostMalloc((void **)&c->h_stage, n * sizeof(float),
+                                     hipHostMallocDefault) != hipSuccess) {
+        c->h_stage = nullptr;
+        return fail(c, MRT_E_NOMEM, "pinned read-back buffer (%zu floats)", n);
+    }
+    for (hipEvent_t &e : c->ev_rb)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            return fail(c, MRT_E_HIP, "read-back events");
         }
-        for (hipEvent_t &e : c->ev_rb)
-            if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-                e = nullptr;
-                return fail(c, MRT_E_HIP, "read-back events");
-            }
+    if (!c->pool) {
         // a 1080p frame is 6.2 M doubles: the add runs on up to 8 threads
         unsigned nt = std::thread::hardware_concurrency();
         nt = std::max(1u, std::min(nt, 8u));
         if (n < (1u << 20))
             nt = 1;
-        c->pool = new (std::nothrow) HostPool();
-        if (!c->pool)
+        HostPool *pool = new (std::nothrow) HostPool();
+        if (!pool)
             return fail(c, MRT_E_NOMEM, "read-back threads");
-        c->pool->start(nt);
+        try {
+            pool->start(nt);
+        } catch (...) {
+            // the threads that did start are joined by the destructor
+            delete pool;
+            return fail(c, MRT_E_NOMEM, "read-back threads");
+        }
+        c->pool = pool;
     }
     // get_result adds one dispatch's samples per call: the device sums are
     // handed over and zeroed in stream order, so the next render starts
@@ -2402,9 +2411,15 @@ extern "C" int mrt_batch_commit(mrt_batch *b)
                 brc[i] = mrt_bvh4_build(&bv[i], &bv4[i]);
         }
     };
+    // threads that cannot be started leave their share to the others (the
+    // scenes are claimed from one counter), at worst to this thread
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nth; t++)
-        pool.emplace_back(work);
+    try {
+        pool.reserve(nth);
+        for (unsigned t = 1; t < nth; t++)
+            pool.emplace_back(work);
+    } catch (...) {
+    }
     work();
     for (std::thread &t : pool)
         t.join();
