@@ -716,12 +716,26 @@ __device__ __forceinline__ bool traverse(const SceneDev &S, V3 o, V3 d,
         }
         if (k0 == INF)
             return pop();
-        if (k3 != INF)
-            push(c3);
-        if (k2 != INF)
-            push(c2);
-        if (k1 != INF)
-            push(c1);
+        // the hit children after the nearest, farthest first: they are a
+        // prefix of the four (sorted, or moved to the front), so with np of
+        // them after the nearest, slot sp + j takes child np - j.  Three
+        // slots are written whatever np is (those at sp + np and above are
+        // free), without a branch per child, while they all lie in LDS.
+        const uint32_t np = (uint32_t)(k1 != INF) + (uint32_t)(k2 != INF)
+                          + (uint32_t)(k3 != INF);
+        if (sp + 3 <= SS) {
+            lds_code[sp * stride] = (uint32_t)(np == 3u ? c3 : (np == 2u ? c2 : c1));
+            lds_code[(sp + 1) * stride] = (uint32_t)(np == 3u ? c2 : c1);
+            lds_code[(sp + 2) * stride] = (uint32_t)c1;
+            sp += (int)np;
+        } else {
+            if (k3 != INF)
+                push(c3);
+            if (k2 != INF)
+                push(c2);
+            if (k1 != INF)
+                push(c1);
+        }
         return c0;
     };
     // One primitive record; true when an any-hit query is answered.
