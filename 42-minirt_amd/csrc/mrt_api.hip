@@ -1389,26 +1389,55 @@ struct Timer {
 // counter block ctr.  Bounce b: its closest-hit rays and the shadow rays of
 // bounce b - 1 in one launch, then the light sums of bounce b - 1, then
 // the shade of bounce b; a last launch takes the shadow rays of the last
-// bounce.
-static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
-                       const SceneDev &S, bool count, uint32_t max_depth,
-                       uint32_t P, uint32_t off, uint32_t rpp, uint32_t *ctr,
-                       const StackOvf &ovf, float4 *tape = nullptr,
-                       hipEvent_t ev_after = nullptr, uint32_t ev_bounce = 0)
-{
-    const size_t so = (size_t)off * rpp;
-    const uint32_t cap_shadow = P * rpp;
-    tm.st = st;
-    LAUNCH(c, tm, K_RAYGEN,
-           launch_raygen(st, fp, c->q_o[0] + off, c->q_d[0] + off,
-                         c->q_t[0] + off, c->acc + off, P, ctr));
-    const bool lit = !c->lights.empty() || c->has_env || c->has_photons;
-    const uint32_t gb = std::min((P + SHADE_BLOCK - 1) / SHADE_BLOCK,
-                                 c->grid_blocks);
-    ShadeBufs Bp;
-    FrameParams fpp = fp;
-    memset(&Bp, 0, sizeof(Bp));
-    for (uint32_t b = 0; b <= max_depth; b++) {
+// bounce.  start() enqueues the raygen, bounce(b) bounce b's launches, so
+// a caller can interleave the lanes' launches bounce by bounce.
+struct LaneRun {
+    mrt_ctx *c;
+    hipStream_t st;
+    Timer *tm;
+    FrameParams fp;
+    const SceneDev *S;
+    bool count;
+    uint32_t max_depth, P, off, rpp;
+    uint32_t *ctr;
+    StackOvf ovf;
+    float4 *tape = nullptr;
+    hipEvent_t ev_after = nullptr;   // recorded once bounce ev_bounce is traced
+    uint32_t ev_bounce = 0;
+    // set by start()
+    size_t so = 0;
+    uint32_t cap_shadow = 0, gb = 0;
+    bool lit = false;
+    ShadeBufs Bp;                    // the previous bounce's shade, for its light sums
+    FrameParams fpp;
+
+    int start()
+    {
+        so = (size_t)off * rpp;
+        cap_shadow = P * rpp;
+        lit = !c->lights.empty() || c->has_env || c->has_photons;
+        gb = std::min((P + SHADE_BLOCK - 1) / SHADE_BLOCK, c->grid_blocks);
+        memset(&Bp, 0, sizeof(Bp));
+        fpp = fp;
+        tm->st = st;
+        LAUNCH(c, (*tm), K_RAYGEN,
+               launch_raygen(st, fp, c->q_o[0] + off, c->q_d[0] + off,
+                             c->q_t[0] + off, c->acc + off, P, ctr));
+        tm->st = nullptr;
+        return MRT_OK;
+    }
+
+    int bounce(uint32_t b)
+    {
+        tm->st = st;
+        const int rc = bounce_launches(b);
+        tm->st = nullptr;
+        return rc;
+    }
+
+private:
+    int bounce_launches(uint32_t b)
+    {
         const bool last = b == max_depth;
         const int cur = b & 1, nxt = cur ^ 1;
         TraceWork w;
@@ -1423,8 +1452,8 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         w.closest = !last;
         w.shadow_ctr = (b > 0 && lit) ? ctr_shadow(b - 1) : -1;
         if (w.closest || w.shadow_ctr >= 0)
-            LAUNCH(c, tm, K_TRACE,
-                   launch_trace(st, c->trace_blocks, count, S, ovf, w, ctr,
+            LAUNCH(c, (*tm), K_TRACE,
+                   launch_trace(st, c->trace_blocks, count, *S, ovf, w, ctr,
                                 c->d_stats, c->trace_top, c->trace_refill,
                                 c->refill_blocks));
         // the next pass may start (ev_after) once this bounce is traced
@@ -1433,11 +1462,10 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
             ev_after = nullptr;
         }
         if (b > 0 && lit)
-            LAUNCH(c, tm, K_RESOLVE, launch_resolve(st, gb, fpp, Bp, ctr));
+            LAUNCH(c, (*tm), K_RESOLVE, launch_resolve(st, gb, fpp, Bp, ctr));
         if (last) {
-            LAUNCH(c, tm, K_ADVANCE,
-                   launch_advance(st, ctr, c->d_stats, P, b));
-            break;
+            LAUNCH(c, (*tm), K_ADVANCE, launch_advance(st, ctr, c->d_stats, P, b));
+            return MRT_OK;
         }
         fp.bounce = b;
         fp.last_bounce = (b + 1 == max_depth) ? 1u : 0u;
@@ -1470,17 +1498,31 @@ static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
         B.cap_shadow = cap_shadow;
         B.tape = tape;
         B.tape_P = P;
-        LAUNCH(c, tm, K_SHADE, launch_shade(st, gb, fp, B, ctr));
+        LAUNCH(c, (*tm), K_SHADE, launch_shade(st, gb, fp, B, ctr));
         if (c->has_photons)
-            LAUNCH(c, tm, K_CAUSTIC,
+            LAUNCH(c, (*tm), K_CAUSTIC,
                    launch_caustic(st, c->grid_blocks, fp, B, ctr, count,
                                   c->d_stats));
-        LAUNCH(c, tm, K_ADVANCE, launch_advance(st, ctr, c->d_stats, P, b));
+        LAUNCH(c, (*tm), K_ADVANCE, launch_advance(st, ctr, c->d_stats, P, b));
         Bp = B;
         fpp = fp;
+        return MRT_OK;
     }
-    tm.st = nullptr;
-    return MRT_OK;
+};
+
+// A whole lane, bounce after bounce.
+static int render_lane(mrt_ctx *c, hipStream_t st, Timer &tm, FrameParams fp,
+                       const SceneDev &S, bool count, uint32_t max_depth,
+                       uint32_t P, uint32_t off, uint32_t rpp, uint32_t *ctr,
+                       const StackOvf &ovf, float4 *tape = nullptr,
+                       hipEvent_t ev_after = nullptr, uint32_t ev_bounce = 0)
+{
+    LaneRun lr{c, st, &tm, fp, &S, count, max_depth, P, off, rpp, ctr, ovf,
+               tape, ev_after, ev_bounce};
+    int rc = lr.start();
+    for (uint32_t b = 0; rc == MRT_OK && b <= max_depth; b++)
+        rc = lr.bounce(b);
+    return rc;
 }
 
 // Lanes of a render of n_samples in passes of nb samples (P64 paths, S64
@@ -1584,22 +1626,33 @@ extern "C" int mrt_render(mrt_ctx *c, uint32_t first_sample, uint32_t n_samples,
         // part l: slots [o0, o0 + Pl) of the pass, numbered as in the whole
         // pass (fp.slot0), in the pass's queue entries from o0, with lane
         // l's counters and stack columns; the pass's sums after every part
+        // The parts' launches are enqueued bounce by bounce, part after
+        // part: a tick starts with the GPU idle, and a part enqueued whole
+        // would hold the next part back by its ~35 launches' host time.
         const uint32_t P = n_samples * WH;
         FrameParams fl = fp;
         fl.sample0 = first_sample;
+        LaneRun parts[MAX_LANES];
         for (uint32_t p = 0; p < Ls; p++) {
             const uint32_t o0 = (uint32_t)((uint64_t)P * p / Ls);
             const uint32_t Pl = (uint32_t)((uint64_t)P * (p + 1) / Ls) - o0;
-            hipStream_t st = p ? c->lane_stream[p] : c->stream;
             fl.slot0 = o0;
-            rc = render_lane(c, st, tm, fl, S, count, max_depth, Pl, o0, rpp,
-                             c->d_ctr + (size_t)p * CTR_COUNT, stack_ovf(c, p));
+            parts[p] = LaneRun{c, p ? c->lane_stream[p] : c->stream, &tm, fl, &S,
+                               count, max_depth, Pl, o0, rpp,
+                               c->d_ctr + (size_t)p * CTR_COUNT, stack_ovf(c, p)};
+            rc = parts[p].start();
             if (rc != MRT_OK)
                 return rc;
-            if (p) {
-                HIPCHK(c, hipEventRecord(c->ev_join[p], st));
-                HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join[p], 0));
+        }
+        for (uint32_t b = 0; b <= max_depth; b++)
+            for (uint32_t p = 0; p < Ls; p++) {
+                rc = parts[p].bounce(b);
+                if (rc != MRT_OK)
+                    return rc;
             }
+        for (uint32_t p = 1; p < Ls; p++) {
+            HIPCHK(c, hipEventRecord(c->ev_join[p], parts[p].st));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join[p], 0));
         }
         LAUNCH(c, tm, K_ACCUM,
                launch_accumulate(c->stream, c->acc, c->d_accum, WH, n_samples,
@@ -1724,7 +1777,13 @@ extern "C" int mrt_readback_add(mrt_ctx *c, double *float_render)
     std::atomic<int> bad{0};
     c->pool->run([&](unsigned t) {
         for (int k = 0; k < K; k++) {
-            if (hipEventSynchronize(c->ev_rb[k]) != hipSuccess) {
+            // polled, not a blocking wait: a tick's chunks arrive within a
+            // millisecond, and a blocked thread's wake-up costs a good part
+            // of that
+            hipError_t e;
+            while ((e = hipEventQuery(c->ev_rb[k])) == hipErrorNotReady)
+                std::this_thread::yield();
+            if (e != hipSuccess) {
                 bad = 1;
                 return;
             }
