@@ -350,18 +350,22 @@ k_trace_top(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
 // wave's slowest ray ends: secondary rays of one wave differ several times
 // over in traversal length.  Refills happen when REFILL_MIN lanes are idle
 // (or all of them), so their loads and set-up run for many lanes at once.
-// The traversal is traverse()'s, one node or one pair of leaf records per
-// iteration, with the ray's kind per lane: closest-hit rays sort the hit
-// children, shadow rays keep slot order and stop at the first hit, as
-// there.
+// The traversal is traverse()'s, with the ray's kind per lane: closest-hit
+// rays sort the hit children, shadow rays keep slot order and stop at the
+// first hit, as there.  An iteration is a node step for the lanes at inner
+// nodes, then a leaf step for the wave: the next one or two records of
+// every lane at a leaf are tested on the wave's lanes, one record per lane
+// (the rest of a leaf stays the lane's code for the next iteration).
 constexpr uint32_t REFILL_MIN = 16;
 
-// 6 waves per SIMD (78 VGPRs, no scratch): 1,379 against 1,225 Msamples/s
-// for the 8-wave k_trace_top on the 1080p mesh workload (one lane); at 8
-// waves it spilled 68 B per lane and halved the rate, at 5 it hid less
-// latency.  REFILL_MIN 16 against 8 / 24: 1,367 / 1,364.  A leaf takes one
-// loop iteration per pair of records, the rest of it staying the lane's
-// code (1,251 -> 1,379 against an inner loop over the whole leaf).
+// 6 waves per SIMD (80 VGPRs; one register spilled outside the step loop):
+// 1,379 against 1,225 Msamples/s for the 8-wave k_trace_top on the 1080p
+// mesh workload (one lane) before the wave-wide leaf step; at 8 waves it
+// spilled 68 B per lane and halved the rate, at 5 it hid less latency.
+// REFILL_MIN 16 against 8 / 24: 1,367 / 1,364.  The wave-wide leaf step
+// against each lane testing its own pair: 1,580 -> 1,698 Msamples/s (the
+// leaf code ran once per record slot for the few lanes at leaves, about as
+// many instructions as the node step).
 //
 // BATCH: the rays of many scenes (triangles and spheres, one sidedness for
 // all) whose trees share the node array.  A ray starts at its scene's root,
@@ -384,6 +388,9 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
     __shared__ uint32_t s_code[SS * TRACE_BLOCK];
     __shared__ float4 s_top[TOP_NODES * 7];
     __shared__ uint32_t s_root[BATCH ? BATCH_LDS_SCENES : 1];
+    // per wave: the owner (lane | record << 6) of each of its lanes' leaf
+    // tests in the current step
+    __shared__ uint32_t s_job[TRACE_BLOCK];
     // the top ntop nodes of one tree, nodes [top_base, top_base + ntop):
     // a single scene's (top_base 0), or those of the scene that holds most
     // of a batch pass's paths (each scene's tree starts at its root, top
@@ -507,126 +514,171 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
         // (1,506 -> 1,547 Msamples/s against one loop with the refill at its
         // top)
         for (;;) {
-            if (item != HIT_NONE) {
-                // one traversal step (none for a ray of a scene without a BVH,
-                // which starts at DONE: only its planes are tested below)
-                bool found = false;
-                if (code == DONE) {
-                } else if (code >= 0) {
-                    float4 lx, hx, ly, hy, lz, hz, cw;
-                    const uint32_t rel = BATCH ? (uint32_t)code - top_base
-                                               : (uint32_t)code;
-                    if (rel < ntop) {
-                        const float4 *n = s_top + (size_t)rel * 7;
-                        lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
-                        cw = n[6];
+            bool found = false;
+            // one node step per lane at an inner node (a ray of a scene
+            // without a BVH starts at DONE: only its planes are tested below)
+            if (item != HIT_NONE && code >= 0) {
+                float4 lx, hx, ly, hy, lz, hz, cw;
+                const uint32_t rel = BATCH ? (uint32_t)code - top_base
+                                           : (uint32_t)code;
+                if (rel < ntop) {
+                    const float4 *n = s_top + (size_t)rel * 7;
+                    lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
+                    cw = n[6];
+                } else {
+                    const float4 *n = S.nodes + (size_t)code * 8;
+                    lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
+                    cw = n[6];
+                }
+                if (COUNT) {
+                    if (any)
+                        nodes_s++;
+                    else
+                        nodes_c++;
+                }
+                float t0, t1, t2, t3;
+                float k0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv,
+                                  tclose, t0) ? t0 : INF;
+                float k1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv,
+                                  tclose, t1) ? t1 : INF;
+                float k2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv,
+                                  tclose, t2) ? t2 : INF;
+                float k3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv,
+                                  tclose, t3) ? t3 : INF;
+                int32_t c0 = __float_as_int(cw.x), c1 = __float_as_int(cw.y),
+                        c2 = __float_as_int(cw.z), c3 = __float_as_int(cw.w);
+                // shadow rays take the hit children in slot order (any
+                // hit ends them): keyed by slot, the same network puts
+                // them first in that order, so a wave that holds both
+                // kinds of ray runs one branch-free sort
+                if (any) {
+                    k0 = k0 != INF ? 0.0f : INF;
+                    k1 = k1 != INF ? 1.0f : INF;
+                    k2 = k2 != INF ? 2.0f : INF;
+                    k3 = k3 != INF ? 3.0f : INF;
+                }
+                cx(k0, c0, k1, c1);
+                cx(k2, c2, k3, c3);
+                cx(k0, c0, k2, c2);
+                cx(k1, c1, k3, c3);
+                cx(k1, c1, k2, c2);
+                if (k0 == INF) {
+                    code = pop();
+                } else {
+                    // the hit children after the nearest, farthest
+                    // first: hit keys are a prefix of the sorted four, so
+                    // with np of them after the nearest, slot sp + j takes
+                    // child np - j.  Three slots are written whatever np
+                    // is (those at sp + np and above are free), without a
+                    // branch per child, while they all lie in LDS
+                    const uint32_t np = (uint32_t)(k1 != INF) + (uint32_t)(k2 != INF)
+                                      + (uint32_t)(k3 != INF);
+                    if (sp + 3 <= SS) {
+                        lds[(uint32_t)sp * TRACE_BLOCK] =
+                            (uint32_t)(np == 3u ? c3 : (np == 2u ? c2 : c1));
+                        lds[(uint32_t)(sp + 1) * TRACE_BLOCK] = (uint32_t)(np == 3u ? c2 : c1);
+                        lds[(uint32_t)(sp + 2) * TRACE_BLOCK] = (uint32_t)c1;
+                        sp += (int)np;
                     } else {
-                        const float4 *n = S.nodes + (size_t)code * 8;
-                        lx = n[0]; hx = n[1]; ly = n[2]; hy = n[3]; lz = n[4]; hz = n[5];
-                        cw = n[6];
+                        if (k3 != INF)
+                            push(c3);
+                        if (k2 != INF)
+                            push(c2);
+                        if (k1 != INF)
+                            push(c1);
                     }
+                    code = c0;
+                }
+            }
+            // Leaf step, after the node step of the same iteration: each lane
+            // at a leaf hands its next one or two records to the wave's
+            // lanes, one record per lane, so the triangle tests of the whole
+            // wave run once instead of once per record slot of the lanes
+            // that hold a leaf.  A test lane reads its owner's ray by a lane
+            // shuffle; the owner reads the results back and takes them in
+            // record order, as a test of the records one after the other
+            // does (the same hit, the same counts, an any-hit query stopping
+            // at the first hit).  Owners whose records do not fit in the 64
+            // lanes keep their leaf for the next iteration.
+            const bool at_leaf = item != HIT_NONE && code < 0 && code != DONE;
+            const uint64_t lm = __ballot(at_leaf);
+            if (lm) {
+                const uint32_t v = ~(uint32_t)code, first = v >> 3, cnt = (v & 7u) + 1u;
+                const bool two = at_leaf && cnt > 1u;
+                const uint64_t m2 = __ballot(two);
+                const uint64_t below = (1ull << lane) - 1ull;
+                const uint32_t pre = (uint32_t)(__popcll(lm & below) + __popcll(m2 & below));
+                const bool served = at_leaf && pre + 1u + (uint32_t)two <= 64u;
+                const uint32_t J = (uint32_t)(__popcll(__ballot(served))
+                                              + __popcll(__ballot(served && two)));
+                uint32_t *tab = s_job + (threadIdx.x & ~63u);
+                if (served) {
+                    tab[pre] = lane;
+                    if (two)
+                        tab[pre + 1u] = lane | 64u;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t job = tab[lane];
+                const uint32_t ow = job & 63u;
+                const V3 jo = v3(__shfl(o.x, ow, 64), __shfl(o.y, ow, 64), __shfl(o.z, ow, 64));
+                const V3 jd = v3(__shfl(d.x, ow, 64), __shfl(d.y, ow, 64), __shfl(d.z, ow, 64));
+                const uint32_t jrec = __shfl(first, ow, 64) + (job >> 6);
+                float tt = INF, uu = 0.0f, vv = 0.0f;
+                if (lane < J) {
+                    const float4 *pr = S.prims + (size_t)jrec * 3;
+                    const float4 A = pr[0], Bq = pr[1], Cq = pr[2];
+                    float th;
+                    bool h;
+                    if (__float_as_uint(Bq.w) == MRT_PRIMKIND_SPHERE)
+                        h = isect_sph(jo, jd, A, Bq, th);
+                    else
+                        h = isect_tri(jo, jd, A, Bq, Cq, S.one_sided != 0, th, uu, vv);
+                    if (h)
+                        tt = th;
+                }
+                const float t0 = __shfl(tt, pre, 64), u0 = __shfl(uu, pre, 64),
+                            w0 = __shfl(vv, pre, 64);
+                const float t1 = __shfl(tt, pre + 1u, 64), u1 = __shfl(uu, pre + 1u, 64),
+                            w1 = __shfl(vv, pre + 1u, 64);
+                if (served) {
                     if (COUNT) {
                         if (any)
-                            nodes_s++;
+                            prims_s++;
                         else
-                            nodes_c++;
+                            prims_c++;
                     }
-                    float t0, t1, t2, t3;
-                    float k0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv,
-                                      tclose, t0) ? t0 : INF;
-                    float k1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv,
-                                      tclose, t1) ? t1 : INF;
-                    float k2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv,
-                                      tclose, t2) ? t2 : INF;
-                    float k3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv,
-                                      tclose, t3) ? t3 : INF;
-                    int32_t c0 = __float_as_int(cw.x), c1 = __float_as_int(cw.y),
-                            c2 = __float_as_int(cw.z), c3 = __float_as_int(cw.w);
-                    // shadow rays take the hit children in slot order (any
-                    // hit ends them): keyed by slot, the same network puts
-                    // them first in that order, so a wave that holds both
-                    // kinds of ray runs one branch-free sort
-                    if (any) {
-                        k0 = k0 != INF ? 0.0f : INF;
-                        k1 = k1 != INF ? 1.0f : INF;
-                        k2 = k2 != INF ? 2.0f : INF;
-                        k3 = k3 != INF ? 3.0f : INF;
+                    if (t0 < tclose) {
+                        tclose = t0;
+                        hit = true;
+                        hslot = first;
+                        bu = u0;
+                        bv = w0;
+                        found = any;
                     }
-                    cx(k0, c0, k1, c1);
-                    cx(k2, c2, k3, c3);
-                    cx(k0, c0, k2, c2);
-                    cx(k1, c1, k3, c3);
-                    cx(k1, c1, k2, c2);
-                    if (k0 == INF) {
-                        code = pop();
-                    } else {
-                        // the hit children after the nearest, farthest
-                        // first: hit keys are a prefix of the sorted four, so
-                        // with np of them after the nearest, slot sp + j takes
-                        // child np - j.  Three slots are written whatever np
-                        // is (those at sp + np and above are free), without a
-                        // branch per child, while they all lie in LDS
-                        const uint32_t np = (uint32_t)(k1 != INF) + (uint32_t)(k2 != INF)
-                                          + (uint32_t)(k3 != INF);
-                        if (sp + 3 <= SS) {
-                            lds[(uint32_t)sp * TRACE_BLOCK] =
-                                (uint32_t)(np == 3u ? c3 : (np == 2u ? c2 : c1));
-                            lds[(uint32_t)(sp + 1) * TRACE_BLOCK] = (uint32_t)(np == 3u ? c2 : c1);
-                            lds[(uint32_t)(sp + 2) * TRACE_BLOCK] = (uint32_t)c1;
-                            sp += (int)np;
-                        } else {
-                            if (k3 != INF)
-                                push(c3);
-                            if (k2 != INF)
-                                push(c2);
-                            if (k1 != INF)
-                                push(c1);
-                        }
-                        code = c0;
-                    }
-                } else {
-                    // a leaf's next two records (one pair per iteration: a lane
-                    // with more left keeps the rest of the leaf as its code, so no
-                    // lane of the wave waits in an inner loop over a long leaf)
-                    auto test = [&](float4 A, float4 Bq, float4 Cq, uint32_t slot) -> bool {
+                    if (!found && two) {
                         if (COUNT) {
                             if (any)
                                 prims_s++;
                             else
                                 prims_c++;
                         }
-                        float tt, uu = 0.0f, vv = 0.0f;
-                        bool h;
-                        if (__float_as_uint(Bq.w) == MRT_PRIMKIND_SPHERE)
-                            h = isect_sph(o, d, A, Bq, tt);
-                        else
-                            h = isect_tri(o, d, A, Bq, Cq, S.one_sided != 0, tt, uu, vv);
-                        if (h && tt < tclose) {
-                            tclose = tt;
+                        if (t1 < tclose) {
+                            tclose = t1;
                             hit = true;
-                            hslot = slot;
-                            bu = uu;
-                            bv = vv;
-                            return any;
+                            hslot = first + 1u;
+                            bu = u1;
+                            bv = w1;
+                            found = any;
                         }
-                        return false;
-                    };
-                    const uint32_t v = ~(uint32_t)code, first = v >> 3, cnt = (v & 7u) + 1u;
-                    const float4 *p = S.prims + (size_t)first * 3;
-                    float4 A0 = p[0], B0 = p[1], C0 = p[2], A1, B1, C1;
-                    const bool two = cnt > 1;
-                    if (two) {
-                        A1 = p[3];
-                        B1 = p[4];
-                        C1 = p[5];
                     }
-                    found = test(A0, B0, C0, first);
-                    if (!found && two)
-                        found = test(A1, B1, C1, first + 1);
                     // the records after this pair: the same leaf code two on
                     if (!found)
                         code = cnt > 2 ? (int32_t)~(((first + 2u) << 3) | (cnt - 3u)) : pop();
                 }
+            }
+            if (item != HIT_NONE) {
                 if (found || code == DONE) {
                     // the ray's planes: the scene's, or in a batch with
                     // planes those of the ray's scene (S.n_planes is then
