@@ -232,39 +232,56 @@ def batched_leg(M, args):
     tuning = b.autotune(args.depth)
     log(f"batched: autotune {time.perf_counter() - t0:.1f} s -> {tuning}")
     b.render(args.depth)                       # warm-up frame (graph capture)
-    # the mean of two timed renders of the whole batch
-    t_runs = []
-    for _ in range(2):
-        b.clear()
+
+    def batch_run(bt):
+        bt.clear()
         t0 = time.perf_counter()
-        b.render(args.depth)
-        b.readback(0)                          # waits for the stream
-        t_runs.append(time.perf_counter() - t0)
-    t_batch = sum(t_runs) / len(t_runs)
+        bt.render(args.depth)
+        bt.readback(0)                         # waits for the stream
+        return time.perf_counter() - t0
+
+    def plain_pass(size, builder=None):
+        """64 sequential plain launches: per scene a context with the
+        library's defaults, uploaded and its queues reserved outside the
+        timed region, then one mrt_render of all its samples (timed, with
+        its stream drained); contexts are released one after the other, as
+        64 of them at the default path budget would not fit in HBM
+        together.  Returns the summed render time."""
+        t = 0.0
+        for k, (hs, d) in enumerate(zip(scenes, desc)):
+            w, h, spp = size(d)
+            with M.Renderer(w, h) as r:
+                if builder is not None:
+                    r.set_option(M.OPT_BUILDER, builder)
+                if args.lanes is not None:
+                    r.set_option(M.OPT_LANES, args.lanes)
+                r.upload(hs)
+                r.set_seed(d["rng_seed"])
+                r.reserve(spp)
+                if k == 0:
+                    r.render(0, 1, args.depth)     # warm-up (code objects)
+                    r.clear()
+                r.synchronize()
+                t0 = time.perf_counter()
+                r.render(0, spp, args.depth)
+                r.synchronize()
+                t += time.perf_counter() - t0
+        return t
+
+    # timed in the order plain, batch, batch, plain, so that a drift of the
+    # GPU's clocks over the leg weighs on both sides alike; each side's
+    # figure is its faster run (a first pass of short plain renders has
+    # run 4x slower than the second)
+    full = lambda d: (d["w"], d["h"], d["spp"])
+    t_plains = [plain_pass(full)]
+    t_runs = [batch_run(b), batch_run(b)]
+    t_plains.append(plain_pass(full))
+    t_batch = min(t_runs)
+    t_plain = min(t_plains)
     log(f"batched: {samples / t_batch / 1e6:.1f} Msamples/s")
+    log(f"plain sequential: {samples / t_plain / 1e6:.1f} Msamples/s "
+        f"(passes {', '.join(f'{t:.3f}' for t in t_plains)} s)")
     b.close()
-    # 64 sequential plain launches: per scene a context with the library's
-    # defaults, uploaded and its queues reserved outside the timed region,
-    # then one mrt_render of all its samples (timed, with its stream
-    # drained); contexts are released one after the other, as 64 of them
-    # at the default path budget would not fit in HBM together
-    t_plain = 0.0
-    for k, (hs, d) in enumerate(zip(scenes, desc)):
-        with M.Renderer(d["w"], d["h"]) as r:
-            if args.lanes is not None:
-                r.set_option(M.OPT_LANES, args.lanes)
-            r.upload(hs)
-            r.set_seed(d["rng_seed"])
-            r.reserve(d["spp"])
-            if k == 0:
-                r.render(0, 1, args.depth)     # warm-up (code objects)
-                r.clear()
-            r.synchronize()
-            t0 = time.perf_counter()
-            r.render(0, d["spp"], args.depth)
-            r.synchronize()
-            t_plain += time.perf_counter() - t0
-    log(f"plain sequential: {samples / t_plain / 1e6:.1f} Msamples/s")
     # the same 64 geometries as small renders (256^2, 16 spp, 1 M paths
     # each), where a plain render is short and its launches' tails weigh:
     # both sides on device-built trees (fast uploads, the same trees)
@@ -277,34 +294,22 @@ def batched_leg(M, args):
     bs_.commit()
     small["tuning"] = bs_.autotune(args.depth)
     bs_.render(args.depth)
-    bs_.clear()
-    t0 = time.perf_counter()
-    bs_.render(args.depth)
-    bs_.readback(0)
-    t_sb = time.perf_counter() - t0
+    sized = lambda d: (sw, sh, sspp)
+    t_sps = [plain_pass(sized, M.BUILDER_DEVICE)]
+    t_sbs = [batch_run(bs_), batch_run(bs_)]
+    t_sps.append(plain_pass(sized, M.BUILDER_DEVICE))
     bs_.close()
-    t_sp = 0.0
-    for k, (hs, d) in enumerate(zip(scenes, desc)):
-        with M.Renderer(sw, sh) as r:
-            r.set_option(M.OPT_BUILDER, M.BUILDER_DEVICE)
-            r.upload(hs)
-            r.set_seed(d["rng_seed"])
-            r.reserve(sspp)
-            if k == 0:
-                r.render(0, 1, args.depth)
-                r.clear()
-            r.synchronize()
-            t0 = time.perf_counter()
-            r.render(0, sspp, args.depth)
-            r.synchronize()
-            t_sp += time.perf_counter() - t0
+    t_sb = min(t_sbs)
+    t_sp = min(t_sps)
     ssamples = len(desc) * sw * sh * sspp
     small.update({
         "workload": f"the same {len(desc)} scenes at {sw}x{sh}, {sspp} spp",
         "samples": ssamples,
         "msamples_per_s": round(ssamples / t_sb / 1e6, 2),
         "plain_sequential_msamples_per_s": round(ssamples / t_sp / 1e6, 2),
-        "speedup_vs_plain": round(t_sp / t_sb, 3)})
+        "speedup_vs_plain": round(t_sp / t_sb, 3),
+        "batched_runs_s": [round(t, 4) for t in t_sbs],
+        "plain_passes_s": [round(t, 4) for t in t_sps]})
     log(f"small batch: {small['msamples_per_s']} vs plain "
         f"{small['plain_sequential_msamples_per_s']} Msamples/s")
     return {
@@ -319,6 +324,8 @@ def batched_leg(M, args):
         "speedup_vs_plain": round(t_plain / t_batch, 3),
         "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
         "batched_runs_s": [round(t, 3) for t in t_runs],
+        "plain_passes_s": [round(t, 3) for t in t_plains],
+        "order": "plain, batch, batch, plain; each side its faster run",
         "commit_s": round(t_commit, 2),
         "commit_device_builder_s": round(t_commit_dev, 2),
         "tuning": tuning,
