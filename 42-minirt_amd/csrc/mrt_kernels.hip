@@ -677,6 +677,11 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                     if (!found)
                         code = cnt > 2 ? (int32_t)~(((first + 2u) << 3) | (cnt - 3u)) : pop();
                 }
+                // everything this step loaded or shuffled has been used by
+                // now; saying so keeps the compiler from taking those
+                // registers as pending in the next iteration, where it would
+                // drain all memory operations in the middle of the node fetch
+                __builtin_amdgcn_s_waitcnt(0);
             }
             if (item != HIT_NONE) {
                 if (found || code == DONE) {
@@ -691,6 +696,12 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                                    : __float_as_uint(w.q_t[item].w)];
                         pl = S.planes + 2 * (size_t)b.fp.plane0;
                         npl = b.n_planes;
+                        // its loads end here: otherwise the compiler takes
+                        // their registers as pending on every pass through
+                        // the loop and drains all memory operations before
+                        // the next node's last load (about 12 % of a batch
+                        // launch)
+                        __builtin_amdgcn_s_waitcnt(0);
                     }
                     if (!any) {
                         // closest_hit: the planes after the BVH
@@ -710,12 +721,17 @@ k_trace_refill(SceneDev S, TraceWork w, uint32_t *__restrict__ ctr,
                                                   bu, bv);
                     } else {
                         bool occ = hit;
-                        const float tlim = occ ? 0.0f : w.s_o[item - nc].w;
-                        for (uint32_t k = 0; k < npl && !occ; k++) {
-                            float tp;
-                            if (isect_pln(o, d, pl[2 * k], pl[2 * k + 1], tp)
-                                && tp < tlim)
-                                occ = true;
+                        // the ray's length is read only when planes are to
+                        // be tested (a load left unused keeps its register
+                        // pending for the compiler, as above)
+                        if (!occ && npl) {
+                            const float tlim = w.s_o[item - nc].w;
+                            for (uint32_t k = 0; k < npl && !occ; k++) {
+                                float tp;
+                                if (isect_pln(o, d, pl[2 * k], pl[2 * k + 1], tp)
+                                    && tp < tlim)
+                                    occ = true;
+                            }
                         }
                         w.vis[item - nc] = occ ? 0 : 1;
                     }
