@@ -69,8 +69,28 @@ def planes_and_one_sphere(spheres):
     return hs
 
 
+def four_screen_triangles():
+    """Two quads (four triangles) across the whole view, one behind the
+    other: every primary ray reaches the tree's leaves in the same step,
+    so a wave holds 128 leaf records at once, more than its 64 lanes test
+    in one leaf step (the owners past the 64th record wait a step)."""
+    hs = M.HostScene.empty()
+    a = hs.add_material(color=(0.8, 0.5, 0.3))
+    b = hs.add_material(color=(0.9, 0.9, 0.9), metallic=1.0, roughness=0.3)
+    for z, m in ((2.0, a), (3.0, b)):
+        p = [(-9.0, -6.0, z), (9.0, -6.0, z), (9.0, 8.0, z), (-9.0, 8.0, z)]
+        hs.add_triangle([p[0], p[1], p[2]], m)
+        hs.add_triangle([p[0], p[2], p[3]], m)
+    hs.add_light((0, 3, -1), 1.0, (255, 255, 255))
+    hs.set_ambient(0.1, (255, 255, 255))
+    hs.set_camera((0, 1, -3), (0, 0, 1), fov=70.0)
+    hs.set_resolution(256, 144)
+    return hs
+
+
 SCENES = {
     "mesh": lambda: M.HostScene.mesh(3, 256, 144, subdiv=6, floor_cells=48),
+    "four_triangles": four_screen_triangles,
     "plain": lambda: M.HostScene.plain(7, 256, 144),
     "planes": lambda: planes_and_one_sphere(False),
     "one_leaf": lambda: planes_and_one_sphere(True),
